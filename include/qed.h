@@ -1,0 +1,239 @@
+/*
+ * qed.h — C ABI of the MI355X exact-diagonalization impurity solver.
+ *
+ * Every entry point returns 0 on success and a nonzero status on failure; the message
+ * of the last failure on the calling thread is returned by qed_last_error(). Pointers
+ * named *_dev are device (HBM) addresses obtained from qed_alloc(); all others are host
+ * addresses. Complex numbers are interleaved (re, im) doubles; matrices are column-major
+ * as the reference's matrix<T> (src_util/matrix.hpp). Indices of spin-orbitals are
+ * 0-based, spin-down orbital a = orbital + L, as ED::new_operator receives them after
+ * the binding's 1-based -> 0-based conversion (src_python/qcm_ED_wrap.hpp:174-215).
+ *
+ * Each declaration names the reference interface it replaces (file:line in pyqcm).
+ */
+#ifndef QED_H
+#define QED_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qed_sector qed_sector;
+
+/* ---- runtime ------------------------------------------------------------------------ */
+
+/* Selects the HIP device and creates the library's stream. Replaces nothing: the
+ * reference has no device. Called implicitly by every other entry point. */
+int qed_init(int device);
+/* Message of the last failure on this thread (src_util/parser.cpp:323-327 qcm_ED_throw). */
+const char* qed_last_error(void);
+int qed_sync(void);
+int qed_alloc(void** ptr_dev, uint64_t bytes);
+int qed_free(void* ptr_dev);
+int qed_memcpy_h2d(void* dst_dev, const void* src, uint64_t bytes);
+int qed_memcpy_d2h(void* dst, const void* src_dev, uint64_t bytes);
+int qed_memcpy_d2d(void* dst_dev, const void* src_dev, uint64_t bytes);
+int qed_memset(void* ptr_dev, int value, uint64_t bytes);
+
+/* ---- basis (bit-exact with the reference) -------------------------------------------- */
+
+/* Half basis of L orbitals with N particles, ascending = colex order; written on the
+ * device by combinatorial-number-system unranking. out may be NULL to query the size.
+ * Replaces ED_halfbasis::ED_halfbasis (src_ed/ED_basis.cpp:115-125). */
+int qed_halfbasis(int L, int N, uint32_t* out, uint32_t* dim);
+/* Ranks of n half states computed on the device (CNS); dim when the popcount is not N.
+ * Replaces ED_halfbasis::index (src_ed/ED_basis.cpp:130-135). */
+int qed_halfbasis_rank(int L, int N, const uint32_t* states, uint64_t n, uint32_t* ranks);
+
+/* ---- one sector of a cluster Hamiltonian (the Hamiltonian<T> seam) ------------------- */
+
+/* Builds H on the factorized sector (n_up, n_dn) of L orbitals per spin from the active
+ * operators of a model: types[k] 0 = one-body, 1 = interaction (density-density),
+ * coefs[k] the operator value, nels[k] element counts, and the concatenated element
+ * lists (er, ec, ev) with er <= ec. Hermitian completion, sign conventions and the
+ * lower-triangle storage follow the reference.
+ * Replaces Hamiltonian_Factorized ctor + HS_ops_map
+ * (src_ed/Hamiltonian/Hamiltonian_Factorized.hpp:31-87, src_ed/Operators/HS_half_operator.hpp:17-67,
+ *  src_ed/Operators/HS_interaction_operator.hpp:221-243). */
+int qed_sector_create(int L, int n_up, int n_dn, int n_ops, const int* types, const double* coefs,
+                      const int* nels, const int* er, const int* ec, const double* ev,
+                      qed_sector** out);
+int qed_sector_destroy(qed_sector* s);
+int qed_sector_dims(const qed_sector* s, uint64_t* dim, uint32_t* dim_up, uint32_t* dim_dn);
+
+/* y_dev += H x_dev. Replaces Hamiltonian_Factorized::mult_add
+ * (src_ed/Hamiltonian/Hamiltonian_Factorized.hpp:50-58). */
+int qed_hv(qed_sector* s, const double* x_dev, double* y_dev);
+/* Same contract on host vectors (copies in and out), the std::vector form of mult_add. */
+int qed_hv_host(qed_sector* s, const double* x, double* y);
+/* d_dev += diagonal of H as the reference's factorized format reports it: interaction
+ * terms only (HS_factorized_operator::diag is empty, src_ed/Operators/HS_factorized_operator.hpp:103).
+ * Replaces Hamiltonian_Factorized::diag (src_ed/Hamiltonian/Hamiltonian_Factorized.hpp:65-68). */
+int qed_diag(qed_sector* s, double* d_dev);
+
+/* The reference's start vector: x = N(0,1) deviates of std::default_random_engine(seed)
+ * through the process-wide normal_distribution, normalized; written to x_dev.
+ * Replaces random() (src_util/vector_num.cpp:114-119). */
+int qed_start_vector(qed_sector* s, int seed, double* x_dev);
+
+/* Lanczos ground state: eigenvalue pass then eigenvector replay. psi_dev holds the
+ * normalized start vector on entry and the ground state on exit. Fails with the
+ * reference's message beyond max_iter iterations. alpha/beta (may be NULL) receive the
+ * tridiagonal coefficients (capacity max_iter + 1).
+ * Replaces Lanczos / LanczosEigenvalue / LanczosEigenvector (src_ed/Hamiltonian/Lanczos.hpp:12-143). */
+int qed_lanczos_gs(qed_sector* s, double accur, int max_iter, double* psi_dev, double* E0,
+                   int* niter, double* alpha, double* beta);
+
+/* Scalar Lanczos from a normalized psi_dev for the continued-fraction Green function;
+ * a (capacity max_iter) and b (capacity max_iter + 1) receive the coefficients.
+ * Replaces LanczosGreen (src_ed/Hamiltonian/Lanczos.hpp:156-183). */
+int qed_lanczos_green(qed_sector* s, const double* psi_dev, double accur_cf, int max_iter,
+                      double* a, double* b, int* n);
+
+/* Band Lanczos on p start vectors phi_dev (p consecutive sector vectors). M0 <= 0 uses
+ * 14 p ln(dim). On exit M poles: evals[M] and Q (p x M, column-major) = P U.
+ * Capacity of evals/Q is cap columns (cap >= max_iter_BL + p is always enough).
+ * Replaces Hamiltonian::build_Q_matrix + bandLanczos
+ * (src_ed/Hamiltonian/Hamiltonian_base.hpp:229-254, src_ed/Hamiltonian/Lanczos.hpp:214-395). */
+int qed_band_lanczos(qed_sector* s, const double* phi_dev, int p, int M0, int max_iter_BL,
+                     double accur_deflation, double accur_band_lanczos, double* evals, double* Q,
+                     int cap, int* M);
+
+/* Davidson-Liu for the k lowest states (device vectors psi_dev: k consecutive sector
+ * vectors, out). Replaces Davidson (src_ed/Hamiltonian/Davidson.hpp:38-230). */
+int qed_davidson(qed_sector* s, int k, double accur, int max_iter, double* evals, double* psi_dev,
+                 int* niter);
+
+/* y_dev += z * c_a x_dev (pm = -1) or z * c†_a x_dev (pm = +1); x lives in (n_up, n_dn)
+ * of L orbitals per spin, y in the shifted sector. Replaces the factorized branch of
+ * model::create_or_destroy (src_ed/model.cpp:155-180) and Destroy (src_ed/ED_basis.cpp:79-110). */
+int qed_apply_c(int L, int n_up, int n_dn, int a, int pm, double z, const double* x_dev,
+                double* y_dev);
+
+/* ---- Green-function representations ---------------------------------------------- */
+
+/* G(z_i) += sum_m Q_am conj(Q_bm) / (z_i - e_m) for nz frequencies; Q real (L x M col-major),
+ * G (nz x L x L complex col-major, element (b,a) at b + a L as the reference writes it).
+ * Replaces Q_matrix::Green_function + VDVH kernel (src_ed/Q_matrix.hpp:133-151,
+ * src_util/VDVH_kernel.cpp:28-48). */
+int qed_qmatrix_gf(int L, int M, const double* e, const double* Q, int nz, const double* z,
+                   double* G);
+
+/* Continued fractions: n fractions of depth[f] coefficients (a, b concatenated with
+ * offsets off[f]); out[f + n*i] = b0/(z_i - a0 - b1/(z_i - a1 - ...)).
+ * Replaces continued_fraction::evaluate (src_ed/continued_fraction.cpp:52-57). */
+int qed_cf_eval(int n, const int* off, const int* depth, const double* a, const double* b, int nz,
+                const double* z, double* out);
+
+/* ---- lattice k-sums (CPT / VCA / CDMFT) ----------------------------------------- */
+
+/* V(k) = V0 + sum_t Vt[t] exp(2 pi i k . nv[t]) (n x n complex each; nv[t] 3 doubles).
+ * CPT: for every (w, k): Gcpt = Gc(w) (1 - V(k) Gc(w))^-1.
+ *   mode 0: out = full Gcpt (nw x nk x n x n complex)
+ *   mode 1: out = trace of Gcpt (nw x nk complex)
+ *   mode 2: out = sum over k of Gcpt / nk (nw x n x n complex) (projected G, CDMFT host)
+ * Replaces lattice_model_instance::set_V / set_Gcpt / projected_Green_function
+ * (src_qcm/CPT.cpp:22-86, 283-345) and QCM::CPT_Green_function (src_qcm/QCM.cpp:328-344). */
+int qed_cpt_batch(int n, int nw, const double* Gc, const double* V0, int nterm, const double* Vt,
+                  const double* nv, int nk, const double* k, int mode, double* out);
+
+/* VCA: out[w] = (1/nk) sum_k log|det(V(k) Gc(w) - 1)| per frequency, same V(k) model.
+ * Replaces SEF_integrand_k summed by k_integral_grid
+ * (src_qcm/lattice_model_instance.cpp:624-643, src_util/integrate.cpp:177-239). */
+int qed_sef_batch(int n, int nw, const double* Gc, const double* V0, int nterm, const double* Vt,
+                  const double* nv, int nk, const double* k, double* out);
+
+/* CDMFT distance for B independent bath parameter sets (batched bath fit):
+ *   d[b] = scale * sum_w weight[w] || Gamma_b(i w) + Ghost[w] ||_F^2,
+ *   Gamma_b(z)_{ij} = sum_m tcb_b[i,m] tcb_b[j,m] / (z - eb_b[m]).
+ * n = cluster sites, nb = bath orbitals. eb (B x nb), tcb (B x n x nb col-major per set).
+ * Replaces lattice_model_instance::CDMFT_distance (src_qcm/CPT.cpp:519-555). */
+int qed_cdmft_distance_batch(int n, int nb, int nw, const double* w, const double* weight,
+                             const double* Ghost, int B, const double* eb, const double* tcb,
+                             double scale, double* d);
+
+/* ---- the ED solver interface (namespace ED, src_ed/qcm_ED.hpp:23-340) -------------- */
+
+/* ED::new_model (src_ed/qcm_ED.cpp:39-58); trivial symmetry group only. */
+int qed_ed_new_model(const char* name, int n_sites, int n_bath);
+/* ED::new_operator (src_ed/qcm_ED.cpp:73-92): type "one-body" or "interaction";
+ * 0-based indices (the binding converts 1-based labels). */
+int qed_ed_new_operator(const char* model, const char* name, const char* type, int nel,
+                        const int* r, const int* c, const double* v);
+/* ED::new_model_instance (src_ed/qcm_ED.cpp:127-158); sectors "R0:N4:S0/R0:N6:S0". */
+int qed_ed_new_model_instance(const char* model, int nparam, const char* const* names,
+                              const double* values, const char* sectors, uint64_t label);
+int qed_ed_erase_model_instance(uint64_t label);
+/* ED::ground_state_solve (src_ed/qcm_ED.cpp:179-188); sector string into buf. */
+int qed_ed_ground_state_solve(uint64_t label, double* E0, char* buf, int buflen);
+/* ED::Green_function_solve (src_ed/qcm_ED.cpp, model_instance.hpp:517-555). */
+int qed_ed_green_function_solve(uint64_t label);
+/* ED::Green_function (model_instance.hpp:565-635) for nz frequencies at once:
+ * G (nz x d x d complex, column-major), d = dim_GF. */
+int qed_ed_green_function(uint64_t label, int nz, const double* z, int spin_down, double* G);
+/* ED::self_energy (model_instance.hpp:683-700), batched over z. */
+int qed_ed_self_energy(uint64_t label, int nz, const double* z, int spin_down, double* S);
+/* ED::hybridization_function (model_instance.hpp:1004-1018), batched over z. */
+int qed_ed_hybridization_function(uint64_t label, int nz, const double* z, int spin_down,
+                                  double* Gam);
+/* ED::hopping_matrix (cluster part tc, complex d x d). */
+int qed_ed_hopping_matrix(uint64_t label, int spin_down, double* tc);
+/* Bath matrices after diagonalization: eb (nb), tcb (d x nb column-major). */
+int qed_ed_bath(uint64_t label, int spin_down, double* eb, double* tcb);
+/* ED::qmatrix (model_instance.hpp:103): M poles, e (M) and Q (d x M column-major);
+ * pass e = NULL to query M. */
+int qed_ed_qmatrix(uint64_t label, int spin_down, int* M, double* e, double* Q);
+/* ED::Potthoff_functional (src_ed/qcm_ED.cpp:330-334): E0 + SEF_bath. */
+int qed_ed_potthoff_functional(uint64_t label, double* omega);
+/* ED::model_size; dims of the Green function. */
+int qed_ed_model_size(const char* model, int* n_sites, int* n_bath);
+int qed_ed_dim_gf(uint64_t label, int* d);
+/* ED::cluster_averages (model_instance.hpp:148-175): for each active operator, in the
+ * order of names, average and variance. */
+int qed_ed_cluster_averages(uint64_t label, int nop, const char* const* names, double* ave,
+                            double* var);
+/* Global parameters (src_util/global_parameter.cpp:12-91): int/double/bool/char maps. */
+int qed_ed_set_global_int(const char* name, int v);
+int qed_ed_set_global_double(const char* name, double v);
+int qed_ed_set_global_bool(const char* name, int v);
+int qed_ed_set_global_char(const char* name, char v);
+int qed_ed_get_global_double(const char* name, double* v);
+
+/* ---- batched CDMFT impurity solves ---------------------------------------------- */
+
+/* Solves nb independent (cluster + bath) instances of one model topology in one call:
+ * ground state by Lanczos and Green function by band Lanczos for each, all vectors in
+ * one arena, the H.v of all instances issued as one persistent launch per Krylov step
+ * (captured in a HIP graph when graph != 0). params: nb x nparam values in the order of
+ * names; sectors: one sector string per instance. Outputs E0[nb] and the poles of each
+ * instance, packed: Mout[nb], e (nb x cap), Q (nb x d x cap). */
+int qed_batch_solve(const char* model, int nparam, const char* const* names, int nb,
+                    const double* params, const char* const* sectors, int graph, int cap,
+                    double* E0, int* Mout, double* e, double* Q);
+
+/* ---- learned warm start (MLP 3 -> 128 -> 128 -> 16) ------------------------------ */
+
+/* Opaque MLP with parameters resident on the device. */
+typedef struct qed_mlp qed_mlp;
+int qed_mlp_create(int n_in, int n_hidden, int n_out, uint64_t seed, qed_mlp** out);
+int qed_mlp_destroy(qed_mlp* m);
+int qed_mlp_get_params(qed_mlp* m, float* params, int* count);
+int qed_mlp_set_params(qed_mlp* m, const float* params, int count);
+/* Forward pass on n samples (x: n x n_in row-major host), FP32; y: n x n_out. */
+int qed_mlp_forward(qed_mlp* m, int n, const float* x, float* y);
+/* Forward pass with bf16 weights and MFMA (inference). */
+int qed_mlp_forward_bf16(qed_mlp* m, int n, const float* x, float* y);
+/* Mean-squared-error loss and gradient (grad: same layout as params) on one batch. */
+int qed_mlp_loss_grad(qed_mlp* m, int n, const float* x, const float* t, float* loss,
+                      float* grad);
+/* Adam training: epochs over (x, t) with minibatch size bs; loss history per epoch. */
+int qed_mlp_train(qed_mlp* m, int n, const float* x, const float* t, int epochs, int bs, float lr,
+                  uint64_t seed, float* loss_hist);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QED_H */

@@ -1,0 +1,515 @@
+"""ORACLE — test infrastructure only (never imported by the product path).
+
+Python half of the CPU oracle for the ED impurity-solver hot path:
+
+* ctypes access to ``liboracle.so`` (the C++ restatement in ``qed_oracle.cpp``);
+* an independent Jordan-Wigner brute force (dense fermion matrices) used to pin the
+  restatement on small clusters;
+* numpy restatements of the reference's band Lanczos, Q-matrix post-processing,
+  continued fractions and the small dense lattice kernels (CPT, SEF, CDMFT distance).
+
+Reference provenance of each function is given in its docstring. Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg import this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-f", os.path.join(_HERE, "Makefile")])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        P = ctypes.POINTER
+        dptr = P(ctypes.c_double)
+        iptr = P(ctypes.c_int)
+        L.or_halfbasis.argtypes = [ctypes.c_int, ctypes.c_int, P(ctypes.c_uint32)]
+        L.or_halfbasis.restype = ctypes.c_int
+        L.or_sector_new.argtypes = [ctypes.c_int] * 4 + [iptr, dptr, iptr, iptr, iptr, dptr]
+        L.or_sector_new.restype = ctypes.c_void_p
+        L.or_sector_free.argtypes = [ctypes.c_void_p]
+        L.or_sector_dim.argtypes = [ctypes.c_void_p]
+        L.or_sector_dim.restype = ctypes.c_uint64
+        L.or_hv.argtypes = [ctypes.c_void_p, dptr, dptr]
+        L.or_hv_omp.argtypes = [ctypes.c_void_p, dptr, dptr, ctypes.c_int]
+        L.or_start_vector.argtypes = [ctypes.c_uint64, ctypes.c_int, dptr]
+        L.or_reset_normal.argtypes = []
+        L.or_lanczos_gs.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, dptr, dptr, iptr,
+                                    dptr, dptr]
+        L.or_lanczos_gs.restype = ctypes.c_int
+        L.or_lanczos_green.argtypes = [ctypes.c_void_p, dptr, ctypes.c_double, ctypes.c_int, dptr, dptr]
+        L.or_lanczos_green.restype = ctypes.c_int
+        L.or_create_destroy.argtypes = [ctypes.c_int] * 5 + [dptr, dptr, ctypes.c_double]
+        L.or_create_destroy.restype = ctypes.c_int
+        L.or_dstev.argtypes = [ctypes.c_int, dptr, dptr, dptr]
+        L.or_dstev.restype = ctypes.c_int
+        L.or_last_error.restype = ctypes.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+# ----------------------------------------------------------------------------------------
+# model description shared by oracle and tests
+
+
+@dataclass
+class Operator:
+    """A cluster operator as ED::new_operator receives it: 0-based spin-orbital element list
+    (down orbital = orbital + L) with r <= c, and its type ('one-body' or 'interaction')."""
+    name: str
+    kind: str
+    elements: list  # [(r, c, v)]
+
+
+@dataclass
+class ClusterModel:
+    L: int                      # orbitals per spin (sites + bath)
+    operators: dict = field(default_factory=dict)
+
+    def add(self, name, kind, elements):
+        self.operators[name] = Operator(name, kind, list(elements))
+
+    def flat(self, values):
+        """Flattened element arrays for the active operators (reference order = insertion)."""
+        types, coefs, nels, er, ec, ev = [], [], [], [], [], []
+        for name, op in self.operators.items():
+            if name not in values or values[name] == 0.0:
+                continue
+            types.append(0 if op.kind == "one-body" else 1)
+            coefs.append(float(values[name]))
+            nels.append(len(op.elements))
+            for (r, c, v) in op.elements:
+                er.append(r)
+                ec.append(c)
+                ev.append(v)
+        return (np.array(types, np.int32), np.array(coefs, np.float64), np.array(nels, np.int32),
+                np.array(er, np.int32), np.array(ec, np.int32), np.array(ev, np.float64))
+
+
+def hubbard_cluster(sites, bonds, t=1.0, tp_bonds=(), tp=0.0, U=0.0, mu=0.0, n_bath=0):
+    """Cluster model with pyqcm's lattice conventions: hopping amplitude -1 per bond
+    (src_qcm/lattice_model.cpp:968-1019 with amplitude -1), mu as -n per orbital
+    (src_qcm/lattice_model.cpp:286-295), Hubbard (i, i+L, 1) (lattice_model.cpp:929-933)."""
+    L = sites + n_bath
+    m = ClusterModel(L)
+    el = []
+    for (i, j) in bonds:
+        a, b = min(i, j), max(i, j)
+        el += [(a, b, -1.0), (a + L, b + L, -1.0)]
+    m.add("t", "one-body", el)
+    if tp_bonds:
+        el = []
+        for (i, j) in tp_bonds:
+            a, b = min(i, j), max(i, j)
+            el += [(a, b, -1.0), (a + L, b + L, -1.0)]
+        m.add("tp", "one-body", el)
+    m.add("U", "interaction", [(i, i + L, 1.0) for i in range(sites)])
+    m.add("mu", "one-body", [(i, i, -1.0) for i in range(sites)] + [(i + L, i + L, -1.0) for i in range(sites)])
+    vals = {"t": t, "U": U, "mu": mu}
+    if tp_bonds:
+        vals["tp"] = tp
+    return m, vals
+
+
+def square_bonds(nx, ny):
+    """Open-boundary nearest and next-nearest neighbour bonds of an nx x ny cluster,
+    site index x + nx*y."""
+    nn, nnn = [], []
+    idx = lambda x, y: x + nx * y
+    for y in range(ny):
+        for x in range(nx):
+            if x + 1 < nx:
+                nn.append((idx(x, y), idx(x + 1, y)))
+            if y + 1 < ny:
+                nn.append((idx(x, y), idx(x, y + 1)))
+            if x + 1 < nx and y + 1 < ny:
+                nnn.append((idx(x, y), idx(x + 1, y + 1)))
+            if x - 1 >= 0 and y + 1 < ny:
+                nnn.append((idx(x, y), idx(x - 1, y + 1)))
+    return nn, nnn
+
+
+# ----------------------------------------------------------------------------------------
+# C++ restatement access
+
+
+def halfbasis(L, N):
+    n = lib().or_halfbasis(L, N, None)
+    out = np.zeros(n, np.uint32)
+    lib().or_halfbasis(L, N, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    return out
+
+
+class Sector:
+    """One (N_up, N_dn) sector of a cluster model in the factorized basis."""
+
+    def __init__(self, model: ClusterModel, values: dict, nup: int, ndn: int):
+        self.model, self.values, self.nup, self.ndn = model, values, nup, ndn
+        arrs = model.flat(values)
+        self._keep = arrs
+        t, c, n, r, cc, v = arrs
+        self.h = lib().or_sector_new(model.L, nup, ndn, len(t), _ip(t), _dp(c), _ip(n), _ip(r), _ip(cc), _dp(v))
+        if not self.h:
+            raise RuntimeError(lib().or_last_error().decode())
+        self.dim = int(lib().or_sector_dim(self.h))
+        self.du = len(halfbasis(model.L, nup))
+        self.dd = len(halfbasis(model.L, ndn))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_sector_free(self.h)
+
+    def hv(self, x, y=None):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.dim) if y is None else np.array(y, np.float64, copy=True)
+        lib().or_hv(self.h, _dp(x), _dp(y))
+        return y
+
+    def hv_omp(self, x, y=None, threads=0):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.dim) if y is None else np.array(y, np.float64, copy=True)
+        lib().or_hv_omp(self.h, _dp(x), _dp(y), threads)
+        return y
+
+    def start_vector(self, seed=0):
+        x = np.zeros(self.dim)
+        lib().or_start_vector(self.dim, seed, _dp(x))
+        return x
+
+    def lanczos_gs(self, seed=0, accur=1e-12, max_iter=1000, start=None):
+        gs = self.start_vector(seed) if start is None else np.array(start, np.float64, copy=True)
+        E0 = np.zeros(1)
+        nit = np.zeros(1, np.int32)
+        a = np.zeros(max_iter + 2)
+        b = np.zeros(max_iter + 2)
+        rc = lib().or_lanczos_gs(self.h, accur, max_iter, _dp(gs), _dp(E0), _ip(nit), _dp(a), _dp(b))
+        if rc != 0:
+            raise RuntimeError(lib().or_last_error().decode())
+        n = int(nit[0])
+        return float(E0[0]), gs, n, a[:n].copy(), b[:n].copy()
+
+    def lanczos_green(self, psi, accur_cf=0.01, max_iter=400):
+        psi = np.ascontiguousarray(psi, np.float64)
+        a = np.zeros(max_iter + 2)
+        b = np.zeros(max_iter + 2)
+        n = lib().or_lanczos_green(self.h, _dp(psi), accur_cf, max_iter, _dp(a), _dp(b))
+        return a[:n].copy(), b[:n + 1].copy()
+
+
+def create_destroy(L, nup, ndn, a, pm, x, y=None, z=1.0):
+    """y += z c_a x (pm=-1) or z c†_a x (pm=+1); a is a spin-orbital index."""
+    tup, tdn = nup, ndn
+    if a >= L:
+        tdn += pm
+    else:
+        tup += pm
+    dt = len(halfbasis(L, tup)) * len(halfbasis(L, tdn))
+    y = np.zeros(dt) if y is None else y
+    x = np.ascontiguousarray(x, np.float64)
+    lib().or_create_destroy(L, nup, ndn, a, pm, _dp(x), _dp(y), z)
+    return y
+
+
+def dstev(d, e, vectors=True):
+    d = np.array(d, np.float64, copy=True)
+    n = len(d)
+    e2 = np.zeros(max(n, 1))
+    e2[:n - 1] = e[:n - 1]
+    z = np.zeros((n, n)) if vectors else None
+    info = lib().or_dstev(n, _dp(d), _dp(e2), _dp(z) if vectors else None)
+    if info != 0:
+        raise RuntimeError("dstev info %d" % info)
+    # LAPACK writes column-major, so row k of the C-ordered array is eigenvector k
+    return (d, z) if vectors else d
+
+
+# ----------------------------------------------------------------------------------------
+# independent brute force (Jordan-Wigner), small clusters only
+
+
+def _fock_ops(L):
+    """Annihilation operators c_a (a < 2L) on the 2^(2L) Fock space, standard Jordan-Wigner
+    string over lower orbital indices (up orbitals 0..L-1 before down orbitals L..2L-1)."""
+    import scipy.sparse as sp
+    n = 2 * L
+    dimF = 1 << n
+    states = np.arange(dimF, dtype=np.int64)
+    ops = []
+    for a in range(n):
+        occ = (states >> a) & 1
+        src = states[occ == 1]
+        tgt = src ^ (1 << a)
+        below = src & ((1 << a) - 1)
+        par = np.array([bin(int(s)).count("1") & 1 for s in below])
+        val = np.where(par, -1.0, 1.0)
+        ops.append(sp.csr_matrix((val, (tgt, src)), shape=(dimF, dimF)))
+    return ops
+
+
+def sector_states(L, nup, ndn):
+    """Fock-space labels of the factorized sector basis, in the factorized index order
+    I = i_up + dim_up * i_dn (i_up, i_dn ranks in ascending order)."""
+    bu = [s for s in range(1 << L) if bin(s).count("1") == nup]
+    bd = [s for s in range(1 << L) if bin(s).count("1") == ndn]
+    return np.array([u | (d << L) for d in bd for u in bu], np.int64)
+
+
+def dense_hamiltonian(model: ClusterModel, values, nup, ndn):
+    """Dense H on one sector, built from Jordan-Wigner matrices (independent of the
+    restatement's half-operator tables)."""
+    L = model.L
+    c = _fock_ops(L)
+    cd = [m.T.tocsr() for m in c]
+    dimF = 1 << (2 * L)
+    import scipy.sparse as sp
+    H = sp.csr_matrix((dimF, dimF))
+    for name, op in model.operators.items():
+        if name not in values or values[name] == 0.0:
+            continue
+        z = values[name]
+        for (r, cc, v) in op.elements:
+            if op.kind == "one-body":
+                if r < cc:
+                    H = H + z * v * (cd[r] @ c[cc] + cd[cc] @ c[r])
+                elif r == cc:
+                    H = H + z * v * (cd[r] @ c[r])
+            else:
+                a, b = min(r, cc), max(r, cc)
+                H = H + z * v * ((cd[a] @ c[a]) @ (cd[b] @ c[b]))
+    idx = sector_states(L, nup, ndn)
+    return H[idx][:, idx].toarray(), (c, cd)
+
+
+def dense_gf(model, values, nup, ndn, orbitals, zs):
+    """Brute-force cluster Green function G_ab(z) = <c_a (z-H+E0)^-1 c†_b> + <c†_b (z+H-E0)^-1 c_a>
+    for spin-up orbitals, from dense sector Hamiltonians (T=0, nondegenerate ground state)."""
+    L = model.L
+    H0, (c, cd) = dense_hamiltonian(model, values, nup, ndn)
+    e, U = np.linalg.eigh(H0)
+    E0, psi = e[0], U[:, 0]
+    i0 = sector_states(L, nup, ndn)
+    dimF = 1 << (2 * L)
+    full = np.zeros(dimF)
+    full[i0] = psi
+    Hp, _ = dense_hamiltonian(model, values, nup + 1, ndn)
+    Hm, _ = dense_hamiltonian(model, values, nup - 1, ndn)
+    ip = sector_states(L, nup + 1, ndn)
+    im = sector_states(L, nup - 1, ndn)
+    phip = np.array([(cd[a] @ full)[ip] for a in orbitals])
+    phim = np.array([(c[a] @ full)[im] for a in orbitals])
+    ep, Up = np.linalg.eigh(Hp)
+    em, Um = np.linalg.eigh(Hm)
+    Qp = phip @ Up          # (n_orb, dim+)
+    Qm = phim @ Um
+    out = []
+    for z in zs:
+        G = (Qp / (z - (ep - E0))) @ Qp.T + (Qm / (z + (em - E0))) @ Qm.T
+        out.append(G)
+    return E0, np.array(out)
+
+
+# ----------------------------------------------------------------------------------------
+# band Lanczos and Q-matrix (numpy restatement)
+
+
+def band_lanczos(hv, phi, accur_deflation=1e-7, accur_band_lanczos=1e-12, max_iter_BL=600, M0=None):
+    """Freund band Lanczos as in src_ed/Hamiltonian/Lanczos.hpp:214-395 with the
+    iteration budget of src_ed/Hamiltonian/Hamiltonian_base.hpp:237. Returns (e, Q) with
+    Q = P U (p x M), or None when no iteration ran."""
+    phi = [np.asarray(v, np.float64) for v in phi]
+    p0 = len(phi)
+    dim = len(phi[0])
+    if M0 is None:
+        M0 = int(14 * p0 * np.log(1.0 * dim))
+    pc = p0
+    if M0 > max_iter_BL:
+        M0 = max_iter_BL
+    M0p = (M0 // p0 + 1) * p0
+    M0 = M0p - pc
+    v = [None] * (M0p + 1)
+    for i in range(p0):
+        v[i] = phi[i].copy()
+    t = np.zeros((M0, M0))
+    P = np.zeros((p0, M0))
+    defl = np.zeros(M0, np.int32)
+    I = []
+    ev_old = 1e12
+    j = 0
+    while j < M0:
+        z = np.linalg.norm(v[j])
+        if z < accur_deflation:
+            if j >= pc:
+                I.append(j - pc)
+                defl[j - pc] = 1
+            pc -= 1
+            if pc == 0:
+                break
+            for k in range(j, j + pc):
+                v[k] = v[k + 1]
+            v[j + pc] = None
+            continue
+        v[j] = v[j] / z
+        if j >= pc:
+            t[j, j - pc] = z
+        for k in range(p0):
+            P[k, j] = phi[k] @ v[j]
+        for k in range(j + 1, j + pc):
+            zz = v[j] @ v[k]
+            v[k] = v[k] - zz * v[j]
+            if k >= pc:
+                t[j, k - pc] = zz
+        w = hv(v[j])
+        k0 = j - pc if j > pc else 0
+        for k in range(k0, j):
+            t[k, j] = t[j, k]
+            w = w - t[k, j] * v[k]
+        for k in I:
+            ww = v[k] @ w
+            w = w - ww * v[k]
+            t[k, j] = ww
+        ww = v[j] @ w
+        w = w - ww * v[j]
+        t[j, j] = ww
+        for k in I:
+            t[j, k] = t[k, j]
+        v[j + pc] = w
+        if j >= 4 * p0 and j % p0 == 0:
+            evs = np.linalg.eigvalsh(t[:j, :j])
+            test = abs(evs[0] - ev_old)
+            ev_old = evs[0]
+            if test < accur_band_lanczos:
+                break
+        j += 1
+    if j == 0:
+        return None
+    e, U = np.linalg.eigh(t[:j, :j])
+    return e, P[:, :j] @ U
+
+
+def dense_qmatrix(hv, dim, phi):
+    """Full-diagonalization Lehmann form used below max_dim_full
+    (src_ed/Hamiltonian/Hamiltonian_Dense.hpp:157-176): Q = phi . U."""
+    H = np.array([hv(np.eye(dim)[i]) for i in range(dim)]).T
+    e, U = np.linalg.eigh(H)
+    return e, np.array(phi) @ U
+
+
+def streamline(e, Q, wtol=1e-5, vtol=1e-10):
+    """Merges poles closer than wtol and drops contributions below vtol
+    (src_ed/Q_matrix.hpp:180-223, 285-318)."""
+    if vtol < 1e-12:
+        return e, Q
+    L, M = Q.shape
+    e2, cols = [], []
+    i = 0
+    while i < M:
+        j = i + 1
+        while j < M and abs(e[j] - e[i]) <= wtol:
+            j += 1
+        n = j - i
+        if n == 1:
+            col = Q[:, i]
+            if np.sum(np.abs(col) ** 2) >= vtol:
+                cols.append(col.reshape(L, 1))
+                e2.append(e[i])
+        else:
+            blk = Q[:, i:j]
+            QQ = blk @ blk.conj().T
+            d, Uq = np.linalg.eigh(QQ)
+            keep = [k for k in range(L) if d[k] > vtol]
+            for k in keep:
+                cols.append((Uq[:, k] * np.sqrt(d[k])).reshape(L, 1))
+                e2.append(e[i])
+        i = j
+    if not cols:
+        return np.zeros(0), np.zeros((L, 0))
+    return np.array(e2), np.hstack(cols)
+
+
+def check_norm(Q, threshold=1e-5, norm=1.0):
+    """Unitarity error of a Q-matrix (src_ed/Q_matrix.hpp:235-260)."""
+    S = Q @ Q.conj().T
+    D = S - norm * np.eye(S.shape[0])
+    return float(np.sum(np.abs(D) ** 2).real)
+
+
+def qmatrix_gf(e, Q, z):
+    """G_ab(z) = sum_i Q_ai conj(Q_bi)/(z - e_i) (src_ed/Q_matrix.hpp:133-151)."""
+    return (Q / (z - e)) @ Q.conj().T
+
+
+def cf_coefficients(a, b, e0, norm, create):
+    """src_ed/continued_fraction.cpp:33-39."""
+    a = np.array(a, np.float64, copy=True)
+    b = np.array(b, np.float64, copy=True) ** 2
+    b[0] = norm
+    if create:
+        a = a - e0
+    else:
+        a = -a + e0
+    return a, b
+
+
+def cf_evaluate(a, b, z):
+    """src_ed/continued_fraction.cpp:52-57 (bottom-up)."""
+    G = 0.0 + 0.0j
+    for i in range(len(a) - 1, -1, -1):
+        G = b[i] / (z - a[i] - G)
+    return G
+
+
+# ----------------------------------------------------------------------------------------
+# lattice kernels (dense numpy, one (k, w) at a time)
+
+
+def cpt_green(Gc, V):
+    """G_cpt = G_c (1 - V G_c)^-1 (src_qcm/CPT.cpp:76-86)."""
+    n = Gc.shape[0]
+    return Gc @ np.linalg.inv(np.eye(n) - V @ Gc)
+
+
+def sef_integrand(Gc, V, factor=2.0):
+    """log|det(V G - 1)| times 2 for spin-degenerate mixing (src_qcm/lattice_model_instance.cpp:624-643)."""
+    n = Gc.shape[0]
+    return factor * np.log(abs(np.linalg.det(V @ Gc - np.eye(n))))
+
+
+def hybridization(w, eb, tcb):
+    """Gamma_ab(w) = sum_i tcb(a,i) conj(tcb(b,i)) / (w - eb_i) (src_ed/model_instance.hpp:1004-1018)."""
+    return (tcb / (w - eb)) @ tcb.conj().T
+
+
+def cdmft_distance(freqs, weights, G_host, eb, tcb, mixing_factor=2.0):
+    """src_qcm/CPT.cpp:519-555 for one cluster, spin-degenerate (mixing 0)."""
+    dist = 0.0
+    dim = G_host[0].shape[0]
+    for i, wv in enumerate(freqs):
+        g = hybridization(1j * wv, eb, tcb) + G_host[i]
+        dist += np.sum(np.abs(g) ** 2) * weights[i]
+    dist *= mixing_factor
+    dist *= freqs[1] - freqs[0]
+    dist /= dim * dim
+    return dist

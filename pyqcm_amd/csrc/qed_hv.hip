@@ -1,0 +1,608 @@
+// Sector construction (device-side ranking by the combinatorial number system) and the
+// on-the-fly factorized H.v kernels.
+//
+// Layout in HBM: a sector vector is the reference's factorized order I = i_up + du * i_dn
+// (src_ed/ED_basis.cpp:142-165), i.e. a dd x du row-major FP64 matrix X[i_dn][i_up].
+//   up-spin hops act inside one row  (gathers within X[r, :])
+//   down-spin hops combine rows      (Y[r, :] += v * X[r', :])
+// Hop tables are CSR over half states with packed uint32 entries
+// (target rank | value index << 20) and a small table of signed amplitudes.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <random>
+
+#include "../../include/qed.h"
+#include "qed_sector.hpp"
+
+namespace qed {
+
+namespace {
+
+constexpr uint32_t kTargetBits = 20;
+constexpr uint32_t kTargetMask = (1u << kTargetBits) - 1u;
+constexpr uint32_t kMaxVals = 1u << (32 - kTargetBits);
+
+struct Bond {
+  int from, to;  // c†_to c_from
+};
+
+__global__ void k_unrank(uint32_t* out, uint32_t n, int L, int N, const uint32_t* binom) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = cns_unrank(i, L, N, binom);
+}
+
+__global__ void k_rank(const uint32_t* states, uint32_t* ranks, uint64_t n, int N, uint32_t dim,
+                       uint32_t fullmask, const uint32_t* binom) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t s = states[i];
+  ranks[i] = ((s & ~fullmask) == 0 && __popc(s) == N) ? cns_rank(s, binom) : dim;
+}
+
+// number of allowed hops per half state
+__global__ void k_hop_count(const uint32_t* bin, uint32_t dim, const Bond* bonds, int nb,
+                            uint32_t* cnt) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim) return;
+  uint32_t s = bin[i], c = 0;
+  for (int b = 0; b < nb; ++b) {
+    uint32_t mf = 1u << bonds[b].from, mt = 1u << bonds[b].to;
+    c += ((s & mf) && !(s & mt)) ? 1u : 0u;
+  }
+  cnt[i] = c;
+}
+
+// hop entries (target rank by CNS, Jordan-Wigner sign) and the one-body diagonal
+__global__ void k_hop_fill(const uint32_t* bin, uint32_t dim, const Bond* bonds, int nb,
+                           const uint32_t* ptr, uint32_t* ent, const int* dsite,
+                           const double* dval, int nd, double* diag, const uint32_t* binom) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim) return;
+  uint32_t s = bin[i], k = ptr[i];
+  for (int b = 0; b < nb; ++b) {
+    int f = bonds[b].from, t = bonds[b].to;
+    uint32_t mf = 1u << f, mt = 1u << t;
+    if (!(s & mf) || (s & mt)) continue;
+    int lo = f < t ? f : t, hi = f < t ? t : f;
+    uint32_t between = ((1u << hi) - 1u) & ~((1u << (lo + 1)) - 1u);
+    int neg = __popc(s & between) & 1;
+    uint32_t j = cns_rank(s ^ mf ^ mt, binom);
+    ent[k++] = j | ((uint32_t)(2 * b + neg) << kTargetBits);
+  }
+  double d = 0.0;
+  for (int q = 0; q < nd; ++q)
+    if (s & (1u << dsite[q])) d += dval[q];
+  diag[i] = d;
+}
+
+// density-density energy of the factorized basis state (su, sd)
+__device__ __forceinline__ double interaction(const HvView& v, uint32_t su, uint32_t sd) {
+  if (v.hubbard_uniform) return v.hub_U * (double)__popc(su & sd & v.hub_sites);
+  uint64_t w = (uint64_t)sd | ((uint64_t)su << 32);
+  double acc = 0.0;
+  for (int k = 0; k < v.n_int; ++k) {
+    uint64_t m = v.int_mask[k];
+    if ((w & m) == m) acc += v.int_val[k];
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------------------
+// H.v, one workgroup per down row r: X[r, :] staged in LDS for the up-spin gathers, the
+// down-spin row combinations read straight from HBM (coalesced along the row).
+template <int BS, bool DOWN, bool LDSX>
+__global__ __launch_bounds__(BS) void k_hv_rows(HvView v, const double* __restrict__ x,
+                                                double* __restrict__ y) {
+  extern __shared__ double smem[];
+  const uint32_t du = v.du;
+  double* vals_s = smem;
+  double* xs = smem + v.up_nvals;
+  const uint32_t r = blockIdx.x;
+  const double* xr = x + (uint64_t)r * du;
+  for (uint32_t i = threadIdx.x; i < v.up_nvals; i += BS) vals_s[i] = v.up_vals[i];
+  if (LDSX)
+    for (uint32_t c = threadIdx.x; c < du; c += BS) xs[c] = xr[c];
+  __syncthreads();
+  const double* xsrc = LDSX ? xs : xr;
+  const uint32_t sd = v.bdn[r];
+  const double dg = v.diag_dn[r];
+  uint32_t d0 = 0, d1 = 0;
+  if (DOWN) {
+    d0 = v.dn_ptr[r];
+    d1 = v.dn_ptr[r + 1];
+  }
+  double* yr = y + (uint64_t)r * du;
+  for (uint32_t c = threadIdx.x; c < du; c += BS) {
+    const uint32_t su = v.bup[c];
+    double acc = (dg + v.diag_up[c] + interaction(v, su, sd)) * xsrc[c];
+    const uint32_t k0 = v.up_ptr[c], k1 = v.up_ptr[c + 1];
+    for (uint32_t k = k0; k < k1; ++k) {
+      const uint32_t e = v.up_ent[k];
+      acc += vals_s[e >> kTargetBits] * xsrc[e & kTargetMask];
+    }
+    if (DOWN) {
+      for (uint32_t k = d0; k < d1; ++k) {
+        const uint32_t e = v.dn_ent[k];
+        acc += v.dn_vals[e >> kTargetBits] * x[(uint64_t)(e & kTargetMask) * du + c];
+      }
+    }
+    yr[c] += acc;
+  }
+}
+
+// H.v down-spin term only, tiled by column chunks of W doubles so that one XCD works on
+// one chunk (dd x W x 8 bytes) at a time and serves the ~22 row re-reads from its L2.
+// Workgroup w runs on XCD w % 8; the k-th workgroup of an XCD takes chunk
+// (k / nrb) * 8 + xcd and row block k % nrb.
+template <int W>
+__global__ __launch_bounds__(256) void k_hv_down_chunks(HvView v, const double* __restrict__ x,
+                                                        double* __restrict__ y, uint32_t nchunks,
+                                                        uint32_t rows_per_block, uint32_t nrb) {
+  const uint32_t w = blockIdx.x;
+  const uint32_t xcd = w & 7u, kk = w >> 3;
+  const uint32_t chunk = (kk / nrb) * 8u + xcd;
+  const uint32_t rb = kk % nrb;
+  if (chunk >= nchunks) return;
+  constexpr int RPW = 64 / W;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint32_t c = chunk * W + (uint32_t)(lane % W);
+  const uint32_t sub = (uint32_t)(lane / W);
+  const uint32_t du = v.du, dd = v.dd;
+  const uint32_t r0 = rb * rows_per_block;
+  for (uint32_t rr = (uint32_t)wid * RPW + sub; rr < rows_per_block; rr += 4u * RPW) {
+    const uint32_t r = r0 + rr;
+    if (r >= dd) break;
+    if (c < du) {
+      double acc = 0.0;
+      const uint32_t k0 = v.dn_ptr[r], k1 = v.dn_ptr[r + 1];
+      for (uint32_t k = k0; k < k1; ++k) {
+        const uint32_t e = v.dn_ent[k];
+        acc += v.dn_vals[e >> kTargetBits] * x[(uint64_t)(e & kTargetMask) * du + c];
+      }
+      y[(uint64_t)r * du + c] += acc;
+    }
+  }
+}
+
+__global__ void k_diag_interaction(HvView v, double* d, uint64_t D) {
+  uint64_t I = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= D) return;
+  uint32_t iu = (uint32_t)(I % v.du), id = (uint32_t)(I / v.du);
+  d[I] += interaction(v, v.bup[iu], v.bdn[id]);
+}
+
+// y[T] += z * sign * x[S(T)] over the target sector (creation) or the target of a
+// destruction, both as gathers: c_a maps source basis states injectively.
+__global__ void k_apply_c(const uint32_t* tu, const uint32_t* td, uint32_t tdu, uint64_t DT,
+                          int L, int Nsu, int Nsd, int site, int down, int pm, double z,
+                          const double* __restrict__ x, uint32_t sdu, double* __restrict__ y,
+                          const uint32_t* binom) {
+  uint64_t T = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (T >= DT) return;
+  uint32_t iu = (uint32_t)(T % tdu), id = (uint32_t)(T / tdu);
+  uint32_t su = tu[iu], sd = td[id];
+  uint32_t m = 1u << site;
+  // source state: creation -> target had the orbital occupied, source = target minus it;
+  //               destruction -> source = target plus the orbital.
+  uint32_t ssu = su, ssd = sd;
+  if (pm > 0) {
+    if (down) { if (!(sd & m)) return; ssd = sd ^ m; }
+    else      { if (!(su & m)) return; ssu = su ^ m; }
+  } else {
+    if (down) { if (sd & m) return; ssd = sd | m; }
+    else      { if (su & m) return; ssu = su | m; }
+  }
+  // parity of c_a acting on the state that holds orbital a (the target for creation,
+  // the source for destruction): same-spin orbitals below a, plus all up orbitals for a
+  // down orbital (up orbitals are ordered first)
+  uint32_t hu = (pm > 0) ? su : ssu, hd = (pm > 0) ? sd : ssd;
+  int par = down ? (__popc(hd & (m - 1u)) + __popc(hu)) : __popc(hu & (m - 1u));
+  double sgn = (par & 1) ? -1.0 : 1.0;
+  uint32_t ju = cns_rank(ssu, binom), jd = cns_rank(ssd, binom);
+  (void)Nsu;
+  (void)Nsd;
+  y[T] += z * sgn * x[(uint64_t)jd * sdu + ju];
+}
+
+struct HalfTables {
+  DevBuf bin, ptr, ent, vals, diag;
+  uint32_t dim = 0, nnz = 0, nvals = 0, maxdeg = 0;
+};
+
+HalfTables build_half_tables(int L, int N, const std::vector<Bond>& bonds,
+                             const std::vector<double>& bval, const std::vector<int>& dsite,
+                             const std::vector<double>& dval) {
+  HalfTables h;
+  const auto& B = binom_host();
+  if (N < 0 || N > L) return h;
+  h.dim = B[L * 33 + N];
+  if (h.dim == 0) return h;
+  if (h.dim > kTargetMask) fail("half basis too large for the packed hop tables");
+  if (2 * bonds.size() > kMaxVals) fail("too many distinct hopping amplitudes in one sector");
+  hipStream_t st = stream();
+  const uint32_t* bd = binom_dev();
+  h.bin.alloc(h.dim * sizeof(uint32_t));
+  k_unrank<<<(h.dim + 255) / 256, 256, 0, st>>>(h.bin.as<uint32_t>(), h.dim, L, N, bd);
+  DevBuf bonds_d = upload(bonds.empty() ? std::vector<Bond>{{0, 0}} : bonds);
+  DevBuf cnt(h.dim * sizeof(uint32_t));
+  k_hop_count<<<(h.dim + 255) / 256, 256, 0, st>>>(h.bin.as<uint32_t>(), h.dim,
+                                                   bonds_d.as<Bond>(), (int)bonds.size(),
+                                                   cnt.as<uint32_t>());
+  std::vector<uint32_t> c(h.dim), ptr(h.dim + 1, 0);
+  QED_HIP(hipMemcpyAsync(c.data(), cnt.p, h.dim * 4, hipMemcpyDeviceToHost, st));
+  QED_HIP(hipStreamSynchronize(st));
+  for (uint32_t i = 0; i < h.dim; ++i) {
+    ptr[i + 1] = ptr[i] + c[i];
+    h.maxdeg = std::max(h.maxdeg, c[i]);
+  }
+  h.nnz = ptr[h.dim];
+  h.ptr = upload(ptr);
+  h.ent.alloc(std::max<uint32_t>(h.nnz, 1) * sizeof(uint32_t));
+  std::vector<double> vals(2 * bonds.size() + 1, 0.0);
+  for (size_t b = 0; b < bonds.size(); ++b) {
+    vals[2 * b] = bval[b];
+    vals[2 * b + 1] = -bval[b];
+  }
+  h.nvals = (uint32_t)vals.size();
+  h.vals = upload(vals);
+  DevBuf ds = upload(dsite.empty() ? std::vector<int>{0} : dsite);
+  DevBuf dv = upload(dval.empty() ? std::vector<double>{0.0} : dval);
+  h.diag.alloc(h.dim * sizeof(double));
+  k_hop_fill<<<(h.dim + 255) / 256, 256, 0, st>>>(
+      h.bin.as<uint32_t>(), h.dim, bonds_d.as<Bond>(), (int)bonds.size(), h.ptr.as<uint32_t>(),
+      h.ent.as<uint32_t>(), ds.as<int>(), dv.as<double>(), (int)dsite.size(), h.diag.as<double>(),
+      bd);
+  QED_HIP(hipStreamSynchronize(st));
+  return h;
+}
+
+int env_int(const char* name, int dflt) {
+  const char* s = std::getenv(name);
+  return s ? std::atoi(s) : dflt;
+}
+
+}  // namespace
+
+HvView view(const qed_sector& s) {
+  HvView v;
+  v.bup = s.bup.as<uint32_t>();
+  v.bdn = s.bdn.as<uint32_t>();
+  v.up_ptr = s.up_ptr.as<uint32_t>();
+  v.up_ent = s.up_ent.as<uint32_t>();
+  v.dn_ptr = s.dn_ptr.as<uint32_t>();
+  v.dn_ent = s.dn_ent.as<uint32_t>();
+  v.up_vals = s.up_vals.as<double>();
+  v.dn_vals = s.dn_vals.as<double>();
+  v.diag_up = s.diag_up.as<double>();
+  v.diag_dn = s.diag_dn.as<double>();
+  v.int_mask = s.int_mask_d.as<uint64_t>();
+  v.int_val = s.int_val_d.as<double>();
+  v.du = s.du;
+  v.dd = s.dd;
+  v.up_nvals = s.up_nvals;
+  v.n_int = (int)s.int_mask.size();
+  v.hubbard_uniform = s.hubbard_uniform ? 1 : 0;
+  v.hub_U = s.hub_U;
+  v.hub_sites = s.hub_sites;
+  return v;
+}
+
+static size_t lds_budget() { return 160 * 1024 - 1024; }
+
+static void set_lds_attributes() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    const int bytes = (int)lds_budget();
+    (void)hipFuncSetAttribute((const void*)k_hv_rows<1024, true, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void*)k_hv_rows<1024, false, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void*)k_hv_rows<256, true, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void*)k_hv_rows<256, false, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  });
+}
+
+void hv(const qed_sector& s, const double* x, double* y, int variant) {
+  if (s.D == 0) return;
+  set_lds_attributes();
+  if (variant < 0) variant = s.hv_variant;
+  if (variant < 0) variant = env_int("QED_HV_VARIANT", 0);
+  HvView v = view(s);
+  hipStream_t st = stream();
+  const size_t lds_x = (size_t)s.du * sizeof(double) + s.up_nvals * sizeof(double);
+  const bool fits = lds_x <= lds_budget();
+  const size_t shm = fits ? lds_x : s.up_nvals * sizeof(double);
+  auto rows = [&](bool down) {
+    if (s.du >= 4096) {
+      if (fits) {
+        if (down) k_hv_rows<1024, true, true><<<s.dd, 1024, shm, st>>>(v, x, y);
+        else k_hv_rows<1024, false, true><<<s.dd, 1024, shm, st>>>(v, x, y);
+      } else {
+        if (down) k_hv_rows<1024, true, false><<<s.dd, 1024, shm, st>>>(v, x, y);
+        else k_hv_rows<1024, false, false><<<s.dd, 1024, shm, st>>>(v, x, y);
+      }
+    } else {
+      if (down) k_hv_rows<256, true, true><<<s.dd, 256, shm, st>>>(v, x, y);
+      else k_hv_rows<256, false, true><<<s.dd, 256, shm, st>>>(v, x, y);
+    }
+  };
+  if (variant == 0) {
+    rows(true);
+  } else {
+    rows(false);
+    const int W = variant == 1 ? 32 : (variant == 2 ? 16 : 64);
+    const uint32_t nchunks = (s.du + W - 1) / W;
+    const uint32_t rpb = (uint32_t)env_int("QED_HV_RPB", 64);
+    const uint32_t nrb = (s.dd + rpb - 1) / rpb;
+    const uint64_t groups = ((nchunks + 7) / 8) * 8ull * nrb;
+    if (W == 32)
+      k_hv_down_chunks<32><<<(uint32_t)groups, 256, 0, st>>>(v, x, y, nchunks, rpb, nrb);
+    else if (W == 16)
+      k_hv_down_chunks<16><<<(uint32_t)groups, 256, 0, st>>>(v, x, y, nchunks, rpb, nrb);
+    else
+      k_hv_down_chunks<64><<<(uint32_t)groups, 256, 0, st>>>(v, x, y, nchunks, rpb, nrb);
+  }
+  QED_HIP(hipGetLastError());
+}
+
+}  // namespace qed
+
+using namespace qed;
+
+namespace {
+
+// start-vector state shared by every sector, as the reference keeps one process-wide
+// normal_distribution (src_ed/Hamiltonian/Lanczos.cpp:7)
+std::normal_distribution<double> g_normal(0.0, 1.0);
+std::mutex g_normal_mutex;
+
+}  // namespace
+
+extern "C" {
+
+int qed_halfbasis(int L, int N, uint32_t* out, uint32_t* dim) {
+  return guard([&] {
+    if (L < 0 || L > 32) fail("number of orbitals out of range");
+    const auto& B = binom_host();
+    uint32_t n = (N < 0 || N > L) ? 0 : B[L * 33 + N];
+    *dim = n;
+    if (!out || n == 0) return;
+    DevBuf d(n * sizeof(uint32_t));
+    k_unrank<<<(n + 255) / 256, 256, 0, stream()>>>(d.as<uint32_t>(), n, L, N, binom_dev());
+    QED_HIP(hipMemcpyAsync(out, d.p, n * 4, hipMemcpyDeviceToHost, stream()));
+    QED_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int qed_halfbasis_rank(int L, int N, const uint32_t* states, uint64_t n, uint32_t* ranks) {
+  return guard([&] {
+    if (L < 0 || L > 32) fail("number of orbitals out of range");
+    const auto& B = binom_host();
+    uint32_t dim = (N < 0 || N > L) ? 0 : B[L * 33 + N];
+    if (n == 0) return;
+    DevBuf s(n * 4), r(n * 4);
+    QED_HIP(hipMemcpyAsync(s.p, states, n * 4, hipMemcpyHostToDevice, stream()));
+    uint32_t full = L >= 32 ? 0xffffffffu : ((1u << L) - 1u);
+    k_rank<<<(uint32_t)((n + 255) / 256), 256, 0, stream()>>>(s.as<uint32_t>(), r.as<uint32_t>(), n,
+                                                              N, dim, full, binom_dev());
+    QED_HIP(hipMemcpyAsync(ranks, r.p, n * 4, hipMemcpyDeviceToHost, stream()));
+    QED_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int qed_sector_create(int L, int n_up, int n_dn, int n_ops, const int* types, const double* coefs,
+                      const int* nels, const int* er, const int* ec, const double* ev,
+                      qed_sector** out) {
+  return guard([&] {
+    *out = nullptr;
+    if (L <= 0 || L > 32) fail("number of orbitals out of range");
+    auto s = std::make_unique<qed_sector>();
+    s->L = L;
+    s->nup = n_up;
+    s->ndn = n_dn;
+    // per-spin bond lists from the Hermitian-completed element lists
+    std::vector<Bond> bu, bd;
+    std::vector<double> vu, vd;
+    std::vector<int> dsu, dsd;
+    std::vector<double> dvu, dvd;
+    int off = 0;
+    for (int k = 0; k < n_ops; ++k) {
+      const double z = coefs[k];
+      for (int i = 0; i < nels[k]; ++i) {
+        int r = er[off + i], c = ec[off + i];
+        double v = ev[off + i];
+        if (r < 0 || c < 0 || r >= 2 * L || c >= 2 * L) fail("operator element index out of range");
+        if (types[k] == 0) {
+          if (r > c) continue;  // only r <= c is read (one_body_operator ctor)
+          if ((r < L) != (c < L)) fail("spin-flip one-body terms are not factorizable");
+          if (std::fabs(v) < 1.0e-12) continue;
+          bool dn = r >= L;
+          int rs = dn ? r - L : r, cs = dn ? c - L : c;
+          if (r == c) {
+            (dn ? dsd : dsu).push_back(rs);
+            (dn ? dvd : dvu).push_back(v * z);
+          } else {
+            auto& B = dn ? bd : bu;
+            auto& V = dn ? vd : vu;
+            B.push_back({cs, rs});
+            V.push_back(v * z);
+            B.push_back({rs, cs});
+            V.push_back(v * z);
+          }
+        } else if (types[k] == 1) {
+          int a = std::min(r, c), b = std::max(r, c);
+          uint64_t m = (a >= L ? (uint64_t(1) << (a - L + 32)) : (uint64_t(1) << a)) |
+                       (b >= L ? (uint64_t(1) << (b - L + 32)) : (uint64_t(1) << b));
+          s->int_mask.push_back(m);
+          s->int_val.push_back(v * z);
+        } else {
+          fail("unsupported operator type in the factorized format");
+        }
+      }
+      off += nels[k];
+    }
+    // uniform Hubbard detection: elements (i, i+L) with one common value, each site once
+    s->hubbard_uniform = true;
+    s->hub_sites = 0;
+    s->hub_U = s->int_val.empty() ? 0.0 : s->int_val[0];
+    for (size_t q = 0; q < s->int_mask.size(); ++q) {
+      uint64_t m = s->int_mask[q];
+      uint32_t lo = (uint32_t)m, hi = (uint32_t)(m >> 32);
+      if (lo != hi || __builtin_popcount(lo) != 1 || (s->hub_sites & lo) || s->int_val[q] != s->hub_U) {
+        s->hubbard_uniform = false;
+        break;
+      }
+      s->hub_sites |= lo;
+    }
+    if (!s->hubbard_uniform) s->hub_sites = 0;
+    HalfTables hu = build_half_tables(L, n_up, bu, vu, dsu, dvu);
+    HalfTables hd = build_half_tables(L, n_dn, bd, vd, dsd, dvd);
+    s->du = hu.dim;
+    s->dd = hd.dim;
+    s->D = (uint64_t)s->du * s->dd;
+    s->bup = std::move(hu.bin);
+    s->bdn = std::move(hd.bin);
+    s->up_ptr = std::move(hu.ptr);
+    s->dn_ptr = std::move(hd.ptr);
+    s->up_ent = std::move(hu.ent);
+    s->dn_ent = std::move(hd.ent);
+    s->up_vals = std::move(hu.vals);
+    s->dn_vals = std::move(hd.vals);
+    s->diag_up = std::move(hu.diag);
+    s->diag_dn = std::move(hd.diag);
+    s->up_nnz = hu.nnz;
+    s->dn_nnz = hd.nnz;
+    s->up_nvals = hu.nvals;
+    s->dn_nvals = hd.nvals;
+    s->up_maxdeg = hu.maxdeg;
+    s->dn_maxdeg = hd.maxdeg;
+    s->int_mask_d = upload(s->int_mask.empty() ? std::vector<uint64_t>{0} : s->int_mask);
+    s->int_val_d = upload(s->int_val.empty() ? std::vector<double>{0.0} : s->int_val);
+    s->red.alloc(4096 * sizeof(double) * 8);
+    s->red_out.alloc(64 * sizeof(double));
+    QED_HIP(hipStreamSynchronize(stream()));
+    *out = s.release();
+  });
+}
+
+int qed_sector_destroy(qed_sector* s) {
+  return guard([&] {
+    QED_HIP(hipStreamSynchronize(stream()));
+    delete s;
+  });
+}
+
+int qed_sector_dims(const qed_sector* s, uint64_t* dim, uint32_t* du, uint32_t* dd) {
+  return guard([&] {
+    if (dim) *dim = s->D;
+    if (du) *du = s->du;
+    if (dd) *dd = s->dd;
+  });
+}
+
+int qed_hv(qed_sector* s, const double* x, double* y) {
+  return guard([&] { hv(*s, x, y); });
+}
+
+int qed_hv_variant(qed_sector* s, int variant, const double* x, double* y) {
+  return guard([&] { hv(*s, x, y, variant); });
+}
+
+// Times `reps` launches of H.v (after `warmup`) with HIP events on the library stream;
+// *ms = average milliseconds per launch.
+int qed_hv_timed(qed_sector* s, int variant, const double* x, double* y, int warmup, int reps,
+                 double* ms) {
+  return guard([&] {
+    for (int i = 0; i < warmup; ++i) hv(*s, x, y, variant);
+    hipEvent_t e0, e1;
+    QED_HIP(hipEventCreate(&e0));
+    QED_HIP(hipEventCreate(&e1));
+    QED_HIP(hipEventRecord(e0, stream()));
+    for (int i = 0; i < reps; ++i) hv(*s, x, y, variant);
+    QED_HIP(hipEventRecord(e1, stream()));
+    QED_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    QED_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / (reps > 0 ? reps : 1);
+  });
+}
+
+int qed_hv_host(qed_sector* s, const double* x, double* y) {
+  return guard([&] {
+    size_t bytes = s->D * sizeof(double);
+    DevBuf dx(bytes), dy(bytes);
+    QED_HIP(hipMemcpyAsync(dx.p, x, bytes, hipMemcpyHostToDevice, stream()));
+    QED_HIP(hipMemcpyAsync(dy.p, y, bytes, hipMemcpyHostToDevice, stream()));
+    hv(*s, dx.as<double>(), dy.as<double>());
+    QED_HIP(hipMemcpyAsync(y, dy.p, bytes, hipMemcpyDeviceToHost, stream()));
+    QED_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int qed_diag(qed_sector* s, double* d) {
+  return guard([&] {
+    if (s->D == 0) return;
+    k_diag_interaction<<<(uint32_t)((s->D + 255) / 256), 256, 0, stream()>>>(view(*s), d, s->D);
+    QED_HIP(hipGetLastError());
+  });
+}
+
+int qed_start_vector(qed_sector* s, int seed, double* x_dev) {
+  return guard([&] {
+    std::vector<double> x(s->D);
+    {
+      std::lock_guard<std::mutex> lk(g_normal_mutex);
+      std::default_random_engine gen((unsigned)seed);
+      for (uint64_t i = 0; i < s->D; ++i) x[i] = 0.0 + g_normal(gen);
+    }
+    double z = 0.0;
+    for (double v : x) z += v * v;
+    z = std::sqrt(z);
+    if (z >= 1.0e-12)
+      for (double& v : x) v *= 1.0 / z;
+    QED_HIP(hipMemcpyAsync(x_dev, x.data(), s->D * 8, hipMemcpyHostToDevice, stream()));
+    QED_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int qed_reset_start_vector_state(void) {
+  return guard([] {
+    std::lock_guard<std::mutex> lk(g_normal_mutex);
+    g_normal.reset();
+  });
+}
+
+int qed_apply_c(int L, int n_up, int n_dn, int a, int pm, double z, const double* x, double* y) {
+  return guard([&] {
+    if (a < 0 || a >= 2 * L) fail("orbital index out of range");
+    if (pm != 1 && pm != -1) fail("pm must be +1 or -1");
+    const auto& B = binom_host();
+    bool down = a >= L;
+    int site = down ? a - L : a;
+    int tup = n_up + (down ? 0 : pm), tdn = n_dn + (down ? pm : 0);
+    if (tup < 0 || tup > L || tdn < 0 || tdn > L || n_up < 0 || n_up > L || n_dn < 0 || n_dn > L)
+      return;
+    uint32_t tdu = B[L * 33 + tup], tdd = B[L * 33 + tdn], sdu = B[L * 33 + n_up];
+    uint64_t DT = (uint64_t)tdu * tdd;
+    if (DT == 0) return;
+    DevBuf tu(tdu * 4), td(tdd * 4);
+    k_unrank<<<(tdu + 255) / 256, 256, 0, stream()>>>(tu.as<uint32_t>(), tdu, L, tup, binom_dev());
+    k_unrank<<<(tdd + 255) / 256, 256, 0, stream()>>>(td.as<uint32_t>(), tdd, L, tdn, binom_dev());
+    k_apply_c<<<(uint32_t)((DT + 255) / 256), 256, 0, stream()>>>(
+        tu.as<uint32_t>(), td.as<uint32_t>(), tdu, DT, L, n_up, n_dn, site, down ? 1 : 0, pm, z,
+        x, sdu, y, binom_dev());
+    QED_HIP(hipGetLastError());
+    QED_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,563 @@
+// Krylov solvers on device vectors: BLAS-1 with wave64 shuffle reductions, Lanczos for
+// the ground state (eigenvalue pass + eigenvector replay), the scalar Lanczos of the
+// continued-fraction Green function and Freund's band Lanczos. The recurrence scalars
+// (alpha, beta, the projected band matrix) and their small eigenproblems stay in FP64 on
+// the host; every O(D) operation runs on the device.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+#include "../../include/qed.h"
+#include "qed_linalg.hpp"
+#include "qed_sector.hpp"
+
+namespace qed {
+
+namespace {
+
+constexpr int RB = 256;
+constexpr int kMaxMulti = 64;
+constexpr uint32_t kMaxBlocks = 2048;
+
+struct PtrList {
+  const double* p[kMaxMulti];
+};
+struct MutPtrList {
+  double* p[kMaxMulti];
+};
+
+uint32_t red_blocks(uint64_t n) {
+  uint64_t b = (n + (uint64_t)RB * 8 - 1) / ((uint64_t)RB * 8);
+  return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(b, 1), kMaxBlocks);
+}
+
+// scratch for block partials and final scalars (library calls are serialized)
+DevBuf& scratch() {
+  static DevBuf b((size_t)kMaxBlocks * (kMaxMulti + 1) * sizeof(double) + 4096);
+  return b;
+}
+std::vector<double>& host_scratch() {
+  static std::vector<double> h;
+  return h;
+}
+
+__global__ __launch_bounds__(RB) void k_multi_dot(const double* __restrict__ x, PtrList ys, int m,
+                                                  uint64_t n, double* __restrict__ part) {
+  __shared__ double sh[RB / 64];
+  double acc[8];
+  for (int base = 0; base < m; base += 8) {
+    int cnt = min(8, m - base);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * RB) {
+      double xv = x[i];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q < cnt) acc[q] += xv * ys.p[base + q][i];
+    }
+    for (int q = 0; q < cnt; ++q) {
+      double s = block_sum(acc[q], sh);
+      if (threadIdx.x == 0) part[(size_t)(base + q) * gridDim.x + blockIdx.x] = s;
+    }
+  }
+}
+
+__global__ void k_finish(const double* __restrict__ part, int nb, int m, double* __restrict__ out) {
+  __shared__ double sh[RB / 64];
+  for (int j = 0; j < m; ++j) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += RB) s += part[(size_t)j * nb + i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[j] = s;
+  }
+}
+
+__global__ void k_axpy(double a, const double* __restrict__ x, double* __restrict__ y, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+
+__global__ void k_scal(double a, double* __restrict__ x, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    x[i] *= a;
+}
+
+__global__ void k_scal2(double ar, double* __restrict__ r, double aq, double* __restrict__ q,
+                        uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    r[i] *= ar;
+    q[i] *= aq;
+  }
+}
+
+// y += a x, partial sums of y^2
+__global__ __launch_bounds__(RB) void k_axpy_nrm(double a, const double* __restrict__ x,
+                                                 double* __restrict__ y, uint64_t n,
+                                                 double* __restrict__ part) {
+  __shared__ double sh[RB / 64];
+  double acc = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * RB) {
+    double v = y[i] + a * x[i];
+    y[i] = v;
+    acc += v * v;
+  }
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// eigenvector replay step: q -= a r; (r, q) <- (q / b, -b r); psi += c r_new
+__global__ void k_replay(double a, double b, double c, double* __restrict__ r,
+                         double* __restrict__ q, double* __restrict__ psi, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    double rv = r[i];
+    double qv = q[i] - a * rv;
+    double rn = qv / b;
+    r[i] = rn;
+    q[i] = -b * rv;
+    psi[i] += c * rn;
+  }
+}
+
+// continued-fraction Lanczos step: (psi, q) <- (q / b, -b psi)
+__global__ void k_swapscale(double b, double* __restrict__ psi, double* __restrict__ q, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    double t = psi[i];
+    psi[i] = q[i] / b;
+    q[i] = -b * t;
+  }
+}
+
+// y -= sum_k c[k] xs[k]
+__global__ void k_multi_axpy(PtrList xs, const double* __restrict__ c, int m, double* __restrict__ y,
+                             uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    double v = y[i];
+    for (int k = 0; k < m; ++k) v -= c[k] * xs.p[k][i];
+    y[i] = v;
+  }
+}
+
+// ys[k] -= c[k] x
+__global__ void k_multi_axpy_rev(MutPtrList ys, const double* __restrict__ c, int m,
+                                 const double* __restrict__ x, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    double xv = x[i];
+    for (int k = 0; k < m; ++k) ys.p[k][i] -= c[k] * xv;
+  }
+}
+
+uint32_t stream_blocks(uint64_t n) {
+  uint64_t b = (n + 255) / 256;
+  return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(b, 1), (uint64_t)device_cu_count() * 16);
+}
+
+// lowest eigenvalue of the Lanczos tridiagonal and its normalized eigenvector
+// (QL eigenvalues, then inverse iteration for the vector)
+void tridiag_lowest(const std::vector<double>& a, const std::vector<double>& b, double* e0,
+                    std::vector<double>& vec) {
+  int n = (int)a.size();
+  vec.assign(n, 1.0);
+  if (n == 1) {
+    *e0 = a[0];
+    return;
+  }
+  std::vector<double> d(a), e(b.begin(), b.begin() + (n - 1));
+  tridiag_eigen(n, d.data(), e.data(), nullptr);
+  double lam = d[0];
+  *e0 = lam;
+  // inverse iteration on (T - lam) with partial pivoting
+  double scale = 0.0;
+  for (int i = 0; i < n; ++i) scale = std::max(scale, std::fabs(a[i]) + (i < n - 1 ? std::fabs(b[i]) : 0.0));
+  double shift = lam - 1e-14 * std::max(scale, 1.0);
+  for (int it = 0; it < 3; ++it) {
+    // solve (T - shift I) x = vec; Gaussian elimination on the tridiagonal with pivoting
+    std::vector<double> dl(n - 1), dm(n), du(n - 1), du2(n, 0.0), rhs(vec);
+    for (int i = 0; i < n; ++i) dm[i] = a[i] - shift;
+    for (int i = 0; i < n - 1; ++i) dl[i] = du[i] = b[i];
+    std::vector<int> swapped(n, 0);
+    for (int i = 0; i < n - 1; ++i) {
+      if (std::fabs(dm[i]) >= std::fabs(dl[i])) {
+        if (dm[i] == 0.0) dm[i] = 1e-300;
+        double f = dl[i] / dm[i];
+        dm[i + 1] -= f * du[i];
+        rhs[i + 1] -= f * rhs[i];
+        if (i < n - 2) du2[i] = 0.0;
+      } else {
+        double f = dm[i] / dl[i];
+        dm[i] = dl[i];
+        double t = dm[i + 1];
+        dm[i + 1] = du[i] - f * t;
+        if (i < n - 2) {
+          du2[i] = du[i + 1];
+          du[i + 1] = -f * du2[i];
+        }
+        du[i] = t;
+        std::swap(rhs[i], rhs[i + 1]);
+        rhs[i + 1] -= f * rhs[i];
+      }
+    }
+    if (dm[n - 1] == 0.0) dm[n - 1] = 1e-300;
+    std::vector<double> x(n);
+    x[n - 1] = rhs[n - 1] / dm[n - 1];
+    if (n >= 2) x[n - 2] = (rhs[n - 2] - du[n - 2] * x[n - 1]) / dm[n - 2];
+    for (int i = n - 3; i >= 0; --i) x[i] = (rhs[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / dm[i];
+    double nrm = 0.0;
+    for (double v : x) nrm += v * v;
+    nrm = std::sqrt(nrm);
+    for (int i = 0; i < n; ++i) vec[i] = x[i] / nrm;
+  }
+  // sign convention: first component positive (sign does not affect the Ritz residual)
+  if (vec[0] < 0)
+    for (double& v : vec) v = -v;
+}
+
+}  // namespace
+
+std::recursive_mutex& lib_mutex() {
+  static std::recursive_mutex m;
+  return m;
+}
+
+void multi_dot(const double* x, const double* const* ys, int m, uint64_t n, double* out) {
+  if (m > kMaxMulti) fail("multi_dot: too many vectors");
+  if (m == 0) return;
+  if (n == 0) {
+    std::fill(out, out + m, 0.0);
+    return;
+  }
+  PtrList l{};
+  for (int k = 0; k < m; ++k) l.p[k] = ys[k];
+  uint32_t nb = red_blocks(n);
+  double* part = scratch().as<double>();
+  double* fin = part + (size_t)kMaxBlocks * kMaxMulti;
+  k_multi_dot<<<nb, RB, 0, stream()>>>(x, l, m, n, part);
+  k_finish<<<1, RB, 0, stream()>>>(part, nb, m, fin);
+  QED_HIP(hipMemcpyAsync(out, fin, m * sizeof(double), hipMemcpyDeviceToHost, stream()));
+  QED_HIP(hipStreamSynchronize(stream()));
+}
+
+double dot(const double* x, const double* y, uint64_t n) {
+  double r;
+  multi_dot(x, &y, 1, n, &r);
+  return r;
+}
+
+double nrm2(const double* x, uint64_t n) { return std::sqrt(dot(x, x, n)); }
+
+void axpy(double a, const double* x, double* y, uint64_t n) {
+  if (n) k_axpy<<<stream_blocks(n), 256, 0, stream()>>>(a, x, y, n);
+}
+void scal(double a, double* x, uint64_t n) {
+  if (n) k_scal<<<stream_blocks(n), 256, 0, stream()>>>(a, x, n);
+}
+void scal2(double ar, double* r, double aq, double* q, uint64_t n) {
+  if (n) k_scal2<<<stream_blocks(n), 256, 0, stream()>>>(ar, r, aq, q, n);
+}
+
+double axpy_nrm2(double a, const double* x, double* y, uint64_t n) {
+  uint32_t nb = red_blocks(n);
+  double* part = scratch().as<double>();
+  double* fin = part + (size_t)kMaxBlocks * kMaxMulti;
+  k_axpy_nrm<<<nb, RB, 0, stream()>>>(a, x, y, n, part);
+  k_finish<<<1, RB, 0, stream()>>>(part, nb, 1, fin);
+  double s;
+  QED_HIP(hipMemcpyAsync(&s, fin, sizeof(double), hipMemcpyDeviceToHost, stream()));
+  QED_HIP(hipStreamSynchronize(stream()));
+  return std::sqrt(s);
+}
+
+void multi_axpy(const double* c, const double* const* xs, int m, double* y, uint64_t n) {
+  if (m == 0 || n == 0) return;
+  if (m > kMaxMulti) fail("multi_axpy: too many vectors");
+  PtrList l{};
+  for (int k = 0; k < m; ++k) l.p[k] = xs[k];
+  double* cd = scratch().as<double>() + (size_t)kMaxBlocks * kMaxMulti + 64;
+  QED_HIP(hipMemcpyAsync(cd, c, m * sizeof(double), hipMemcpyHostToDevice, stream()));
+  k_multi_axpy<<<stream_blocks(n), 256, 0, stream()>>>(l, cd, m, y, n);
+  QED_HIP(hipStreamSynchronize(stream()));
+}
+
+void multi_axpy_rev(const double* c, const double* x, double* const* ys, int m, uint64_t n) {
+  if (m == 0 || n == 0) return;
+  if (m > kMaxMulti) fail("multi_axpy_rev: too many vectors");
+  MutPtrList l{};
+  for (int k = 0; k < m; ++k) l.p[k] = ys[k];
+  double* cd = scratch().as<double>() + (size_t)kMaxBlocks * kMaxMulti + 64;
+  QED_HIP(hipMemcpyAsync(cd, c, m * sizeof(double), hipMemcpyHostToDevice, stream()));
+  k_multi_axpy_rev<<<stream_blocks(n), 256, 0, stream()>>>(l, cd, m, x, n);
+  QED_HIP(hipStreamSynchronize(stream()));
+}
+
+void lanczos_gs(qed_sector& s, double accur, int max_iter, double* psi, double* E0, int* niter,
+                std::vector<double>* alpha_out, std::vector<double>* beta_out) {
+  const uint64_t n = s.D;
+  DevBuf rb(n * 8), qb(n * 8);
+  double* r = rb.as<double>();
+  double* q = qb.as<double>();
+  QED_HIP(hipMemcpyAsync(r, psi, n * 8, hipMemcpyDeviceToDevice, stream()));
+  QED_HIP(hipMemsetAsync(q, 0, n * 8, stream()));
+  std::vector<double> alpha, beta, evec;
+  double e0 = 0.0;
+  double ritz = 2.0 * accur;
+  while (ritz > accur) {
+    if (!beta.empty()) scal2(1.0 / beta.back(), r, -beta.back(), q, n);
+    hv(s, r, q);
+    std::swap(r, q);
+    double a = dot(q, r, n);
+    alpha.push_back(a);
+    beta.push_back(axpy_nrm2(-a, q, r, n));
+    tridiag_lowest(alpha, beta, &e0, evec);
+    ritz = std::fabs(evec.back() * beta.back());
+    if ((int)alpha.size() > max_iter)
+      fail("Lanczos procedure exceeded " + std::to_string(max_iter) + " iterations");
+  }
+  *E0 = e0;
+  *niter = (int)beta.size();
+  if (alpha_out) *alpha_out = alpha;
+  if (beta_out) *beta_out = beta;
+  // eigenvector replay from the saved start vector
+  QED_HIP(hipMemcpyAsync(r, psi, n * 8, hipMemcpyDeviceToDevice, stream()));
+  QED_HIP(hipMemsetAsync(q, 0, n * 8, stream()));
+  scal(evec[0], psi, n);
+  for (size_t j = 1; j < evec.size(); ++j) {
+    hv(s, r, q);
+    k_replay<<<stream_blocks(n), 256, 0, stream()>>>(alpha[j - 1], beta[j - 1], evec[j], r, q, psi, n);
+  }
+  QED_HIP(hipStreamSynchronize(stream()));
+}
+
+// ---------------------------------------------------------------------------------------
+// band Lanczos (Freund), device vectors, following src_ed/Hamiltonian/Lanczos.hpp:214-395
+
+struct VecPool {
+  uint64_t n;
+  std::vector<std::unique_ptr<DevBuf>> owned;
+  std::vector<double*> free_list;
+  explicit VecPool(uint64_t n_) : n(n_) {}
+  double* get() {
+    if (!free_list.empty()) {
+      double* p = free_list.back();
+      free_list.pop_back();
+      return p;
+    }
+    owned.emplace_back(new DevBuf(n * sizeof(double)));
+    return owned.back()->as<double>();
+  }
+  void put(double* p) {
+    if (p) free_list.push_back(p);
+  }
+};
+
+int band_lanczos(qed_sector& s, const double* phi, int p0, int M0, int max_iter_BL, double accur_deflation,
+                 double accur_band_lanczos, std::vector<double>& evals, std::vector<double>& Q) {
+  const uint64_t n = s.D;
+  if (n == 0 || p0 == 0) return 0;
+  if (M0 <= 0) M0 = (int)(14 * p0 * std::log(1.0 * n));
+  if (M0 <= 1) fail("band Lanczos: too few iterations requested");
+  int pc = p0;
+  if (M0 > max_iter_BL) M0 = max_iter_BL;
+  int M0p = (M0 / p0 + 1) * p0;
+  M0 = M0p - pc;
+  VecPool pool(n);
+  std::vector<double*> v(M0p + 1, nullptr);
+  std::vector<const double*> ph(p0);
+  for (int i = 0; i < p0; ++i) ph[i] = phi + (uint64_t)i * n;
+  for (int i = 0; i < p0; ++i) {
+    v[i] = pool.get();
+    QED_HIP(hipMemcpyAsync(v[i], ph[i], n * 8, hipMemcpyDeviceToDevice, stream()));
+  }
+  std::vector<double> t((size_t)M0 * M0, 0.0), P((size_t)p0 * M0, 0.0);
+  auto T = [&](int i, int j) -> double& { return t[(size_t)j * M0 + i]; };
+  std::vector<int> defl(M0, 0), I;
+  double ev_old = 1e12;
+  int j = 0;
+  std::vector<double> tmp(kMaxMulti);
+  while (j < M0) {
+    // norm of v[j] and its overlaps with the start vectors in one pass
+    std::vector<const double*> ys(p0 + 1);
+    ys[0] = v[j];
+    for (int k = 0; k < p0; ++k) ys[k + 1] = ph[k];
+    std::vector<double> dd(p0 + 1);
+    multi_dot(v[j], ys.data(), p0 + 1, n, dd.data());
+    double z = std::sqrt(std::max(dd[0], 0.0));
+    if (z < accur_deflation) {
+      if (j >= pc) {
+        I.push_back(j - pc);
+        defl[j - pc] = 1;
+      }
+      pc--;
+      if (pc == 0) break;
+      pool.put(v[j]);
+      for (int k = j; k < j + pc; ++k) v[k] = v[k + 1];
+      v[j + pc] = nullptr;
+      continue;
+    }
+    scal(1.0 / z, v[j], n);
+    if (j >= pc) T(j, j - pc) = z;
+    for (int k = 0; k < p0; ++k) P[(size_t)j * p0 + k] = dd[k + 1] / z;
+    // orthogonalize the pending block against v[j]
+    int nk = 0;
+    std::vector<const double*> blk;
+    std::vector<double*> blk_mut;
+    for (int k = j + 1; k < j + pc; ++k) {
+      blk.push_back(v[k]);
+      blk_mut.push_back(v[k]);
+      ++nk;
+    }
+    if (nk) {
+      std::vector<double> zz(nk);
+      multi_dot(v[j], blk.data(), nk, n, zz.data());
+      multi_axpy_rev(zz.data(), v[j], blk_mut.data(), nk, n);
+      for (int q = 0; q < nk; ++q) {
+        int k = j + 1 + q;
+        if (k >= pc) T(j, k - pc) = zz[q];
+      }
+    }
+    double* w = pool.get();
+    QED_HIP(hipMemsetAsync(w, 0, n * 8, stream()));
+    hv(s, v[j], w);
+    int k0 = 0;
+    if (j > pc) {
+      k0 = j - pc;
+      if (defl[k0 - 1] == 0) {
+        pool.put(v[k0 - 1]);
+        v[k0 - 1] = nullptr;
+      }
+    }
+    {
+      std::vector<const double*> xs;
+      std::vector<double> cs;
+      for (int k = k0; k < j; ++k) {
+        T(k, j) = T(j, k);
+        xs.push_back(v[k]);
+        cs.push_back(T(k, j));
+      }
+      multi_axpy(cs.data(), xs.data(), (int)xs.size(), w, n);
+    }
+    for (int k : I) {
+      double ww = dot(v[k], w, n);
+      axpy(-ww, v[k], w, n);
+      T(k, j) = ww;
+    }
+    double ww = dot(v[j], w, n);
+    axpy(-ww, v[j], w, n);
+    T(j, j) = ww;
+    for (int k : I) T(j, k) = T(k, j);
+    v[j + pc] = w;
+    if (j >= 4 * p0 && j % p0 == 0) {
+      std::vector<double> Hpr((size_t)j * j), evs(j);
+      for (int c = 0; c < j; ++c)
+        for (int r = 0; r < j; ++r) Hpr[(size_t)c * j + r] = T(r, c);
+      sym_eigen(j, Hpr.data(), evs.data(), false);
+      double test = std::fabs(evs[0] - ev_old);
+      ev_old = evs[0];
+      if (test < accur_band_lanczos) break;
+    }
+    ++j;
+  }
+  QED_HIP(hipStreamSynchronize(stream()));
+  if (j == 0) return 0;
+  std::vector<double> Hpr((size_t)j * j);
+  for (int c = 0; c < j; ++c)
+    for (int r = 0; r < j; ++r) Hpr[(size_t)c * j + r] = T(r, c);
+  evals.assign(j, 0.0);
+  sym_eigen(j, Hpr.data(), evals.data(), true);
+  // Q = P U  (p0 x j)
+  Q.assign((size_t)p0 * j, 0.0);
+  for (int c = 0; c < j; ++c)
+    for (int m = 0; m < j; ++m) {
+      double u = Hpr[(size_t)c * j + m];
+      if (u == 0.0) continue;
+      for (int k = 0; k < p0; ++k) Q[(size_t)c * p0 + k] += P[(size_t)m * p0 + k] * u;
+    }
+  return j;
+}
+
+void lanczos_green(qed_sector& s, const double* psi_in, double accur_cf, int max_iter,
+                   std::vector<double>& a, std::vector<double>& b) {
+  const uint64_t n = s.D;
+  DevBuf pb(n * 8), qb(n * 8);
+  double* psi = pb.as<double>();
+  double* q = qb.as<double>();
+  QED_HIP(hipMemcpyAsync(psi, psi_in, n * 8, hipMemcpyDeviceToDevice, stream()));
+  QED_HIP(hipMemsetAsync(q, 0, n * 8, stream()));
+  a.clear();
+  b.assign(1, 1.0);
+  double prod = 1.0;
+  while (prod > accur_cf && std::fabs(b.back()) > accur_cf && (int)a.size() < max_iter) {
+    if (!a.empty()) k_swapscale<<<stream_blocks(n), 256, 0, stream()>>>(b.back(), psi, q, n);
+    hv(s, psi, q);
+    double z = dot(psi, q, n);
+    a.push_back(z);
+    double zz = axpy_nrm2(-z, psi, q, n);
+    b.push_back(zz);
+    prod *= std::fabs(zz);
+  }
+}
+
+}  // namespace qed
+
+using namespace qed;
+
+extern "C" {
+
+int qed_lanczos_gs(qed_sector* s, double accur, int max_iter, double* psi, double* E0, int* niter,
+                   double* alpha, double* beta) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    std::vector<double> a, b;
+    lanczos_gs(*s, accur, max_iter, psi, E0, niter, &a, &b);
+    if (alpha) std::copy(a.begin(), a.end(), alpha);
+    if (beta) std::copy(b.begin(), b.end(), beta);
+  });
+}
+
+int qed_lanczos_green(qed_sector* s, const double* psi, double accur_cf, int max_iter, double* a,
+                      double* b, int* n) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    std::vector<double> av, bv;
+    lanczos_green(*s, psi, accur_cf, max_iter, av, bv);
+    std::copy(av.begin(), av.end(), a);
+    std::copy(bv.begin(), bv.end(), b);
+    *n = (int)av.size();
+  });
+}
+
+int qed_band_lanczos(qed_sector* s, const double* phi, int p, int M0, int max_iter_BL,
+                     double accur_deflation, double accur_band_lanczos, double* evals, double* Q,
+                     int cap, int* M) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    std::vector<double> e, q;
+    int m = band_lanczos(*s, phi, p, M0, max_iter_BL, accur_deflation, accur_band_lanczos, e, q);
+    if (m > cap) fail("band Lanczos: output capacity too small");
+    *M = m;
+    std::copy(e.begin(), e.end(), evals);
+    std::copy(q.begin(), q.end(), Q);
+  });
+}
+
+int qed_blas_dot(const double* x, const double* y, uint64_t n, double* out) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] { *out = dot(x, y, n); });
+}
+
+int qed_util_tridiag_eigen(int n, double* d, const double* e, double* z) {
+  return guard([&] { tridiag_eigen(n, d, e, z); });
+}
+
+int qed_util_sym_eigen(int n, double* a, double* w, int vectors) {
+  return guard([&] { sym_eigen(n, a, w, vectors != 0); });
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// One factorized sector (N_up, N_dn) of a cluster Hamiltonian resident on the device.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "qed_common.hpp"
+
+struct qed_sector {
+  int L = 0, nup = 0, ndn = 0;
+  uint32_t du = 0, dd = 0;  // half-basis dimensions (up = fast index)
+  uint64_t D = 0;           // du * dd
+
+  // half bases (ascending bit strings) and hop tables per spin, CSR over half states:
+  // ptr[i]..ptr[i+1] index packed entries (target rank | value index << 20)
+  qed::DevBuf bup, bdn;
+  qed::DevBuf up_ptr, up_ent, dn_ptr, dn_ent;
+  qed::DevBuf up_vals, dn_vals;   // distinct signed amplitudes (double)
+  qed::DevBuf diag_up, diag_dn;   // one-body diagonal per half state
+  uint32_t up_nnz = 0, dn_nnz = 0, up_nvals = 0, dn_nvals = 0;
+  uint32_t up_maxdeg = 0, dn_maxdeg = 0;
+
+  // density-density interaction. Word layout of the reference's factorized basis:
+  // w = dn_state | up_state << 32; element (r, c) active when both mask bits are set.
+  std::vector<uint64_t> int_mask;
+  std::vector<double> int_val;
+  qed::DevBuf int_mask_d, int_val_d;
+  bool hubbard_uniform = true;  // all elements (i, i+L) with one value -> U * popcount
+  double hub_U = 0.0;
+  uint32_t hub_sites = 0;
+  std::vector<double> int_diag_host;  // unused unless requested
+
+  int hv_variant = -1;  // -1 = automatic
+
+  // scratch for reductions
+  qed::DevBuf red;      // partial sums
+  qed::DevBuf red_out;  // final scalars (device)
+};
+
+namespace qed {
+
+// Kernel-facing view of the sector (passed by value).
+struct HvView {
+  const uint32_t* bup;
+  const uint32_t* bdn;
+  const uint32_t* up_ptr;
+  const uint32_t* up_ent;
+  const uint32_t* dn_ptr;
+  const uint32_t* dn_ent;
+  const double* up_vals;
+  const double* dn_vals;
+  const double* diag_up;
+  const double* diag_dn;
+  const uint64_t* int_mask;
+  const double* int_val;
+  uint32_t du, dd;
+  uint32_t up_nvals;
+  int n_int;
+  int hubbard_uniform;
+  double hub_U;
+  uint32_t hub_sites;
+};
+
+HvView view(const qed_sector& s);
+
+// y += H x on the library stream (device vectors). variant -1 chooses automatically.
+void hv(const qed_sector& s, const double* x, double* y, int variant = -1);
+// y = alpha*y + H x, with <x, y_out> accumulated into dot_partial when non-null:
+// the fused Lanczos pass.
+void hv_fused(const qed_sector& s, const double* x, double* y, double beta_y, double* dot_out);
+
+// BLAS-1 on the library stream. dot/nrm2 return host scalars (synchronizing).
+double dot(const double* x, const double* y, uint64_t n);
+double nrm2(const double* x, uint64_t n);
+void axpy(double a, const double* x, double* y, uint64_t n);
+void scal(double a, double* x, uint64_t n);
+// r *= ar; q *= aq (one pass)
+void scal2(double ar, double* r, double aq, double* q, uint64_t n);
+// y += a x and returns ||y|| (one pass, synchronizing)
+double axpy_nrm2(double a, const double* x, double* y, uint64_t n);
+// multi-dot: out[k] = <x, ys[k]> for k < m (one pass over x)
+void multi_dot(const double* x, const double* const* ys, int m, uint64_t n, double* out);
+// y -= sum_k c[k] xs[k] (one pass)
+void multi_axpy(const double* c, const double* const* xs, int m, double* y, uint64_t n);
+// ys[k] -= c[k] x for each k (one pass over x)
+void multi_axpy_rev(const double* c, const double* x, double* const* ys, int m, uint64_t n);
+
+void lanczos_gs(qed_sector& s, double accur, int max_iter, double* psi, double* E0, int* niter,
+                std::vector<double>* alpha, std::vector<double>* beta);
+
+}  // namespace qed
