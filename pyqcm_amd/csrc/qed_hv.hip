@@ -169,6 +169,78 @@ __global__ __launch_bounds__(256) void k_hv_down_chunks(HvView v, const double* 
   }
 }
 
+// CSR -> ELL, padding with zero-amplitude self hops (value index zv holds 0.0)
+__global__ void k_ell_colmajor(const uint32_t* ptr, const uint32_t* ent, uint32_t dim, uint32_t W,
+                               uint32_t zv, uint32_t* out) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= dim) return;
+  uint32_t k0 = ptr[c], n = ptr[c + 1] - k0;
+  for (uint32_t k = 0; k < W; ++k)
+    out[(uint64_t)k * dim + c] = (k < n) ? ent[k0 + k] : (c | (zv << kTargetBits));
+}
+
+__global__ void k_ell_rowmajor(const uint32_t* ptr, const uint32_t* ent, uint32_t dim, uint32_t W,
+                               uint32_t zv, uint32_t* out) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= dim) return;
+  uint32_t k0 = ptr[r], n = ptr[r + 1] - k0;
+  for (uint32_t k = 0; k < W; ++k)
+    out[(uint64_t)r * W + k] = (k < n) ? ent[k0 + k] : (r | (zv << kTargetBits));
+}
+
+// H.v, one workgroup per down row r. Up-spin hops: ELL entries (coalesced per lane) gather
+// from the row staged in LDS. Down-spin hops: the row's neighbour list is uniform across
+// the workgroup; 8 neighbour rows are requested before any of them is consumed so that
+// each wave keeps 8 row loads in flight.
+constexpr int kG = 8;
+template <int BS, bool LDSX>
+__global__ __launch_bounds__(BS) void k_hv_ell(HvView v, const uint32_t* __restrict__ up_ell,
+                                               uint32_t Wu, const uint32_t* __restrict__ dn_ell,
+                                               uint32_t Wd, const double* __restrict__ x,
+                                               double* __restrict__ y) {
+  extern __shared__ double smem[];
+  const uint32_t du = v.du;
+  const uint32_t nvu = v.up_nvals;
+  double* vals_s = smem;
+  double* xs = smem + nvu;
+  const uint32_t r = blockIdx.x;
+  const double* xr = x + (uint64_t)r * du;
+  for (uint32_t i = threadIdx.x; i < nvu; i += BS) vals_s[i] = v.up_vals[i];
+  if (LDSX)
+    for (uint32_t c = threadIdx.x; c < du; c += BS) xs[c] = xr[c];
+  __syncthreads();
+  const double* xsrc = LDSX ? xs : xr;
+  const uint32_t sd = v.bdn[r];
+  const double dg = v.diag_dn[r];
+  const uint32_t dcnt = v.dn_ptr[r + 1] - v.dn_ptr[r];
+  const uint32_t dgroups = (dcnt + kG - 1) / kG * kG;
+  const uint32_t* dl = dn_ell + (uint64_t)r * Wd;
+  double* yr = y + (uint64_t)r * du;
+  for (uint32_t c = threadIdx.x; c < du; c += BS) {
+    const uint32_t su = v.bup[c];
+    double acc = (dg + v.diag_up[c] + interaction(v, su, sd)) * xsrc[c];
+    double acc2 = 0.0;
+    for (uint32_t g = 0; g < Wu; g += kG) {
+      uint32_t e[kG];
+#pragma unroll
+      for (int q = 0; q < kG; ++q) e[q] = up_ell[(uint64_t)(g + q) * du + c];
+#pragma unroll
+      for (int q = 0; q < kG; ++q) acc += vals_s[e[q] >> kTargetBits] * xsrc[e[q] & kTargetMask];
+    }
+    for (uint32_t g = 0; g < dgroups; g += kG) {
+      uint32_t e[kG];
+      double xv[kG];
+#pragma unroll
+      for (int q = 0; q < kG; ++q) e[q] = dl[g + q];
+#pragma unroll
+      for (int q = 0; q < kG; ++q) xv[q] = x[(uint64_t)(e[q] & kTargetMask) * du + c];
+#pragma unroll
+      for (int q = 0; q < kG; ++q) acc2 += v.dn_vals[e[q] >> kTargetBits] * xv[q];
+    }
+    yr[c] += acc + acc2;
+  }
+}
+
 __global__ void k_diag_interaction(HvView v, double* d, uint64_t D) {
   uint64_t I = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= D) return;
@@ -306,6 +378,10 @@ static void set_lds_attributes() {
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     (void)hipFuncSetAttribute((const void*)k_hv_rows<256, false, true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void*)k_hv_ell<1024, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void*)k_hv_ell<256, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   });
 }
 
@@ -313,7 +389,7 @@ void hv(const qed_sector& s, const double* x, double* y, int variant) {
   if (s.D == 0) return;
   set_lds_attributes();
   if (variant < 0) variant = s.hv_variant;
-  if (variant < 0) variant = env_int("QED_HV_VARIANT", 0);
+  if (variant < 0) variant = env_int("QED_HV_VARIANT", 4);
   HvView v = view(s);
   hipStream_t st = stream();
   const size_t lds_x = (size_t)s.du * sizeof(double) + s.up_nvals * sizeof(double);
@@ -333,7 +409,19 @@ void hv(const qed_sector& s, const double* x, double* y, int variant) {
       else k_hv_rows<256, false, true><<<s.dd, 256, shm, st>>>(v, x, y);
     }
   };
-  if (variant == 0) {
+  if (variant == 4) {
+    const size_t lds_e = (size_t)s.du * sizeof(double) + s.up_nvals * sizeof(double);
+    const bool fe = lds_e <= lds_budget();
+    const size_t sh = fe ? lds_e : s.up_nvals * sizeof(double);
+    const uint32_t* ue = s.up_ell.as<uint32_t>();
+    const uint32_t* de = s.dn_ell.as<uint32_t>();
+    if (s.du >= 4096) {
+      if (fe) k_hv_ell<1024, true><<<s.dd, 1024, sh, st>>>(v, ue, s.up_w, de, s.dn_w, x, y);
+      else k_hv_ell<1024, false><<<s.dd, 1024, sh, st>>>(v, ue, s.up_w, de, s.dn_w, x, y);
+    } else {
+      k_hv_ell<256, true><<<s.dd, 256, sh, st>>>(v, ue, s.up_w, de, s.dn_w, x, y);
+    }
+  } else if (variant == 0) {
     rows(true);
   } else {
     rows(false);
@@ -483,6 +571,20 @@ int qed_sector_create(int L, int n_up, int n_dn, int n_ops, const int* types, co
     s->dn_nvals = hd.nvals;
     s->up_maxdeg = hu.maxdeg;
     s->dn_maxdeg = hd.maxdeg;
+    // ELL copies (entries padded to multiples of 8 with zero-amplitude self hops)
+    s->up_w = (s->up_maxdeg + 7) / 8 * 8;
+    s->dn_w = (s->dn_maxdeg + 7) / 8 * 8;
+    if (s->du && s->dd) {
+      s->up_ell.alloc(std::max<uint64_t>((uint64_t)s->up_w * s->du, 1) * 4);
+      s->dn_ell.alloc(std::max<uint64_t>((uint64_t)s->dn_w * s->dd, 1) * 4);
+      k_ell_colmajor<<<(s->du + 255) / 256, 256, 0, stream()>>>(
+          s->up_ptr.as<uint32_t>(), s->up_ent.as<uint32_t>(), s->du, s->up_w, s->up_nvals - 1,
+          s->up_ell.as<uint32_t>());
+      k_ell_rowmajor<<<(s->dd + 255) / 256, 256, 0, stream()>>>(
+          s->dn_ptr.as<uint32_t>(), s->dn_ent.as<uint32_t>(), s->dd, s->dn_w, s->dn_nvals - 1,
+          s->dn_ell.as<uint32_t>());
+      QED_HIP(hipGetLastError());
+    }
     s->int_mask_d = upload(s->int_mask.empty() ? std::vector<uint64_t>{0} : s->int_mask);
     s->int_val_d = upload(s->int_val.empty() ? std::vector<double>{0.0} : s->int_val);
     s->red.alloc(4096 * sizeof(double) * 8);

@@ -30,6 +30,11 @@ struct qed_sector {
   uint32_t hub_sites = 0;
   std::vector<double> int_diag_host;  // unused unless requested
 
+  // ELL copies of the hop tables, padded to multiples of 8 entries with zero-amplitude
+  // self hops: up column-major (entry k of column c at k*du + c), down row-major
+  qed::DevBuf up_ell, dn_ell;
+  uint32_t up_w = 0, dn_w = 0;
+
   int hv_variant = -1;  // -1 = automatic
 
   // scratch for reductions
