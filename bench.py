@@ -66,7 +66,7 @@ def main():
     import ctypes
     ms = np.zeros(1)
     if args.sweep:
-        for v in (0, 1, 4):
+        for v in (0, 1, 4, 5):
             _lib.call("qed_hv_timed", S.h, v, dx.ptr, dy.ptr, 3, 10, _lib.dp(ms))
             log("variant %d: %.3f ms per H.v -> %.0f GB/s" % (v, ms[0], bytes_per / ms[0] / 1e6))
 

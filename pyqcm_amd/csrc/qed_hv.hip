@@ -241,6 +241,69 @@ __global__ __launch_bounds__(BS) void k_hv_ell(HvView v, const uint32_t* __restr
   }
 }
 
+// Same product, two adjacent columns per thread: 16-byte loads of the down-spin neighbour
+// rows and of the ELL entry pairs (needs an even up dimension).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_hv_ell2(HvView v, const uint32_t* __restrict__ up_ell,
+                                                uint32_t Wu, const uint32_t* __restrict__ dn_ell,
+                                                uint32_t Wd, const double* __restrict__ x,
+                                                double* __restrict__ y) {
+  extern __shared__ double smem[];
+  const uint32_t du = v.du;
+  const uint32_t nvu = v.up_nvals;
+  double* vals_s = smem;
+  double* xs = smem + ((nvu + 1u) & ~1u);  // 16-byte aligned row
+  const uint32_t r = blockIdx.x;
+  const double* xr = x + (uint64_t)r * du;
+  const uint32_t dh = du / 2;
+  for (uint32_t i = threadIdx.x; i < nvu; i += BS) vals_s[i] = v.up_vals[i];
+  for (uint32_t c = threadIdx.x; c < dh; c += BS)
+    reinterpret_cast<double2*>(xs)[c] = reinterpret_cast<const double2*>(xr)[c];
+  __syncthreads();
+  const uint32_t sd = v.bdn[r];
+  const double dg = v.diag_dn[r];
+  const uint32_t dcnt = v.dn_ptr[r + 1] - v.dn_ptr[r];
+  const uint32_t dgroups = (dcnt + kG - 1) / kG * kG;
+  const uint32_t* dl = dn_ell + (uint64_t)r * Wd;
+  double2* yr = reinterpret_cast<double2*>(y + (uint64_t)r * du);
+  for (uint32_t c2 = threadIdx.x; c2 < dh; c2 += BS) {
+    const uint32_t c = 2 * c2;
+    double a0 = (dg + v.diag_up[c] + interaction(v, v.bup[c], sd)) * xs[c];
+    double a1 = (dg + v.diag_up[c + 1] + interaction(v, v.bup[c + 1], sd)) * xs[c + 1];
+    for (uint32_t g = 0; g < Wu; g += kG) {
+      uint2 e[kG];
+#pragma unroll
+      for (int q = 0; q < kG; ++q)
+        e[q] = *reinterpret_cast<const uint2*>(up_ell + (uint64_t)(g + q) * du + c);
+#pragma unroll
+      for (int q = 0; q < kG; ++q) {
+        a0 += vals_s[e[q].x >> kTargetBits] * xs[e[q].x & kTargetMask];
+        a1 += vals_s[e[q].y >> kTargetBits] * xs[e[q].y & kTargetMask];
+      }
+    }
+    double b0 = 0.0, b1 = 0.0;
+    for (uint32_t g = 0; g < dgroups; g += kG) {
+      uint32_t e[kG];
+      double2 xv[kG];
+#pragma unroll
+      for (int q = 0; q < kG; ++q) e[q] = dl[g + q];
+#pragma unroll
+      for (int q = 0; q < kG; ++q)
+        xv[q] = *reinterpret_cast<const double2*>(x + (uint64_t)(e[q] & kTargetMask) * du + c);
+#pragma unroll
+      for (int q = 0; q < kG; ++q) {
+        const double cf = v.dn_vals[e[q] >> kTargetBits];
+        b0 += cf * xv[q].x;
+        b1 += cf * xv[q].y;
+      }
+    }
+    double2 yo = yr[c2];
+    yo.x += a0 + b0;
+    yo.y += a1 + b1;
+    yr[c2] = yo;
+  }
+}
+
 __global__ void k_diag_interaction(HvView v, double* d, uint64_t D) {
   uint64_t I = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= D) return;
@@ -333,10 +396,6 @@ HalfTables build_half_tables(int L, int N, const std::vector<Bond>& bonds,
   return h;
 }
 
-int env_int(const char* name, int dflt) {
-  const char* s = std::getenv(name);
-  return s ? std::atoi(s) : dflt;
-}
 
 }  // namespace
 
@@ -382,60 +441,72 @@ static void set_lds_attributes() {
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     (void)hipFuncSetAttribute((const void*)k_hv_ell<256, true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void*)k_hv_ell2<512>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   });
 }
 
+// Kernel variants of y += H x:
+//   0  one workgroup per row, CSR hop lists (straightforward form, kept as a baseline)
+//   1  two passes: up-spin + diagonal per row, then the down-spin term by L2-sized column chunks
+//   4  one workgroup per row, ELL tables, 8-deep load groups, one column per thread
+//   5  as 4 with two columns per thread and 16-byte loads (default; falls back to 4 when the
+//      up dimension is odd or small, or the row does not fit in LDS)
 void hv(const qed_sector& s, const double* x, double* y, int variant) {
   if (s.D == 0) return;
   set_lds_attributes();
   if (variant < 0) variant = s.hv_variant;
-  if (variant < 0) variant = env_int("QED_HV_VARIANT", 4);
+  if (variant < 0) variant = 5;
   HvView v = view(s);
   hipStream_t st = stream();
-  const size_t lds_x = (size_t)s.du * sizeof(double) + s.up_nvals * sizeof(double);
-  const bool fits = lds_x <= lds_budget();
-  const size_t shm = fits ? lds_x : s.up_nvals * sizeof(double);
-  auto rows = [&](bool down) {
-    if (s.du >= 4096) {
-      if (fits) {
-        if (down) k_hv_rows<1024, true, true><<<s.dd, 1024, shm, st>>>(v, x, y);
-        else k_hv_rows<1024, false, true><<<s.dd, 1024, shm, st>>>(v, x, y);
-      } else {
-        if (down) k_hv_rows<1024, true, false><<<s.dd, 1024, shm, st>>>(v, x, y);
-        else k_hv_rows<1024, false, false><<<s.dd, 1024, shm, st>>>(v, x, y);
-      }
-    } else {
-      if (down) k_hv_rows<256, true, true><<<s.dd, 256, shm, st>>>(v, x, y);
-      else k_hv_rows<256, false, true><<<s.dd, 256, shm, st>>>(v, x, y);
+  const uint32_t* ue = s.up_ell.as<uint32_t>();
+  const uint32_t* de = s.dn_ell.as<uint32_t>();
+  if (variant == 5) {
+    const size_t lds2 = ((size_t)s.du + s.up_nvals + 1) * sizeof(double);
+    if (s.du % 2 == 0 && s.du >= 4096 && lds2 <= lds_budget()) {
+      k_hv_ell2<512><<<s.dd, 512, lds2, st>>>(v, ue, s.up_w, de, s.dn_w, x, y);
+      QED_HIP(hipGetLastError());
+      return;
     }
-  };
+    variant = 4;
+  }
   if (variant == 4) {
-    const size_t lds_e = (size_t)s.du * sizeof(double) + s.up_nvals * sizeof(double);
-    const bool fe = lds_e <= lds_budget();
-    const size_t sh = fe ? lds_e : s.up_nvals * sizeof(double);
-    const uint32_t* ue = s.up_ell.as<uint32_t>();
-    const uint32_t* de = s.dn_ell.as<uint32_t>();
+    const size_t lds = ((size_t)s.du + s.up_nvals) * sizeof(double);
+    const bool fits = lds <= lds_budget();
+    const size_t sh = fits ? lds : s.up_nvals * sizeof(double);
     if (s.du >= 4096) {
-      if (fe) k_hv_ell<1024, true><<<s.dd, 1024, sh, st>>>(v, ue, s.up_w, de, s.dn_w, x, y);
+      if (fits) k_hv_ell<1024, true><<<s.dd, 1024, sh, st>>>(v, ue, s.up_w, de, s.dn_w, x, y);
       else k_hv_ell<1024, false><<<s.dd, 1024, sh, st>>>(v, ue, s.up_w, de, s.dn_w, x, y);
     } else {
       k_hv_ell<256, true><<<s.dd, 256, sh, st>>>(v, ue, s.up_w, de, s.dn_w, x, y);
     }
-  } else if (variant == 0) {
-    rows(true);
+    QED_HIP(hipGetLastError());
+    return;
+  }
+  if (variant != 0 && variant != 1) fail("unknown H.v kernel variant " + std::to_string(variant));
+  const size_t lds_x = ((size_t)s.du + s.up_nvals) * sizeof(double);
+  const bool fits = lds_x <= lds_budget();
+  const size_t shm = fits ? lds_x : s.up_nvals * sizeof(double);
+  const bool down = variant == 0;
+  if (s.du >= 4096) {
+    if (fits) {
+      if (down) k_hv_rows<1024, true, true><<<s.dd, 1024, shm, st>>>(v, x, y);
+      else k_hv_rows<1024, false, true><<<s.dd, 1024, shm, st>>>(v, x, y);
+    } else {
+      if (down) k_hv_rows<1024, true, false><<<s.dd, 1024, shm, st>>>(v, x, y);
+      else k_hv_rows<1024, false, false><<<s.dd, 1024, shm, st>>>(v, x, y);
+    }
   } else {
-    rows(false);
-    const int W = variant == 1 ? 32 : (variant == 2 ? 16 : 64);
+    if (down) k_hv_rows<256, true, true><<<s.dd, 256, shm, st>>>(v, x, y);
+    else k_hv_rows<256, false, true><<<s.dd, 256, shm, st>>>(v, x, y);
+  }
+  if (variant == 1) {
+    constexpr int W = 32;
     const uint32_t nchunks = (s.du + W - 1) / W;
-    const uint32_t rpb = (uint32_t)env_int("QED_HV_RPB", 64);
+    const uint32_t rpb = 64;
     const uint32_t nrb = (s.dd + rpb - 1) / rpb;
     const uint64_t groups = ((nchunks + 7) / 8) * 8ull * nrb;
-    if (W == 32)
-      k_hv_down_chunks<32><<<(uint32_t)groups, 256, 0, st>>>(v, x, y, nchunks, rpb, nrb);
-    else if (W == 16)
-      k_hv_down_chunks<16><<<(uint32_t)groups, 256, 0, st>>>(v, x, y, nchunks, rpb, nrb);
-    else
-      k_hv_down_chunks<64><<<(uint32_t)groups, 256, 0, st>>>(v, x, y, nchunks, rpb, nrb);
+    k_hv_down_chunks<W><<<(uint32_t)groups, 256, 0, st>>>(v, x, y, nchunks, rpb, nrb);
   }
   QED_HIP(hipGetLastError());
 }

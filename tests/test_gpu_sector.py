@@ -57,7 +57,7 @@ HV_MODELS = {
 }
 
 
-@pytest.mark.parametrize("variant", [0, 1, 4])
+@pytest.mark.parametrize("variant", [0, 1, 4, 5])
 @pytest.mark.parametrize("name", sorted(HV_MODELS))
 def test_hv_matches_oracle(gpu_lib, name, variant):
     make, sectors = HV_MODELS[name]
@@ -87,7 +87,7 @@ def test_hv_16_sites_half_filling(gpu_lib):
     ref = So.hv(x)
     from pyqcm_amd._lib import DeviceArray
     dx = DeviceArray.from_host(x)
-    for variant in (0, 4):
+    for variant in (0, 4, 5):
         dy = DeviceArray(Sg.dim).zeros()
         Sg.hv_dev(dx, dy, variant)
         got = dy.to_host()
