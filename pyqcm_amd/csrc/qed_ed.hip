@@ -1,0 +1,976 @@
+// The ED solver interface (the reference's namespace ED, src_ed/qcm_ED.hpp:23-340) on top of
+// the device sector API: model registry, instances, the sector loop and Boltzmann weights,
+// Lehmann (band Lanczos / full diagonalization) and continued-fraction Green functions,
+// self-energy, hybridization function and the Potthoff functional. Trivial symmetry group,
+// normal or up/down mixing, real Hilbert space.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/qed.h"
+#include "qed_linalg.hpp"
+#include "qed_sector.hpp"
+
+namespace qed {
+namespace ed {
+
+using cd = std::complex<double>;
+
+// ---- global parameters (src_util/global_parameter.cpp:12-91 defaults) ---------------
+struct Globals {
+  std::map<std::string, int> i{{"seed", 0},           {"max_iter_lanczos", 1000}, {"max_iter_BL", 600},
+                               {"max_iter_CF", 400},  {"max_dim_full", 256},      {"n_states", 1},
+                               {"kgrid_side", 32},    {"GF_lookup_depth", 64}};
+  std::map<std::string, double> d{{"accur_lanczos", 1e-12},     {"accur_band_lanczos", 1e-12},
+                                  {"accur_deflation", 1e-7},    {"accur_Q_matrix", 1e-5},
+                                  {"Qmatrix_wtol", 1e-5},       {"Qmatrix_vtol", 1e-10},
+                                  {"temperature", 0.0},         {"minimum_weight", 0.01},
+                                  {"accur_Davidson", 1e-5},     {"accur_continued_fraction", 0.01},
+                                  {"band_lanczos_minimum_gap", 1e-5}};
+  std::map<std::string, bool> b{{"verb_ED", false}, {"no_degenerate_BL", false}, {"continued_fraction", false}};
+  std::map<std::string, char> c{{"GF_method", 'L'}, {"Ground_state_method", 'L'}, {"Hamiltonian_format", 'F'}};
+};
+Globals& G() {
+  static Globals g;
+  return g;
+}
+int gi(const char* n) { return G().i.at(n); }
+double gd(const char* n) { return G().d.at(n); }
+char gc(const char* n) { return G().c.at(n); }
+bool gb(const char* n) { return G().b.at(n); }
+
+constexpr double SMALL_VALUE = 1.0e-12;
+constexpr double MIDDLE_VALUE = 1.0e-8;
+constexpr double MIN_GAP = 1.0e-6;
+
+struct Op {
+  std::string name, type;
+  std::vector<int> r, c;
+  std::vector<double> v;
+  int target = 1;  // 1 cluster, 2 bath, 3 cluster-bath (src_ed/Operators/Hermitian_operator.hpp)
+  bool interaction = false;
+};
+
+struct Model {
+  std::string name;
+  int n_sites = 0, n_bath = 0, L = 0;
+  std::vector<Op> ops;
+  std::map<std::string, size_t> idx;
+  int mixing = -1;
+  bool closed = false;
+};
+
+struct QMat {
+  int L = 0;
+  std::vector<double> e;  // M
+  std::vector<double> Q;  // L x M column-major
+  int M() const { return (int)e.size(); }
+};
+
+struct CFrac {
+  std::vector<double> a, b;  // after the continued_fraction transform
+  bool set = false;
+};
+
+struct State {
+  int nup = 0, ndn = 0;
+  double energy = 0.0, weight = 1.0;
+  DevBuf psi;
+  QMat gf[2];
+  std::vector<CFrac> cfe[2], cfh[2];  // n x n (o1 >= o2), electron / hole parts
+};
+
+struct Instance {
+  uint64_t label = 0;
+  Model* model = nullptr;
+  std::map<std::string, double> value;
+  std::vector<std::pair<int, int>> sectors;  // (N, S)
+  int mixing = 0;
+  bool correlated = false, gs_solved = false, gf_solved = false;
+  bool cf = false;
+  std::vector<std::unique_ptr<State>> states;
+  double GS_energy = 0.0, E0 = 0.0, SEF_bath = 0.0;
+  // one-body matrices (index 0 up, 1 down), bath diagonalized
+  std::vector<double> tc[2], tb[2], tcb[2];  // tc n x n, tb nb x nb, tcb n x nb (column-major)
+  std::vector<double> eb[2], tcb_d[2];       // diagonal bath energies and rotated hybridization
+  std::string gs_string;
+};
+
+std::map<std::string, std::unique_ptr<Model>>& models() {
+  static std::map<std::string, std::unique_ptr<Model>> m;
+  return m;
+}
+std::map<uint64_t, std::unique_ptr<Instance>>& instances() {
+  static std::map<uint64_t, std::unique_ptr<Instance>> m;
+  return m;
+}
+
+Instance& inst(uint64_t label) {
+  auto it = instances().find(label);
+  if (it == instances().end())
+    fail("The cluster instance label " + std::to_string(label) + " is out of range.");
+  return *it->second;
+}
+
+struct SectorDeleter {
+  void operator()(qed_sector* s) const { qed_sector_destroy(s); }
+};
+using SectorPtr = std::unique_ptr<qed_sector, SectorDeleter>;
+
+// flattened element lists of the active operators (value != 0), insertion order
+void flatten(const Instance& I, std::vector<int>& types, std::vector<double>& coefs, std::vector<int>& nels,
+             std::vector<int>& er, std::vector<int>& ec, std::vector<double>& ev, const std::string* only = nullptr) {
+  for (const Op& op : I.model->ops) {
+    double z;
+    if (only) {
+      if (op.name != *only) continue;
+      z = 1.0;
+    } else {
+      auto it = I.value.find(op.name);
+      if (it == I.value.end() || it->second == 0.0) continue;
+      z = it->second;
+    }
+    types.push_back(op.interaction ? 1 : 0);
+    coefs.push_back(z);
+    nels.push_back((int)op.v.size());
+    er.insert(er.end(), op.r.begin(), op.r.end());
+    ec.insert(ec.end(), op.c.begin(), op.c.end());
+    ev.insert(ev.end(), op.v.begin(), op.v.end());
+  }
+}
+
+SectorPtr make_sector(const Instance& I, int nup, int ndn, const std::string* only = nullptr) {
+  std::vector<int> types, nels, er, ec;
+  std::vector<double> coefs, ev;
+  flatten(I, types, coefs, nels, er, ec, ev, only);
+  qed_sector* s = nullptr;
+  if (qed_sector_create(I.model->L, nup, ndn, (int)types.size(), types.data(), coefs.data(), nels.data(),
+                        er.data(), ec.data(), ev.data(), &s) != 0)
+    fail(qed_last_error());
+  return SectorPtr(s);
+}
+
+// ---- one-body matrices (model_instance::set_hopping_matrix, model_instance.hpp:424-475) --
+void set_hopping_matrix(Instance& I) {
+  Model& M = *I.model;
+  const int ns = M.n_sites, nb = M.n_bath, L = M.L;
+  I.SEF_bath = 0.0;
+  const int nspin = (I.mixing & 4) ? 2 : 1;
+  for (int sp = 0; sp < 2; ++sp) {
+    I.tc[sp].assign((size_t)ns * ns, 0.0);
+    I.tb[sp].assign((size_t)nb * nb, 0.0);
+    I.tcb[sp].assign((size_t)ns * nb, 0.0);
+  }
+  for (const Op& op : M.ops) {
+    if (op.interaction) continue;
+    auto it = I.value.find(op.name);
+    if (it == I.value.end() || it->second == 0.0) continue;
+    const double z = it->second;
+    for (size_t q = 0; q < op.v.size(); ++q) {
+      int r = op.r[q], c = op.c[q];
+      double v = op.v[q] * z;
+      if (r > c) continue;
+      int sp = r >= L ? 1 : 0;
+      int rr = r - sp * L, cc = c - sp * L;
+      auto add = [&](int a, int b, double val) {
+        if (a < ns && b < ns) I.tc[sp][a + (size_t)b * ns] += val;
+        else if (a >= ns && b >= ns) I.tb[sp][(a - ns) + (size_t)(b - ns) * nb] += val;
+        else if (a < ns && b >= ns) I.tcb[sp][a + (size_t)(b - ns) * ns] += val;
+      };
+      add(rr, cc, v);
+      if (rr != cc) add(cc, rr, v);
+    }
+  }
+  for (int sp = 0; sp < 2; ++sp) {
+    I.eb[sp].assign(nb, 0.0);
+    I.tcb_d[sp] = I.tcb[sp];
+    if (nb == 0) continue;
+    std::vector<double> a(I.tb[sp]), w(nb);
+    bool diag = true;
+    for (int i = 0; i < nb && diag; ++i)
+      for (int j = 0; j < nb; ++j)
+        if (i != j && a[i + (size_t)j * nb] != 0.0) {
+          diag = false;
+          break;
+        }
+    if (diag) {
+      for (int i = 0; i < nb; ++i) I.eb[sp][i] = a[i + (size_t)i * nb];
+    } else {
+      sym_eigen(nb, a.data(), w.data(), true);
+      I.eb[sp] = w;
+      std::vector<double> t((size_t)ns * nb, 0.0);
+      for (int i = 0; i < ns; ++i)
+        for (int m = 0; m < nb; ++m) {
+          double acc = 0.0;
+          for (int k = 0; k < nb; ++k) acc += I.tcb[sp][i + (size_t)k * ns] * a[k + (size_t)m * nb];
+          t[i + (size_t)m * ns] = acc;
+        }
+      I.tcb_d[sp] = t;
+    }
+    if (sp < nspin)
+      for (int i = 0; i < nb; ++i)
+        if (I.eb[sp][i] < 0) I.SEF_bath -= I.eb[sp][i];
+  }
+  if (I.mixing == 0) I.SEF_bath *= 2.0;
+}
+
+// ---- Q-matrix post-processing (src_ed/Q_matrix.hpp) ------------------------------------
+QMat streamline(const QMat& q) {
+  const double wtol = gd("Qmatrix_wtol"), vtol = gd("Qmatrix_vtol");
+  if (vtol < 1e-12) return q;
+  QMat out;
+  out.L = q.L;
+  const int L = q.L, M = q.M();
+  for (int i = 0; i < M;) {
+    int j = i + 1;
+    while (j < M && std::fabs(q.e[j] - q.e[i]) <= wtol) ++j;
+    const int n = j - i;
+    if (n == 1) {
+      double s = 0.0;
+      for (int a = 0; a < L; ++a) s += q.Q[a + (size_t)i * L] * q.Q[a + (size_t)i * L];
+      if (s >= vtol) {
+        out.e.push_back(q.e[i]);
+        out.Q.insert(out.Q.end(), q.Q.begin() + (size_t)i * L, q.Q.begin() + (size_t)(i + 1) * L);
+      }
+    } else {
+      std::vector<double> QQ((size_t)L * L, 0.0), d(L);
+      for (int a = 0; a < L; ++a)
+        for (int b = 0; b < L; ++b) {
+          double s = 0.0;
+          for (int k = i; k < j; ++k) s += q.Q[a + (size_t)k * L] * q.Q[b + (size_t)k * L];
+          QQ[a + (size_t)b * L] = s;
+        }
+      sym_eigen(L, QQ.data(), d.data(), true);
+      for (int k = 0; k < L; ++k)
+        if (d[k] > vtol) {
+          out.e.push_back(q.e[i]);
+          double sq = std::sqrt(d[k]);
+          for (int a = 0; a < L; ++a) out.Q.push_back(QQ[a + (size_t)k * L] * sq);
+        }
+    }
+    i = j;
+  }
+  return out;
+}
+
+void check_norm(const QMat& q, double threshold) {
+  const int L = q.L, M = q.M();
+  double ztot = 0.0;
+  for (int i = 0; i < L; ++i) {
+    double z = 0.0;
+    for (int k = 0; k < M; ++k) z += q.Q[i + (size_t)k * L] * q.Q[i + (size_t)k * L];
+    ztot += (z - 1.0) * (z - 1.0);
+    for (int j = 0; j < i; ++j) {
+      double zz = 0.0;
+      for (int k = 0; k < M; ++k) zz += q.Q[i + (size_t)k * L] * q.Q[j + (size_t)k * L];
+      ztot += 2.0 * zz * zz;
+    }
+  }
+  if (ztot > threshold) fail("Q-matrix does not satisfy unitary condition by " + std::to_string(ztot));
+}
+
+void append(QMat& a, const QMat& b) {
+  if (b.M() == 0) return;
+  if (a.M() == 0) {
+    a = b;
+    return;
+  }
+  a.e.insert(a.e.end(), b.e.begin(), b.e.end());
+  a.Q.insert(a.Q.end(), b.Q.begin(), b.Q.end());
+}
+
+// Lehmann form of one sector from the start vectors phi (p x D on the device)
+QMat sector_qmatrix(qed_sector& T, const double* phi, int p) {
+  QMat q;
+  q.L = p;
+  const uint64_t D = T.D;
+  if (D < (uint64_t)gi("max_dim_full")) {
+    // full diagonalization (Hamiltonian_Dense::build_Q_matrix, Hamiltonian_Dense.hpp:157-176)
+    const int n = (int)D;
+    std::vector<double> H((size_t)n * n), e(n), ph((size_t)p * n);
+    DevBuf ex(D * 8), hy(D * 8);
+    for (int c = 0; c < n; ++c) {
+      QED_HIP(hipMemsetAsync(ex.p, 0, D * 8, stream()));
+      QED_HIP(hipMemsetAsync(hy.p, 0, D * 8, stream()));
+      const double one = 1.0;
+      QED_HIP(hipMemcpyAsync(ex.as<double>() + c, &one, 8, hipMemcpyHostToDevice, stream()));
+      hv(T, ex.as<double>(), hy.as<double>());
+      QED_HIP(hipMemcpyAsync(&H[(size_t)c * n], hy.p, D * 8, hipMemcpyDeviceToHost, stream()));
+    }
+    QED_HIP(hipMemcpyAsync(ph.data(), phi, (size_t)p * D * 8, hipMemcpyDeviceToHost, stream()));
+    QED_HIP(hipStreamSynchronize(stream()));
+    sym_eigen(n, H.data(), e.data(), true);
+    q.e = e;
+    q.Q.assign((size_t)p * n, 0.0);
+    for (int m = 0; m < n; ++m)
+      for (int i = 0; i < p; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < n; ++k) s += ph[(size_t)i * n + k] * H[k + (size_t)m * n];
+        q.Q[i + (size_t)m * p] = s;
+      }
+    return q;
+  }
+  std::vector<double> e, Q;
+  int M0 = (int)(14 * p * std::log(1.0 * D));
+  int m = band_lanczos(T, phi, p, M0, gi("max_iter_BL"), gd("accur_deflation"), gd("accur_band_lanczos"), e, Q);
+  q.e = e;
+  q.Q = Q;
+  (void)m;
+  return q;
+}
+
+bool sector_valid(const Instance& I, int nup, int ndn) {
+  const int L = I.model->L;
+  return nup >= 0 && ndn >= 0 && nup <= L && ndn <= L;
+}
+
+void build_qmatrix(Instance& I, State& st, int spin) {
+  Model& M = *I.model;
+  const int p = M.n_sites, L = M.L;
+  QMat parts[2];
+  for (int s = 0; s < 2; ++s) {
+    const int pm = 2 * s - 1;
+    const int tnup = st.nup + (spin == 0 ? pm : 0), tndn = st.ndn + (spin == 1 ? pm : 0);
+    if (!sector_valid(I, tnup, tndn)) continue;
+    SectorPtr T = make_sector(I, tnup, tndn);
+    if (T->D == 0) continue;
+    DevBuf phi((size_t)p * T->D * 8);
+    QED_HIP(hipMemsetAsync(phi.p, 0, (size_t)p * T->D * 8, stream()));
+    for (int i = 0; i < p; ++i)
+      if (qed_apply_c(L, st.nup, st.ndn, i + spin * L, pm, 1.0, st.psi.as<double>(),
+                      phi.as<double>() + (uint64_t)i * T->D) != 0)
+        fail(qed_last_error());
+    QMat q = sector_qmatrix(*T, phi.as<double>(), p);
+    for (double& e : q.e) {
+      e -= st.energy;
+      if (pm == -1) e = -e;
+    }
+    parts[s] = streamline(q);
+  }
+  QMat tot;
+  tot.L = p;
+  append(tot, parts[0]);
+  append(tot, parts[1]);
+  tot.L = p;
+  check_norm(tot, gd("accur_Q_matrix"));
+  const double sw = std::sqrt(st.weight);
+  for (double& v : tot.Q) v *= sw;
+  st.gf[spin] = tot;
+}
+
+// continued fractions (model_instance<double>::build_cf, src_ed/model_instance.cpp:34-99)
+void build_cf(Instance& I, State& st, int spin) {
+  Model& M = *I.model;
+  const int n = M.n_sites, L = M.L;
+  st.cfe[spin].assign((size_t)n * n, CFrac());
+  st.cfh[spin].assign((size_t)n * n, CFrac());
+  for (int pm = -1; pm < 2; pm += 2) {
+    const int tnup = st.nup + (spin == 0 ? pm : 0), tndn = st.ndn + (spin == 1 ? pm : 0);
+    if (!sector_valid(I, tnup, tndn)) continue;
+    SectorPtr T = make_sector(I, tnup, tndn);
+    if (T->D == 0) continue;
+    DevBuf psi(T->D * 8);
+    for (int o1 = 0; o1 < n; ++o1)
+      for (int o2 = 0; o2 <= o1; ++o2) {
+        QED_HIP(hipMemsetAsync(psi.p, 0, T->D * 8, stream()));
+        if (qed_apply_c(L, st.nup, st.ndn, o1 + spin * L, pm, 1.0, st.psi.as<double>(), psi.as<double>()) != 0)
+          fail(qed_last_error());
+        if (o1 != o2 &&
+            qed_apply_c(L, st.nup, st.ndn, o2 + spin * L, pm, 1.0, st.psi.as<double>(), psi.as<double>()) != 0)
+          fail(qed_last_error());
+        double nrm = dot(psi.as<double>(), psi.as<double>(), T->D);
+        if (nrm <= 1e-14) continue;
+        scal(1.0 / std::sqrt(nrm), psi.as<double>(), T->D);
+        std::vector<double> a, b;
+        lanczos_green(*T, psi.as<double>(), gd("accur_continued_fraction"), gi("max_iter_CF"), a, b);
+        CFrac f;
+        for (size_t i = 0; i < b.size(); ++i) b[i] *= b[i];
+        b[0] = nrm * st.weight;
+        for (double& x : a) x = (pm == 1) ? x - st.energy : -x + st.energy;
+        f.a = a;
+        f.b = b;
+        f.set = true;
+        (pm == 1 ? st.cfe[spin] : st.cfh[spin])[o1 + (size_t)o2 * n] = f;
+      }
+  }
+}
+
+std::string sector_name(int N, int S) {
+  return "R0:N" + std::to_string(N) + ":S" + std::to_string(S);
+}
+
+// ---- ground state (model_instance::low_energy_states, model_instance.hpp:245-417) ------
+void one_body_solve(Instance& I) {
+  Model& M = *I.model;
+  const int ns = M.n_sites, nb = M.n_bath, d = ns + nb;
+  I.GS_energy = 0.0;
+  auto st = std::make_unique<State>();
+  st->weight = 1.0;
+  const int nspin = (I.mixing & 4) ? 2 : 1;
+  for (int sp = 0; sp < nspin; ++sp) {
+    std::vector<double> H((size_t)d * d, 0.0), e(d);
+    for (int a = 0; a < ns; ++a)
+      for (int b = 0; b < ns; ++b) H[a + (size_t)b * d] = I.tc[sp][a + (size_t)b * ns];
+    for (int a = 0; a < nb; ++a)
+      for (int b = 0; b < nb; ++b) H[(ns + a) + (size_t)(ns + b) * d] = I.tb[sp][a + (size_t)b * nb];
+    for (int a = 0; a < ns; ++a)
+      for (int b = 0; b < nb; ++b) {
+        H[a + (size_t)(ns + b) * d] = I.tcb[sp][a + (size_t)b * ns];
+        H[(ns + b) + (size_t)a * d] = I.tcb[sp][a + (size_t)b * ns];
+      }
+    sym_eigen(d, H.data(), e.data(), true);
+    QMat q;
+    q.L = ns;
+    q.e = e;
+    q.Q.assign((size_t)ns * d, 0.0);
+    for (int m = 0; m < d; ++m)
+      for (int a = 0; a < ns; ++a) q.Q[a + (size_t)m * ns] = H[a + (size_t)m * d];
+    st->gf[sp] = q;
+    for (double x : st->gf[0].e)
+      if (x < 0) I.GS_energy += x;
+  }
+  if (I.mixing == 0) I.GS_energy *= 2.0;
+  st->energy = I.GS_energy;
+  I.E0 = I.GS_energy;
+  I.states.clear();
+  I.states.push_back(std::move(st));
+  I.gs_solved = I.gf_solved = true;
+  I.gs_string = "uncorrelated";
+}
+
+void low_energy_states(Instance& I) {
+  if (I.gs_solved) return;
+  if (!I.correlated) {
+    one_body_solve(I);
+    return;
+  }
+  I.gs_solved = true;
+  const double temperature = gd("temperature");
+  double max_gap = -std::log(gd("minimum_weight")) * temperature;
+  if (max_gap < MIN_GAP) max_gap = MIN_GAP;
+  I.GS_energy = 1.0e12;
+  std::vector<std::unique_ptr<State>> states;
+  for (auto& sec : I.sectors) {
+    const int N = sec.first, S = sec.second;
+    const int nup = (N + S) / 2, ndn = (N - S) / 2;
+    if (!sector_valid(I, nup, ndn)) continue;
+    SectorPtr H = make_sector(I, nup, ndn);
+    if (H->D == 0) continue;
+    int n_states = std::max(1, gi("n_states"));
+    if ((uint64_t)n_states > H->D) n_states = (int)H->D;
+    char method = gc("Ground_state_method");
+    if (method == 'L' && n_states > 1) method = 'D';
+    std::vector<double> evals;
+    std::vector<DevBuf> vecs;
+    try {
+      if (method == 'D') {
+        DevBuf all((size_t)n_states * H->D * 8);
+        int nit = 0;
+        davidson(*H, n_states, gd("accur_Davidson"), gi("max_iter_lanczos"), evals, all.as<double>(), &nit);
+        for (int k = 0; k < n_states; ++k) {
+          DevBuf v(H->D * 8);
+          QED_HIP(hipMemcpyAsync(v.p, all.as<double>() + (uint64_t)k * H->D, H->D * 8, hipMemcpyDeviceToDevice,
+                                 stream()));
+          vecs.push_back(std::move(v));
+        }
+      } else if (method == 'L') {
+        DevBuf v(H->D * 8);
+        if (qed_start_vector(H.get(), gi("seed"), v.as<double>()) != 0) fail(qed_last_error());
+        double E = 0.0;
+        int nit = 0;
+        lanczos_gs(*H, gd("accur_lanczos"), gi("max_iter_lanczos"), v.as<double>(), &E, &nit, nullptr, nullptr);
+        evals.push_back(E);
+        vecs.push_back(std::move(v));
+      } else {
+        fail("Unknown method in Ground_state_method global parameter");
+      }
+    } catch (const std::exception& e) {
+      fail(std::string(e.what()) + "\n  in cluster model '" + I.model->name + "', instance '" + I.model->name +
+           "_" + std::to_string(I.label) + "', sector '" + sector_name(N, S) + "'");
+    }
+    if (evals[0] < I.GS_energy) I.GS_energy = evals[0];
+    for (size_t k = 0; k < evals.size(); ++k) {
+      if (evals[k] - I.GS_energy > max_gap) continue;
+      auto st = std::make_unique<State>();
+      st->nup = nup;
+      st->ndn = ndn;
+      st->energy = evals[k];
+      st->psi = std::move(vecs[k]);
+      states.push_back(std::move(st));
+    }
+  }
+  // weights (compute_weights, model_instance.hpp:356-417)
+  if (temperature < MIDDLE_VALUE) {
+    double e0 = 1e12;
+    for (auto& s : states) e0 = std::min(e0, s->energy);
+    std::vector<std::unique_ptr<State>> kept;
+    for (auto& s : states)
+      if (s->energy - e0 <= SMALL_VALUE) kept.push_back(std::move(s));
+    states.swap(kept);
+    if (states.empty()) fail("instance " + I.model->name + "_" + std::to_string(I.label) + " has no low-energy state!");
+    for (auto& s : states) s->weight = 1.0 / states.size();
+  } else {
+    std::vector<std::unique_ptr<State>> kept;
+    for (auto& s : states)
+      if (s->energy - I.GS_energy <= max_gap) kept.push_back(std::move(s));
+    states.swap(kept);
+    double Z = 0.0;
+    for (auto& s : states) {
+      s->weight = std::exp(-(s->energy - I.GS_energy) / temperature);
+      Z += s->weight;
+    }
+    for (auto& s : states) s->weight /= Z;
+  }
+  const double minw = gd("minimum_weight");
+  std::vector<std::unique_ptr<State>> kept;
+  double tot = 0.0;
+  for (auto& s : states)
+    if (s->weight >= minw) {
+      tot += s->weight;
+      kept.push_back(std::move(s));
+    }
+  for (auto& s : kept) s->weight /= tot;
+  I.states.swap(kept);
+  bool updown = false;
+  for (auto& s : I.states)
+    if (s->nup != s->ndn) updown = true;
+  if (I.mixing == 0 && updown) I.mixing = 4;
+  I.GS_energy = 1.0e18;
+  I.E0 = 0.0;
+  std::ostringstream o;
+  for (auto& s : I.states) {
+    I.E0 += s->weight * s->energy;
+    I.GS_energy = std::min(I.GS_energy, s->energy);
+    if (o.tellp() > 0) o << "/";
+    o << sector_name(s->nup + s->ndn, s->nup - s->ndn) << ":1";
+  }
+  I.gs_string = o.str();
+  set_hopping_matrix(I);
+}
+
+void green_function_solve(Instance& I) {
+  if (I.gf_solved) return;
+  low_energy_states(I);
+  if (!I.correlated) return;
+  I.cf = gc("GF_method") == 'F' || gb("continued_fraction");
+  for (auto& s : I.states) {
+    if (I.cf) build_cf(I, *s, 0);
+    else build_qmatrix(I, *s, 0);
+    if (I.mixing & 4) {
+      if (I.cf) build_cf(I, *s, 1);
+      else build_qmatrix(I, *s, 1);
+    }
+  }
+  I.gf_solved = true;
+}
+
+int spin_index(const Instance& I, int spin_down) {
+  if (spin_down && !(I.mixing & 4) && I.mixing != 0)
+    fail("spin_down=True impossible with Hilbert space mixing " + std::to_string(I.mixing));
+  return (spin_down && (I.mixing & 4)) ? 1 : 0;
+}
+
+// G(z) for nz frequencies, d x d complex column-major each
+void green_function(Instance& I, int nz, const double* z, int spin_down, double* out) {
+  green_function_solve(I);
+  const int sp = spin_index(I, spin_down);
+  const int d = I.model->n_sites;
+  std::fill(out, out + (size_t)2 * nz * d * d, 0.0);
+  for (auto& s : I.states) {
+    if (!I.cf || !I.correlated) {
+      const QMat& q = s->gf[sp];
+      if (q.M() == 0) continue;
+      if (qed_qmatrix_gf(d, q.M(), q.e.data(), q.Q.data(), nz, z, out) != 0) fail(qed_last_error());
+    } else {
+      // continued fractions: diagonal entries and the (o1 + o2) combinations
+      std::vector<int> off, dep;
+      std::vector<double> aa, bb;
+      std::vector<int> which;  // (o1, o2, e/h)
+      for (int h = 0; h < 2; ++h) {
+        auto& cf = h ? s->cfh[sp] : s->cfe[sp];
+        for (int o1 = 0; o1 < d; ++o1)
+          for (int o2 = 0; o2 <= o1; ++o2) {
+            const CFrac& f = cf[o1 + (size_t)o2 * d];
+            if (!f.set) continue;
+            off.push_back((int)aa.size());
+            dep.push_back((int)f.a.size());
+            aa.insert(aa.end(), f.a.begin(), f.a.end());
+            bb.insert(bb.end(), f.b.begin(), f.b.begin() + f.a.size());
+            which.push_back(o1 + d * o2);
+          }
+      }
+      const int nf = (int)off.size();
+      if (nf == 0) continue;
+      std::vector<double> vals((size_t)2 * nf * nz);
+      if (qed_cf_eval(nf, off.data(), dep.data(), aa.data(), bb.data(), nz, z, vals.data()) != 0)
+        fail(qed_last_error());
+      for (int iz = 0; iz < nz; ++iz) {
+        std::vector<cd> comb((size_t)d * d, 0.0);
+        for (int f = 0; f < nf; ++f) comb[which[f]] += cd(vals[2 * ((size_t)iz * nf + f)], vals[2 * ((size_t)iz * nf + f) + 1]);
+        double* g = out + (size_t)2 * iz * d * d;
+        for (int o1 = 0; o1 < d; ++o1) {
+          g[2 * (o1 + (size_t)o1 * d)] += comb[o1 + (size_t)o1 * d].real();
+          g[2 * (o1 + (size_t)o1 * d) + 1] += comb[o1 + (size_t)o1 * d].imag();
+        }
+        for (int o1 = 0; o1 < d; ++o1)
+          for (int o2 = 0; o2 < o1; ++o2) {
+            cd t = 0.5 * (comb[o1 + (size_t)o2 * d] - comb[o1 + (size_t)o1 * d] - comb[o2 + (size_t)o2 * d]);
+            g[2 * (o1 + (size_t)o2 * d)] += t.real();
+            g[2 * (o1 + (size_t)o2 * d) + 1] += t.imag();
+            g[2 * (o2 + (size_t)o1 * d)] += t.real();
+            g[2 * (o2 + (size_t)o1 * d) + 1] += t.imag();
+          }
+      }
+    }
+  }
+}
+
+void hybridization(Instance& I, int nz, const double* z, int spin_down, double* out) {
+  low_energy_states(I);
+  const int sp = (spin_down && (I.mixing & 4)) ? 1 : 0;
+  const int d = I.model->n_sites, nb = I.model->n_bath;
+  for (int iz = 0; iz < nz; ++iz) {
+    cd w(z[2 * iz], z[2 * iz + 1]);
+    double* g = out + (size_t)2 * iz * d * d;
+    for (int a = 0; a < d; ++a)
+      for (int b = 0; b < d; ++b) {
+        cd s = 0.0;
+        for (int i = 0; i < nb; ++i)
+          s += I.tcb_d[sp][a + (size_t)i * d] * I.tcb_d[sp][b + (size_t)i * d] / (w - I.eb[sp][i]);
+        g[2 * (a + (size_t)b * d)] = s.real();
+        g[2 * (a + (size_t)b * d) + 1] = s.imag();
+      }
+  }
+}
+
+}  // namespace ed
+}  // namespace qed
+
+using namespace qed;
+using namespace qed::ed;
+
+namespace {
+std::vector<std::string> split(const std::string& s, char c) {
+  std::vector<std::string> out;
+  std::string cur;
+  for (char ch : s) {
+    if (ch == c) {
+      out.push_back(cur);
+      cur.clear();
+    } else {
+      cur += ch;
+    }
+  }
+  out.push_back(cur);
+  return out;
+}
+
+// "R0:N12:S0" -> (N, S)
+std::pair<int, int> parse_sector(const std::string& s) {
+  int R = -1, N = -1000, S = -1000;
+  for (auto& tok : split(s, ':')) {
+    if (tok.empty()) continue;
+    char k = tok[0];
+    std::string v = tok.substr(1);
+    try {
+      if (k == 'R') R = std::stoi(v);
+      else if (k == 'N') N = std::stoi(v);
+      else if (k == 'S') S = std::stoi(v);
+    } catch (...) {
+      fail("sector " + s + " is not valid");
+    }
+  }
+  if (R != 0 || N == -1000 || S == -1000) fail("sector " + s + " is not valid");
+  if ((N + S) % 2 != 0 || S > N || S < -N || N < 0) fail("sector " + s + " is not valid");
+  return {N, S};
+}
+}  // namespace
+
+extern "C" {
+
+int qed_ed_new_model(const char* name, int n_sites, int n_bath) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    if (models().count(name)) fail(std::string("the name '") + name + "' has already been used for another cluster model");
+    if (n_sites <= 0 || n_bath < 0 || n_sites + n_bath > 32) fail("invalid model size");
+    auto m = std::make_unique<Model>();
+    m->name = name;
+    m->n_sites = n_sites;
+    m->n_bath = n_bath;
+    m->L = n_sites + n_bath;
+    models()[name] = std::move(m);
+  });
+}
+
+int qed_ed_new_operator(const char* model, const char* name, const char* type, int nel, const int* r,
+                        const int* c, const double* v) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    if (nel == 0) return;
+    auto it = models().find(model);
+    if (it == models().end()) fail(std::string("The model ") + model + " is not defined. Check spelling.");
+    Model& M = *it->second;
+    if (M.closed) fail("model " + M.name + " has already been instantiated and is closed for modifications");
+    std::string t(type);
+    Op op;
+    op.name = name;
+    op.type = t;
+    if (t == "interaction") op.interaction = true;
+    else if (t != "one-body") fail("type of operator " + op.name + " is not supported by the factorized solver");
+    bool touches_bath = false, touches_site = false, hybrid = false;
+    for (int i = 0; i < nel; ++i) {
+      int a = r[i], b = c[i];
+      if (a < 0 || b < 0 || a >= 2 * M.L || b >= 2 * M.L) fail("operator element index out of range");
+      if (op.interaction && a > b) std::swap(a, b);
+      op.r.push_back(a);
+      op.c.push_back(b);
+      op.v.push_back(v[i]);
+      bool ab = (a % M.L) >= M.n_sites, bb = (b % M.L) >= M.n_sites;
+      touches_bath |= ab || bb;
+      touches_site |= !ab || !bb;
+      hybrid |= ab != bb;
+    }
+    op.target = hybrid ? 3 : (touches_bath && !touches_site ? 2 : 1);
+    if (M.idx.count(op.name)) {
+      M.ops[M.idx[op.name]] = op;
+    } else {
+      M.idx[op.name] = M.ops.size();
+      M.ops.push_back(op);
+    }
+  });
+}
+
+int qed_ed_new_model_instance(const char* model, int nparam, const char* const* names, const double* values,
+                              const char* sectors, uint64_t label) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    auto it = models().find(model);
+    if (it == models().end())
+      fail(std::string("The model ") + model + " is not defined and so no model instance based on it is allowed.");
+    Model& M = *it->second;
+    M.closed = true;
+    auto I = std::make_unique<Instance>();
+    I->label = label;
+    I->model = &M;
+    for (int k = 0; k < nparam; ++k)
+      if (M.idx.count(names[k])) I->value[names[k]] = values[k];
+    if (I->value.empty()) fail("ED model instance (label = " + std::to_string(label) + ") for model " + M.name + " has no parameters!");
+    // mixing: normal unless spin-asymmetric one-body terms or S != 0 sectors
+    int mixing = 0;
+    for (auto& kv : I->value) {
+      const Op& op = M.ops[M.idx[kv.first]];
+      if (op.interaction && kv.second != 0.0) I->correlated = true;
+      if (op.interaction || kv.second == 0.0) continue;
+      std::map<std::pair<int, int>, double> up, dn;
+      for (size_t q = 0; q < op.v.size(); ++q) {
+        if (op.r[q] < M.L) up[{op.r[q], op.c[q]}] += op.v[q];
+        else dn[{op.r[q] - M.L, op.c[q] - M.L}] += op.v[q];
+      }
+      if (up != dn) mixing = 4;
+    }
+    for (auto& s : split(sectors, '/')) {
+      if (s.empty()) continue;
+      auto ns = parse_sector(s);
+      if (ns.second != 0 && mixing == 0) mixing = 4;
+      I->sectors.push_back(ns);
+    }
+    if (I->sectors.empty()) fail("no target sectors were specified for model " + M.name);
+    if (M.mixing == -1) M.mixing = mixing;
+    else if ((mixing | M.mixing) != M.mixing)
+      fail("The new instance mixing state of model " + M.name + " (" + std::to_string(mixing) +
+           ") is not compatible with the first instance mixing (" + std::to_string(M.mixing) + ")");
+    else mixing = M.mixing;
+    I->mixing = mixing;
+    set_hopping_matrix(*I);
+    instances()[label] = std::move(I);
+  });
+}
+
+int qed_ed_erase_model_instance(uint64_t label) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] { instances().erase(label); });
+}
+
+int qed_ed_ground_state_solve(uint64_t label, double* E0, char* buf, int buflen) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    Instance& I = inst(label);
+    low_energy_states(I);
+    *E0 = I.GS_energy;
+    if (buf && buflen > 0) {
+      std::strncpy(buf, I.gs_string.c_str(), buflen - 1);
+      buf[buflen - 1] = 0;
+    }
+  });
+}
+
+int qed_ed_green_function_solve(uint64_t label) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] { green_function_solve(inst(label)); });
+}
+
+int qed_ed_green_function(uint64_t label, int nz, const double* z, int spin_down, double* Gout) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] { green_function(inst(label), nz, z, spin_down, Gout); });
+}
+
+int qed_ed_hybridization_function(uint64_t label, int nz, const double* z, int spin_down, double* Gam) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] { hybridization(inst(label), nz, z, spin_down, Gam); });
+}
+
+int qed_ed_self_energy(uint64_t label, int nz, const double* z, int spin_down, double* S) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    Instance& I = inst(label);
+    const int d = I.model->n_sites;
+    std::vector<double> G((size_t)2 * nz * d * d), H((size_t)2 * nz * d * d);
+    green_function(I, nz, z, spin_down, G.data());
+    hybridization(I, nz, z, spin_down, H.data());
+    const int sp = (spin_down && (I.mixing & 4)) ? 1 : 0;
+    for (int iz = 0; iz < nz; ++iz) {
+      std::vector<cd> g((size_t)d * d);
+      for (size_t q = 0; q < (size_t)d * d; ++q) g[q] = cd(G[2 * ((size_t)iz * d * d + q)], G[2 * ((size_t)iz * d * d + q) + 1]);
+      if (!cinverse(d, g.data())) fail("cluster Green function is singular");
+      cd w(z[2 * iz], z[2 * iz + 1]);
+      for (int a = 0; a < d; ++a)
+        for (int b = 0; b < d; ++b) {
+          size_t q = a + (size_t)b * d;
+          cd s = -(g[q] + I.tc[sp][q] + (I.model->n_bath ? cd(H[2 * ((size_t)iz * d * d + q)], H[2 * ((size_t)iz * d * d + q) + 1]) : cd(0.0)));
+          if (a == b) s += w;
+          S[2 * ((size_t)iz * d * d + q)] = s.real();
+          S[2 * ((size_t)iz * d * d + q) + 1] = s.imag();
+        }
+    }
+  });
+}
+
+int qed_ed_hopping_matrix(uint64_t label, int spin_down, double* tc) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    Instance& I = inst(label);
+    const int sp = (spin_down && (I.mixing & 4)) ? 1 : 0;
+    const int d = I.model->n_sites;
+    for (size_t q = 0; q < (size_t)d * d; ++q) {
+      tc[2 * q] = I.tc[sp][q];
+      tc[2 * q + 1] = 0.0;
+    }
+  });
+}
+
+int qed_ed_bath(uint64_t label, int spin_down, double* eb, double* tcb) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    Instance& I = inst(label);
+    const int sp = (spin_down && (I.mixing & 4)) ? 1 : 0;
+    std::copy(I.eb[sp].begin(), I.eb[sp].end(), eb);
+    std::copy(I.tcb_d[sp].begin(), I.tcb_d[sp].end(), tcb);
+  });
+}
+
+int qed_ed_qmatrix(uint64_t label, int spin_down, int* M, double* e, double* Q) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    Instance& I = inst(label);
+    green_function_solve(I);
+    if (I.cf && I.correlated) fail("the Q-matrix is not available with the continued-fraction Green function");
+    const int sp = spin_index(I, spin_down);
+    const int d = I.model->n_sites;
+    int tot = 0;
+    for (auto& s : I.states) tot += s->gf[sp].M();
+    *M = tot;
+    if (!e) return;
+    int off = 0;
+    for (auto& s : I.states) {
+      const QMat& q = s->gf[sp];
+      std::copy(q.e.begin(), q.e.end(), e + off);
+      std::copy(q.Q.begin(), q.Q.end(), Q + (size_t)off * d);
+      off += q.M();
+    }
+  });
+}
+
+int qed_ed_potthoff_functional(uint64_t label, double* omega) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    Instance& I = inst(label);
+    green_function_solve(I);
+    *omega = I.SEF_bath + I.E0;
+  });
+}
+
+int qed_ed_model_size(const char* model, int* n_sites, int* n_bath) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    auto it = models().find(model);
+    if (it == models().end()) fail(std::string("model ") + model + " does not exist!");
+    *n_sites = it->second->n_sites;
+    *n_bath = it->second->n_bath;
+  });
+}
+
+int qed_ed_dim_gf(uint64_t label, int* d) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] { *d = inst(label).model->n_sites; });
+}
+
+int qed_ed_cluster_averages(uint64_t label, int nop, const char* const* names, double* ave, double* var) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    Instance& I = inst(label);
+    low_energy_states(I);
+    if (!I.correlated) fail("cluster averages of an uncorrelated instance come from the Green function");
+    for (int k = 0; k < nop; ++k) {
+      std::string nm(names[k]);
+      if (!I.value.count(nm)) fail("operator " + nm + " is not activated in instance " + std::to_string(label));
+      const Op& op = I.model->ops[I.model->idx.at(nm)];
+      double a = 0.0, a2 = 0.0;
+      for (auto& s : I.states) {
+        SectorPtr H = make_sector(I, s->nup, s->ndn, &nm);
+        DevBuf y(H->D * 8);
+        QED_HIP(hipMemsetAsync(y.p, 0, H->D * 8, stream()));
+        hv(*H, s->psi.as<double>(), y.as<double>());
+        a += s->weight * dot(y.as<double>(), s->psi.as<double>(), H->D);
+        a2 += s->weight * dot(y.as<double>(), y.as<double>(), H->D);
+      }
+      double v = a2 - a * a;
+      if (op.target == 1) {
+        double norm = 1.0 / I.model->n_sites;
+        if (nm == "mu") norm *= -1.0;
+        a *= norm;
+        v *= norm * norm;
+      }
+      ave[k] = a;
+      var[k] = v;
+    }
+  });
+}
+
+int qed_ed_set_global_int(const char* name, int v) {
+  return guard([&] { G().i[name] = v; });
+}
+int qed_ed_set_global_double(const char* name, double v) {
+  return guard([&] { G().d[name] = v; });
+}
+int qed_ed_set_global_bool(const char* name, int v) {
+  return guard([&] { G().b[name] = v != 0; });
+}
+int qed_ed_set_global_char(const char* name, char v) {
+  return guard([&] { G().c[name] = v; });
+}
+int qed_ed_get_global_double(const char* name, double* v) {
+  return guard([&] {
+    auto it = G().d.find(name);
+    if (it == G().d.end()) fail(std::string("unknown global parameter ") + name);
+    *v = it->second;
+  });
+}
+
+}  // extern "C"

@@ -504,6 +504,224 @@ void lanczos_green(qed_sector& s, const double* psi_in, double accur_cf, int max
   }
 }
 
+
+// ---------------------------------------------------------------------------------------
+// Davidson-Liu for the K lowest states (src_ed/Hamiltonian/Davidson.hpp:38-230)
+
+namespace {
+
+// out_k[i] = sum_p C[p + m k] in_p[i], k < K (outputs distinct from inputs)
+__global__ void k_combine(PtrList in, int m, MutPtrList out, int K, const double* __restrict__ C,
+                          uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    double v[kMaxMulti];
+    for (int p = 0; p < m; ++p) v[p] = in.p[p][i];
+    for (int k = 0; k < K; ++k) {
+      double acc = 0.0;
+      for (int p = 0; p < m; ++p) acc += C[p + m * k] * v[p];
+      out.p[k][i] = acc;
+    }
+  }
+}
+
+// c = r / (mu - d + tiny) with d the diagonal of H as the factorized format reports it
+__global__ void k_precondition(HvView v, const double* __restrict__ r, double mu, double tiny,
+                               double* __restrict__ c, uint64_t n, const double* __restrict__ d) {
+  for (uint64_t I = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n;
+       I += (uint64_t)gridDim.x * blockDim.x)
+    c[I] = r[I] / (-d[I] + mu + tiny);
+}
+
+__global__ void k_diag_only_interaction(HvView v, double* d, uint64_t D) {
+  for (uint64_t I = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; I < D;
+       I += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t iu = (uint32_t)(I % v.du), id = (uint32_t)(I / v.du);
+    uint32_t su = v.bup[iu], sd = v.bdn[id];
+    double e = 0.0;
+    if (v.hubbard_uniform) {
+      e = v.hub_U * (double)__popc(su & sd & v.hub_sites);
+    } else {
+      uint64_t w = (uint64_t)sd | ((uint64_t)su << 32);
+      for (int k = 0; k < v.n_int; ++k)
+        if ((w & v.int_mask[k]) == v.int_mask[k]) e += v.int_val[k];
+    }
+    d[I] = e;
+  }
+}
+
+void combine(const std::vector<double*>& in, const std::vector<double>& C, int K,
+             const std::vector<double*>& out, uint64_t n) {
+  const int m = (int)in.size();
+  if (m > kMaxMulti || K > kMaxMulti) fail("Davidson: subspace too large");
+  PtrList li{};
+  MutPtrList lo{};
+  for (int p = 0; p < m; ++p) li.p[p] = in[p];
+  for (int k = 0; k < K; ++k) lo.p[k] = out[k];
+  double* cd = scratch().as<double>() + (size_t)kMaxBlocks * kMaxMulti + 64;
+  if (C.size() > 512) fail("Davidson: coefficient block too large");
+  QED_HIP(hipMemcpyAsync(cd, C.data(), C.size() * sizeof(double), hipMemcpyHostToDevice, stream()));
+  k_combine<<<stream_blocks(n), 256, 0, stream()>>>(li, m, lo, K, cd, n);
+  QED_HIP(hipStreamSynchronize(stream()));
+}
+
+// find_lowest (src_util/vector_num.hpp:242-264): insertion into a sorted list of nm
+// entries, later indices placed before earlier equal values
+std::vector<size_t> find_lowest(const std::vector<double>& x, size_t nm) {
+  std::vector<double> minv(nm);
+  std::vector<size_t> idx(nm, 0);
+  for (size_t i = 0; i < nm; ++i) minv[i] = 1.0e12 + 1.0 * i;
+  for (size_t i = 0; i < x.size(); ++i) {
+    int j;
+    for (j = (int)nm - 1; j >= 0; --j)
+      if (x[i] > minv[j]) break;
+    ++j;
+    if (j < (int)nm) {
+      for (int k = (int)nm - 1; k > j; --k) {
+        minv[k] = minv[k - 1];
+        idx[k] = idx[k - 1];
+      }
+      minv[j] = x[i];
+      idx[j] = i;
+    }
+  }
+  return idx;
+}
+
+}  // namespace
+
+void davidson(qed_sector& s, int K, double accur, int max_iter, std::vector<double>& val,
+              double* xout, int* niter) {
+  const uint64_t n = s.D;
+  const double tiny = 1e-14;
+  if ((uint64_t)K > n) K = (int)n;
+  val.assign(K, 0.0);
+  VecPool pool(n);
+  double* d = pool.get();
+  k_diag_only_interaction<<<stream_blocks(n), 256, 0, stream()>>>(view(s), d, n);
+  std::vector<double> dh(n);
+  QED_HIP(hipMemcpyAsync(dh.data(), d, n * 8, hipMemcpyDeviceToHost, stream()));
+  QED_HIP(hipStreamSynchronize(stream()));
+  std::vector<size_t> low = find_lowest(dh, (size_t)K);
+  dh.clear();
+  dh.shrink_to_fit();
+  std::vector<double*> b, h, x(K);
+  for (int k = 0; k < K; ++k) {
+    double* bk = pool.get();
+    QED_HIP(hipMemsetAsync(bk, 0, n * 8, stream()));
+    const double one = 1.0;
+    QED_HIP(hipMemcpyAsync(bk + low[k], &one, 8, hipMemcpyHostToDevice, stream()));
+    double* hk = pool.get();
+    QED_HIP(hipMemsetAsync(hk, 0, n * 8, stream()));
+    hv(s, bk, hk);
+    b.push_back(bk);
+    h.push_back(hk);
+    x[k] = pool.get();
+  }
+  QED_HIP(hipStreamSynchronize(stream()));
+  int m = K;
+  std::vector<double> bhb((size_t)m * m);
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j)
+      QED_HIP(hipMemcpy(&bhb[(size_t)j * m + i], h[j] + low[i], 8, hipMemcpyDeviceToHost));
+  int iter;
+  std::vector<double*> r(K, nullptr);
+  for (iter = 0; iter < max_iter; ++iter) {
+    std::vector<double> a(bhb), mu(m);
+    sym_eigen(m, a.data(), mu.data(), true);  // columns of a are eigenvectors
+    std::vector<double> C((size_t)m * K);
+    for (int k = 0; k < K; ++k)
+      for (int p = 0; p < m; ++p) C[p + (size_t)m * k] = a[(size_t)k * m + p];
+    combine(b, C, K, x, n);
+    std::vector<double*> rr(K);
+    for (int k = 0; k < K; ++k) rr[k] = pool.get();
+    combine(h, C, K, rr, n);
+    std::vector<int> unconv;
+    for (int k = 0; k < K; ++k) {
+      double res = axpy_nrm2(-mu[k], x[k], rr[k], n);
+      if (res > accur) unconv.push_back(k);
+    }
+    const int Q = (int)unconv.size();
+    if (Q == 0) {
+      for (int k = 0; k < K; ++k) {
+        val[k] = mu[k];
+        pool.put(rr[k]);
+      }
+      break;
+    }
+    // new subspace: Ritz vectors, previous basis vectors of unconverged roots, corrections
+    std::vector<double*> nb;
+    for (int k = 0; k < K; ++k) {
+      double* t = pool.get();
+      QED_HIP(hipMemcpyAsync(t, x[k], n * 8, hipMemcpyDeviceToDevice, stream()));
+      nb.push_back(t);
+    }
+    std::vector<bool> keep_b(b.size(), false);
+    for (int k : unconv) {
+      nb.push_back(b[k]);
+      keep_b[k] = true;
+    }
+    for (int k : unconv) {
+      k_precondition<<<stream_blocks(n), 256, 0, stream()>>>(view(s), rr[k], mu[k], tiny, rr[k], n, d);
+      nb.push_back(rr[k]);
+      rr[k] = nullptr;
+    }
+    for (int k = 0; k < K; ++k) pool.put(rr[k]);
+    for (size_t p = 0; p < b.size(); ++p)
+      if (!keep_b[p]) pool.put(b[p]);
+    for (double* hp : h) pool.put(hp);
+    h.clear();
+    // Gram-Schmidt in the reference's order (vectors below tiny are removed)
+    b = nb;
+    for (size_t i = 0; i < b.size(); ++i) {
+      double z = nrm2(b[i], n);
+      if (z < tiny) {
+        pool.put(b[i]);
+        b[i] = b.back();
+        b.pop_back();
+        --i;
+        continue;
+      }
+      scal(1.0 / z, b[i], n);
+      int cnt = (int)b.size() - (int)(i + 1);
+      if (cnt > 0) {
+        std::vector<const double*> ys(b.begin() + i + 1, b.end());
+        std::vector<double> zz(cnt);
+        multi_dot(b[i], ys.data(), cnt, n, zz.data());
+        std::vector<double*> ym(b.begin() + i + 1, b.end());
+        multi_axpy_rev(zz.data(), b[i], ym.data(), cnt, n);
+      }
+    }
+    if ((int)b.size() == K) {
+      for (int k = 0; k < K; ++k) val[k] = mu[k];
+      break;
+    }
+    m = (int)b.size();
+    for (int p = 0; p < m; ++p) {
+      double* hp = pool.get();
+      QED_HIP(hipMemsetAsync(hp, 0, n * 8, stream()));
+      hv(s, b[p], hp);
+      h.push_back(hp);
+    }
+    bhb.assign((size_t)m * m, 0.0);
+    for (int i = 0; i < m; ++i) {
+      std::vector<const double*> hs(h.begin(), h.begin() + i + 1);
+      std::vector<double> zz(i + 1);
+      multi_dot(b[i], hs.data(), i + 1, n, zz.data());
+      for (int k = 0; k <= i; ++k) {
+        bhb[(size_t)k * m + i] = zz[k];
+        bhb[(size_t)i * m + k] = zz[k];
+      }
+    }
+    for (int k = 0; k < K; ++k) val[k] = mu[k];
+  }
+  if (iter == max_iter) fail("Davidson method failed to converge!");
+  *niter = iter;
+  for (int k = 0; k < K; ++k)
+    QED_HIP(hipMemcpyAsync(xout + (uint64_t)k * n, x[k], n * 8, hipMemcpyDeviceToDevice, stream()));
+  QED_HIP(hipStreamSynchronize(stream()));
+}
+
 }  // namespace qed
 
 using namespace qed;
@@ -544,6 +762,16 @@ int qed_band_lanczos(qed_sector* s, const double* phi, int p, int M0, int max_it
     *M = m;
     std::copy(e.begin(), e.end(), evals);
     std::copy(q.begin(), q.end(), Q);
+  });
+}
+
+int qed_davidson(qed_sector* s, int k, double accur, int max_iter, double* evals, double* psi,
+                 int* niter) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    std::vector<double> e;
+    davidson(*s, k, accur, max_iter, e, psi, niter);
+    std::copy(e.begin(), e.end(), evals);
   });
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "qed_common.hpp"
@@ -92,5 +93,13 @@ void multi_axpy_rev(const double* c, const double* x, double* const* ys, int m, 
 
 void lanczos_gs(qed_sector& s, double accur, int max_iter, double* psi, double* E0, int* niter,
                 std::vector<double>* alpha, std::vector<double>* beta);
+int band_lanczos(qed_sector& s, const double* phi, int p0, int M0, int max_iter_BL,
+                 double accur_deflation, double accur_band_lanczos, std::vector<double>& evals,
+                 std::vector<double>& Q);
+void lanczos_green(qed_sector& s, const double* psi, double accur_cf, int max_iter,
+                   std::vector<double>& a, std::vector<double>& b);
+void davidson(qed_sector& s, int k, double accur, int max_iter, std::vector<double>& evals,
+              double* psi, int* niter);
+std::recursive_mutex& lib_mutex();
 
 }  // namespace qed

@@ -1,0 +1,211 @@
+"""The ``qcm`` functions pyqcm's Python layer calls on the ED path, backed by libqed.so.
+
+Names, argument meaning, defaults and errors follow the reference binding
+(src_python/qcm_ED_wrap.hpp:27-381, src_python/qcm_wrap.hpp:292-910): operator element
+labels are 1-based with r <= c, anomalous elements are scaled by 1/2 as there (anomalous
+operators themselves are not supported by the factorized solver and are rejected),
+failures raise RuntimeError with the library's message.
+
+Beyond the reference's one-frequency calls, every Green-function entry point also takes
+an array of frequencies and evaluates them in one device batch.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, dp, ip
+
+_instance_values: dict[int, dict] = {}
+_instance_model: dict[int, str] = {}
+_model_sizes: dict[str, tuple] = {}
+
+
+def _b(s):
+    return s.encode() if isinstance(s, str) else s
+
+
+# ---------------------------------------------------------------- models and operators
+
+def new_model(name, n_sites, n_bath, generators=None, bath_irrep=False):
+    """ED::new_model (src_ed/qcm_ED.cpp:39-58). Only the trivial point group is supported:
+    the factorized solver requires it (src_ed/model_instance_base.cpp:87-88)."""
+    if generators:
+        raise RuntimeError("the factorized format is not compatible with a nontrivial point group symmetry")
+    call("qed_ed_new_model", _b(name), int(n_sites), int(n_bath))
+    _model_sizes[name] = (int(n_sites), int(n_bath))
+
+
+def model_sizeC(name):
+    ns, nb = ctypes.c_int(), ctypes.c_int()
+    call("qed_ed_model_size", _b(name), ctypes.byref(ns), ctypes.byref(nb))
+    return ns.value, nb.value, 1
+
+
+def new_operator(model, name, type, elements):
+    """ED::new_operator with the binding's label conversion: elements (r, c, v) are
+    1-based, r <= c (src_python/qcm_ED_wrap.hpp:174-215)."""
+    r, c, v = [], [], []
+    fac = 0.5 if type == "anomalous" else 1.0
+    check_upper = type != "general_interaction"
+    for i, el in enumerate(elements):
+        if len(el) != 3:
+            raise RuntimeError("element %d of argument 4 of 'new_operator' should be a 3-tuple" % i)
+        a, b, val = int(el[0]), int(el[1]), float(el[2])
+        if a > b and check_upper:
+            raise RuntimeError("the first index of element %d of argument 4 of 'new_operator' cannot be "
+                               "bigger than the second index" % i)
+        if a == 0 or b == 0:
+            raise RuntimeError("indices in matrix elements of operators are labelled starting at 1, not at 0.")
+        r.append(a - 1)
+        c.append(b - 1)
+        v.append(fac * val)
+    r = np.array(r, np.int32)
+    c = np.array(c, np.int32)
+    v = np.array(v, np.float64)
+    call("qed_ed_new_operator", _b(model), _b(name), _b(type), len(v), ip(r), ip(c), dp(v))
+
+
+def new_model_instanceC(name, values, sectors, label=0):
+    """ED::new_model_instance (src_ed/qcm_ED.cpp:127-158)."""
+    names = list(values.keys())
+    arr = (ctypes.c_char_p * len(names))(*[_b(n) for n in names])
+    vals = np.array([float(values[n]) for n in names], np.float64)
+    call("qed_ed_new_model_instance", _b(name), len(names), arr, dp(vals), _b(sectors), int(label))
+    _instance_values[int(label)] = dict(values)
+    _instance_model[int(label)] = name
+
+
+def erase_model_instance(label=0):
+    call("qed_ed_erase_model_instance", int(label))
+    _instance_values.pop(int(label), None)
+
+
+# ---------------------------------------------------------------- solving
+
+def ground_state_solve(label=0):
+    """Returns (energy, Hilbert space sector string)."""
+    e = np.zeros(1)
+    buf = ctypes.create_string_buffer(4096)
+    call("qed_ed_ground_state_solve", int(label), dp(e), buf, 4096)
+    return float(e[0]), buf.value.decode()
+
+
+def Green_function_solveC(label=0):
+    call("qed_ed_green_function_solve", int(label))
+
+
+def Green_function_dimensionC(label=0):
+    d = ctypes.c_int()
+    call("qed_ed_dim_gf", int(label), ctypes.byref(d))
+    return d.value
+
+
+def _zarr(z):
+    z = np.atleast_1d(np.asarray(z, np.complex128))
+    return z, np.ascontiguousarray(np.stack([z.real, z.imag], -1).ravel())
+
+
+def _batched(fn, z, spin_down, label):
+    zs, zz = _zarr(z)
+    d = Green_function_dimensionC(label)
+    out = np.zeros(2 * len(zs) * d * d)
+    call(fn, int(label), len(zs), dp(zz), int(spin_down), dp(out))
+    g = out.view(np.complex128).reshape(len(zs), d, d)
+    # column-major per matrix -> (row, col) arrays
+    g = np.transpose(g, (0, 2, 1)).copy()
+    return g[0] if np.ndim(z) == 0 else g
+
+
+def Green_function(z, spin_down=0, label=0, blocks=0):
+    """G(z) (dim_GF x dim_GF); z may be an array of frequencies (batched)."""
+    return _batched("qed_ed_green_function", z, spin_down, label)
+
+
+def self_energyC(z, spin_down=0, label=0):
+    return _batched("qed_ed_self_energy", z, spin_down, label)
+
+
+def hybridization_functionC(z, spin_down=0, label=0):
+    return _batched("qed_ed_hybridization_function", z, spin_down, label)
+
+
+def hopping_matrix(spin_down=0, diag=0, label=0, full=0):
+    d = Green_function_dimensionC(label)
+    out = np.zeros(2 * d * d)
+    call("qed_ed_hopping_matrix", int(label), int(spin_down), dp(out))
+    return out.view(np.complex128).reshape(d, d).T.copy()
+
+
+def bath(label=0, spin_down=0):
+    """Diagonalized bath: energies (nb) and cluster-bath hybridization (d x nb)."""
+    d = Green_function_dimensionC(label)
+    nb = model_sizeC(_instance_model[int(label)])[1]
+    eb = np.zeros(max(nb, 1))
+    t = np.zeros(max(d * nb, 1))
+    call("qed_ed_bath", int(label), int(spin_down), dp(eb), dp(t))
+    return eb[:nb], t[:d * nb].reshape(nb, d).T.copy()
+
+
+def qmatrix(label=0, spin_down=0):
+    """Lehmann representation: (eigenvalues M, M x L matrix) as the reference binding."""
+    M = ctypes.c_int()
+    call("qed_ed_qmatrix", int(label), int(spin_down), ctypes.byref(M), None, None)
+    m = M.value
+    d = Green_function_dimensionC(label)
+    e = np.zeros(max(m, 1))
+    Q = np.zeros(max(m * d, 1))
+    call("qed_ed_qmatrix", int(label), int(spin_down), ctypes.byref(M), dp(e), dp(Q))
+    return e[:m].copy(), Q[:m * d].reshape(m, d).astype(np.complex128)
+
+
+def Potthoff_functional(label=0):
+    """ED part of the Potthoff functional: E0 + SEF_bath (src_ed/qcm_ED.cpp:330-334)."""
+    w = np.zeros(1)
+    call("qed_ed_potthoff_functional", int(label), dp(w))
+    return float(w[0])
+
+
+def cluster_averages(label=0):
+    """dict name -> (average, variance) over the active operators."""
+    names = list(_instance_values.get(int(label), {}).keys())
+    if not names:
+        return {}
+    arr = (ctypes.c_char_p * len(names))(*[_b(n) for n in names])
+    ave = np.zeros(len(names))
+    var = np.zeros(len(names))
+    call("qed_ed_cluster_averages", int(label), len(names), arr, dp(ave), dp(var))
+    return {n: (float(a), float(v)) for n, a, v in zip(names, ave, var)}
+
+
+# ---------------------------------------------------------------- global parameters
+
+_BOOL = {"verb_ED", "no_degenerate_BL", "continued_fraction"}
+
+
+def set_global_parameter(name, value=None):
+    """Same typing rules as the reference binding: None -> bool true; int -> bool or int;
+    float -> double; str -> char."""
+    if value is None:
+        call("qed_ed_set_global_bool", _b(name), 1)
+    elif isinstance(value, bool) or (isinstance(value, int) and name in _BOOL):
+        call("qed_ed_set_global_bool", _b(name), int(bool(value)))
+    elif isinstance(value, int):
+        call("qed_ed_set_global_int", _b(name), int(value))
+    elif isinstance(value, float):
+        call("qed_ed_set_global_double", _b(name), float(value))
+    elif isinstance(value, str):
+        call("qed_ed_set_global_char", _b(name), ctypes.c_char(value.encode()[0]))
+    else:
+        raise RuntimeError("unknown type of global_parameter")
+
+
+set_global_parameterC = set_global_parameter
+
+
+def get_global_double(name):
+    v = np.zeros(1)
+    call("qed_ed_get_global_double", _b(name), dp(v))
+    return float(v[0])

@@ -1,0 +1,291 @@
+"""ED interface (qcm.* / namespace ED) on the GPU against the oracle, brute force and the
+reference's recorded outputs; G(z), continued fractions and the lattice k-sum kernels."""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import models as TM
+
+pytestmark = pytest.mark.gpu
+
+_counter = itertools.count(1000)
+
+
+def register(model, values, sectors, label=None, name=None):
+    """Registers an oracle ClusterModel through the qcm functions (1-based labels)."""
+    from pyqcm_amd import qcm
+    name = name or "m%d" % next(_counter)
+    ns = model.n_sites if hasattr(model, "n_sites") else None
+    if ns is None:
+        ns = model.L
+    nb = model.L - ns
+    qcm.new_model(name, ns, nb)
+    for opname, op in model.operators.items():
+        qcm.new_operator(name, opname, op.kind, [(r + 1, c + 1, v) for (r, c, v) in op.elements])
+    label = next(_counter) if label is None else label
+    qcm.new_model_instanceC(name, values, sectors, label)
+    return label
+
+
+def with_sites(model, n_sites):
+    model.n_sites = n_sites
+    return model
+
+
+# ------------------------------------------------------------------ ground state
+
+def test_chain4_reference_values(gpu_lib, golden):
+    """examples/1D_4/GS.tsv row 1: E0, sector, and the cluster averages/variances."""
+    from pyqcm_amd import qcm
+    row = golden["chain4"]["rows"][0]
+    m, vals = TM.chain(4, U=4.0, mu=2.0)
+    lab = register(with_sites(m, 4), vals, "R0:N4:S0")
+    E0, sec = qcm.ground_state_solve(lab)
+    assert abs(E0 - float(row["E0_1"])) < 1e-7
+    assert sec == row["sector_1"]
+    ave = qcm.cluster_averages(lab)
+    for op in ("t", "U", "mu"):
+        assert abs(ave[op][0] - float(row["ave_%s_1" % op])) < 2e-7, op
+        assert abs(ave[op][1] - float(row["var_%s_1" % op])) < 2e-7, op
+
+
+def test_plaquette_8bath_reference(gpu_lib, golden):
+    """S_format.ref: two target sectors, ground state in N=12; E0 = -75.0353."""
+    from pyqcm_amd import qcm
+    g = golden["plaquette_8bath"]
+    m, vals = TM.plaquette_bath(g["params"])
+    lab = register(with_sites(m, 4), vals, "R0:N12:S0/R0:N8:S0")
+    E0, sec = qcm.ground_state_solve(lab)
+    assert abs(E0 - g["E0"]) < 6e-5
+    assert sec.startswith("R0:N12:S0")
+
+
+# ------------------------------------------------------------------ Green function
+
+def test_green_function_dense_path_matches_bruteforce(gpu_lib):
+    """Sectors below max_dim_full use full diagonalization: exact Lehmann form."""
+    from pyqcm_amd import qcm
+    m, vals = TM.chain(4, tp=-0.3, U=3.0, mu=1.2)
+    lab = register(with_sites(m, 4), vals, "R0:N4:S0")
+    zs = np.array([0.05j, 0.5j, 2j, 1.0 + 0.1j, -2.0 + 0.3j])
+    G = qcm.Green_function(zs, 0, lab)
+    E0, Gref = O.dense_gf(m, vals, 2, 2, [0, 1, 2, 3], zs)
+    np.testing.assert_allclose(G, Gref, atol=1e-11)
+    e, Q = qcm.qmatrix(lab)
+    np.testing.assert_allclose(Q.T.conj() @ Q, np.eye(4), atol=1e-9)
+
+
+def test_green_function_band_lanczos_vs_oracle(gpu_lib):
+    """3x3 Hubbard: band Lanczos on the device vs the oracle's restatement (same algorithm)
+    on the imaginary axis, within 1e-8; and close to exact."""
+    from pyqcm_amd import qcm
+    m, vals = TM.square(3, 3, tp=-0.3, U=8.0, mu=4.0)
+    lab = register(with_sites(m, 9), vals, "R0:N9:S1")
+    zs = 1j * np.array([0.05, 0.2, 0.7, 2.0, 6.0])
+    G = qcm.Green_function(zs, 0, lab)
+    # oracle: ground state (N=9, S=1 -> 5 up, 4 down), band Lanczos on c_i|0>, c+_i|0>
+    So = O.Sector(m, vals, 5, 4)
+    E0, gs, *_ = So.lanczos_gs()
+    es, Qs = [], []
+    for pm in (-1, 1):
+        To = O.Sector(m, vals, 5 + pm, 4)
+        phi = [O.create_destroy(9, 5, 4, a, pm, gs) for a in range(9)]
+        e, Q = O.band_lanczos(To.hv, phi)
+        e = e - E0
+        if pm == -1:
+            e = -e
+        e, Q = O.streamline(e, Q)
+        es.append(e)
+        Qs.append(Q)
+    e = np.concatenate(es)
+    Q = np.hstack(Qs)
+    for i, z in enumerate(zs):
+        ref = O.qmatrix_gf(e, Q, z)
+        err = np.abs(G[i] - ref).max()
+        assert err <= 1e-8 * max(1.0, np.abs(ref).max()), (z, err, np.abs(ref).max())
+    Gd = qcm.Green_function(zs, 1, lab)  # S=1 sector -> up/down mixing, spin-down GF exists
+    assert Gd.shape == G.shape
+
+
+def test_self_energy_and_hybridization(gpu_lib):
+    from pyqcm_amd import qcm
+    p = {"U": 5.0, "mu": 2.5, **{"eb%d" % x: 0.5 * x - 1.3 for x in range(1, 5)},
+         **{"tb%d" % x: 0.3 + 0.1 * x for x in range(1, 5)}}
+    m, vals = TM.plaquette_bath(p, n_bath=4)
+    lab = register(with_sites(m, 4), vals, "R0:N8:S0")
+    zs = 1j * np.array([0.1, 1.0, 5.0])
+    G = qcm.Green_function(zs, 0, lab)
+    Gam = qcm.hybridization_functionC(zs, 0, lab)
+    S = qcm.self_energyC(zs, 0, lab)
+    tc = qcm.hopping_matrix(0, 0, lab)
+    # bath (diagonal here): Gamma_ab = sum_x tb_x s_a(x) s_b(x) / (z - eb_x)
+    seq = TM.PLAQ_SEQ
+    for i, z in enumerate(zs):
+        ref = np.zeros((4, 4), complex)
+        for x in range(1, 5):
+            s = np.array(seq[x], float) * p["tb%d" % x]
+            ref += np.outer(s, s) / (z - p["eb%d" % x])
+        np.testing.assert_allclose(Gam[i], ref, atol=1e-13)
+        np.testing.assert_allclose(S[i], z * np.eye(4) - tc - Gam[i] - np.linalg.inv(G[i]), atol=1e-10)
+    np.testing.assert_allclose(tc, -2.5 * np.eye(4), atol=1e-14)
+    # Potthoff functional (ED part): E0 + SEF_bath, SEF_bath = -2 sum_{eb<0} eb (mixing 0)
+    E0, _ = qcm.ground_state_solve(lab)
+    sef_bath = -2 * sum(v for k, v in p.items() if k.startswith("eb") and v < 0)
+    assert abs(qcm.Potthoff_functional(lab) - (E0 + sef_bath)) < 1e-10
+
+
+def test_continued_fraction_gf(gpu_lib):
+    """GF_method 'F' (scalar Lanczos per orbital pair) against brute force; the CF stop
+    criterion is tightened so that the fraction is exhaustive on this small cluster."""
+    from pyqcm_amd import qcm
+    m, vals = TM.chain(4, U=4.0, mu=2.0)
+    qcm.set_global_parameter("GF_method", "F")
+    qcm.set_global_parameter("accur_continued_fraction", 1e-12)
+    try:
+        lab = register(with_sites(m, 4), vals, "R0:N4:S0")
+        zs = 1j * np.array([0.1, 1.0, 3.0])
+        G = qcm.Green_function(zs, 0, lab)
+    finally:
+        qcm.set_global_parameter("GF_method", "L")
+        qcm.set_global_parameter("accur_continued_fraction", 0.01)
+    _, Gref = O.dense_gf(m, vals, 2, 2, [0, 1, 2, 3], zs)
+    np.testing.assert_allclose(G, Gref, atol=1e-9)
+
+
+def test_plaquette_poles_match_reference_irreps(gpu_lib, golden):
+    """S_format.ref stores the Lehmann poles per C2v irrep. Our trivial-group G, rotated to
+    the irrep basis, must reproduce each irrep's G(iw) from the recorded poles."""
+    from pyqcm_amd import qcm
+    g = golden["plaquette_8bath"]
+    m, vals = TM.plaquette_bath(g["params"])
+    lab = register(with_sites(m, 4), vals, "R0:N12:S0")
+    ws = np.array([0.1, 0.5, 1.0, 2.0, 5.0])
+    G = qcm.Green_function(1j * ws, 0, lab)
+    S = 0.5 * np.array([[1, 1, 1, 1], [1, -1, 1, -1], [1, 1, -1, -1], [1, -1, -1, 1]], float)
+    Girr = np.array([[S[r] @ G[i] @ S[r] for r in range(4)] for i in range(len(ws))])
+    for blk in g["blocks"][:4]:
+        e = np.array(blk["e"])
+        q = np.array(blk["Q"])[:, 0, 0]
+        Gref = np.array([np.sum(q * q / (1j * w - e)) for w in ws])
+        err = min(np.abs(Girr[:, r] - Gref).max() for r in range(4))
+        assert err < 2e-5, err
+
+
+# ------------------------------------------------------------------ G(z) and CF kernels
+
+def test_qmatrix_gf_kernel(gpu_lib):
+    from pyqcm_amd import _lib
+    rng = np.random.default_rng(0)
+    L, M, nz = 12, 700, 37
+    e = rng.uniform(-8, 8, M)
+    Q = rng.standard_normal((L, M)) / np.sqrt(M)
+    z = rng.uniform(-5, 5, nz) + 1j * rng.uniform(0.01, 2, nz)
+    zz = np.stack([z.real, z.imag], -1).ravel()
+    out = np.zeros(2 * nz * L * L)
+    _lib.call("qed_qmatrix_gf", L, M, _lib.dp(e), _lib.dp(np.asfortranarray(Q).ravel(order="F")), nz,
+              _lib.dp(zz), _lib.dp(out))
+    G = out.view(np.complex128).reshape(nz, L, L).transpose(0, 2, 1)
+    for i in range(nz):
+        np.testing.assert_allclose(G[i], O.qmatrix_gf(e, Q, z[i]), atol=1e-12)
+
+
+def test_cf_eval_kernel(gpu_lib):
+    from pyqcm_amd import _lib
+    rng = np.random.default_rng(1)
+    depths = [1, 5, 40, 400]
+    a = [rng.uniform(-3, 3, d) for d in depths]
+    b = [np.abs(rng.standard_normal(d)) for d in depths]
+    off = np.cumsum([0] + depths[:-1]).astype(np.int32)
+    z = np.array([0.3j, 1.0 + 0.05j, -2 + 1j])
+    zz = np.stack([z.real, z.imag], -1).ravel()
+    out = np.zeros(2 * len(depths) * len(z))
+    _lib.call("qed_cf_eval", len(depths), _lib.ip(off), _lib.ip(np.array(depths, np.int32)),
+              _lib.dp(np.concatenate(a)), _lib.dp(np.concatenate(b)), len(z), _lib.dp(zz), _lib.dp(out))
+    got = out.view(np.complex128).reshape(len(z), len(depths))
+    for iz in range(len(z)):
+        for f in range(len(depths)):
+            assert abs(got[iz, f] - O.cf_evaluate(a[f], b[f], z[iz])) < 1e-12
+
+
+# ------------------------------------------------------------------ lattice k-sums
+
+def _lattice_case(n, seed):
+    rng = np.random.default_rng(seed)
+    nw, nterm = 6, 3
+    Gc = []
+    for _ in range(nw):
+        A = rng.standard_normal((n, n))
+        A = A + A.T
+        w = rng.uniform(0.1, 2.0)
+        Gc.append(np.linalg.inv(1j * w * np.eye(n) - A))
+    Gc = np.array(Gc)
+    V0 = rng.standard_normal((n, n)) * 0.3
+    V0 = V0 + V0.T
+    Vt = (rng.standard_normal((nterm, n, n)) + 1j * rng.standard_normal((nterm, n, n))) * 0.2
+    nv = rng.integers(-1, 2, (nterm, 3)).astype(float)
+    k = rng.uniform(0, 1, (17, 3))
+    return Gc, V0.astype(complex), Vt, nv, k
+
+
+def _cm(a):
+    """(..., n, n) complex -> interleaved column-major doubles."""
+    a = np.ascontiguousarray(np.swapaxes(a, -1, -2))
+    return a.view(np.float64).ravel()
+
+
+def _vk(V0, Vt, nv, kk):
+    return V0 + sum(Vt[t] * np.exp(2j * np.pi * (kk @ nv[t])) for t in range(len(nv)))
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 6, 12, 16])
+def test_cpt_batch_modes(gpu_lib, n):
+    from pyqcm_amd import _lib
+    Gc, V0, Vt, nv, k = _lattice_case(n, n)
+    nw, nk = len(Gc), len(k)
+    args = (n, nw, _lib.dp(_cm(Gc)), _lib.dp(_cm(V0)), len(nv), _lib.dp(_cm(Vt)),
+            _lib.dp(np.ascontiguousarray(nv.ravel())), nk, _lib.dp(np.ascontiguousarray(k.ravel())))
+    full = np.zeros(2 * nw * nk * n * n)
+    _lib.call("qed_cpt_batch", *args, 0, _lib.dp(full))
+    full = full.view(np.complex128).reshape(nw, nk, n, n).transpose(0, 1, 3, 2)
+    tr = np.zeros(2 * nw * nk)
+    _lib.call("qed_cpt_batch", *args, 1, _lib.dp(tr))
+    tr = tr.view(np.complex128).reshape(nw, nk)
+    avg = np.zeros(2 * nw * n * n)
+    _lib.call("qed_cpt_batch", *args, 2, _lib.dp(avg))
+    avg = avg.view(np.complex128).reshape(nw, n, n).transpose(0, 2, 1)
+    sef = np.zeros(nw)
+    _lib.call("qed_sef_batch", *args, _lib.dp(sef))
+    for w in range(nw):
+        acc = np.zeros((n, n), complex)
+        s = 0.0
+        for q in range(nk):
+            V = _vk(V0, Vt, nv, k[q])
+            ref = O.cpt_green(Gc[w], V)
+            np.testing.assert_allclose(full[w, q], ref, atol=1e-10 * max(1, np.abs(ref).max()))
+            assert abs(tr[w, q] - np.trace(ref)) < 1e-9 * max(1, abs(np.trace(ref)))
+            acc += ref
+            s += O.sef_integrand(Gc[w], V, factor=1.0)
+        np.testing.assert_allclose(avg[w], acc / nk, atol=1e-10 * max(1, np.abs(acc).max()))
+        assert abs(sef[w] - s / nk) < 1e-9 * max(1, abs(s / nk))
+
+
+def test_cdmft_distance_batch(gpu_lib):
+    from pyqcm_amd import _lib
+    rng = np.random.default_rng(3)
+    n, nb, nw, B = 4, 8, 20, 5
+    w = np.sort(rng.uniform(0.05, 10, nw))
+    wt = rng.uniform(0.1, 1, nw)
+    Gh = rng.standard_normal((nw, n, n)) + 1j * rng.standard_normal((nw, n, n))
+    eb = rng.uniform(-3, 3, (B, nb))
+    tcb = rng.uniform(0.1, 1, (B, n, nb))
+    d = np.zeros(B)
+    scale = 2.0 * (w[1] - w[0]) / (n * n)
+    _lib.call("qed_cdmft_distance_batch", n, nb, nw, _lib.dp(w), _lib.dp(wt), _lib.dp(_cm(Gh)), B,
+              _lib.dp(np.ascontiguousarray(eb.ravel())),
+              _lib.dp(np.ascontiguousarray(np.swapaxes(tcb, 1, 2)).ravel()), scale, _lib.dp(d))
+    for b in range(B):
+        ref = O.cdmft_distance(w, wt, Gh, eb[b], tcb[b])
+        assert abs(d[b] - ref) < 1e-10 * abs(ref)
