@@ -1,0 +1,501 @@
+"""Cellular DMFT on the ED solver: the plain loop and its batched form.
+
+Plain loop (one instance), the semantics of pyqcm's CDMFT class:
+  * frequency grid and weights: ``frequency_grid`` (pyqcm/cdmft.py:821-899), default
+    Legendre (1, 10, 5, 10, 5); distance weights are all 1;
+  * one CDMFT step (pyqcm/cdmft.py:479-627): solve the impurity problem at the current
+    bath parameters x, build the host G_host (src_qcm/CPT.cpp:356-424), then minimize the
+    distance (src_qcm/CPT.cpp:519-555) over the bath parameters with adaptive Nelder-Mead
+    from an initial simplex of step ``initial_step`` (pyqcm/cdmft.py:1614-1633; xatol =
+    accur_bath, fatol = accur_dist);
+  * outer iteration: Broyden on F(x) = x - step(x) with xtol 1e-6 (pyqcm/__init__.py:2844-2898);
+  * convergence: ``convergence_manager`` on 'parameters' (||x - x_i|| / len(x) below
+    ``accur`` against the last ``depth`` iterates, pyqcm/cdmft.py:14-106).
+
+Batched loop: B independent CDMFT problems advance in lockstep. The ED solves of one
+outer iteration go to the device together, and the B Nelder-Mead minimizations run as
+coroutines whose trial points are evaluated in one device call per round
+(qed_cdmft_distance_batch per distinct host). Each coroutine follows scipy's adaptive
+Nelder-Mead exactly, so a batched run reproduces the plain run of each instance.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import lattice as LT
+
+
+class MinimizationError(RuntimeError):
+    pass
+
+
+class TooManyIterationsError(RuntimeError):
+    pass
+
+
+# ----------------------------------------------------------------- frequency grid
+
+class frequency_grid:
+    """Imaginary-frequency grid of the distance function (pyqcm/cdmft.py:821-899):
+    'legendre' with specs (w1, w2, n1, n2, n3), 'matsubara' (wc, beta), 'regular'
+    (wc, n1, n2). cdmft_weight is 1 on every frequency."""
+
+    def __init__(self, grid_type="legendre", specs=(1, 10, 5, 10, 5)):
+        if grid_type.lower() == "legendre":
+            w1, w2, n1, n2, n3 = specs
+            self.wr, self.weight = LT.legendre_frequency_grid(w1, w2, n1, n2, n3)
+            self.name = "legendre({:g}-{:g}-{:d}-{:d}-{:d})".format(w1, w2, n1, n2, n3)
+        elif grid_type.lower() == "matsubara":
+            wc, beta = specs
+            self.wr = np.arange(np.pi / beta, wc + 1e-6, 2 * np.pi / beta)
+            self.weight = (2 * np.pi / beta) * np.ones_like(self.wr)
+            self.name = "matsubara({:g}-{:g})".format(wc, beta)
+        elif grid_type == "regular":
+            wc, n1, n2 = specs
+            self.wr, self.weight = regular_frequency_grid(wc, n1, n2)
+            self.name = "regular({:g}-{:d}-{:d})".format(wc, n1, n2)
+        else:
+            raise ValueError("Unknown type of frequency grid " + grid_type)
+        self.nw = len(self.wr)
+        self.cdmft_weight = np.ones_like(self.wr)
+
+
+def regular_frequency_grid(wc, n1, n2):
+    """n1 evenly spaced points in (0, wc) and n2 Gauss-Legendre points on [wc, inf) in
+    u = 1/w (weights with 1/pi)."""
+    dw = wc / n1
+    w = np.concatenate([(np.arange(n1) + 0.5) * dw, np.zeros(n2)])
+    p = np.concatenate([np.full(n1, dw), np.zeros(n2)])
+    x, q = np.polynomial.legendre.leggauss(n2)
+    u = 0.5 * (1.0 / wc) * (x[::-1] + 1.0)
+    w[n1:] = 1.0 / u
+    p[n1:] = 0.5 * (1.0 / wc) * q[::-1] * w[n1:] ** 2
+    return w, p / np.pi
+
+
+# ------------------------------------------------------------ convergence, Broyden
+
+class convergence_manager:
+    """Convergence on a quantity compared with its `depth` predecessors
+    (pyqcm/cdmft.py:14-106)."""
+
+    def __init__(self, name, diff_func, tol, depth=2):
+        self.name, self.diff_func, self.tol, self.depth = name, diff_func, tol, depth
+        self.diff = np.zeros(depth)
+        self.iter = 0
+        self.X = []
+        self.converged = False
+
+    def test(self, x):
+        self.iter += 1
+        if self.iter <= self.depth:
+            self.X.append(x)
+            return False
+        for i in range(self.depth):
+            self.diff[i] = self.diff_func(x, self.X[i])
+        converged = bool(np.all(self.diff[: self.depth] <= self.tol))
+        for i in range(self.depth - 1):
+            self.X[i] = self.X[i + 1]
+        self.X[-1] = x
+        self.converged = converged
+        return converged
+
+
+def parameters_convergence(tol, depth=2):
+    return convergence_manager("parameters", lambda x, y: np.linalg.norm(x - y) / x.shape[0],
+                               tol, depth)
+
+
+def broyden(F, x0, iJ0=0.0, xtol=1e-6, convergence_test=None, maxiter=32, miniter=0):
+    """Broyden root search with the good inverse-Jacobian update and custom convergence
+    tests (pyqcm/__init__.py:2844-2898). Returns (x, iterations, inverse Jacobian)."""
+    n = len(x0)
+    I = (1.0 + iJ0) * np.eye(n) if np.isscalar(iJ0) else np.array(iJ0, float)
+    f0 = F(x0)
+    f = np.copy(f0)
+    x = np.array(x0, float)
+    it = 0
+    while True:
+        dx_vec = -I @ f
+        if np.any(np.isnan(dx_vec)):
+            raise ValueError("NaN produced in Broyden method")
+        x = x + dx_vec
+        if np.linalg.norm(dx_vec) < xtol and it >= miniter:
+            if convergence_test is not None:
+                convergence_test()
+            break
+        if convergence_test is not None and convergence_test() and it >= miniter:
+            break
+        f = F(x)
+        df = f - f0
+        f0 = np.copy(f)
+        T = np.outer(dx_vec - I @ df, dx_vec)
+        I = I + T @ I / (dx_vec @ (I @ df))
+        it += 1
+        if it > maxiter:
+            raise TooManyIterationsError(maxiter)
+    return x, it, I
+
+
+# ------------------------------------------------------------------- Nelder-Mead
+
+def nelder_mead_coroutine(x0, initial_step, xatol, fatol, maxfev):
+    """Adaptive Nelder-Mead as scipy.optimize.minimize(method='Nelder-Mead',
+    options={'adaptive': True, 'initial_simplex': x0 + step e_i, 'xatol', 'fatol',
+    'maxfev'}) performs it (the options used at pyqcm/cdmft.py:1614-1633).
+
+    A generator: it yields 2-D arrays of points to evaluate and receives their values;
+    it returns (x, nfev, success, fun). Driving it point by point reproduces scipy's
+    sequence of evaluations exactly."""
+    x0 = np.asarray(x0, float)
+    N = len(x0)
+    rho, chi, psi, sigma = 1.0, 1.0 + 2.0 / N, 0.75 - 1.0 / (2.0 * N), 1.0 - 1.0 / N
+    sim = np.repeat(x0[None], N + 1, 0)
+    for i in range(N):
+        sim[i + 1, i] += initial_step
+    nfev = 0
+    fsim = np.asarray((yield sim.copy()), float).copy()
+    nfev += N + 1
+    ind = np.argsort(fsim)
+    sim, fsim = sim[ind], fsim[ind]
+    while nfev < maxfev:
+        if (np.max(np.abs(sim[1:] - sim[0])) <= xatol and np.max(np.abs(fsim[0] - fsim[1:])) <= fatol):
+            break
+        xbar = np.add.reduce(sim[:-1], 0) / N
+        xr = (1 + rho) * xbar - rho * sim[-1]
+        fxr = float((yield xr[None])[0])
+        nfev += 1
+        doshrink = False
+        if fxr < fsim[0]:
+            xe = (1 + rho * chi) * xbar - rho * chi * sim[-1]
+            fxe = float((yield xe[None])[0])
+            nfev += 1
+            if fxe < fxr:
+                sim[-1], fsim[-1] = xe, fxe
+            else:
+                sim[-1], fsim[-1] = xr, fxr
+        elif fxr < fsim[-2]:
+            sim[-1], fsim[-1] = xr, fxr
+        else:
+            if fxr < fsim[-1]:
+                xc = (1 + psi * rho) * xbar - psi * rho * sim[-1]
+                fxc = float((yield xc[None])[0])
+                nfev += 1
+                if fxc <= fxr:
+                    sim[-1], fsim[-1] = xc, fxc
+                else:
+                    doshrink = True
+            else:
+                xcc = (1 - psi) * xbar + psi * sim[-1]
+                fxcc = float((yield xcc[None])[0])
+                nfev += 1
+                if fxcc < fsim[-1]:
+                    sim[-1], fsim[-1] = xcc, fxcc
+                else:
+                    doshrink = True
+            if doshrink:
+                pts = sim[0] + sigma * (sim[1:] - sim[0])
+                vals = np.asarray((yield pts), float)
+                nfev += N
+                sim[1:] = pts
+                fsim[1:] = vals
+        ind = np.argsort(fsim)
+        sim, fsim = sim[ind], fsim[ind]
+    return sim[0].copy(), nfev, nfev < maxfev, float(np.min(fsim))
+
+
+def run_coroutines(coros, evaluate):
+    """Drives optimizer coroutines in lockstep. evaluate(list of (index, points)) returns
+    the list of value arrays; returns the list of coroutine results."""
+    n = len(coros)
+    results = [None] * n
+    pending = {}
+    for i, c in enumerate(coros):
+        pending[i] = next(c)
+    while pending:
+        order = list(pending.keys())
+        vals = evaluate([(i, pending[i]) for i in order])
+        nxt = {}
+        for i, v in zip(order, vals):
+            try:
+                nxt[i] = coros[i].send(v)
+            except StopIteration as stop:
+                results[i] = stop.value
+        pending = nxt
+    return results
+
+
+# ------------------------------------------------------------------ the problem
+
+class CDMFTProblem:
+    """A cluster-plus-bath model on a lattice with a set of variational bath parameters.
+
+    `model` is a lattice.LatticeModel whose bath operators were added with
+    add_cluster_operator; `varia` names the bath parameters (optionally with the
+    cluster suffix '_1', as in pyqcm); `sectors` is the target-sector string."""
+
+    def __init__(self, model, varia, sectors, grid=None, kside=32):
+        self.model = model
+        self.varia = [v[:-2] if v.endswith("_1") else v for v in varia]
+        self.sectors = sectors
+        self.grid = grid or frequency_grid()
+        self.kside = kside
+
+    def solve(self, values, x, label=None):
+        vals = dict(values)
+        vals.update(zip(self.varia, np.asarray(x, float)))
+        inst = self.model.instance(vals, self.sectors, label)
+        inst.ground_state()
+        return inst
+
+    def host(self, inst):
+        return inst.cdmft_host(self.grid.wr, self.kside)
+
+    def distance(self, Ghost, X):
+        eb, tcb = self.model.diagonal_bath(self.varia, X)
+        return LT.cdmft_distance_batch(self.grid.wr, self.grid.cdmft_weight, Ghost, eb, tcb)
+
+    def distance_multi(self, hosts, requests):
+        """requests: list of (host index, points (m x nvar)); one device call per
+        distinct host, values returned in request order."""
+        out = []
+        for h, pts in requests:
+            out.append(self.distance(hosts[h], pts))
+        return out
+
+
+class CDMFT:
+    """Plain CDMFT loop for one instance (pyqcm/cdmft.py:194-478), Broyden iteration,
+    Nelder-Mead bath fit, convergence on 'parameters'."""
+
+    def __init__(self, problem, values, x0, maxiter=32, miniter=0, depth=2, accur=1e-4,
+                 accur_bath=1e-6, accur_dist=1e-8, initial_step=0.1,
+                 max_function_eval=1_000_000, iteration="broyden", label=None, verbose=False):
+        self.problem = problem
+        self.values = dict(values)
+        self.x = np.array(x0, float)
+        self.accur_bath, self.accur_dist = accur_bath, accur_dist
+        self.initial_step = initial_step
+        self.maxfev = max_function_eval
+        self.label = label
+        self.verbose = verbose
+        self.conv = parameters_convergence(accur, depth)
+        self.n_solves = 0
+        self.nfev = 0
+        self.dist = None
+        self.history = []
+        self.t_ed = self.t_fit = 0.0
+
+        def F(x):
+            self.x = np.copy(x)
+            self.step()
+            return x - self.x
+
+        def G():
+            return self.conv.test(np.copy(self.x))
+
+        if iteration == "broyden":
+            self.x, self.niter, _ = broyden(F, self.x, 0.0, xtol=1e-6, convergence_test=G,
+                                            maxiter=maxiter, miniter=miniter)
+        elif iteration == "fixed_point":
+            it = 0
+            while True:
+                F(self.x)
+                if G() and it >= miniter:
+                    break
+                it += 1
+                if it > maxiter:
+                    raise TooManyIterationsError(maxiter)
+            self.niter = it
+        else:
+            raise ValueError("iteration must be 'broyden' or 'fixed_point'")
+
+    def step(self):
+        t0 = time.perf_counter()
+        inst = self.problem.solve(self.values, self.x, self.label)
+        Gh = self.problem.host(inst)
+        self.n_solves += 1
+        t1 = time.perf_counter()
+        co = nelder_mead_coroutine(self.x, self.initial_step, self.accur_bath, self.accur_dist,
+                                   self.maxfev)
+        (xo, nfev, ok, fun), = run_coroutines([co], lambda reqs: [self.problem.distance(Gh, p)
+                                                                for _, p in reqs])
+        if not ok:
+            raise MinimizationError("Failure of the optimization of the bath parameters")
+        if np.any(np.isnan(xo)):
+            raise MinimizationError("NaN found in optimization of bath parameters")
+        self.nfev += nfev
+        self.dist = fun
+        self.x = xo
+        self.history.append((np.copy(xo), fun, inst.ground_state()[0]))
+        self.t_ed += t1 - t0
+        self.t_fit += time.perf_counter() - t1
+        if self.verbose:
+            print("CDMFT step %d: distance %.4g, %d evaluations" % (len(self.history), fun, nfev),
+                  flush=True)
+
+
+class BatchedCDMFT:
+    """B independent CDMFT problems in lockstep (same model, different parameter values
+    and starting points). Every instance follows the plain loop's semantics; per outer
+    iteration the solves of the active instances and their bath fits are batched.
+
+    `solver(values_list, x_list)` returns the host functions of the listed instances;
+    the default solves them one after the other through `problem`."""
+
+    def __init__(self, problem, values_list, x0_list, maxiter=32, depth=2, accur=1e-4,
+                 accur_bath=1e-6, accur_dist=1e-8, initial_step=0.1, max_function_eval=1_000_000,
+                 solver=None, verbose=False):
+        self.problem = problem
+        B = len(values_list)
+        self.values = [dict(v) for v in values_list]
+        self.x = [np.array(x, float) for x in x0_list]
+        self.accur_bath, self.accur_dist = accur_bath, accur_dist
+        self.initial_step, self.maxfev = initial_step, max_function_eval
+        self.maxiter = maxiter
+        self.solver = solver or self._solve_sequential
+        self.verbose = verbose
+        self.conv = [parameters_convergence(accur, depth) for _ in range(B)]
+        self.n_solves = np.zeros(B, int)
+        self.nfev = np.zeros(B, int)
+        self.dist = np.full(B, np.nan)
+        self.niter = np.zeros(B, int)
+        self.converged = np.zeros(B, bool)
+        self.failed = np.zeros(B, bool)
+        self.t_ed = self.t_fit = 0.0
+        self._run()
+
+    def _solve_sequential(self, values_list, x_list):
+        hosts = []
+        for v, x in zip(values_list, x_list):
+            inst = self.problem.solve(v, x)
+            hosts.append(self.problem.host(inst))
+        return hosts
+
+    def _steps(self, idx, xs):
+        """One CDMFT step for the instances idx at bath parameters xs: returns new xs."""
+        t0 = time.perf_counter()
+        hosts = self.solver([self.values[i] for i in idx], xs)
+        t1 = time.perf_counter()
+        coros = [nelder_mead_coroutine(x, self.initial_step, self.accur_bath, self.accur_dist,
+                                       self.maxfev) for x in xs]
+
+        def evaluate(reqs):
+            return self.problem.distance_multi(hosts, reqs)
+
+        res = run_coroutines(coros, evaluate)
+        self.t_ed += t1 - t0
+        self.t_fit += time.perf_counter() - t1
+        out = []
+        for j, i in enumerate(idx):
+            xo, nfev, ok, fun = res[j]
+            self.n_solves[i] += 1
+            self.nfev[i] += nfev
+            self.dist[i] = fun
+            if not ok or np.any(np.isnan(xo)):
+                self.failed[i] = True
+            out.append(xo)
+        return out
+
+    def _run(self):
+        """Broyden per instance, advanced in lockstep (pyqcm/__init__.py:2844-2898)."""
+        B = len(self.x)
+        n = len(self.x[0])
+        Ij = [np.eye(n) for _ in range(B)]
+        idx = list(range(B))
+        xs_new = self._steps(idx, [self.x[i] for i in idx])
+        f0 = [self.x[i] - xs_new[j] for j, i in enumerate(idx)]
+        f = [fi.copy() for fi in f0]
+        x = [self.x[i].copy() for i in idx]
+        it = np.zeros(B, int)
+        active = list(range(B))
+        while active:
+            need = []
+            for i in active:
+                if self.failed[i]:
+                    continue
+                dx = -Ij[i] @ f[i]
+                if np.any(np.isnan(dx)):
+                    self.failed[i] = True
+                    continue
+                x[i] = x[i] + dx
+                self.x[i] = x[i]
+                if np.linalg.norm(dx) < 1e-6:
+                    self.conv[i].test(np.copy(x[i]))
+                    self.converged[i] = True
+                    continue
+                if self.conv[i].test(np.copy(x[i])):
+                    self.converged[i] = True
+                    continue
+                need.append((i, dx))
+            if not need:
+                break
+            ids = [i for i, _ in need]
+            new = self._steps(ids, [x[i] for i in ids])
+            nxt = []
+            for (i, dx), xn in zip(need, new):
+                fi = x[i] - xn
+                df = fi - f0[i]
+                f0[i] = fi.copy()
+                f[i] = fi
+                T = np.outer(dx - Ij[i] @ df, dx)
+                Ij[i] = Ij[i] + T @ Ij[i] / (dx @ (Ij[i] @ df))
+                it[i] += 1
+                if it[i] > self.maxiter:
+                    self.failed[i] = True
+                    continue
+                nxt.append(i)
+            active = nxt
+            if self.verbose:
+                print("batched CDMFT: %d active, %d converged" % (len(active), self.converged.sum()),
+                      flush=True)
+        self.niter = it
+
+
+# ------------------------------------------------------------- 2x2 + 8 bath model
+
+PLAQUETTE_SEQ = [[1, 1, 1, 1], [1, 1, 1, 1], [1, 1, -1, -1], [1, 1, -1, -1],
+                 [1, -1, 1, -1], [1, -1, 1, -1], [1, -1, -1, 1], [1, -1, -1, 1]]
+
+
+def plaquette_8b_model(name="clus_2x2_8b"):
+    """2x2 plaquette on the square lattice (t, tp, mu, U) with 8 bath orbitals: eb_x
+    energies and tb_x site-bath hybridizations with the C2v sign patterns of the
+    reference's tests/test_diagonalization/cluster_2x2_2C_8b_C2v_L.py:8-33 (trivial
+    point group here, as the factorized solver requires)."""
+    T = LT.square_tiling(2, 2)
+    M = LT.LatticeModel(name, T, n_bath=8)
+    L = M.L
+    for x in range(1, 9):
+        b = 4 + x
+        M.add_cluster_operator("eb%d" % x, [(b, b, 1.0), (b + L, b + L, 1.0)])
+    for x in range(1, 9):
+        b = 4 + x
+        seq = PLAQUETTE_SEQ[x - 1]
+        el = [(i + 1, b, float(seq[i])) for i in range(4)] + \
+             [(i + 1 + L, b + L, float(seq[i])) for i in range(4)]
+        M.add_cluster_operator("tb%d" % x, el)
+    return M
+
+
+PLAQUETTE_VARIA = ["eb%d" % x for x in range(1, 9)] + ["tb%d" % x for x in range(1, 9)]
+
+
+def canonical_bath(x):
+    """Gauge-fixes 2x2+8-bath parameters: Gamma is invariant under tb_x -> -tb_x and
+    under exchanging the two bath orbitals of one sign pattern; take tb >= 0 and order
+    each pair by bath energy."""
+    x = np.array(x, float)
+    eb, tb = x[..., :8].copy(), np.abs(x[..., 8:])
+    for p in range(4):
+        a, b = 2 * p, 2 * p + 1
+        swap = eb[..., a] > eb[..., b]
+        ea, eb_ = eb[..., a].copy(), eb[..., b].copy()
+        ta, tb_ = tb[..., a].copy(), tb[..., b].copy()
+        eb[..., a] = np.where(swap, eb_, ea)
+        eb[..., b] = np.where(swap, ea, eb_)
+        tb[..., a] = np.where(swap, tb_, ta)
+        tb[..., b] = np.where(swap, ta, tb_)
+    return np.concatenate([eb, tb], -1)

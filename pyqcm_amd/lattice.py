@@ -1,0 +1,443 @@
+"""Lattice side of the quantum cluster methods on this path (CPT, VCA, CDMFT).
+
+A cluster of sites tiles a lattice through a superlattice; lattice operators (hoppings,
+on-site terms, density waves) decompose into an intra-cluster part and inter-cluster
+Fourier terms T_n, t(k) = sum_n T_n exp(2 pi i k.n), with k in the superdual basis and n
+the integer superlattice translation of the second site (src_qcm/CPT.cpp:22-64 set_V).
+
+For one cluster instance (ED model with bath, solved through ``qcm``):
+
+  V(k, w) = t(k) - t_c - Gamma_c(w)                        (src_qcm/CPT.cpp:57-61)
+  G_cpt   = G_c (1 - V G_c)^-1                             (src_qcm/CPT.cpp:76-86)
+
+The frequency-dependent Gamma_c is folded into an effective cluster function
+G_eff = (G_c^-1 + Gamma_c)^-1, so that the device kernels see a k-dependence only through
+V(k) = V0 + sum_n T_n e^{2 pi i k.n} with V0 = T_0(lattice) - t_c. Identities used:
+G_eff^-1 - (t(k) - t_c) = G_c^-1 - V(k, w), and
+log|det(1 - V G_c)| = log|det(1 - (V+Gamma) G_eff)| + log|det G_c| - log|det G_eff|.
+
+All k-sums run on the device (qed_cpt_batch, qed_sef_batch, qed_cdmft_distance_batch);
+this module only assembles the small per-frequency matrices.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib, qcm
+
+
+def _cm(a):
+    """complex (..., n, n) row-major array -> flat float64 buffer, column-major per matrix."""
+    a = np.ascontiguousarray(np.swapaxes(np.asarray(a, np.complex128), -1, -2))
+    return a.view(np.float64).ravel()
+
+
+def _from_cm(buf, shape):
+    n = shape[-1]
+    a = buf.view(np.complex128).reshape(*shape[:-2], n, n)
+    return np.swapaxes(a, -1, -2).copy()
+
+
+# ------------------------------------------------------------------------- tiling
+
+class Tiling:
+    """Cluster sites (integer positions) repeated by a superlattice of dimension 1-3.
+
+    Operators are registered as real one-body matrices per spin (spin up and down equal):
+      hop(name, link, amplitude)   amplitude on c+_i c_j and c+_j c_i, j = i + link
+                                   (pyqcm hopping_operator);
+      onsite(name, amplitude)      amplitude on every site (mu uses -1);
+      density_wave(name, Q)        cos(pi Q.r_i) on site i, both spins
+                                   (src_qcm/lattice_model.cpp:1149-1215, type 'N').
+    """
+
+    def __init__(self, sites, superlattice):
+        self.sites = np.array(sites, np.int64).reshape(-1, 3)
+        self.n = len(self.sites)
+        S = np.array(superlattice, np.float64).reshape(-1, 3)
+        self.dim = S.shape[0]
+        full = np.eye(3)
+        full[: self.dim] = S
+        # complete the basis with unit vectors independent of the superlattice
+        k = self.dim
+        for e in np.eye(3):
+            if k == 3:
+                break
+            trial = full.copy()
+            trial[k] = e
+            if abs(np.linalg.det(trial)) > 1e-9:
+                full = trial
+                k += 1
+        self.S = full
+        self.Sinv = np.linalg.inv(full.T)
+        self.terms = {}  # name -> {n (tuple of dim ints): n x n real matrix}
+        self.n_orb = self.n
+
+    def _zero(self):
+        return tuple([0] * self.dim)
+
+    def _locate(self, pos):
+        for j, r in enumerate(self.sites):
+            c = self.Sinv @ (np.asarray(pos, float) - r)
+            cr = np.round(c)
+            if np.allclose(c, cr, atol=1e-9) and np.all(cr[self.dim:] == 0):
+                return j, tuple(int(x) for x in cr[: self.dim])
+        return None
+
+    def _mat(self, name, nvec):
+        d = self.terms.setdefault(name, {})
+        if nvec not in d:
+            d[nvec] = np.zeros((self.n, self.n))
+        return d[nvec]
+
+    def hop(self, name, link, amplitude=-1.0):
+        link = np.array(list(link) + [0] * (3 - len(link)), np.int64)
+        for i, r in enumerate(self.sites):
+            loc = self._locate(r + link)
+            if loc is None:
+                raise ValueError("site %d + link %s is not a lattice site" % (i, link))
+            j, nvec = loc
+            neg = tuple(-x for x in nvec)
+            self._mat(name, nvec)[i, j] += amplitude
+            self._mat(name, neg)[j, i] += amplitude
+        return self
+
+    def onsite(self, name, amplitude=1.0):
+        M = self._mat(name, self._zero())
+        M[np.diag_indices(self.n)] += amplitude
+        return self
+
+    def density_wave(self, name, Q, amplitude=1.0, phase=0.0):
+        Q = np.array(list(Q) + [0] * (3 - len(Q)), float)
+        M = self._mat(name, self._zero())
+        for i, r in enumerate(self.sites):
+            M[i, i] += amplitude * np.cos(np.pi * (Q @ r) + np.pi * phase)
+        return self
+
+    # ---- decompositions
+
+    def cluster_matrix(self, values):
+        """Intra-cluster one-body matrix T_0 = sum_op value_op T_0^op."""
+        T0 = np.zeros((self.n, self.n))
+        for name, d in self.terms.items():
+            z = values.get(name, 0.0)
+            if z and self._zero() in d:
+                T0 += z * d[self._zero()]
+        return T0
+
+    def cluster_operator_elements(self, name):
+        """Intra-cluster part of operator `name` as 1-based (r, c, v), r <= c, over both
+        spins (site i -> label i+1 for spin up, i+1+n_orb for spin down), the form
+        qcm.new_operator takes."""
+        T0 = self.terms.get(name, {}).get(self._zero())
+        out = []
+        if T0 is None:
+            return out
+        for s in (0, self.n_orb):
+            for i in range(self.n):
+                for j in range(i, self.n):
+                    if T0[i, j] != 0.0:
+                        out.append((i + 1 + s, j + 1 + s, float(T0[i, j])))
+        return out
+
+    def fourier_terms(self, values):
+        """Inter-cluster terms: (nvecs (m x 3) float, mats (m x n x n) complex), n != 0."""
+        acc = {}
+        zero = self._zero()
+        for name, d in self.terms.items():
+            z = values.get(name, 0.0)
+            if not z:
+                continue
+            for nvec, T in d.items():
+                if nvec == zero:
+                    continue
+                acc[nvec] = acc.get(nvec, 0.0) + z * T
+        keys = [k for k in acc if np.any(acc[k] != 0.0)]
+        nv = np.array([list(k) + [0] * (3 - self.dim) for k in keys], float).reshape(-1, 3)
+        mats = np.array([acc[k] for k in keys], complex).reshape(-1, self.n, self.n)
+        return nv, mats
+
+    def tk(self, values, k):
+        """t(k) for wavevectors k (nk x 3), dense host evaluation."""
+        k = np.atleast_2d(np.asarray(k, float))
+        nv, mats = self.fourier_terms(values)
+        T0 = self.cluster_matrix(values).astype(complex)
+        if not len(nv):
+            return np.repeat(T0[None], len(k), 0)
+        ph = np.exp(2j * np.pi * (k @ nv.T))
+        return T0[None] + np.einsum("km,mij->kij", ph, mats)
+
+    def kgrid(self, side, shift=0.5):
+        """Regular grid in the superdual basis, k_a = (i_a + shift)/side (nk x 3)."""
+        axes = [(np.arange(side) + shift) / side for _ in range(self.dim)]
+        mesh = np.meshgrid(*axes, indexing="ij")
+        k = np.zeros((side ** self.dim, 3))
+        for a in range(self.dim):
+            k[:, a] = mesh[a].ravel()
+        return k
+
+
+def square_tiling(nx, ny):
+    """nx x ny cluster (site index x + nx*y) tiling the square lattice, with the
+    operators of the Hubbard model: t (NN), tp (NNN), mu."""
+    sites = [[x, y, 0] for y in range(ny) for x in range(nx)]
+    T = Tiling(sites, [[nx, 0, 0], [0, ny, 0]])
+    T.hop("t", (1, 0, 0)).hop("t", (0, 1, 0))
+    T.hop("tp", (1, 1, 0)).hop("tp", (-1, 1, 0))
+    T.onsite("mu", -1.0)
+    return T
+
+
+def chain_tiling(n):
+    """n-site cluster of a 1D chain: t (NN), tp (NNN), mu, cdw (Q = pi)."""
+    T = Tiling([[x, 0, 0] for x in range(n)], [[n, 0, 0]])
+    T.hop("t", (1, 0, 0)).hop("tp", (2, 0, 0))
+    T.onsite("mu", -1.0)
+    T.density_wave("cdw", (1, 0, 0))
+    return T
+
+
+# --------------------------------------------------------------- device k-sums
+
+def _lattice_call(fn, n, nw, Geff, V0, nv, mats, k, *tail):
+    k = np.ascontiguousarray(np.asarray(k, float).reshape(-1, 3))
+    nt = len(nv)
+    Vt = _cm(mats) if nt else np.zeros(2)
+    nvb = np.ascontiguousarray(nv, float).ravel() if nt else np.zeros(3)
+    _lib.call(fn, n, nw, _lib.dp(_cm(Geff)), _lib.dp(_cm(V0)), nt, _lib.dp(Vt), _lib.dp(nvb),
+              len(k), _lib.dp(k.ravel()), *tail)
+
+
+def cpt_batch(Geff, V0, nv, mats, k, mode="full"):
+    """G_cpt for every (w, k): Geff (nw x n x n), V(k) = V0 + sum_t mats[t] e^{2 pi i k.nv[t]}.
+    mode 'full' -> (nw, nk, n, n); 'trace' -> (nw, nk); 'average' -> (nw, n, n)."""
+    Geff = np.asarray(Geff, np.complex128)
+    nw, n = Geff.shape[0], Geff.shape[1]
+    nk = len(np.asarray(k).reshape(-1, 3))
+    m = {"full": 0, "trace": 1, "average": 2}[mode]
+    size = {0: nw * nk * n * n, 1: nw * nk, 2: nw * n * n}[m]
+    out = np.zeros(2 * size)
+    _lattice_call("qed_cpt_batch", n, nw, Geff, np.asarray(V0, complex), nv, mats, k, m,
+                  _lib.dp(out))
+    if m == 0:
+        return _from_cm(out, (nw, nk, n, n))
+    if m == 1:
+        return out.view(np.complex128).reshape(nw, nk).copy()
+    return _from_cm(out, (nw, n, n))
+
+
+def sef_batch(Geff, V0, nv, mats, k):
+    """(1/nk) sum_k log|det(V(k) Geff(w) - 1)| per frequency."""
+    Geff = np.asarray(Geff, np.complex128)
+    nw, n = Geff.shape[0], Geff.shape[1]
+    out = np.zeros(nw)
+    _lattice_call("qed_sef_batch", n, nw, Geff, np.asarray(V0, complex), nv, mats, k,
+                  _lib.dp(out))
+    return out
+
+
+def cdmft_distance_batch(wr, weights, Ghost, eb, tcb, mixing=0):
+    """Distance function for B bath parameter sets at once (src_qcm/CPT.cpp:519-555):
+    d = f dw / dim^2 sum_w weight(w) ||Gamma(i w) + G_host(i w)||_F^2, f = 2 at mixing 0,
+    dw = wr[1] - wr[0]. eb: (B, nb) bath energies, tcb: (B, n, nb) cluster-bath couplings
+    in the basis that diagonalizes the bath."""
+    wr = np.ascontiguousarray(wr, float)
+    weights = np.ascontiguousarray(weights, float)
+    eb = np.ascontiguousarray(np.atleast_2d(eb), float)
+    tcb = np.asarray(tcb, float)
+    if tcb.ndim == 2:
+        tcb = tcb[None]
+    B, n, nb = tcb.shape
+    if eb.shape != (B, nb) or np.shape(Ghost) != (len(wr), n, n):
+        raise ValueError("cdmft_distance_batch: inconsistent shapes")
+    scale = (2.0 if mixing == 0 else 1.0) * (wr[1] - wr[0]) / (n * n)
+    d = np.zeros(B)
+    tcm = np.ascontiguousarray(np.swapaxes(tcb, 1, 2)).ravel()  # (i, m) at i + n m
+    _lib.call("qed_cdmft_distance_batch", n, nb, len(wr), _lib.dp(wr), _lib.dp(weights),
+              _lib.dp(_cm(Ghost)), B, _lib.dp(eb.ravel()), _lib.dp(tcm), float(scale),
+              _lib.dp(d))
+    return d
+
+
+# -------------------------------------------------------------- frequency grids
+
+def legendre_frequency_grid(w1, w2, n1, n2, n3):
+    """Gauss-Legendre grid on the positive imaginary axis with n1, n2, n3 points in
+    [0, w1], [w1, w2] and [w2, inf) (the last interval mapped by u = 1/w); the weights
+    include the 1/pi of the frequency integral (pyqcm/__init__.py:3040-3066)."""
+    w = np.zeros(n1 + n2 + n3)
+    p = np.zeros(n1 + n2 + n3)
+    x, q = np.polynomial.legendre.leggauss(n1)
+    w[:n1] = 0.5 * w1 * (x + 1.0)
+    p[:n1] = 0.5 * w1 * q
+    x, q = np.polynomial.legendre.leggauss(n2)
+    w[n1:n1 + n2] = w1 + 0.5 * (w2 - w1) * (x + 1.0)
+    p[n1:n1 + n2] = 0.5 * (w2 - w1) * q
+    x, q = np.polynomial.legendre.leggauss(n3)
+    u = 0.5 * (1.0 / w2) * (x[::-1] + 1.0)
+    w[n1 + n2:] = 1.0 / u
+    p[n1 + n2:] = 0.5 * (1.0 / w2) * q[::-1] * w[n1 + n2:] ** 2
+    return w, p / np.pi
+
+
+# ---------------------------------------------------------------- cluster side
+
+class LatticeModel:
+    """A lattice model: a tiling plus the ED cluster model (cluster sites first, then
+    bath orbitals) registered through qcm. `lattice_ops` are the tiling operators whose
+    intra-cluster parts are also cluster operators of the same name."""
+
+    def __init__(self, name, tiling, n_bath=0, lattice_ops=("t", "tp", "mu", "cdw"),
+                 interaction="U"):
+        self.name = name
+        self.tiling = tiling
+        self.n = tiling.n
+        self.n_bath = n_bath
+        tiling.n_orb = self.n + n_bath
+        qcm.new_model(name, self.n, n_bath)
+        self.cluster_ops = []
+        for op in lattice_ops:
+            el = tiling.cluster_operator_elements(op)
+            if el:
+                qcm.new_operator(name, op, "one-body", el)
+                self.cluster_ops.append(op)
+        L = self.L
+        qcm.new_operator(name, interaction, "interaction",
+                         [(i + 1, i + 1 + L, 1.0) for i in range(self.n)])
+        self.cluster_ops.append(interaction)
+        self.bath_ops = {}  # name -> (B (nb x nb), C (n x nb)) spin-up linear blocks
+        self._next_label = 0
+
+    @property
+    def L(self):
+        return self.n + self.n_bath
+
+    def add_cluster_operator(self, name, elements):
+        """One-body cluster operator with 1-based (r, c, v) elements over 2L spin orbitals;
+        its bath-bath and cluster-bath spin-up blocks are recorded for the bath fit."""
+        qcm.new_operator(self.name, name, "one-body", elements)
+        B = np.zeros((self.n_bath, self.n_bath))
+        C = np.zeros((self.n, self.n_bath))
+        for r, c, v in elements:
+            r, c = r - 1, c - 1
+            if r >= self.L or c >= self.L:
+                continue
+            if r >= self.n and c >= self.n:
+                B[r - self.n, c - self.n] += v
+                if r != c:
+                    B[c - self.n, r - self.n] += v
+            elif r < self.n <= c:
+                C[r, c - self.n] += v
+        self.bath_ops[name] = (B, C)
+        self.cluster_ops.append(name)
+
+    def diagonal_bath(self, names, X):
+        """(eb (B x nb), tcb (B x n x nb)) for rows X (B x len(names)) of bath parameters;
+        a non-diagonal bath matrix is diagonalized as in the reference's hybridization
+        function (src_ed/model_instance.hpp:1004-1018)."""
+        X = np.atleast_2d(np.asarray(X, float))
+        E = np.zeros((len(X), self.n_bath, self.n_bath))
+        T = np.zeros((len(X), self.n, self.n_bath))
+        for i, nm in enumerate(names):
+            Bm, Cm = self.bath_ops[nm]
+            E += X[:, i, None, None] * Bm
+            T += X[:, i, None, None] * Cm
+        off = E - np.einsum("bii->bi", E)[..., None] * np.eye(self.n_bath)
+        if not np.any(off):
+            return np.einsum("bii->bi", E).copy(), T
+        eb, U = np.linalg.eigh(E)
+        return eb, T @ U
+
+    def instance(self, values, sectors, label=None):
+        """Solves nothing yet: registers the cluster instance. As in pyqcm, a name with the
+        suffix '_1' is a cluster parameter (e.g. a Weiss field 'cdw_1' or bath 'eb1_1');
+        other names are lattice parameters, inherited by the cluster operator of the same
+        name unless that has its own '_1' value."""
+        if label is None:
+            self._next_label += 1
+            label = 7000 + self._next_label
+        lattice_vals = {k: v for k, v in values.items() if not k.endswith("_1")}
+        cluster_vals = {k: v for k, v in lattice_vals.items() if k in self.cluster_ops}
+        for k, v in values.items():
+            if k.endswith("_1"):
+                if k[:-2] not in self.cluster_ops:
+                    raise ValueError("%s is not a cluster operator of %s" % (k[:-2], self.name))
+                cluster_vals[k[:-2]] = v
+        qcm.new_model_instanceC(self.name, cluster_vals, sectors, label)
+        return LatticeInstance(self, lattice_vals, label)
+
+
+class LatticeInstance:
+    """One cluster instance on the lattice (model_instance + lattice_model_instance)."""
+
+    def __init__(self, model, values, label):
+        self.model = model
+        self.values = values
+        self.label = label
+        self._gs = None
+
+    def ground_state(self):
+        if self._gs is None:
+            self._gs = qcm.ground_state_solve(self.label)
+        return self._gs
+
+    def green(self, z):
+        self.ground_state()
+        return qcm.Green_function(np.atleast_1d(z), 0, self.label)
+
+    def _geff(self, z):
+        G = self.green(z)
+        if self.model.n_bath == 0:
+            return G, G
+        Gam = qcm.hybridization_functionC(np.atleast_1d(z), 0, self.label)
+        return G, np.linalg.inv(np.linalg.inv(G) + Gam)
+
+    def v0(self):
+        """V0 = T_0(lattice) - t_c, the k-independent part of V."""
+        tc = qcm.hopping_matrix(0, 0, self.label).real
+        return self.model.tiling.cluster_matrix(self.values) - tc
+
+    def cpt_green(self, k, z, mode="full"):
+        """CPT Green function at frequencies z and wavevectors k (QCM::CPT_Green_function,
+        src_qcm/QCM.cpp:328-344)."""
+        _, Geff = self._geff(z)
+        nv, mats = self.model.tiling.fourier_terms(self.values)
+        return cpt_batch(Geff, self.v0(), nv, mats, k, mode)
+
+    def sef_integrand(self, z, k):
+        """(1/nk) sum_k log|det(V(k, z) G_c(z) - 1)| per frequency (one spin)."""
+        G, Geff = self._geff(z)
+        nv, mats = self.model.tiling.fourier_terms(self.values)
+        I = sef_batch(Geff, self.v0(), nv, mats, k)
+        if self.model.n_bath:
+            I = I + np.log(np.abs(np.linalg.det(G))) - np.log(np.abs(np.linalg.det(Geff)))
+        return I
+
+    def potthoff_functional(self, grid=(1, 10, 10, 10, 10), kside=64):
+        """Self-energy functional on a fixed (w, k) grid, the grid path of
+        lattice_model_instance::Potthoff_functional (src_qcm/lattice_model_instance.cpp:503-584):
+        Omega = [Omega_c - sum_w weight_w 2 (1/nk) sum_k log|det(V G - 1)|
+                 + sum_i (H_ii - Hc_ii)] / n_sites   (spin-symmetric case),
+        with the k-grid k = i/side (src_util/integrate.cpp:177-239)."""
+        ws, wt = legendre_frequency_grid(*grid)
+        k = self.model.tiling.kgrid(kside, shift=0.0)
+        Iw = 2.0 * self.sef_integrand(1j * ws, k)
+        omega_trace = -np.sum(Iw * wt)
+        omega_diag = -np.trace(self.v0())
+        self.ground_state()
+        omega_clus = qcm.Potthoff_functional(self.label)
+        return (omega_trace - omega_diag + omega_clus) / self.model.n
+
+    def projected_green(self, wr, kside=32):
+        """(1/nk) sum_k G_cpt(k, i w) on k = ((i+1/2)/side, ...) (src_qcm/CPT.cpp:283-345)."""
+        k = self.model.tiling.kgrid(kside, shift=0.5)
+        return self.cpt_green(k, 1j * np.asarray(wr, float), mode="average")
+
+    def cdmft_host(self, wr, kside=32):
+        """G_host(i w) = Gproj^-1 + Sigma_c + t_c - i w (src_qcm/CPT.cpp:356-424)."""
+        z = 1j * np.asarray(wr, float)
+        Gp = self.projected_green(wr, kside)
+        S = qcm.self_energyC(z, 0, self.label)
+        tc = qcm.hopping_matrix(0, 0, self.label)
+        n = Gp.shape[1]
+        return np.linalg.inv(Gp) + S + tc[None] - z[:, None, None] * np.eye(n)[None]

@@ -6,6 +6,8 @@ Sources:
   tests/test_diagonalization/S_format.ref     E0 and Lehmann poles of a 2x2 plaquette + 8 baths
   tests/test_diagonalization/test_pole_distribution.py   its full-precision parameters
   tests/test_diagonalization/S_format_complex.ref        E0 and poles of the 8-site chain
+  examples/1D_4/sef.tsv                       VCA self-energy functional of the 4-site chain
+  examples/1D_2_4b/cdmft.tsv                  converged CDMFT bath of the 2-site + 4-bath chain
 
 Run once in a container where the reference is mounted:
   python tests/golden/extract_reference_values.py /root/reference
@@ -31,6 +33,18 @@ def gs_tsv():
     for ln in lines[1:3]:
         row = dict(zip(head, ln.split("\t")))
         out.append({k: row[k] for k in head if k and k != "githash"})
+    return out
+
+
+def tsv_rows(rel, keys, n=None):
+    lines = [ln for ln in read(rel).splitlines() if ln.strip()]
+    head = lines[0].split("\t")
+    out = []
+    for ln in lines[1:] if n is None else lines[1:1 + n]:
+        if ln.startswith(head[0] + "\t"):
+            continue  # repeated header
+        row = dict(zip(head, ln.split("\t")))
+        out.append({k: row[k].strip() for k in keys if k in row})
     return out
 
 
@@ -102,6 +116,18 @@ def main():
             "source": "tests/test_diagonalization/S_format_complex.ref (model_1D_8.py, ti=1, U=1e-9)",
             "E0": E_c, "sector": sec_c, "blocks": parse_poles(cref),
         },
+    }
+    data["chain4_sef"] = {
+        "source": "examples/1D_4/sef.tsv (model examples/1D_4/model_1D_4.py; V, V0m, V1m = 1e-9)",
+        "rows": tsv_rows("examples/1D_4/sef.tsv", ["omega", "U", "mu", "t", "cdw", "cdw_1", "E0_1",
+                                                    "sector_1"]),
+    }
+    data["chain2_4b_cdmft"] = {
+        "source": "examples/1D_2_4b/cdmft.tsv (model examples/1D_2_4b/model_1D_2_4b.py, "
+                  "start values examples/1D_2_4b/QCM_params.def)",
+        "rows": tsv_rows("examples/1D_2_4b/cdmft.tsv",
+                         ["iterations", "dist_function", "distance", "U", "mu", "t", "eb1_1", "eb2_1",
+                          "tb1_1", "tb2_1", "E0_1", "sector_1", "ave_mu", "version"]),
     }
     with open(OUT, "w") as f:
         json.dump(data, f, indent=1)

@@ -1,0 +1,133 @@
+"""Host-side logic of the lattice/CDMFT layer (no GPU): the Nelder-Mead coroutine against
+scipy, Broyden, frequency grids, the tiling's Fourier decomposition, bath gauge fixing."""
+import numpy as np
+import pytest
+import scipy.optimize
+
+from pyqcm_amd import cdmft as C
+from pyqcm_amd import lattice as LT
+
+
+def _rosen_shifted(x):
+    return scipy.optimize.rosen(x) + 0.1 * np.sum((x - 0.3) ** 2)
+
+
+@pytest.mark.parametrize("n", [2, 5, 16])
+def test_nelder_mead_coroutine_reproduces_scipy(n):
+    """Same options as the reference's bath fit: adaptive, initial simplex x0 + 0.1 e_i."""
+    rng = np.random.default_rng(n)
+    x0 = rng.uniform(-1, 1, n)
+    step, xatol, fatol = 0.1, 1e-6, 1e-8
+    sim = np.repeat(x0[None], n + 1, 0)
+    for i in range(n):
+        sim[i + 1, i] += step
+    calls = []
+
+    def f(x):
+        calls.append(np.copy(x))
+        return _rosen_shifted(x)
+
+    sol = scipy.optimize.minimize(f, x0, method="Nelder-Mead",
+                                  options={"maxfev": 200000, "xatol": xatol, "fatol": fatol,
+                                           "initial_simplex": sim, "adaptive": True, "disp": False})
+    co = C.nelder_mead_coroutine(x0, step, xatol, fatol, 200000)
+    mine = []
+
+    def ev(reqs):
+        out = []
+        for _, pts in reqs:
+            mine.extend(np.copy(p) for p in pts)
+            out.append(np.array([_rosen_shifted(p) for p in pts]))
+        return out
+
+    (x, nfev, ok, fun), = C.run_coroutines([co], ev)
+    assert ok and sol.success
+    assert nfev == sol.nfev == len(calls) == len(mine)
+    np.testing.assert_array_equal(np.array(mine), np.array(calls))
+    np.testing.assert_array_equal(x, sol.x)
+    assert fun == sol.fun
+
+
+def test_run_coroutines_interleaves_independent_problems():
+    rng = np.random.default_rng(3)
+    x0s = [rng.uniform(-1, 1, 4) for _ in range(5)]
+    singles = []
+    for x0 in x0s:
+        co = C.nelder_mead_coroutine(x0, 0.1, 1e-7, 1e-10, 100000)
+        singles.append(C.run_coroutines([co], lambda r: [np.array([_rosen_shifted(p) for p in pts])
+                                                         for _, pts in r])[0])
+    cos = [C.nelder_mead_coroutine(x0, 0.1, 1e-7, 1e-10, 100000) for x0 in x0s]
+    batch = C.run_coroutines(cos, lambda r: [np.array([_rosen_shifted(p) for p in pts])
+                                             for _, pts in r])
+    for a, b in zip(singles, batch):
+        np.testing.assert_array_equal(a[0], b[0])
+        assert a[1] == b[1]
+
+
+def test_broyden_solves_nonlinear_system():
+    A = np.array([[3.0, 0.5, 0.0], [0.2, 2.5, 0.3], [0.0, 0.4, 2.0]])
+    b = np.array([1.0, -0.5, 0.25])
+
+    def step(x):  # fixed-point map x -> x - 0.3 (A x + 0.1 x^3 - b)
+        return x - 0.3 * (A @ x + 0.1 * x ** 3 - b)
+
+    F = lambda x: x - step(x)
+    x, it, _ = C.broyden(F, np.zeros(3), 0.0, xtol=1e-12, maxiter=50)
+    np.testing.assert_allclose(A @ x + 0.1 * x ** 3, b, atol=1e-10)
+
+
+def test_convergence_manager_depth():
+    cm = C.parameters_convergence(1e-3, depth=2)
+    xs = [np.array([1.0, 2.0]), np.array([1.1, 2.0]), np.array([1.1001, 2.0]),
+          np.array([1.1002, 2.0]), np.array([1.1002, 2.0])]
+    res = [cm.test(x) for x in xs]
+    assert res == [False, False, False, True, True]
+
+
+def test_legendre_grid_integrates_rational_functions():
+    """sum_i p_i f(w_i) = (1/pi) int_0^inf f: 1/(1+w^2) -> 1/2, 1/(a^2+w^2) -> 1/(2a)."""
+    w, p = LT.legendre_frequency_grid(1, 10, 10, 10, 10)
+    assert abs(np.sum(p / (1 + w ** 2)) - 0.5) < 1e-6
+    w, p = LT.legendre_frequency_grid(1, 10, 30, 30, 30)
+    assert abs(np.sum(p / (1 + w ** 2)) - 0.5) < 1e-12
+    assert abs(np.sum(p / (0.3 ** 2 + w ** 2)) - 1 / 0.6) < 1e-9
+    w, p = LT.legendre_frequency_grid(1, 10, 5, 10, 5)
+    assert len(w) == 20 and np.all(np.diff(w) > 0)
+    g = C.frequency_grid()
+    assert g.name == "legendre(1-10-5-10-5)" and np.all(g.cdmft_weight == 1)
+
+
+def test_square_tiling_reproduces_dispersion():
+    """Eigenvalues of t(K) for the 2x2 tiling = -2t(cos kx + cos ky) - 4t' cos kx cos ky
+    at the 4 lattice momenta folded onto K."""
+    T = LT.square_tiling(2, 2)
+    vals = {"t": 1.0, "tp": -0.3, "mu": 0.0}
+    rng = np.random.default_rng(0)
+    for K in rng.uniform(0, 1, (5, 2)):
+        ev = np.linalg.eigvalsh(T.tk(vals, [[K[0], K[1], 0]])[0])
+        ks = []
+        for a in range(2):
+            for b in range(2):
+                kx = np.pi * (K[0] + a)  # k = 2 pi (K + G)/2
+                ky = np.pi * (K[1] + b)
+                ks.append(-2 * (np.cos(kx) + np.cos(ky)) + 4 * 0.3 * np.cos(kx) * np.cos(ky))
+        np.testing.assert_allclose(ev, np.sort(ks), atol=1e-12)
+
+
+def test_chain_tiling_cluster_operator_elements():
+    T = LT.chain_tiling(4)
+    T.n_orb = 4
+    el = T.cluster_operator_elements("t")
+    assert (1, 2, -1.0) in el and (5, 6, -1.0) in el and len(el) == 6
+    cdw = T.cluster_operator_elements("cdw")
+    assert [v for (_, _, v) in cdw[:4]] == [1.0, -1.0, 1.0, -1.0]
+
+
+def test_canonical_bath_is_gauge_invariant():
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.uniform(-3, 3, 8), rng.uniform(0.1, 1, 8)])
+    y = x.copy()
+    y[8:] *= np.array([1, -1, 1, 1, -1, 1, -1, 1])
+    y[[0, 1]] = y[[1, 0]]
+    y[[8, 9]] = y[[9, 8]]
+    np.testing.assert_array_equal(C.canonical_bath(x), C.canonical_bath(y))
