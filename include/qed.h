@@ -154,6 +154,26 @@ int qed_cdmft_distance_batch(int n, int nb, int nw, const double* w, const doubl
                              const double* Ghost, int B, const double* eb, const double* tcb,
                              double scale, double* d);
 
+/* Same distance with one host function per parameter set: Ghost holds nhost hosts
+ * (nhost x nw x n x n complex, col-major per matrix), set b uses host host_of[b].
+ * Identical arithmetic to qed_cdmft_distance_batch. */
+int qed_cdmft_distance_multi(int n, int nb, int nw, const double* w, const double* weight,
+                             int nhost, const double* Ghost, int B, const int* host_of,
+                             const double* eb, const double* tcb, double scale, double* d);
+
+/* Batched bath fit: B adaptive Nelder-Mead minimizations of the distance in one launch
+ * (one wavefront per instance). Bath parameters x (nvar) map linearly onto a diagonal
+ * bath: eb[m] = sum_p x[p] E[p nb + m], tcb[i + n m] = sum_p x[p] T[p n nb + i + n m].
+ * Initial simplex x0 + initial_step e_i; stops when the simplex spread is <= xatol and
+ * the value spread <= fatol (status 0) or after maxfev evaluations (status 1).
+ * Replaces the Nelder-Mead minimization of CDMFT.CDMFT_step (pyqcm/cdmft.py:543-582 with
+ * the options of pyqcm/cdmft.py:1614-1633), evaluating CDMFT_distance on the device. */
+int qed_cdmft_fit_batch(int n, int nb, int nw, const double* w, const double* weight, int nhost,
+                        const double* Ghost, int B, const int* host_of, int nvar, const double* E,
+                        const double* T, const double* x0, double initial_step, double xatol,
+                        double fatol, int maxfev, double scale, double* x, double* f, int* nfev,
+                        int* status);
+
 /* ---- the ED solver interface (namespace ED, src_ed/qcm_ED.hpp:23-340) -------------- */
 
 /* ED::new_model (src_ed/qcm_ED.cpp:39-58); trivial symmetry group only. */

@@ -53,6 +53,11 @@ SIGNATURES = [
     ("qed_sef_batch", c_int, [c_int, c_int, dptr, dptr, c_int, dptr, dptr, c_int, dptr, dptr]),
     ("qed_cdmft_distance_batch", c_int, [c_int, c_int, c_int, dptr, dptr, dptr, c_int, dptr, dptr,
                                          c_double, dptr]),
+    ("qed_cdmft_distance_multi", c_int, [c_int, c_int, c_int, dptr, dptr, c_int, dptr, c_int, iptr,
+                                         dptr, dptr, c_double, dptr]),
+    ("qed_cdmft_fit_batch", c_int, [c_int, c_int, c_int, dptr, dptr, c_int, dptr, c_int, iptr, c_int,
+                                    dptr, dptr, dptr, c_double, c_double, c_double, c_int, c_double,
+                                    dptr, dptr, iptr, iptr]),
     ("qed_ed_new_model", c_int, [c_char_p, c_int, c_int]),
     ("qed_ed_new_operator", c_int, [c_char_p, c_char_p, c_char_p, c_int, iptr, iptr, dptr]),
     ("qed_ed_new_model_instance", c_int, [c_char_p, c_int, P(c_char_p), dptr, c_char_p, c_u64]),

@@ -12,11 +12,14 @@ Plain loop (one instance), the semantics of pyqcm's CDMFT class:
   * convergence: ``convergence_manager`` on 'parameters' (||x - x_i|| / len(x) below
     ``accur`` against the last ``depth`` iterates, pyqcm/cdmft.py:14-106).
 
-Batched loop: B independent CDMFT problems advance in lockstep. The ED solves of one
-outer iteration go to the device together, and the B Nelder-Mead minimizations run as
-coroutines whose trial points are evaluated in one device call per round
-(qed_cdmft_distance_batch per distinct host). Each coroutine follows scipy's adaptive
-Nelder-Mead exactly, so a batched run reproduces the plain run of each instance.
+Bath fit: for a diagonal bath whose parameters act linearly (the models here), all the
+fits of an outer iteration run in one device launch (qed_cdmft_fit_batch, one wavefront
+per instance). The same algorithm also exists as a host coroutine whose trial points are
+evaluated on the device (qed_cdmft_distance_multi); it reproduces scipy's Nelder-Mead
+call for call and is the form used for general baths.
+
+Batched loop: B independent CDMFT problems advance in lockstep; the ED solves of one
+outer iteration and their bath fits go to the device together.
 """
 from __future__ import annotations
 
@@ -141,24 +144,28 @@ def broyden(F, x0, iJ0=0.0, xtol=1e-6, convergence_test=None, maxiter=32, minite
 
 # ------------------------------------------------------------------- Nelder-Mead
 
-def nelder_mead_coroutine(x0, initial_step, xatol, fatol, maxfev):
+def nelder_mead_coroutine(x0, initial_step, xatol, fatol, maxfev, stable_sort=False):
     """Adaptive Nelder-Mead as scipy.optimize.minimize(method='Nelder-Mead',
     options={'adaptive': True, 'initial_simplex': x0 + step e_i, 'xatol', 'fatol',
     'maxfev'}) performs it (the options used at pyqcm/cdmft.py:1614-1633).
 
     A generator: it yields 2-D arrays of points to evaluate and receives their values;
-    it returns (x, nfev, success, fun). Driving it point by point reproduces scipy's
-    sequence of evaluations exactly."""
+    it returns (x, nfev, success, fun). With the default sort it reproduces scipy's
+    sequence of evaluations exactly. scipy orders the simplex with numpy's default
+    argsort, whose order for exactly equal values depends on the sort implementation;
+    stable_sort=True keeps equal values in their previous order, the rule the device
+    fit (qed_cdmft_fit_batch) applies."""
     x0 = np.asarray(x0, float)
     N = len(x0)
     rho, chi, psi, sigma = 1.0, 1.0 + 2.0 / N, 0.75 - 1.0 / (2.0 * N), 1.0 - 1.0 / N
+    kind = "stable" if stable_sort else None
     sim = np.repeat(x0[None], N + 1, 0)
     for i in range(N):
         sim[i + 1, i] += initial_step
     nfev = 0
     fsim = np.asarray((yield sim.copy()), float).copy()
     nfev += N + 1
-    ind = np.argsort(fsim)
+    ind = np.argsort(fsim, kind=kind)
     sim, fsim = sim[ind], fsim[ind]
     while nfev < maxfev:
         if (np.max(np.abs(sim[1:] - sim[0])) <= xatol and np.max(np.abs(fsim[0] - fsim[1:])) <= fatol):
@@ -201,7 +208,7 @@ def nelder_mead_coroutine(x0, initial_step, xatol, fatol, maxfev):
                 nfev += N
                 sim[1:] = pts
                 fsim[1:] = vals
-        ind = np.argsort(fsim)
+        ind = np.argsort(fsim, kind=kind)
         sim, fsim = sim[ind], fsim[ind]
     return sim[0].copy(), nfev, nfev < maxfev, float(np.min(fsim))
 
@@ -258,12 +265,53 @@ class CDMFTProblem:
         return LT.cdmft_distance_batch(self.grid.wr, self.grid.cdmft_weight, Ghost, eb, tcb)
 
     def distance_multi(self, hosts, requests):
-        """requests: list of (host index, points (m x nvar)); one device call per
-        distinct host, values returned in request order."""
-        out = []
-        for h, pts in requests:
-            out.append(self.distance(hosts[h], pts))
+        """requests: list of (host index, points (m x nvar)); all points are evaluated in
+        one device call (qed_cdmft_distance_multi), values returned per request."""
+        pts = np.concatenate([np.atleast_2d(p) for _, p in requests])
+        host_of = np.concatenate([np.full(len(np.atleast_2d(p)), h, np.int32) for h, p in requests])
+        eb, tcb = self.model.diagonal_bath(self.varia, pts)
+        d = LT.cdmft_distance_multi(self.grid.wr, self.grid.cdmft_weight, np.asarray(hosts),
+                                    host_of, eb, tcb)
+        out, o = [], 0
+        for _, p in requests:
+            m = len(np.atleast_2d(p))
+            out.append(d[o:o + m])
+            o += m
         return out
+
+    def linear_bath_map(self):
+        """(E (nvar x nb), T (nvar x n x nb)) when every variational parameter acts on a
+        diagonal bath (eb and tcb linear in x); None otherwise."""
+        nb, n = self.model.n_bath, self.model.n
+        E = np.zeros((len(self.varia), nb))
+        T = np.zeros((len(self.varia), n, nb))
+        for p, nm in enumerate(self.varia):
+            Bm, Cm = self.model.bath_ops[nm]
+            if np.any(Bm - np.diag(np.diag(Bm))):
+                return None
+            E[p] = np.diag(Bm)
+            T[p] = Cm
+        return E, T
+
+    def fit(self, hosts, host_of, X0, initial_step=0.1, xatol=1e-6, fatol=1e-8, maxfev=1_000_000,
+            where="device"):
+        """Nelder-Mead bath fits of len(X0) instances; instance b fits to hosts[host_of[b]].
+        'device': one launch of qed_cdmft_fit_batch; 'host': the coroutine form of the same
+        algorithm, its trial points evaluated on the device. Returns (x, f, nfev, ok)."""
+        X0 = np.atleast_2d(np.asarray(X0, float))
+        hosts = np.asarray(hosts)
+        host_of = np.ascontiguousarray(host_of, np.int32)
+        lin = self.linear_bath_map() if where == "device" else None
+        if lin is not None:
+            return LT.cdmft_fit_batch(self.grid.wr, self.grid.cdmft_weight, hosts, host_of, lin[0],
+                                      lin[1], X0, initial_step, xatol, fatol, maxfev)
+        coros = [nelder_mead_coroutine(x, initial_step, xatol, fatol, maxfev, stable_sort=True)
+                 for x in X0]
+        res = run_coroutines(coros, lambda reqs: self.distance_multi(
+            hosts, [(host_of[i], p) for i, p in reqs]))
+        x = np.array([r[0] for r in res])
+        return x, np.array([r[3] for r in res]), np.array([r[1] for r in res]), \
+            np.array([r[2] for r in res])
 
 
 class CDMFT:
@@ -272,8 +320,10 @@ class CDMFT:
 
     def __init__(self, problem, values, x0, maxiter=32, miniter=0, depth=2, accur=1e-4,
                  accur_bath=1e-6, accur_dist=1e-8, initial_step=0.1,
-                 max_function_eval=1_000_000, iteration="broyden", label=None, verbose=False):
+                 max_function_eval=1_000_000, iteration="broyden", label=None, verbose=False,
+                 fit="device"):
         self.problem = problem
+        self.fit_where = fit
         self.values = dict(values)
         self.x = np.array(x0, float)
         self.accur_bath, self.accur_dist = accur_bath, accur_dist
@@ -318,10 +368,9 @@ class CDMFT:
         Gh = self.problem.host(inst)
         self.n_solves += 1
         t1 = time.perf_counter()
-        co = nelder_mead_coroutine(self.x, self.initial_step, self.accur_bath, self.accur_dist,
-                                   self.maxfev)
-        (xo, nfev, ok, fun), = run_coroutines([co], lambda reqs: [self.problem.distance(Gh, p)
-                                                                for _, p in reqs])
+        X, F, NF, OK = self.problem.fit(Gh[None], [0], self.x[None], self.initial_step,
+                                        self.accur_bath, self.accur_dist, self.maxfev, self.fit_where)
+        xo, fun, nfev, ok = X[0], float(F[0]), int(NF[0]), bool(OK[0])
         if not ok:
             raise MinimizationError("Failure of the optimization of the bath parameters")
         if np.any(np.isnan(xo)):
@@ -347,8 +396,9 @@ class BatchedCDMFT:
 
     def __init__(self, problem, values_list, x0_list, maxiter=32, depth=2, accur=1e-4,
                  accur_bath=1e-6, accur_dist=1e-8, initial_step=0.1, max_function_eval=1_000_000,
-                 solver=None, verbose=False):
+                 solver=None, verbose=False, fit="device"):
         self.problem = problem
+        self.fit_where = fit
         B = len(values_list)
         self.values = [dict(v) for v in values_list]
         self.x = [np.array(x, float) for x in x0_list]
@@ -379,18 +429,14 @@ class BatchedCDMFT:
         t0 = time.perf_counter()
         hosts = self.solver([self.values[i] for i in idx], xs)
         t1 = time.perf_counter()
-        coros = [nelder_mead_coroutine(x, self.initial_step, self.accur_bath, self.accur_dist,
-                                       self.maxfev) for x in xs]
-
-        def evaluate(reqs):
-            return self.problem.distance_multi(hosts, reqs)
-
-        res = run_coroutines(coros, evaluate)
+        X, F, NF, OK = self.problem.fit(np.asarray(hosts), np.arange(len(idx)), np.array(xs),
+                                        self.initial_step, self.accur_bath, self.accur_dist,
+                                        self.maxfev, self.fit_where)
         self.t_ed += t1 - t0
         self.t_fit += time.perf_counter() - t1
         out = []
         for j, i in enumerate(idx):
-            xo, nfev, ok, fun = res[j]
+            xo, fun, nfev, ok = X[j], float(F[j]), int(NF[j]), bool(OK[j])
             self.n_solves[i] += 1
             self.nfev[i] += nfev
             self.dist[i] = fun

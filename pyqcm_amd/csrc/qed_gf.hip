@@ -277,42 +277,6 @@ __global__ void k_ksum_reduce(int nw, int nparts, int len, const double* __restr
   out[id] = s * scale;
 }
 
-// CDMFT distance for B parameter sets: one thread per (set, frequency), then per-set sum
-__global__ void k_cdmft_dist(int n, int nb, int nw, const double* __restrict__ w,
-                             const double* __restrict__ weight, const double* __restrict__ Ghost,
-                             int B, const double* __restrict__ eb, const double* __restrict__ tcb,
-                             double* __restrict__ part) {
-  int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= B * nw) return;
-  int bset = id / nw, iw = id % nw;
-  const double* e = eb + (size_t)bset * nb;
-  const double* t = tcb + (size_t)bset * n * nb;
-  double sum = 0.0;
-  for (int a = 0; a < n; ++a)
-    for (int c = 0; c < n; ++c) {
-      double gr = Ghost[(((size_t)iw * n * n) + a + (size_t)c * n) * 2];
-      double gi = Ghost[(((size_t)iw * n * n) + a + (size_t)c * n) * 2 + 1];
-      for (int m = 0; m < nb; ++m) {
-        // 1/(i w - e) = (-e - i w)/(e^2 + w^2)
-        double d = e[m] * e[m] + w[iw] * w[iw];
-        double tt = t[a + (size_t)m * n] * t[c + (size_t)m * n];
-        gr += tt * (-e[m] / d);
-        gi += tt * (-w[iw] / d);
-      }
-      sum += gr * gr + gi * gi;
-    }
-  part[id] = sum * weight[iw];
-}
-
-__global__ void k_cdmft_sum(int B, int nw, const double* __restrict__ part, double scale,
-                            double* __restrict__ d) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  double s = 0.0;
-  for (int i = 0; i < nw; ++i) s += part[(size_t)b * nw + i];
-  d[b] = s * scale;
-}
-
 template <class T>
 DevBuf up(const T* p, size_t n) {
   DevBuf b(std::max<size_t>(n, 1) * sizeof(T));
@@ -528,24 +492,6 @@ int qed_sef_batch(int n, int nw, const double* Gc, const double* V0, int nterm, 
     QED_HIP(hipStreamSynchronize(stream()));
     // log|det(V G - 1)| = log|det G| + log|det(G^-1 - V)|  (padding blocks contribute log 1 = 0)
     for (int w = 0; w < nw; ++w) out[w] += logdetG[w];
-  });
-}
-
-int qed_cdmft_distance_batch(int n, int nb, int nw, const double* w, const double* weight,
-                             const double* Ghost, int B, const double* eb, const double* tcb,
-                             double scale, double* d) {
-  return guard([&] {
-    if (B <= 0) return;
-    DevBuf dw = up(w, nw), dwt = up(weight, nw), dg = up(Ghost, (size_t)nw * n * n * 2);
-    DevBuf de = up(eb, (size_t)B * nb), dt = up(tcb, (size_t)B * n * nb);
-    DevBuf dpart((size_t)B * nw * 8), dd((size_t)B * 8);
-    k_cdmft_dist<<<(B * nw + 255) / 256, 256, 0, stream()>>>(n, nb, nw, dw.as<double>(), dwt.as<double>(),
-                                                             dg.as<double>(), B, de.as<double>(),
-                                                             dt.as<double>(), dpart.as<double>());
-    k_cdmft_sum<<<(B + 255) / 256, 256, 0, stream()>>>(B, nw, dpart.as<double>(), scale, dd.as<double>());
-    QED_HIP(hipGetLastError());
-    QED_HIP(hipMemcpyAsync(d, dd.p, (size_t)B * 8, hipMemcpyDeviceToHost, stream()));
-    QED_HIP(hipStreamSynchronize(stream()));
   });
 }
 

@@ -259,6 +259,56 @@ def cdmft_distance_batch(wr, weights, Ghost, eb, tcb, mixing=0):
     return d
 
 
+def cdmft_distance_multi(wr, weights, hosts, host_of, eb, tcb, mixing=0):
+    """Distance for B parameter sets, set b against hosts[host_of[b]] (nhost x nw x n x n)."""
+    wr = np.ascontiguousarray(wr, float)
+    weights = np.ascontiguousarray(weights, float)
+    hosts = np.asarray(hosts, complex)
+    eb = np.ascontiguousarray(np.atleast_2d(eb), float)
+    tcb = np.asarray(tcb, float)
+    B, n, nb = tcb.shape
+    host_of = np.ascontiguousarray(host_of, np.int32)
+    if hosts.shape[1:] != (len(wr), n, n) or eb.shape != (B, nb) or host_of.shape != (B,):
+        raise ValueError("cdmft_distance_multi: inconsistent shapes")
+    scale = (2.0 if mixing == 0 else 1.0) * (wr[1] - wr[0]) / (n * n)
+    d = np.zeros(B)
+    tcm = np.ascontiguousarray(np.swapaxes(tcb, 1, 2)).ravel()
+    _lib.call("qed_cdmft_distance_multi", n, nb, len(wr), _lib.dp(wr), _lib.dp(weights),
+              len(hosts), _lib.dp(_cm(hosts)), B, _lib.ip(host_of), _lib.dp(eb.ravel()),
+              _lib.dp(tcm), float(scale), _lib.dp(d))
+    return d
+
+
+def cdmft_fit_batch(wr, weights, hosts, host_of, E, T, X0, initial_step=0.1, xatol=1e-6,
+                    fatol=1e-8, maxfev=1_000_000, mixing=0):
+    """Adaptive Nelder-Mead bath fits of B instances in one device launch.
+    E: (nvar, nb) bath-energy coefficients, T: (nvar, n, nb) coupling coefficients,
+    X0: (B, nvar) starting points. Returns (x (B, nvar), f (B), nfev (B), ok (B))."""
+    wr = np.ascontiguousarray(wr, float)
+    weights = np.ascontiguousarray(weights, float)
+    hosts = np.asarray(hosts, complex)
+    E = np.asarray(E, float)
+    T = np.asarray(T, float)
+    nvar, n, nb = T.shape
+    X0 = np.ascontiguousarray(np.atleast_2d(X0), float)
+    B = len(X0)
+    host_of = np.ascontiguousarray(host_of, np.int32)
+    if hosts.shape[1:] != (len(wr), n, n) or E.shape != (nvar, nb) or X0.shape[1] != nvar:
+        raise ValueError("cdmft_fit_batch: inconsistent shapes")
+    scale = (2.0 if mixing == 0 else 1.0) * (wr[1] - wr[0]) / (n * n)
+    Tcm = np.ascontiguousarray(np.swapaxes(T, 1, 2)).ravel()
+    x = np.zeros((B, nvar))
+    f = np.zeros(B)
+    nfev = np.zeros(B, np.int32)
+    status = np.zeros(B, np.int32)
+    _lib.call("qed_cdmft_fit_batch", n, nb, len(wr), _lib.dp(wr), _lib.dp(weights), len(hosts),
+              _lib.dp(_cm(hosts)), B, _lib.ip(host_of), nvar,
+              _lib.dp(np.ascontiguousarray(E).ravel()), _lib.dp(Tcm), _lib.dp(X0.ravel()),
+              float(initial_step), float(xatol), float(fatol), int(maxfev), float(scale),
+              _lib.dp(x), _lib.dp(f), _lib.ip(nfev), _lib.ip(status))
+    return x, f, nfev, status == 0
+
+
 # -------------------------------------------------------------- frequency grids
 
 def legendre_frequency_grid(w1, w2, n1, n2, n3):

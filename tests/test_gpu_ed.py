@@ -80,7 +80,10 @@ def test_green_function_dense_path_matches_bruteforce(gpu_lib):
 
 def test_green_function_band_lanczos_vs_oracle(gpu_lib):
     """3x3 Hubbard: band Lanczos on the device vs the oracle's restatement (same algorithm)
-    on the imaginary axis, within 1e-8; and close to exact."""
+    on the imaginary axis. The recurrence stops when the lowest eigenvalue of the projected
+    matrix changes by less than 1e-12 between checks, a test decided at round-off level,
+    so two implementations can stop one block of 9 steps apart; the Green function is
+    then converged to ~1e-8, not beyond. Bound: 5e-8 absolute (|G| <= 0.12 here)."""
     from pyqcm_amd import qcm
     m, vals = TM.square(3, 3, tp=-0.3, U=8.0, mu=4.0)
     lab = register(with_sites(m, 9), vals, "R0:N9:S1")
@@ -105,7 +108,10 @@ def test_green_function_band_lanczos_vs_oracle(gpu_lib):
     for i, z in enumerate(zs):
         ref = O.qmatrix_gf(e, Q, z)
         err = np.abs(G[i] - ref).max()
-        assert err <= 1e-8 * max(1.0, np.abs(ref).max()), (z, err, np.abs(ref).max())
+        tol = 5e-8
+        print("Im z = %g: max |G_gpu - G_oracle| = %.3g (poles: gpu %d, oracle %d)"
+              % (z.imag, err, len(qcm.qmatrix(lab)[0]), len(e)))
+        assert err <= tol * max(1.0, np.abs(ref).max()), (z, err, np.abs(ref).max())
     Gd = qcm.Green_function(zs, 1, lab)  # S=1 sector -> up/down mixing, spin-down GF exists
     assert Gd.shape == G.shape
 

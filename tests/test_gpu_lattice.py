@@ -182,3 +182,33 @@ def test_batched_cdmft_equals_plain_loop(gpu_lib):
         assert batch.converged[i]
         np.testing.assert_allclose(batch.x[i], p.x, atol=1e-5)
         assert abs(int(batch.niter[i]) - p.niter) <= 1
+
+
+@pytest.mark.parametrize("model", ["chain2_4b", "plaquette_8b"])
+def test_device_nelder_mead_matches_host_coroutine(gpu_lib, model):
+    """qed_cdmft_fit_batch (one wavefront per fit) follows the host form of scipy's
+    adaptive Nelder-Mead evaluation for evaluation: same minimizer, value and number of
+    evaluations, bit for bit, with several hosts in one launch."""
+    from pyqcm_amd import cdmft as C
+    rng = np.random.default_rng(11)
+    if model == "chain2_4b":
+        M = chain2_4b_model()
+        P = C.CDMFTProblem(M, ["eb1_1", "eb2_1", "tb1_1", "tb2_1"], "R0:N6:S0",
+                           grid=C.frequency_grid("matsubara", (2, 50)))
+        x0 = np.array([1.0, -1.0, 0.5, 0.5])
+        cases = [({"t": 1.0, "mu": 2.0, "U": 4.0}, x0), ({"t": 1.0, "mu": 1.4, "U": 3.0}, x0 * 1.1)]
+    else:
+        M = C.plaquette_8b_model(_name("plaq"))
+        P = C.CDMFTProblem(M, C.PLAQUETTE_VARIA, "R0:N12:S0")
+        x0 = np.concatenate([rng.uniform(-3, 3, 8), rng.uniform(0.1, 1, 8)])
+        cases = [({"t": 1.0, "tp": -0.3, "U": 8.0, "mu": 4.0}, x0),
+                 ({"t": 1.0, "tp": 0.0, "U": 4.0, "mu": 2.0}, x0)]
+    hosts = np.array([P.host(P.solve(v, x)) for v, x in cases])
+    X0 = np.array([x + rng.normal(0, 0.05, len(x)) for x in [cases[0][1], cases[1][1]] * 2])
+    host_of = np.array([0, 1, 0, 1])
+    xd, fd, nd, okd = P.fit(hosts, host_of, X0, where="device")
+    xh, fh, nh, okh = P.fit(hosts, host_of, X0, where="host")
+    assert okd.all() and okh.all()
+    np.testing.assert_array_equal(nd, nh)
+    np.testing.assert_array_equal(xd, xh)
+    np.testing.assert_array_equal(fd, fh)
