@@ -366,6 +366,8 @@ class CDMFT:
         t0 = time.perf_counter()
         inst = self.problem.solve(self.values, self.x, self.label)
         Gh = self.problem.host(inst)
+        E0 = inst.ground_state()[0]
+        inst.erase()
         self.n_solves += 1
         t1 = time.perf_counter()
         X, F, NF, OK = self.problem.fit(Gh[None], [0], self.x[None], self.initial_step,
@@ -378,7 +380,7 @@ class CDMFT:
         self.nfev += nfev
         self.dist = fun
         self.x = xo
-        self.history.append((np.copy(xo), fun, inst.ground_state()[0]))
+        self.history.append((np.copy(xo), fun, E0))
         self.t_ed += t1 - t0
         self.t_fit += time.perf_counter() - t1
         if self.verbose:
@@ -422,6 +424,7 @@ class BatchedCDMFT:
         for v, x in zip(values_list, x_list):
             inst = self.problem.solve(v, x)
             hosts.append(self.problem.host(inst))
+            inst.erase()
         return hosts
 
     def _steps(self, idx, xs):

@@ -426,6 +426,10 @@ class LatticeInstance:
         self.label = label
         self._gs = None
 
+    def erase(self):
+        """Releases the cluster instance (its device vectors) in the solver."""
+        qcm.erase_model_instance(self.label)
+
     def ground_state(self):
         if self._gs is None:
             self._gs = qcm.ground_state_solve(self.label)

@@ -1,0 +1,101 @@
+"""Learned warm start for the CDMFT bath fit.
+
+A small MLP (n_in -> H -> H -> n_out, tanh hidden layers) maps model parameters such as
+(U, mu, t') to the bath parameters the plain CDMFT loop converges to. It is trained on
+the device (qed_mlp_train: fused FP32 forward/backward kernels and Adam) on solutions of
+the plain loop, evaluated with BF16 matrix-core inference (qed_mlp_forward_bf16), and
+replaces the starting point x0 of the CDMFT iteration (pyqcm/cdmft.py:300-302). The plain
+loop then runs from that point unchanged: it is both the fallback and the judge of the
+prediction, since convergence is still decided by its own tests.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+class MLP:
+    """Device-resident MLP (include/qed.h qed_mlp_*). Inputs and targets are normalized
+    with the statistics of the training set (kept here)."""
+
+    def __init__(self, n_in=3, n_hidden=128, n_out=16, seed=0):
+        self.n_in, self.n_hidden, self.n_out = n_in, n_hidden, n_out
+        h = ctypes.c_void_p()
+        _lib.call("qed_mlp_create", n_in, n_hidden, n_out, ctypes.c_uint64(seed), ctypes.byref(h))
+        self._h = h
+        self.x_mean = np.zeros(n_in)
+        self.x_std = np.ones(n_in)
+        self.y_mean = np.zeros(n_out)
+        self.y_std = np.ones(n_out)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.call("qed_mlp_destroy", h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def n_params(self):
+        c = ctypes.c_int()
+        _lib.call("qed_mlp_get_params", self._h, None, ctypes.byref(c))
+        return c.value
+
+    def get_params(self):
+        p = np.zeros(self.n_params, np.float32)
+        c = ctypes.c_int()
+        _lib.call("qed_mlp_get_params", self._h, _lib.fp(p), ctypes.byref(c))
+        return p
+
+    def set_params(self, p):
+        p = _f32(p)
+        _lib.call("qed_mlp_set_params", self._h, _lib.fp(p), len(p))
+
+    # raw (normalized-space) calls
+    def forward_raw(self, x, bf16=False):
+        x = _f32(np.atleast_2d(x))
+        y = np.zeros((len(x), self.n_out), np.float32)
+        fn = "qed_mlp_forward_bf16" if bf16 else "qed_mlp_forward"
+        _lib.call(fn, self._h, len(x), _lib.fp(x), _lib.fp(y))
+        return y
+
+    def loss_grad_raw(self, x, t):
+        x, t = _f32(np.atleast_2d(x)), _f32(np.atleast_2d(t))
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        _lib.call("qed_mlp_loss_grad", self._h, len(x), _lib.fp(x), _lib.fp(t), _lib.fp(loss), _lib.fp(g))
+        return float(loss[0]), g
+
+    def train_raw(self, x, t, epochs, batch=64, lr=1e-3, seed=0):
+        x, t = _f32(np.atleast_2d(x)), _f32(np.atleast_2d(t))
+        hist = np.zeros(epochs, np.float32)
+        _lib.call("qed_mlp_train", self._h, len(x), _lib.fp(x), _lib.fp(t), int(epochs), int(batch),
+                  ctypes.c_float(lr), ctypes.c_uint64(seed), _lib.fp(hist))
+        return hist
+
+    # physical-units interface
+    def fit(self, X, Y, epochs=2000, batch=64, lr=1e-3, seed=0):
+        X, Y = np.asarray(X, float), np.asarray(Y, float)
+        self.x_mean, self.x_std = X.mean(0), X.std(0) + 1e-12
+        self.y_mean, self.y_std = Y.mean(0), Y.std(0) + 1e-12
+        return self.train_raw((X - self.x_mean) / self.x_std, (Y - self.y_mean) / self.y_std,
+                              epochs, batch, lr, seed)
+
+    def predict(self, X, bf16=True):
+        X = np.atleast_2d(np.asarray(X, float))
+        y = self.forward_raw((X - self.x_mean) / self.x_std, bf16=bf16).astype(float)
+        return y * self.y_std + self.y_mean
+
+
+def plaquette_features(values):
+    """(U, mu, t') of a parameter dict: the inputs of the 2x2+8-bath warm start."""
+    return np.array([values["U"], values["mu"], values.get("tp", 0.0)], float)

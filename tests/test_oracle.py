@@ -306,3 +306,35 @@ def test_cpt_identity_and_sef():
     np.testing.assert_allclose(np.linalg.inv(Gcpt), np.linalg.inv(Gc) - V, atol=1e-12)
     s = O.sef_integrand(Gc, V)
     assert abs(s - 2 * np.log(abs(np.linalg.det(np.eye(n) - V @ Gc)))) < 1e-12
+
+
+# ---------------------------------------------------------------- warm-start MLP (a26)
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """The oracle's backpropagation against central differences of its own loss."""
+    from oracle import mlp_oracle as MO
+    rng = np.random.default_rng(2)
+    ni, nh, no = 3, 8, 5
+    n_par = nh * ni + nh + nh * nh + nh + no * nh + no
+    p = rng.normal(0, 0.5, n_par)
+    x = rng.normal(size=(7, ni))
+    t = rng.normal(size=(7, no))
+    loss, g = MO.loss_grad(p, x, t, ni, nh, no)
+    h = 1e-6
+    fd = np.zeros(n_par)
+    for i in range(n_par):
+        e = np.zeros(n_par)
+        e[i] = h
+        fd[i] = (MO.loss_grad(p + e, x, t, ni, nh, no)[0] - MO.loss_grad(p - e, x, t, ni, nh, no)[0]) / (2 * h)
+    np.testing.assert_allclose(g, fd, rtol=1e-6, atol=1e-9)
+    # the loss itself is the mean over samples and outputs
+    y, _ = MO.forward(p, x, ni, nh, no)
+    assert abs(loss - np.mean((y - t) ** 2)) < 1e-14
+
+
+def test_mlp_oracle_adam_first_step_is_signed_lr():
+    """Bias-corrected Adam: the first step moves every parameter by lr * g/(|g| + eps')."""
+    from oracle import mlp_oracle as MO
+    g = np.array([0.3, -2.0, 1e-3])
+    p, m, v = MO.adam_step(np.zeros(3), np.zeros(3), np.zeros(3), g, 1, 0.01)
+    np.testing.assert_allclose(p, -0.01 * np.sign(g), rtol=1e-4)
