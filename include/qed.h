@@ -227,8 +227,17 @@ int qed_ed_get_global_double(const char* name, double* v);
  * ground state by Lanczos and Green function by band Lanczos for each, all vectors in
  * one arena, the H.v of all instances issued as one persistent launch per Krylov step
  * (captured in a HIP graph when graph != 0). params: nb x nparam values in the order of
- * names; sectors: one sector string per instance. Outputs E0[nb] and the poles of each
- * instance, packed: Mout[nb], e (nb x cap), Q (nb x d x cap). */
+ * names; sectors: one sector string per instance. Outputs E0[nb] and the spin-up
+ * Lehmann form of each instance, packed: Mout[nb] poles, e (nb x cap),
+ * Q (nb x cap x d, pole-major: Q[(b cap + m) d + i]). New in this library: the batched
+ * form of ED::ground_state_solve + ED::Green_function_solve (src_ed/qcm_ED.cpp). */
+/* Solves n instances already created with qed_ed_new_model_instance together (ground
+ * states and Green functions, same batched engine as qed_batch_solve); afterwards the
+ * qed_ed_* queries on those labels use the stored results. chunk: Lanczos iterations
+ * between stopping tests (0: 16). stats (4 ints, may be null): lockstep Lanczos
+ * iterations, band Lanczos steps, and the autotuned H.v block size and grid. */
+int qed_ed_batch_solve_instances(int n, const uint64_t* labels, int graph, int chunk, int* stats);
+
 int qed_batch_solve(const char* model, int nparam, const char* const* names, int nb,
                     const double* params, const char* const* sectors, int graph, int cap,
                     double* E0, int* Mout, double* e, double* Q);

@@ -79,6 +79,7 @@ SIGNATURES = [
     ("qed_ed_set_global_bool", c_int, [c_char_p, c_int]),
     ("qed_ed_set_global_char", c_int, [c_char_p, c_char]),
     ("qed_ed_get_global_double", c_int, [c_char_p, dptr]),
+    ("qed_ed_batch_solve_instances", c_int, [c_int, P(c_u64), c_int, c_int, iptr]),
     ("qed_batch_solve", c_int, [c_char_p, c_int, P(c_char_p), c_int, dptr, P(c_char_p), c_int, c_int,
                                 dptr, iptr, dptr, dptr]),
     ("qed_mlp_create", c_int, [c_int, c_int, c_int, c_u64, P(c_void_p)]),

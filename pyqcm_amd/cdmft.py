@@ -241,7 +241,8 @@ class CDMFTProblem:
 
     `model` is a lattice.LatticeModel whose bath operators were added with
     add_cluster_operator; `varia` names the bath parameters (optionally with the
-    cluster suffix '_1', as in pyqcm); `sectors` is the target-sector string."""
+    cluster suffix '_1', as in pyqcm); `sectors` is the target-sector string, or a function
+    of the parameter values returning it."""
 
     def __init__(self, model, varia, sectors, grid=None, kside=32):
         self.model = model
@@ -253,12 +254,30 @@ class CDMFTProblem:
     def solve(self, values, x, label=None):
         vals = dict(values)
         vals.update(zip(self.varia, np.asarray(x, float)))
-        inst = self.model.instance(vals, self.sectors, label)
+        inst = self.model.instance(vals, self.sectors_of(values), label)
         inst.ground_state()
         return inst
 
     def host(self, inst):
         return inst.cdmft_host(self.grid.wr, self.kside)
+
+    def solve_batch(self, values_list, x_list, graph=True):
+        """Host functions of many instances: one batched device solve for all of them
+        (qcm.batch_solve), then the k-sums per instance."""
+        from . import qcm
+        insts = []
+        for v, x in zip(values_list, x_list):
+            vals = dict(v)
+            vals.update(zip(self.varia, np.asarray(x, float)))
+            insts.append(self.model.instance(vals, self.sectors_of(v)))
+        self.last_stats = qcm.batch_solve([I.label for I in insts], graph=graph)
+        hosts = [self.host(I) for I in insts]
+        for I in insts:
+            I.erase()
+        return hosts
+
+    def sectors_of(self, values):
+        return self.sectors(values) if callable(self.sectors) else self.sectors
 
     def distance(self, Ghost, X):
         eb, tcb = self.model.diagonal_bath(self.varia, X)
@@ -407,6 +426,8 @@ class BatchedCDMFT:
         self.accur_bath, self.accur_dist = accur_bath, accur_dist
         self.initial_step, self.maxfev = initial_step, max_function_eval
         self.maxiter = maxiter
+        if solver == "batched":
+            solver = problem.solve_batch
         self.solver = solver or self._solve_sequential
         self.verbose = verbose
         self.conv = [parameters_convergence(accur, depth) for _ in range(B)]
@@ -530,6 +551,30 @@ def plaquette_8b_model(name="clus_2x2_8b"):
 
 
 PLAQUETTE_VARIA = ["eb%d" % x for x in range(1, 9)] + ["tb%d" % x for x in range(1, 9)]
+
+
+def plaquette_sectors(values):
+    """Target sectors of the 2x2+8-bath cluster (12 orbitals) for chemical potential mu in
+    [0, U]: the even particle number N0 = 12 + 2 round(4 (mu/U - 1/2)) (8 .. 16) and its
+    even neighbours within 8 .. 16, all at S = 0."""
+    U, mu = float(values["U"]), float(values["mu"])
+    f = mu / U if U > 0 else 0.5
+    n0 = 12 + 2 * int(round(4 * (min(max(f, 0.0), 1.0) - 0.5)))
+    ns = [n for n in (n0 - 2, n0, n0 + 2) if 8 <= n <= 16]
+    return "/".join("R0:N%d:S0" % n for n in ns)
+
+
+def random_plaquette_instances(n, seed):
+    """Seeded (model values, bath parameters) pairs: U ~ U[0.5, 12], mu ~ U[0, U],
+    t = 1, t' ~ U[-0.5, 0.5], bath energies ~ U[-3, 3], hybridizations ~ U[0.1, 1]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        U = rng.uniform(0.5, 12.0)
+        vals = {"t": 1.0, "tp": rng.uniform(-0.5, 0.5), "U": U, "mu": rng.uniform(0.0, U)}
+        x = np.concatenate([rng.uniform(-3, 3, 8), rng.uniform(0.1, 1.0, 8)])
+        out.append((vals, x))
+    return out
 
 
 def canonical_bath(x):

@@ -1,0 +1,977 @@
+// Batched Krylov solvers: many sectors (cluster instances x Hilbert-space sectors) advance
+// together, so that every step of the recurrence is one launch per operation for all of
+// them instead of one per sector.
+//
+//  * Ground state: Lanczos for all sectors in lockstep. Each iteration is a fixed kernel
+//    sequence (batched H.v, fused dot, fused axpy+norm) driven by a device-side
+//    iteration counter; two iterations (both parities of the vector rotation) are
+//    captured once in a hipGraph and replayed chunk by chunk. The recurrence scalars stay
+//    on the device; after each chunk the host reads them, applies the stopping test of
+//    the scalar Lanczos (|beta_j e_j| <= accur, src_ed/Hamiltonian/Lanczos.hpp) to every
+//    new step, and retires the converged sectors. The ground-state vector is then
+//    rebuilt for all sectors by a batched replay of the recurrence.
+//  * Green function: Freund's band Lanczos (src_ed/Hamiltonian/Lanczos.hpp:214-395) for
+//    all (instance, particle/hole) sectors. The host keeps each sector's band matrix and
+//    deflation logic; one synchronization per step serves all sectors, and every vector
+//    operation of the step is a single batched launch over the sectors.
+//
+// Descriptor-based kernels: each sector ("job") has a descriptor with its vectors and a
+// block range; block -> job by binary search. Reductions are two-pass and deterministic.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/qed.h"
+#include "qed_common.hpp"
+#include "qed_linalg.hpp"
+#include "qed_sector.hpp"
+
+namespace qed {
+namespace {
+
+constexpr int kBT = 256;               // threads per block
+constexpr uint64_t kChunk = kBT * 8;   // elements per block
+constexpr int kBM = 20;                // vectors per multi-operation
+
+__host__ __device__ inline uint32_t nblocks(uint64_t n) { return (uint32_t)((n + kChunk - 1) / kChunk); }
+
+template <class D>
+__device__ __forceinline__ int find_job(const D* __restrict__ d, int nj, uint32_t b) {
+  int lo = 0, hi = nj - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].blk0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kBT / 64; ++i) s += red[i];
+  __syncthreads();
+  return s;
+}
+
+// ---- multi-dot: out[q] = <x, y[q]> -------------------------------------------------------
+struct MDot {
+  const double* x;
+  const double* y[kBM];
+  double* out;  // device, m values
+  int m;
+  uint64_t n;
+  uint32_t blk0;
+};
+
+__global__ __launch_bounds__(kBT) void k_mdot_part(const MDot* __restrict__ d, int nj,
+                                                   double* __restrict__ part) {
+  __shared__ double red[kBT / 64];
+  const uint32_t b = blockIdx.x;
+  const int j = find_job(d, nj, b);
+  const MDot& D = d[j];
+  const uint64_t i0 = (uint64_t)(b - D.blk0) * kChunk;
+  const uint64_t i1 = min(i0 + kChunk, D.n);
+  for (int q = 0; q < D.m; ++q) {
+    double s = 0.0;
+    for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBT) s += D.x[i] * D.y[q][i];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) part[(size_t)b * kBM + q] = s;
+  }
+}
+
+__global__ void k_mdot_final(const MDot* __restrict__ d, int nj, const double* __restrict__ part) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nj * kBM) return;
+  const int j = id / kBM, q = id % kBM;
+  const MDot& D = d[j];
+  if (q >= D.m) return;
+  const uint32_t nb = nblocks(D.n);
+  double s = 0.0;
+  for (uint32_t b = 0; b < nb; ++b) s += part[(size_t)(D.blk0 + b) * kBM + q];
+  D.out[q] = s;
+}
+
+// ---- multi-axpy: y += sgn * sum_q (*c[q]) x[q]; optionally y *= scale first -------------
+struct MAxpy {
+  double* y;
+  const double* x[kBM];
+  const double* c[kBM];  // device scalars
+  double sgn;
+  double scale;  // applied to y before the sum
+  int m;
+  uint64_t n;
+  uint32_t blk0;
+};
+
+__global__ __launch_bounds__(kBT) void k_maxpy(const MAxpy* __restrict__ d, int nj) {
+  const uint32_t b = blockIdx.x;
+  const int j = find_job(d, nj, b);
+  const MAxpy& D = d[j];
+  const uint64_t i0 = (uint64_t)(b - D.blk0) * kChunk;
+  const uint64_t i1 = min(i0 + kChunk, D.n);
+  double c[kBM];
+  for (int q = 0; q < D.m; ++q) c[q] = D.sgn * *D.c[q];
+  for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBT) {
+    double v = D.y[i] * D.scale;
+    for (int q = 0; q < D.m; ++q) v += c[q] * D.x[q][i];
+    D.y[i] = v;
+  }
+}
+
+// ---- reverse multi-axpy: y[q] -= (*c[q]) x ----------------------------------------------
+struct MRev {
+  const double* x;
+  double* y[kBM];
+  const double* c[kBM];
+  int m;
+  uint64_t n;
+  uint32_t blk0;
+};
+
+__global__ __launch_bounds__(kBT) void k_mrev(const MRev* __restrict__ d, int nj) {
+  const uint32_t b = blockIdx.x;
+  const int j = find_job(d, nj, b);
+  const MRev& D = d[j];
+  const uint64_t i0 = (uint64_t)(b - D.blk0) * kChunk;
+  const uint64_t i1 = min(i0 + kChunk, D.n);
+  for (int q = 0; q < D.m; ++q) {
+    const double c = *D.c[q];
+    double* y = D.y[q];
+    for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBT) y[i] -= c * D.x[i];
+  }
+}
+
+// ---- fill: x = 0 ------------------------------------------------------------------------
+struct VFill {
+  double* x;
+  uint64_t n;
+  uint32_t blk0;
+};
+
+__global__ __launch_bounds__(kBT) void k_fill0(const VFill* __restrict__ d, int nj) {
+  const uint32_t b = blockIdx.x;
+  const int j = find_job(d, nj, b);
+  const VFill& D = d[j];
+  const uint64_t i0 = (uint64_t)(b - D.blk0) * kChunk;
+  const uint64_t i1 = min(i0 + kChunk, D.n);
+  for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBT) D.x[i] = 0.0;
+}
+
+// ---- Lanczos (ground state) ------------------------------------------------------------
+// vectors A, B per sector; at iteration k (parity p = k & 1) r = p ? B : A, q = p ? A : B.
+struct LJob {
+  double* v[2];
+  double* alpha;  // per-iteration scalars (device)
+  double* beta;
+  uint64_t n;
+  uint32_t blk0;
+};
+
+// k > 0: r /= beta[k-1]; q *= -beta[k-1]
+__global__ __launch_bounds__(kBT) void k_lz_scal(const LJob* __restrict__ d, int nj, int parity,
+                                                 const int* __restrict__ kdev,
+                                                 const int* __restrict__ active) {
+  const int k = *kdev;
+  if (k == 0) return;
+  const uint32_t b = blockIdx.x;
+  const int j = find_job(d, nj, b);
+  if (!active[j]) return;
+  const LJob& D = d[j];
+  double* r = D.v[parity];
+  double* q = D.v[parity ^ 1];
+  const double bt = D.beta[k - 1];
+  const double inv = 1.0 / bt;
+  const uint64_t i0 = (uint64_t)(b - D.blk0) * kChunk;
+  const uint64_t i1 = min(i0 + kChunk, D.n);
+  for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBT) {
+    r[i] *= inv;
+    q[i] *= -bt;
+  }
+}
+
+// partials of <r, q> (r = v_k, q = H v_k - beta v_{k-1})
+__global__ __launch_bounds__(kBT) void k_lz_dot(const LJob* __restrict__ d, int nj, int parity,
+                                                const int* __restrict__ active, double* __restrict__ part) {
+  __shared__ double red[kBT / 64];
+  const uint32_t b = blockIdx.x;
+  const int j = find_job(d, nj, b);
+  const LJob& D = d[j];
+  double s = 0.0;
+  if (active[j]) {
+    const double* r = D.v[parity];
+    const double* q = D.v[parity ^ 1];
+    const uint64_t i0 = (uint64_t)(b - D.blk0) * kChunk;
+    const uint64_t i1 = min(i0 + kChunk, D.n);
+    for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBT) s += r[i] * q[i];
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) part[b] = s;
+}
+
+// alpha[k] = sum of the partials (one thread per job)
+__global__ void k_lz_alpha(const LJob* __restrict__ d, int nj, const int* __restrict__ kdev,
+                           const int* __restrict__ active, const double* __restrict__ part) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nj || !active[j]) return;
+  const LJob& D = d[j];
+  const uint32_t nb = nblocks(D.n);
+  double s = 0.0;
+  for (uint32_t b = 0; b < nb; ++b) s += part[D.blk0 + b];
+  D.alpha[*kdev] = s;
+}
+
+// q -= alpha[k] r, partials of |q|^2
+__global__ __launch_bounds__(kBT) void k_lz_axpy(const LJob* __restrict__ d, int nj, int parity,
+                                                 const int* __restrict__ kdev,
+                                                 const int* __restrict__ active,
+                                                 double* __restrict__ part) {
+  __shared__ double red[kBT / 64];
+  const uint32_t b = blockIdx.x;
+  const int j = find_job(d, nj, b);
+  const LJob& D = d[j];
+  double s = 0.0;
+  if (active[j]) {
+    const double a = D.alpha[*kdev];
+    const double* r = D.v[parity];
+    double* q = D.v[parity ^ 1];
+    const uint64_t i0 = (uint64_t)(b - D.blk0) * kChunk;
+    const uint64_t i1 = min(i0 + kChunk, D.n);
+    for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBT) {
+      const double v = q[i] - a * r[i];
+      q[i] = v;
+      s += v * v;
+    }
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) part[b] = s;
+}
+
+// beta[k] = sqrt(sum of partials)
+__global__ void k_lz_beta(const LJob* __restrict__ d, int nj, const int* __restrict__ kdev,
+                          const int* __restrict__ active, const double* __restrict__ part) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nj || !active[j]) return;
+  const LJob& D = d[j];
+  const uint32_t nb = nblocks(D.n);
+  double s = 0.0;
+  for (uint32_t b = 0; b < nb; ++b) s += part[D.blk0 + b];
+  D.beta[*kdev] = std::sqrt(s);
+}
+
+__global__ void k_lz_next(int* __restrict__ kdev) {
+  if (threadIdx.x == 0) *kdev = *kdev + 1;
+}
+
+// eigenvector replay: q -= a r; (r, q) <- (q / b, -b r); psi += c r_new, for step i < len
+struct RJob {
+  double* r;
+  double* q;
+  double* psi;
+  const double* alpha;  // device, len
+  const double* beta;
+  const double* evec;
+  int len;
+  uint64_t n;
+  uint32_t blk0;
+};
+
+__global__ __launch_bounds__(kBT) void k_replay_batch(const RJob* __restrict__ d, int nj, int i) {
+  const uint32_t b = blockIdx.x;
+  const int j = find_job(d, nj, b);
+  const RJob& D = d[j];
+  if (i >= D.len) return;
+  const double a = D.alpha[i - 1], bt = D.beta[i - 1], c = D.evec[i];
+  const uint64_t i0 = (uint64_t)(b - D.blk0) * kChunk;
+  const uint64_t i1 = min(i0 + kChunk, D.n);
+  for (uint64_t t = i0 + threadIdx.x; t < i1; t += kBT) {
+    const double rv = D.r[t];
+    const double qv = D.q[t] - a * rv;
+    const double rn = qv / bt;
+    D.r[t] = rn;
+    D.q[t] = -bt * rv;
+    D.psi[t] += c * rn;
+  }
+}
+
+struct PsiScale {
+  double* psi;
+  double c;
+  uint64_t n;
+  uint32_t blk0;
+};
+__global__ __launch_bounds__(kBT) void k_psi_scale(const PsiScale* __restrict__ d, int nj) {
+  const uint32_t b = blockIdx.x;
+  const int j = find_job(d, nj, b);
+  const PsiScale& D = d[j];
+  const uint64_t i0 = (uint64_t)(b - D.blk0) * kChunk;
+  const uint64_t i1 = min(i0 + kChunk, D.n);
+  for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBT) D.psi[i] *= D.c;
+}
+
+// ---- host helpers ------------------------------------------------------------------------
+template <class D>
+struct Table {
+  std::vector<D> h;
+  DevBuf d;
+  uint32_t total = 0;
+  void clear() {
+    h.clear();
+    total = 0;
+  }
+  void add(D v, uint64_t n) {
+    v.blk0 = total;
+    total += nblocks(n);
+    h.push_back(v);
+  }
+  D* upload() {
+    const size_t bytes = std::max<size_t>(1, h.size()) * sizeof(D);
+    if (d.bytes < bytes) d.alloc(bytes * 2);
+    if (!h.empty()) QED_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(D), hipMemcpyHostToDevice, stream()));
+    return d.as<D>();
+  }
+};
+
+void ensure(DevBuf& b, size_t bytes) {
+  if (b.bytes < bytes) b.alloc(bytes);
+}
+
+template <class F>
+void parallel_for(int n, F f) {
+  const int nt = std::max(1, std::min<int>(n, (int)std::min<unsigned>(16u, std::thread::hardware_concurrency())));
+  if (nt <= 1 || n < 4) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  std::atomic<int> next{0};
+  std::exception_ptr err;
+  std::mutex mu;
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&] {
+      try {
+        for (int i = next++; i < n; i = next++) f(i);
+      } catch (...) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!err) err = std::current_exception();
+        next = n;
+      }
+    });
+  for (auto& t : th) t.join();
+  if (err) std::rethrow_exception(err);
+}
+
+}  // namespace
+
+// ========================================================================================
+// Batched Lanczos ground state. psi[j] holds the start vector (device) and receives the
+// ground-state vector. E0[j], niter[j] out. Returns the H.v launch configuration used.
+void lanczos_gs_batch(const std::vector<qed_sector*>& S, const std::vector<double*>& psi, double accur,
+                      int max_iter, bool use_graph, int chunk, std::vector<double>& E0,
+                      std::vector<int>& niter, BatchStats* stats) {
+  const int nj = (int)S.size();
+  E0.assign(nj, 0.0);
+  niter.assign(nj, 0);
+  if (nj == 0) return;
+  if (chunk < 2) chunk = 2;
+  chunk += chunk & 1;
+  const int ld = max_iter + chunk + 2;
+  // vectors: A = copy of the start vector, B = 0
+  std::vector<std::unique_ptr<DevBuf>> A(nj), Bv(nj);
+  DevBuf scal((size_t)nj * ld * 2 * sizeof(double));
+  double* alpha = scal.as<double>();
+  double* beta = alpha + (size_t)nj * ld;
+  Table<LJob> T;
+  Table<VFill> F;
+  for (int j = 0; j < nj; ++j) {
+    const uint64_t n = S[j]->D;
+    A[j].reset(new DevBuf(n * 8));
+    Bv[j].reset(new DevBuf(n * 8));
+    QED_HIP(hipMemcpyAsync(A[j]->p, psi[j], n * 8, hipMemcpyDeviceToDevice, stream()));
+    T.add(LJob{{A[j]->as<double>(), Bv[j]->as<double>()}, alpha + (size_t)j * ld, beta + (size_t)j * ld, n, 0}, n);
+    F.add(VFill{Bv[j]->as<double>(), n, 0}, n);
+  }
+  LJob* dT = T.upload();
+  VFill* dF = F.upload();
+  k_fill0<<<F.total, kBT, 0, stream()>>>(dF, nj);
+  // H.v batch tables for both parities: parity p: x = r, y = q
+  std::vector<const qed_sector*> cs(S.begin(), S.end());
+  HvBatch HB;
+  hv_batch_prepare(HB, cs);
+  for (int j = 0; j < nj; ++j) {
+    HB.host[j].x[0] = A[j]->as<double>();
+    HB.host[j].y[0] = Bv[j]->as<double>();
+    HB.host[j].x[1] = Bv[j]->as<double>();
+    HB.host[j].y[1] = A[j]->as<double>();
+  }
+  hv_batch_upload(HB);
+  std::vector<int> active(nj, 1);
+  DevBuf dact((size_t)nj * sizeof(int)), dk(sizeof(int)), part((size_t)T.total * sizeof(double));
+  QED_HIP(hipMemcpyAsync(dact.p, active.data(), nj * sizeof(int), hipMemcpyHostToDevice, stream()));
+  const int zero = 0;
+  QED_HIP(hipMemcpyAsync(dk.p, &zero, sizeof(int), hipMemcpyHostToDevice, stream()));
+  // autotune the batched H.v on scratch vectors (results discarded)
+  HvBatchConfig cfg;
+  {
+    std::vector<std::unique_ptr<DevBuf>> sx(nj), sy(nj);
+    HvBatch tb;
+    hv_batch_prepare(tb, cs);
+    for (int j = 0; j < nj; ++j) {
+      sx[j].reset(new DevBuf(S[j]->D * 8));
+      sy[j].reset(new DevBuf(S[j]->D * 8));
+      tb.host[j].x[0] = tb.host[j].x[1] = A[j]->as<double>();
+      tb.host[j].y[0] = tb.host[j].y[1] = sy[j]->as<double>();
+    }
+    hv_batch_upload(tb);
+    cfg = hv_batch_tune(tb, dact.as<int>());
+    QED_HIP(hipStreamSynchronize(stream()));
+  }
+  if (stats) stats->hv_config = cfg;
+  const int* act = dact.as<int>();
+  int* kd = dk.as<int>();
+  double* pd = part.as<double>();
+  const int nb1 = (nj + 255) / 256;
+  auto iteration = [&](int parity) {
+    k_lz_scal<<<T.total, kBT, 0, stream()>>>(dT, nj, parity, kd, act);
+    hv_batch(HB, cfg, parity, act);
+    k_lz_dot<<<T.total, kBT, 0, stream()>>>(dT, nj, parity, act, pd);
+    k_lz_alpha<<<nb1, 256, 0, stream()>>>(dT, nj, kd, act, pd);
+    k_lz_axpy<<<T.total, kBT, 0, stream()>>>(dT, nj, parity, kd, act, pd);
+    k_lz_beta<<<nb1, 256, 0, stream()>>>(dT, nj, kd, act, pd);
+    k_lz_next<<<1, 64, 0, stream()>>>(kd);
+  };
+  hipGraphExec_t exec = nullptr;
+  if (use_graph) {
+    hipGraph_t g;
+    QED_HIP(hipStreamSynchronize(stream()));
+    QED_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+    iteration(0);
+    iteration(1);
+    QED_HIP(hipStreamEndCapture(stream(), &g));
+    QED_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    QED_HIP(hipGraphDestroy(g));
+  }
+  std::vector<double> ha((size_t)nj * ld), hb((size_t)nj * ld);
+  std::vector<std::vector<double>> evec(nj);
+  std::vector<int> done_at(nj, -1);
+  int k = 0;
+  int remaining = nj;
+  while (remaining > 0) {
+    for (int c = 0; c < chunk; c += 2) {
+      if (exec) {
+        QED_HIP(hipGraphLaunch(exec, stream()));
+      } else {
+        iteration(0);
+        iteration(1);
+      }
+    }
+    QED_HIP(hipGetLastError());
+    const int k1 = k + chunk;
+    QED_HIP(hipMemcpyAsync(ha.data(), alpha, ha.size() * 8, hipMemcpyDeviceToHost, stream()));
+    QED_HIP(hipMemcpyAsync(hb.data(), beta, hb.size() * 8, hipMemcpyDeviceToHost, stream()));
+    QED_HIP(hipStreamSynchronize(stream()));
+    // stopping test of the scalar Lanczos for the new steps m = k+1 .. k1 of every active
+    // sector: ritz(m) = |beta_m e_m^T y_min(T_m)|. The chunk's last step is tested first;
+    // the steps before it only when it has converged.
+    std::vector<int> newly(nj, -1);
+    parallel_for(nj, [&](int j) {
+      if (!active[j]) return;
+      std::vector<double> a, b, ev;
+      double e0 = 0.0;
+      auto ritz_at = [&](int m) {
+        a.assign(ha.begin() + (size_t)j * ld, ha.begin() + (size_t)j * ld + m);
+        b.assign(hb.begin() + (size_t)j * ld, hb.begin() + (size_t)j * ld + m);
+        tridiag_lowest(a, b, &e0, ev);
+        return std::fabs(ev.back() * b.back());
+      };
+      if (ritz_at(k1) > accur) return;
+      int m = k + 1;
+      while (m < k1 && ritz_at(m) > accur) ++m;
+      ritz_at(m);
+      newly[j] = m;
+      evec[j] = ev;
+      E0[j] = e0;
+    });
+    for (int j = 0; j < nj; ++j)
+      if (newly[j] > 0) {
+        if (newly[j] > max_iter)
+          fail("Lanczos procedure exceeded " + std::to_string(max_iter) + " iterations");
+        done_at[j] = newly[j];
+        active[j] = 0;
+        --remaining;
+      }
+    k = k1;
+    if (remaining > 0 && k >= max_iter)
+      fail("Lanczos procedure exceeded " + std::to_string(max_iter) + " iterations");
+    QED_HIP(hipMemcpyAsync(dact.p, active.data(), nj * sizeof(int), hipMemcpyHostToDevice, stream()));
+  }
+  if (exec) QED_HIP(hipGraphExecDestroy(exec));
+  if (stats) {
+    stats->lanczos_iterations = k;
+  }
+  // ground-state vectors: replay the recurrence with the stored scalars
+  int maxlen = 0;
+  std::vector<DevBuf> ev_dev;
+  Table<RJob> R;
+  Table<PsiScale> PS;
+  std::vector<int> ones(nj, 1);
+  for (int j = 0; j < nj; ++j) {
+    niter[j] = done_at[j];
+    const uint64_t n = S[j]->D;
+    ev_dev.emplace_back(evec[j].size() * 8);
+    QED_HIP(hipMemcpyAsync(ev_dev.back().p, evec[j].data(), evec[j].size() * 8, hipMemcpyHostToDevice, stream()));
+    // r = start vector (copy kept in psi), q = 0, psi *= evec[0]
+    QED_HIP(hipMemcpyAsync(A[j]->p, psi[j], n * 8, hipMemcpyDeviceToDevice, stream()));
+    R.add(RJob{A[j]->as<double>(), Bv[j]->as<double>(), psi[j], alpha + (size_t)j * ld, beta + (size_t)j * ld,
+               ev_dev.back().as<double>(), (int)evec[j].size(), n, 0},
+          n);
+    PS.add(PsiScale{psi[j], evec[j][0], n, 0}, n);
+    maxlen = std::max(maxlen, (int)evec[j].size());
+  }
+  k_fill0<<<F.total, kBT, 0, stream()>>>(dF, nj);
+  PsiScale* dPS = PS.upload();
+  k_psi_scale<<<PS.total, kBT, 0, stream()>>>(dPS, nj);
+  RJob* dR = R.upload();
+  QED_HIP(hipMemcpyAsync(dact.p, ones.data(), nj * sizeof(int), hipMemcpyHostToDevice, stream()));
+  // replay H.v: x = r (A), y = q (B): parity 0 of HB
+  for (int i = 1; i < maxlen; ++i) {
+    for (int j = 0; j < nj; ++j) active[j] = i < (int)evec[j].size();
+    QED_HIP(hipMemcpyAsync(dact.p, active.data(), nj * sizeof(int), hipMemcpyHostToDevice, stream()));
+    hv_batch(HB, cfg, 0, act);
+    k_replay_batch<<<R.total, kBT, 0, stream()>>>(dR, nj, i);
+  }
+  QED_HIP(hipGetLastError());
+  QED_HIP(hipStreamSynchronize(stream()));
+}
+
+// ========================================================================================
+// Batched band Lanczos (Freund), one host state per job, one synchronization per step.
+namespace {
+
+struct BandJob {
+  const qed_sector* s = nullptr;
+  uint64_t n = 0;
+  int p0 = 0, pc = 0, M0 = 0, M0p = 0, j = 0;
+  bool done = false, fresh = true;
+  std::vector<double*> v;
+  std::vector<const double*> ph;
+  std::vector<double> t, P;
+  std::vector<int> defl, I;
+  double ev_old = 1e12;
+  std::vector<std::unique_ptr<DevBuf>> owned;
+  std::vector<double*> free_list;
+  // device scalars of the step: [0..p0] multi-dot, [p0+1 .. p0+pc] pending-block overlaps,
+  // then the deflated-vector overlaps and T(j, j)
+  DevBuf sc;
+  double* scd = nullptr;
+  std::vector<double> sch;
+  // pending host reads from the previous step
+  int prev_j = -1, prev_nk = 0, prev_pc = 0, prev_nI = 0, nk = 0;
+  bool check_pending = false;
+  double z = 0.0;
+  std::vector<double> hcoef;  // host coefficients uploaded for phase 4
+  double& T(int a, int b) { return t[(size_t)b * M0 + a]; }
+  double* get() {
+    if (!free_list.empty()) {
+      double* p = free_list.back();
+      free_list.pop_back();
+      return p;
+    }
+    owned.emplace_back(new DevBuf(n * 8));
+    return owned.back()->as<double>();
+  }
+  void put(double* p) {
+    if (p) free_list.push_back(p);
+  }
+};
+
+constexpr int kSc = 4 * kBM + 8;  // device scalars per job and step
+
+}  // namespace
+
+void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<const double*>& phi,
+                        const std::vector<int>& p, int max_iter_BL, double accur_deflation,
+                        double accur_band_lanczos, std::vector<std::vector<double>>& evals,
+                        std::vector<std::vector<double>>& Q, BatchStats* stats) {
+  const int nj = (int)S.size();
+  evals.assign(nj, {});
+  Q.assign(nj, {});
+  std::vector<BandJob> J(nj);
+  std::vector<const qed_sector*> cs;
+  std::vector<int> hv_of(nj, -1);
+  for (int i = 0; i < nj; ++i) {
+    BandJob& b = J[i];
+    b.s = S[i];
+    b.n = S[i]->D;
+    b.p0 = b.pc = p[i];
+    if (b.n == 0 || b.p0 == 0) {
+      b.done = true;
+      continue;
+    }
+    if (b.p0 + 1 > kBM) fail("batched band Lanczos: too many start vectors");
+    int M0 = (int)(14 * b.p0 * std::log(1.0 * b.n));
+    if (M0 <= 1) fail("band Lanczos: too few iterations requested");
+    if (M0 > max_iter_BL) M0 = max_iter_BL;
+    b.M0p = (M0 / b.p0 + 1) * b.p0;
+    b.M0 = b.M0p - b.pc;
+    b.v.assign(b.M0p + 1, nullptr);
+    for (int k = 0; k < b.p0; ++k) b.ph.push_back(phi[i] + (uint64_t)k * b.n);
+    for (int k = 0; k < b.p0; ++k) {
+      b.v[k] = b.get();
+      QED_HIP(hipMemcpyAsync(b.v[k], b.ph[k], b.n * 8, hipMemcpyDeviceToDevice, stream()));
+    }
+    b.t.assign((size_t)b.M0 * b.M0, 0.0);
+    b.P.assign((size_t)b.p0 * b.M0, 0.0);
+    b.defl.assign(b.M0, 0);
+    b.sc.alloc(kSc * sizeof(double));
+    b.scd = b.sc.as<double>();
+    b.sch.assign(kSc, 0.0);
+    hv_of[i] = (int)cs.size();
+    cs.push_back(S[i]);
+  }
+  HvBatch HB;
+  if (!cs.empty()) hv_batch_prepare(HB, cs);
+  DevBuf dact(std::max<size_t>(1, cs.size()) * sizeof(int));
+  DevBuf part;
+  Table<MDot> TD;
+  Table<MAxpy> TA;
+  Table<MRev> TR;
+  Table<VFill> TF;
+  HvBatchConfig cfg;
+  bool tuned = false;
+  int steps = 0;
+  std::vector<int> act(cs.size(), 0);
+  auto run_mdot = [&]() {
+    if (TD.h.empty()) return;
+    MDot* d = TD.upload();
+    ensure(part, (size_t)TD.total * kBM * sizeof(double));
+    k_mdot_part<<<TD.total, kBT, 0, stream()>>>(d, (int)TD.h.size(), part.as<double>());
+    k_mdot_final<<<((int)TD.h.size() * kBM + 255) / 256, 256, 0, stream()>>>(d, (int)TD.h.size(), part.as<double>());
+  };
+  auto run_maxpy = [&]() {
+    if (TA.h.empty()) return;
+    MAxpy* d = TA.upload();
+    k_maxpy<<<TA.total, kBT, 0, stream()>>>(d, (int)TA.h.size());
+  };
+  auto run_mrev = [&]() {
+    if (TR.h.empty()) return;
+    MRev* d = TR.upload();
+    k_mrev<<<TR.total, kBT, 0, stream()>>>(d, (int)TR.h.size());
+  };
+  while (true) {
+    // ---- phase 1: norm of v[j] and overlaps with the start vectors (all live jobs) ----
+    TD.clear();
+    bool live = false;
+    for (auto& b : J) {
+      if (b.done) continue;
+      live = true;
+      if (b.j >= b.M0) continue;
+      if (!b.v[b.j]) fail("batched band Lanczos: missing Krylov vector");
+      MDot d{};
+      d.x = b.v[b.j];
+      d.y[0] = b.v[b.j];
+      for (int k = 0; k < b.p0; ++k) d.y[k + 1] = b.ph[k];
+      d.m = b.p0 + 1;
+      d.n = b.n;
+      d.out = b.scd;
+      TD.add(d, b.n);
+    }
+    if (!live) break;
+    run_mdot();
+    for (auto& b : J)
+      if (!b.done) QED_HIP(hipMemcpyAsync(b.sch.data(), b.scd, kSc * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    QED_HIP(hipStreamSynchronize(stream()));
+    ++steps;
+    // ---- host: complete the previous step's band entries and its eigenvalue test, then
+    //      take this step's deflation decision
+    std::vector<int> proceed(nj, 0);
+    parallel_for(nj, [&](int i) {
+      BandJob& b = J[i];
+      if (b.done) return;
+      const double* s = b.sch.data();
+      if (b.prev_j >= 0) {
+        // previous step: [kBM, kBM + nk) pending-block overlaps, [2 kBM, 2 kBM + nI)
+        // deflated-vector overlaps, [3 kBM] T(j, j); phase 1 wrote only [0, p0]
+        const int pj = b.prev_j;
+        for (int q = 0; q < b.prev_nk; ++q) {
+          const int kk = pj + 1 + q;
+          if (kk >= b.prev_pc) b.T(pj, kk - b.prev_pc) = s[kBM + q];
+        }
+        for (int q = 0; q < b.prev_nI; ++q) b.T(b.I[q], pj) = s[2 * kBM + q];
+        b.T(pj, pj) = s[3 * kBM];
+        for (int kk : b.I) b.T(pj, kk) = b.T(kk, pj);
+        b.prev_j = -1;
+        if (b.check_pending) {
+          b.check_pending = false;
+          std::vector<double> H((size_t)pj * pj), e(pj);
+          for (int c = 0; c < pj; ++c)
+            for (int r = 0; r < pj; ++r) H[(size_t)c * pj + r] = b.T(r, c);
+          sym_eigen(pj, H.data(), e.data(), false);
+          const double test = std::fabs(e[0] - b.ev_old);
+          b.ev_old = e[0];
+          if (test < accur_band_lanczos) {
+            b.j = pj;  // stop with the pj x pj projected matrix
+            b.done = true;
+            return;
+          }
+        }
+        if (b.j >= b.M0) {
+          b.done = true;
+          return;
+        }
+      }
+      const double z = std::sqrt(std::max(s[0], 0.0));
+      if (z < accur_deflation) {
+        if (b.j >= b.pc) {
+          b.I.push_back(b.j - b.pc);
+          b.defl[b.j - b.pc] = 1;
+        }
+        b.pc--;
+        if (b.pc == 0) {
+          b.done = true;
+          return;
+        }
+        b.put(b.v[b.j]);
+        for (int k = b.j; k < b.j + b.pc; ++k) b.v[k] = b.v[k + 1];
+        b.v[b.j + b.pc] = nullptr;
+        return;  // phase 1 again at the same j
+      }
+      if (b.j >= b.pc) b.T(b.j, b.j - b.pc) = z;
+      for (int k = 0; k < b.p0; ++k) b.P[(size_t)b.j * b.p0 + k] = s[k + 1] / z;
+      b.z = z;
+      proceed[i] = 1;
+    });
+    // ---- phase 2: v[j] /= z; pending block orthogonalized against v[j] ----
+    // scale: y = y * (1/z) with no sum (MAxpy with m = 0)
+    TA.clear();
+    for (int i = 0; i < nj; ++i) {
+      if (!proceed[i]) continue;
+      BandJob& b = J[i];
+      MAxpy a{};
+      a.y = b.v[b.j];
+      a.m = 0;
+      a.sgn = 1.0;
+      a.scale = 1.0 / b.z;
+      a.n = b.n;
+      TA.add(a, b.n);
+    }
+    run_maxpy();
+    TD.clear();
+    TR.clear();
+    for (int i = 0; i < nj; ++i) {
+      if (!proceed[i]) continue;
+      BandJob& b = J[i];
+      const int nk = b.pc - 1;
+      b.nk = nk;
+      if (nk <= 0) continue;
+      MDot d{};
+      d.x = b.v[b.j];
+      for (int q = 0; q < nk; ++q) d.y[q] = b.v[b.j + 1 + q];
+      d.m = nk;
+      d.n = b.n;
+      d.out = b.scd + kBM;
+      TD.add(d, b.n);
+      MRev r{};
+      r.x = b.v[b.j];
+      for (int q = 0; q < nk; ++q) {
+        r.y[q] = b.v[b.j + 1 + q];
+        r.c[q] = b.scd + kBM + q;
+      }
+      r.m = nk;
+      r.n = b.n;
+      TR.add(r, b.n);
+    }
+    run_mdot();
+    run_mrev();
+    // ---- phase 3: w = H v[j] ----
+    TF.clear();
+    std::vector<double*> w(nj, nullptr);
+    for (int i = 0; i < nj; ++i) {
+      BandJob& b = J[i];
+      if (proceed[i]) {
+        w[i] = b.get();
+        TF.add(VFill{w[i], b.n, 0}, b.n);
+      }
+    }
+    if (!TF.h.empty()) {
+      VFill* d = TF.upload();
+      k_fill0<<<TF.total, kBT, 0, stream()>>>(d, (int)TF.h.size());
+    }
+    for (size_t q = 0; q < cs.size(); ++q) act[q] = 0;
+    for (int i = 0; i < nj; ++i) {
+      if (!proceed[i]) continue;
+      const int h = hv_of[i];
+      act[h] = 1;
+      HB.host[h].x[0] = J[i].v[J[i].j];
+      HB.host[h].y[0] = w[i];
+    }
+    if (!cs.empty()) {
+      hv_batch_upload(HB);
+      QED_HIP(hipMemcpyAsync(dact.p, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, stream()));
+      if (!tuned) {
+        // tune on scratch outputs (the trial products must not land in w)
+        std::vector<std::unique_ptr<DevBuf>> sy(cs.size());
+        HvBatch tb;
+        hv_batch_prepare(tb, cs);
+        for (size_t q = 0; q < cs.size(); ++q) {
+          sy[q].reset(new DevBuf(cs[q]->D * 8));
+          tb.host[q].x[0] = tb.host[q].x[1] = HB.host[q].x[0] ? HB.host[q].x[0] : sy[q]->as<double>();
+          tb.host[q].y[0] = tb.host[q].y[1] = sy[q]->as<double>();
+        }
+        hv_batch_upload(tb);
+        cfg = hv_batch_tune(tb, dact.as<int>());
+        QED_HIP(hipStreamSynchronize(stream()));
+        tuned = true;
+        if (stats) stats->hv_config = cfg;
+      }
+      hv_batch(HB, cfg, 0, dact.as<int>());
+    }
+    // ---- phase 4: w -= sum_k T(k, j) v[k] over the band; release the oldest vector ----
+    TA.clear();
+    for (int i = 0; i < nj; ++i) {
+      if (!proceed[i]) continue;
+      BandJob& b = J[i];
+      int k0 = 0;
+      if (b.j > b.pc) {
+        k0 = b.j - b.pc;
+        if (b.defl[k0 - 1] == 0) {
+          b.put(b.v[k0 - 1]);
+          b.v[k0 - 1] = nullptr;
+        }
+      }
+      // T(k, j) = T(j, k): k = j - pc is z; k = j + 1 + q - pc (0 <= q < nk) is the q-th
+      // pending-block overlap of phase 2, still on the device; other entries are on the host
+      MAxpy a{};
+      a.y = w[i];
+      a.sgn = -1.0;
+      a.scale = 1.0;
+      a.n = b.n;
+      int m = 0;
+      b.hcoef.clear();
+      std::vector<int> hslot;
+      for (int k = k0; k < b.j; ++k) {
+        if (m >= kBM) fail("batched band Lanczos: band too wide");
+        a.x[m] = b.v[k];
+        const int q = k + b.pc - b.j - 1;
+        if (k == b.j - b.pc) {
+          hslot.push_back(m);
+          b.hcoef.push_back(b.z);
+        } else if (q >= 0 && q < b.nk) {
+          a.c[m] = b.scd + kBM + q;
+        } else {
+          hslot.push_back(m);
+          b.hcoef.push_back(b.T(b.j, k));
+        }
+        ++m;
+      }
+      a.m = m;
+      if (!b.hcoef.empty()) {
+        double* slot = b.scd + 3 * kBM + 8;
+        QED_HIP(hipMemcpyAsync(slot, b.hcoef.data(), b.hcoef.size() * 8, hipMemcpyHostToDevice, stream()));
+        for (size_t u = 0; u < hslot.size(); ++u) a.c[hslot[u]] = slot + u;
+      }
+      if (m > 0) TA.add(a, b.n);
+    }
+    run_maxpy();
+    // ---- phase 5: deflated vectors, one at a time (w -= <v_k, w> v_k) ----
+    int maxI = 0;
+    for (int i = 0; i < nj; ++i)
+      if (proceed[i]) maxI = std::max(maxI, (int)J[i].I.size());
+    for (int u = 0; u < maxI; ++u) {
+      TD.clear();
+      TA.clear();
+      for (int i = 0; i < nj; ++i) {
+        if (!proceed[i] || u >= (int)J[i].I.size()) continue;
+        BandJob& b = J[i];
+        MDot d{};
+        d.x = b.v[b.I[u]];
+        d.y[0] = w[i];
+        d.m = 1;
+        d.n = b.n;
+        d.out = b.scd + 2 * kBM + u;
+        TD.add(d, b.n);
+        MAxpy a{};
+        a.y = w[i];
+        a.x[0] = b.v[b.I[u]];
+        a.c[0] = b.scd + 2 * kBM + u;
+        a.sgn = -1.0;
+        a.scale = 1.0;
+        a.m = 1;
+        a.n = b.n;
+        TA.add(a, b.n);
+      }
+      run_mdot();
+      run_maxpy();
+    }
+    // ---- phase 6: T(j, j) = <v_j, w>; w -= T(j, j) v_j ----
+    TD.clear();
+    TA.clear();
+    for (int i = 0; i < nj; ++i) {
+      if (!proceed[i]) continue;
+      BandJob& b = J[i];
+      if ((int)b.I.size() > kBM) fail("batched band Lanczos: too many deflations");
+      MDot d{};
+      d.x = b.v[b.j];
+      d.y[0] = w[i];
+      d.m = 1;
+      d.n = b.n;
+      d.out = b.scd + 3 * kBM;
+      TD.add(d, b.n);
+      MAxpy a{};
+      a.y = w[i];
+      a.x[0] = b.v[b.j];
+      a.c[0] = b.scd + 3 * kBM;
+      a.sgn = -1.0;
+      a.scale = 1.0;
+      a.m = 1;
+      a.n = b.n;
+      TA.add(a, b.n);
+    }
+    run_mdot();
+    run_maxpy();
+    for (int i = 0; i < nj; ++i) {
+      if (!proceed[i]) continue;
+      BandJob& b = J[i];
+      b.v[b.j + b.pc] = w[i];
+      b.prev_j = b.j;
+      b.prev_pc = b.pc;
+      b.prev_nk = b.nk;
+      b.prev_nI = (int)b.I.size();
+      b.check_pending = (b.j >= 4 * b.p0 && b.j % b.p0 == 0);
+      ++b.j;
+    }
+    // (the next phase 1 reads this step's device scalars, completes T and runs the test)
+  }
+  QED_HIP(hipStreamSynchronize(stream()));
+  if (stats) stats->band_steps = steps;
+  // projected matrices: eigen-decomposition and Q = P U (host, in parallel)
+  parallel_for(nj, [&](int i) {
+    BandJob& b = J[i];
+    const int jj = b.j;
+    if (b.n == 0 || b.p0 == 0 || jj == 0) return;
+    std::vector<double> H((size_t)jj * jj);
+    for (int c = 0; c < jj; ++c)
+      for (int r = 0; r < jj; ++r) H[(size_t)c * jj + r] = b.T(r, c);
+    evals[i].assign(jj, 0.0);
+    sym_eigen(jj, H.data(), evals[i].data(), true);
+    Q[i].assign((size_t)b.p0 * jj, 0.0);
+    for (int c = 0; c < jj; ++c)
+      for (int m = 0; m < jj; ++m) {
+        const double u = H[(size_t)c * jj + m];
+        if (u == 0.0) continue;
+        for (int k = 0; k < b.p0; ++k) Q[i][(size_t)c * b.p0 + k] += b.P[(size_t)m * b.p0 + k] * u;
+      }
+  });
+}
+
+}  // namespace qed

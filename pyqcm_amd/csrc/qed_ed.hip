@@ -445,6 +445,23 @@ void one_body_solve(Instance& I) {
   I.gs_string = "uncorrelated";
 }
 
+void finish_states(Instance& I, std::vector<std::unique_ptr<State>>& states);
+
+// keeps the states of one sector within max_gap of the lowest energy seen so far
+void add_states(Instance& I, int nup, int ndn, const std::vector<double>& evals, std::vector<DevBuf>& vecs,
+                double max_gap, std::vector<std::unique_ptr<State>>& states) {
+  if (evals[0] < I.GS_energy) I.GS_energy = evals[0];
+  for (size_t k = 0; k < evals.size(); ++k) {
+    if (evals[k] - I.GS_energy > max_gap) continue;
+    auto st = std::make_unique<State>();
+    st->nup = nup;
+    st->ndn = ndn;
+    st->energy = evals[k];
+    st->psi = std::move(vecs[k]);
+    states.push_back(std::move(st));
+  }
+}
+
 void low_energy_states(Instance& I) {
   if (I.gs_solved) return;
   if (!I.correlated) {
@@ -495,18 +512,16 @@ void low_energy_states(Instance& I) {
       fail(std::string(e.what()) + "\n  in cluster model '" + I.model->name + "', instance '" + I.model->name +
            "_" + std::to_string(I.label) + "', sector '" + sector_name(N, S) + "'");
     }
-    if (evals[0] < I.GS_energy) I.GS_energy = evals[0];
-    for (size_t k = 0; k < evals.size(); ++k) {
-      if (evals[k] - I.GS_energy > max_gap) continue;
-      auto st = std::make_unique<State>();
-      st->nup = nup;
-      st->ndn = ndn;
-      st->energy = evals[k];
-      st->psi = std::move(vecs[k]);
-      states.push_back(std::move(st));
-    }
+    add_states(I, nup, ndn, evals, vecs, max_gap, states);
   }
-  // weights (compute_weights, model_instance.hpp:356-417)
+  finish_states(I, states);
+}
+
+// weights and selection of the low-energy states (compute_weights, model_instance.hpp:356-417)
+void finish_states(Instance& I, std::vector<std::unique_ptr<State>>& states) {
+  const double temperature = gd("temperature");
+  double max_gap = -std::log(gd("minimum_weight")) * temperature;
+  if (max_gap < MIN_GAP) max_gap = MIN_GAP;
   if (temperature < MIDDLE_VALUE) {
     double e0 = 1e12;
     for (auto& s : states) e0 = std::min(e0, s->energy);
@@ -569,6 +584,164 @@ void green_function_solve(Instance& I) {
     }
   }
   I.gf_solved = true;
+}
+
+// ---- batched solve of many instances -------------------------------------------------
+// Same results as low_energy_states + green_function_solve applied to each instance in
+// turn (start vectors are drawn in the same order), with the Krylov work of all
+// instances batched: one lockstep Lanczos for every (instance, sector) ground state and
+// one band Lanczos pass for every (instance, state, spin, particle/hole) Green function
+// sector. Instances that need another method (Davidson, several states per sector,
+// continued fractions) are solved one by one.
+void batch_solve(std::vector<Instance*>& insts, bool graph, int chunk, BatchStats* stats) {
+  struct GS {
+    Instance* I;
+    int nup, ndn;
+    SectorPtr H;
+    DevBuf v;
+  };
+  std::vector<GS> gs;
+  std::vector<Instance*> batched;
+  for (Instance* I : insts) {
+    if (I->gs_solved) continue;
+    const bool plain = I->correlated && gc("Ground_state_method") == 'L' && std::max(1, gi("n_states")) == 1;
+    if (!plain) {
+      low_energy_states(*I);
+      continue;
+    }
+    I->gs_solved = true;
+    I->GS_energy = 1.0e12;
+    batched.push_back(I);
+    for (auto& sec : I->sectors) {
+      const int N = sec.first, S = sec.second;
+      const int nup = (N + S) / 2, ndn = (N - S) / 2;
+      if (!sector_valid(*I, nup, ndn)) continue;
+      GS g{I, nup, ndn, make_sector(*I, nup, ndn), DevBuf()};
+      if (g.H->D == 0) continue;
+      g.v.alloc(g.H->D * 8);
+      if (qed_start_vector(g.H.get(), gi("seed"), g.v.as<double>()) != 0) fail(qed_last_error());
+      gs.push_back(std::move(g));
+    }
+  }
+  {
+    std::vector<qed_sector*> S;
+    std::vector<double*> psi;
+    for (auto& g : gs) {
+      S.push_back(g.H.get());
+      psi.push_back(g.v.as<double>());
+    }
+    std::vector<double> E;
+    std::vector<int> nit;
+    lanczos_gs_batch(S, psi, gd("accur_lanczos"), gi("max_iter_lanczos"), graph, chunk, E, nit, stats);
+    const double temperature = gd("temperature");
+    double max_gap = -std::log(gd("minimum_weight")) * temperature;
+    if (max_gap < MIN_GAP) max_gap = MIN_GAP;
+    std::map<Instance*, std::vector<std::unique_ptr<State>>> states;
+    for (size_t k = 0; k < gs.size(); ++k) {
+      std::vector<double> ev{E[k]};
+      std::vector<DevBuf> vv;
+      vv.push_back(std::move(gs[k].v));
+      add_states(*gs[k].I, gs[k].nup, gs[k].ndn, ev, vv, max_gap, states[gs[k].I]);
+    }
+    for (Instance* I : batched) finish_states(*I, states[I]);
+  }
+  gs.clear();
+  // Green functions
+  struct GF {
+    Instance* I;
+    State* st;
+    int spin, part;  // part 0: holes (pm = -1), 1: particles
+    SectorPtr T;
+    DevBuf phi;
+  };
+  std::vector<GF> gf;
+  for (Instance* I : insts) {
+    if (I->gf_solved || !I->correlated) continue;
+    I->cf = gc("GF_method") == 'F' || gb("continued_fraction");
+    if (I->cf) {
+      green_function_solve(*I);
+      continue;
+    }
+    Model& M = *I->model;
+    const int p = M.n_sites, L = M.L;
+    for (auto& stp : I->states) {
+      State& st = *stp;
+      for (int spin = 0; spin < ((I->mixing & 4) ? 2 : 1); ++spin)
+        for (int part = 0; part < 2; ++part) {
+          const int pm = 2 * part - 1;
+          const int tnup = st.nup + (spin == 0 ? pm : 0), tndn = st.ndn + (spin == 1 ? pm : 0);
+          GF g{I, &st, spin, part, SectorPtr(), DevBuf()};
+          if (sector_valid(*I, tnup, tndn)) {
+            g.T = make_sector(*I, tnup, tndn);
+            if (g.T->D > 0) {
+              g.phi.alloc((size_t)p * g.T->D * 8);
+              QED_HIP(hipMemsetAsync(g.phi.p, 0, (size_t)p * g.T->D * 8, stream()));
+              for (int i = 0; i < p; ++i)
+                if (qed_apply_c(L, st.nup, st.ndn, i + spin * L, pm, 1.0, st.psi.as<double>(),
+                                g.phi.as<double>() + (uint64_t)i * g.T->D) != 0)
+                  fail(qed_last_error());
+            }
+          }
+          gf.push_back(std::move(g));
+        }
+    }
+  }
+  std::vector<QMat> res(gf.size());
+  std::vector<qed_sector*> S;
+  std::vector<const double*> phi;
+  std::vector<int> pv, which;
+  for (size_t k = 0; k < gf.size(); ++k) {
+    GF& g = gf[k];
+    if (!g.T || g.T->D == 0) continue;
+    const int p = g.I->model->n_sites;
+    if (g.T->D < (uint64_t)gi("max_dim_full")) {
+      res[k] = sector_qmatrix(*g.T, g.phi.as<double>(), p);
+      continue;
+    }
+    S.push_back(g.T.get());
+    phi.push_back(g.phi.as<double>());
+    pv.push_back(p);
+    which.push_back((int)k);
+  }
+  if (!S.empty()) {
+    std::vector<std::vector<double>> ev, Q;
+    band_lanczos_batch(S, phi, pv, gi("max_iter_BL"), gd("accur_deflation"), gd("accur_band_lanczos"), ev, Q,
+                       stats);
+    for (size_t u = 0; u < which.size(); ++u) {
+      QMat& q = res[which[u]];
+      q.L = pv[u];
+      q.e = std::move(ev[u]);
+      q.Q = std::move(Q[u]);
+    }
+  }
+  // assemble per (instance, state, spin) exactly as build_qmatrix does
+  for (size_t k = 0; k < gf.size(); k += 2) {
+    GF& h = gf[k];
+    QMat parts[2];
+    for (int part = 0; part < 2; ++part) {
+      GF& g = gf[k + part];
+      if (!g.T || g.T->D == 0) continue;
+      QMat q = std::move(res[k + part]);
+      q.L = g.I->model->n_sites;
+      for (double& e : q.e) {
+        e -= g.st->energy;
+        if (part == 0) e = -e;
+      }
+      parts[part] = streamline(q);
+    }
+    QMat tot;
+    tot.L = h.I->model->n_sites;
+    append(tot, parts[0]);
+    append(tot, parts[1]);
+    tot.L = h.I->model->n_sites;
+    check_norm(tot, gd("accur_Q_matrix"));
+    const double sw = std::sqrt(h.st->weight);
+    for (double& v : tot.Q) v *= sw;
+    h.st->gf[h.spin] = tot;
+  }
+  for (Instance* I : insts)
+    if (I->correlated && !I->cf) I->gf_solved = true;
+  QED_HIP(hipStreamSynchronize(stream()));
 }
 
 int spin_index(const Instance& I, int spin_down) {
@@ -814,6 +987,67 @@ int qed_ed_ground_state_solve(uint64_t label, double* E0, char* buf, int buflen)
 int qed_ed_green_function_solve(uint64_t label) {
   std::lock_guard<std::recursive_mutex> lk(lib_mutex());
   return guard([&] { green_function_solve(inst(label)); });
+}
+
+int qed_ed_batch_solve_instances(int n, const uint64_t* labels, int graph, int chunk, int* stats) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    std::vector<Instance*> v;
+    for (int i = 0; i < n; ++i) v.push_back(&inst(labels[i]));
+    BatchStats st;
+    batch_solve(v, graph != 0, chunk > 0 ? chunk : 16, &st);
+    if (stats) {
+      stats[0] = st.lanczos_iterations;
+      stats[1] = st.band_steps;
+      stats[2] = st.hv_config.block;
+      stats[3] = (int)st.hv_config.grid;
+    }
+  });
+}
+
+int qed_batch_solve(const char* model, int nparam, const char* const* names, int nb, const double* params,
+                    const char* const* sectors, int graph, int cap, double* E0, int* Mout, double* e,
+                    double* Q) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    auto it = models().find(model);
+    if (it == models().end()) fail(std::string("The model ") + model + " is not defined");
+    const int d = it->second->n_sites;
+    // temporary instances under labels no user instance takes
+    const uint64_t base = 0xB000000000000000ull;
+    std::vector<uint64_t> labels(nb);
+    for (int b = 0; b < nb; ++b) {
+      labels[b] = base + (uint64_t)b;
+      instances().erase(labels[b]);
+      if (qed_ed_new_model_instance(model, nparam, names, params + (size_t)b * nparam, sectors[b], labels[b]) != 0)
+        fail(qed_last_error());
+    }
+    struct Cleanup {
+      std::vector<uint64_t>& l;
+      ~Cleanup() {
+        for (uint64_t x : l) instances().erase(x);
+      }
+    } cleanup{labels};
+    std::vector<Instance*> v;
+    for (uint64_t x : labels) v.push_back(&inst(x));
+    BatchStats st;
+    batch_solve(v, graph != 0, 16, &st);
+    for (int b = 0; b < nb; ++b) {
+      Instance& I = *v[b];
+      E0[b] = I.GS_energy;
+      int m = 0;
+      for (auto& s : I.states) {
+        const QMat& q = s->gf[0];
+        for (int k = 0; k < q.M(); ++k) {
+          if (m >= cap) fail("qed_batch_solve: more poles than the output capacity");
+          e[(size_t)b * cap + m] = q.e[k];
+          for (int i = 0; i < d; ++i) Q[((size_t)b * cap + m) * d + i] = q.Q[i + (size_t)k * d];
+          ++m;
+        }
+      }
+      Mout[b] = m;
+    }
+  });
 }
 
 int qed_ed_green_function(uint64_t label, int nz, const double* z, int spin_down, double* Gout) {

@@ -193,17 +193,18 @@ __global__ void k_ell_rowmajor(const uint32_t* ptr, const uint32_t* ent, uint32_
 // the workgroup; 8 neighbour rows are requested before any of them is consumed so that
 // each wave keeps 8 row loads in flight.
 constexpr int kG = 8;
+
+// Row r of y += H x for one sector: the body shared by the single-sector kernel and the
+// batched multi-sector kernel. smem: nvu amplitudes, then (LDSX) the row X[r, :].
 template <int BS, bool LDSX>
-__global__ __launch_bounds__(BS) void k_hv_ell(HvView v, const uint32_t* __restrict__ up_ell,
-                                               uint32_t Wu, const uint32_t* __restrict__ dn_ell,
-                                               uint32_t Wd, const double* __restrict__ x,
-                                               double* __restrict__ y) {
-  extern __shared__ double smem[];
+__device__ __forceinline__ void ell_row(const HvView& v, const uint32_t* __restrict__ up_ell,
+                                        uint32_t Wu, const uint32_t* __restrict__ dn_ell,
+                                        uint32_t Wd, const double* __restrict__ x,
+                                        double* __restrict__ y, uint32_t r, double* smem) {
   const uint32_t du = v.du;
   const uint32_t nvu = v.up_nvals;
   double* vals_s = smem;
   double* xs = smem + nvu;
-  const uint32_t r = blockIdx.x;
   const double* xr = x + (uint64_t)r * du;
   for (uint32_t i = threadIdx.x; i < nvu; i += BS) vals_s[i] = v.up_vals[i];
   if (LDSX)
@@ -238,6 +239,45 @@ __global__ __launch_bounds__(BS) void k_hv_ell(HvView v, const uint32_t* __restr
       for (int q = 0; q < kG; ++q) acc2 += v.dn_vals[e[q] >> kTargetBits] * xv[q];
     }
     yr[c] += acc + acc2;
+  }
+}
+
+// H.v, one workgroup per down row r. Up-spin hops: ELL entries (coalesced per lane) gather
+// from the row staged in LDS. Down-spin hops: the row's neighbour list is uniform across
+// the workgroup; 8 neighbour rows are requested before any of them is consumed so that
+// each wave keeps 8 row loads in flight.
+template <int BS, bool LDSX>
+__global__ __launch_bounds__(BS) void k_hv_ell(HvView v, const uint32_t* __restrict__ up_ell,
+                                               uint32_t Wu, const uint32_t* __restrict__ dn_ell,
+                                               uint32_t Wd, const double* __restrict__ x,
+                                               double* __restrict__ y) {
+  extern __shared__ double smem[];
+  ell_row<BS, LDSX>(v, up_ell, Wu, dn_ell, Wd, x, y, blockIdx.x, smem);
+}
+
+// Batched H.v over many sectors ("jobs") in one persistent launch: the rows of all jobs
+// form one tile list (job j owns tiles row0[j] .. row0[j] + dd_j - 1); a fixed grid walks
+// it with a stride. Jobs whose active flag is 0 are skipped. Parity selects which of the
+// job's two vector pairs is (x, y), so that an alternating recurrence needs only two job
+// tables.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_hv_ell_batch(const HvJob* __restrict__ jobs, int njobs,
+                                                     uint32_t ntiles, int parity,
+                                                     const int* __restrict__ active) {
+  extern __shared__ double smem[];
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {  // last job with row0 <= t
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].row0 <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const HvJob& J = jobs[lo];
+    if (active[lo]) {
+      ell_row<BS, true>(J.v, J.up_ell, J.Wu, J.dn_ell, J.Wd, J.x[parity], J.y[parity], t - J.row0,
+                        smem);
+    }
+    __syncthreads();
   }
 }
 
@@ -508,6 +548,111 @@ void hv(const qed_sector& s, const double* x, double* y, int variant) {
     const uint64_t groups = ((nchunks + 7) / 8) * 8ull * nrb;
     k_hv_down_chunks<W><<<(uint32_t)groups, 256, 0, st>>>(v, x, y, nchunks, rpb, nrb);
   }
+  QED_HIP(hipGetLastError());
+}
+
+
+// ---- batched launch with autotuned (block size, grid) ------------------------------------
+namespace {
+struct TuneKey {
+  std::vector<uint64_t> sig;
+  bool operator<(const TuneKey& o) const { return sig < o.sig; }
+};
+std::map<TuneKey, HvBatchConfig>& tune_cache() {
+  static std::map<TuneKey, HvBatchConfig> c;
+  return c;
+}
+
+void launch_batch(const HvBatch& B, const HvBatchConfig& cfg, int parity, const int* active,
+                  hipStream_t st) {
+  const uint32_t grid = cfg.grid ? std::min<uint32_t>(cfg.grid, B.ntiles) : B.ntiles;
+  if (cfg.block == 1024)
+    k_hv_ell_batch<1024><<<grid, 1024, B.lds, st>>>(B.jobs, B.njobs, B.ntiles, parity, active);
+  else if (cfg.block == 512)
+    k_hv_ell_batch<512><<<grid, 512, B.lds, st>>>(B.jobs, B.njobs, B.ntiles, parity, active);
+  else
+    k_hv_ell_batch<256><<<grid, 256, B.lds, st>>>(B.jobs, B.njobs, B.ntiles, parity, active);
+}
+}  // namespace
+
+void hv_batch_prepare(HvBatch& B, const std::vector<const qed_sector*>& S) {
+  std::vector<HvJob> h(S.size());
+  uint32_t row = 0;
+  size_t lds = 0;
+  for (size_t j = 0; j < S.size(); ++j) {
+    const qed_sector& s = *S[j];
+    HvJob& J = h[j];
+    J.v = view(s);
+    J.up_ell = s.up_ell.as<uint32_t>();
+    J.dn_ell = s.dn_ell.as<uint32_t>();
+    J.Wu = s.up_w;
+    J.Wd = s.dn_w;
+    J.row0 = row;
+    J.x[0] = J.x[1] = nullptr;
+    J.y[0] = J.y[1] = nullptr;
+    row += s.dd;
+    lds = std::max(lds, ((size_t)s.du + s.up_nvals) * sizeof(double));
+  }
+  if (lds > lds_budget()) fail("batched H.v: a sector row does not fit in LDS");
+  B.host = h;
+  B.njobs = (int)S.size();
+  B.ntiles = row;
+  B.lds = lds;
+  B.dev.alloc(std::max<size_t>(1, h.size()) * sizeof(HvJob) * 2);
+  B.jobs = B.dev.as<HvJob>();
+  B.sig.clear();
+  for (const qed_sector* s : S) {
+    B.sig.push_back(s->D);
+    B.sig.push_back(s->up_w | ((uint64_t)s->dn_w << 32));
+  }
+}
+
+void hv_batch_upload(HvBatch& B) {
+  QED_HIP(hipMemcpyAsync(B.jobs, B.host.data(), B.host.size() * sizeof(HvJob), hipMemcpyHostToDevice,
+                         stream()));
+}
+
+HvBatchConfig hv_batch_tune(const HvBatch& B, const int* active, bool force) {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    const int bytes = (int)lds_budget();
+    (void)hipFuncSetAttribute((const void*)k_hv_ell_batch<256>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void*)k_hv_ell_batch<512>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void*)k_hv_ell_batch<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  });
+  TuneKey key{B.sig};
+  auto it = tune_cache().find(key);
+  if (it != tune_cache().end() && !force) return it->second;
+  // candidates: block size x persistent grid (multiples of the CU count) or one tile per block
+  const int cus = device_cu_count();
+  std::vector<HvBatchConfig> cand;
+  for (int bs : {256, 512, 1024})
+    for (int mult : {0, 2, 4, 8}) cand.push_back({bs, mult ? (uint32_t)(cus * mult) : 0u, 0.0f});
+  hipEvent_t e0, e1;
+  QED_HIP(hipEventCreate(&e0));
+  QED_HIP(hipEventCreate(&e1));
+  HvBatchConfig best = cand[0];
+  best.ms = 1e30f;
+  for (auto& c : cand) {
+    launch_batch(B, c, 0, active, stream());  // warm-up (y accumulates; callers pass scratch y)
+    QED_HIP(hipEventRecord(e0, stream()));
+    for (int r = 0; r < 3; ++r) launch_batch(B, c, 0, active, stream());
+    QED_HIP(hipEventRecord(e1, stream()));
+    QED_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    QED_HIP(hipEventElapsedTime(&ms, e0, e1));
+    c.ms = ms / 3.0f;
+    if (c.ms < best.ms) best = c;
+  }
+  QED_HIP(hipEventDestroy(e0));
+  QED_HIP(hipEventDestroy(e1));
+  QED_HIP(hipGetLastError());
+  tune_cache()[key] = best;
+  return best;
+}
+
+void hv_batch(const HvBatch& B, const HvBatchConfig& cfg, int parity, const int* active) {
+  launch_batch(B, cfg, parity, active, stream());
   QED_HIP(hipGetLastError());
 }
 

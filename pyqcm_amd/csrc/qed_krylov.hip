@@ -160,6 +160,8 @@ uint32_t stream_blocks(uint64_t n) {
   return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(b, 1), (uint64_t)device_cu_count() * 16);
 }
 
+}  // namespace
+
 // lowest eigenvalue of the Lanczos tridiagonal and its normalized eigenvector
 // (QL eigenvalues, then inverse iteration for the vector)
 void tridiag_lowest(const std::vector<double>& a, const std::vector<double>& b, double* e0,
@@ -219,6 +221,8 @@ void tridiag_lowest(const std::vector<double>& a, const std::vector<double>& b, 
   if (vec[0] < 0)
     for (double& v : vec) v = -v;
 }
+
+namespace {
 
 }  // namespace
 

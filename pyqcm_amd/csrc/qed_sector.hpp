@@ -69,6 +69,60 @@ struct HvView {
 
 HvView view(const qed_sector& s);
 
+// one sector of a batched H.v: y[parity] += H x[parity]
+struct HvJob {
+  HvView v;
+  const uint32_t* up_ell;
+  const uint32_t* dn_ell;
+  uint32_t Wu, Wd;
+  uint32_t row0;  // first tile (down row) of this job in the batch
+  const double* x[2];
+  double* y[2];
+};
+
+struct HvBatch {
+  std::vector<HvJob> host;  // fill x / y, then hv_batch_upload
+  DevBuf dev;
+  HvJob* jobs = nullptr;
+  int njobs = 0;
+  uint32_t ntiles = 0;
+  size_t lds = 0;
+  std::vector<uint64_t> sig;  // shape signature (autotuner cache key)
+};
+
+struct HvBatchConfig {
+  int block = 256;
+  uint32_t grid = 0;  // 0: one workgroup per tile; else a persistent grid of this size
+  float ms = 0.0f;
+};
+
+void hv_batch_prepare(HvBatch& B, const std::vector<const qed_sector*>& S);
+void hv_batch_upload(HvBatch& B);
+// times the candidate launch shapes on this batch (y[0] of every job receives the sums of
+// the trial products: pass scratch vectors) and caches the fastest by shape signature
+HvBatchConfig hv_batch_tune(const HvBatch& B, const int* active, bool force = false);
+void hv_batch(const HvBatch& B, const HvBatchConfig& cfg, int parity, const int* active);
+
+struct BatchStats {
+  HvBatchConfig hv_config;
+  int lanczos_iterations = 0;  // lockstep iterations of the ground-state pass
+  int band_steps = 0;          // synchronized steps of the band Lanczos pass
+};
+
+// Lanczos ground states of many sectors in lockstep; psi[j]: start vector in, ground
+// state out; the stopping test runs every `chunk` iterations (graph: replay a captured
+// two-iteration hipGraph).
+void lanczos_gs_batch(const std::vector<qed_sector*>& S, const std::vector<double*>& psi, double accur,
+                      int max_iter, bool use_graph, int chunk, std::vector<double>& E0,
+                      std::vector<int>& niter, BatchStats* stats);
+// band Lanczos of many sectors (start vectors phi[j]: p[j] x D_j on the device)
+void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<const double*>& phi,
+                        const std::vector<int>& p, int max_iter_BL, double accur_deflation,
+                        double accur_band_lanczos, std::vector<std::vector<double>>& evals,
+                        std::vector<std::vector<double>>& Q, BatchStats* stats);
+void tridiag_lowest(const std::vector<double>& a, const std::vector<double>& b, double* e0,
+                    std::vector<double>& vec);
+
 // y += H x on the library stream (device vectors). variant -1 chooses automatically.
 void hv(const qed_sector& s, const double* x, double* y, int variant = -1);
 // y = alpha*y + H x, with <x, y_out> accumulated into dot_partial when non-null:

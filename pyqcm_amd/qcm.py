@@ -93,6 +93,18 @@ def ground_state_solve(label=0):
     return float(e[0]), buf.value.decode()
 
 
+def batch_solve(labels, graph=True, chunk=16):
+    """Ground states and Green functions of several instances in one batched device solve
+    (same results as solving them one after the other). Returns the solver statistics
+    (lockstep Lanczos iterations, band Lanczos steps, H.v block size, H.v grid)."""
+    labels = np.ascontiguousarray(labels, np.uint64)
+    st = np.zeros(4, np.int32)
+    call("qed_ed_batch_solve_instances", len(labels),
+         labels.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), int(bool(graph)), int(chunk), ip(st))
+    return {"lanczos_iterations": int(st[0]), "band_steps": int(st[1]), "hv_block": int(st[2]),
+            "hv_grid": int(st[3])}
+
+
 def Green_function_solveC(label=0):
     call("qed_ed_green_function_solve", int(label))
 

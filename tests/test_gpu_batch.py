@@ -1,0 +1,105 @@
+"""Batched impurity solves (many cluster+bath instances in one device solve) against the
+same instances solved one after the other."""
+import ctypes
+import itertools
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+_names = itertools.count(1)
+
+
+def _setup(n, seed):
+    from pyqcm_amd import cdmft as C
+    M = C.plaquette_8b_model("plaq_batch_%d" % next(_names))
+    cases = C.random_plaquette_instances(n, seed)
+    return C, M, cases
+
+
+def _values(C, vals, x):
+    v = dict(vals)
+    v.update(zip(C.PLAQUETTE_VARIA, x))
+    return v
+
+
+def test_batch_solve_matches_sequential(gpu_lib):
+    """6 instances with uneven sectors (N = 8..16): E0 within 1e-10 relative, same
+    ground-state sectors, G(i w) within 1e-7 (the band Lanczos stopping step is decided at
+    round-off level and may differ by one block between the two paths)."""
+    from pyqcm_amd import _lib, qcm
+    C, M, cases = _setup(6, 21)
+    zs = 1j * np.array([0.1, 0.5, 1.5, 5.0])
+    seq, bat = [], []
+    for vals, x in cases:
+        v = _values(C, vals, x)
+        seq.append(M.instance(v, C.plaquette_sectors(vals)))
+        bat.append(M.instance(v, C.plaquette_sectors(vals)))
+    _lib.call("qed_reset_start_vector_state")
+    Gs = []
+    for I in seq:
+        I.ground_state()
+        Gs.append(qcm.Green_function(zs, 0, I.label))
+    _lib.call("qed_reset_start_vector_state")
+    stats = qcm.batch_solve([I.label for I in bat], graph=True)
+    print("batch stats", stats)
+    for I, J, G in zip(seq, bat, Gs):
+        E, sec = I.ground_state()
+        Eb, secb = qcm.ground_state_solve(J.label)
+        assert secb == sec
+        assert abs(Eb - E) <= 1e-10 * abs(E), (E, Eb)
+        Gb = qcm.Green_function(zs, 0, J.label)
+        err = np.abs(Gb - G).max()
+        print(sec, "E0 %.12f  |dE| %.2e  max|dG| %.2e" % (E, abs(Eb - E), err))
+        assert err <= 1e-7 * max(1.0, np.abs(G).max())
+    for I in seq + bat:
+        I.erase()
+
+
+def test_batch_graph_replay_equals_direct_launches(gpu_lib):
+    from pyqcm_amd import _lib, qcm
+    C, M, cases = _setup(3, 5)
+    res = []
+    for graph in (True, False):
+        insts = [M.instance(_values(C, v, x), C.plaquette_sectors(v)) for v, x in cases]
+        _lib.call("qed_reset_start_vector_state")
+        qcm.batch_solve([I.label for I in insts], graph=graph)
+        res.append([(qcm.ground_state_solve(I.label)[0], qcm.qmatrix(I.label)) for I in insts])
+        for I in insts:
+            I.erase()
+    for (e1, (p1, q1)), (e2, (p2, q2)) in zip(*res):
+        assert e1 == e2
+        np.testing.assert_array_equal(p1, p2)
+        np.testing.assert_array_equal(q1, q2)
+
+
+def test_qed_batch_solve_abi_matches_instances(gpu_lib):
+    """The packed C entry point: E0 and the Lehmann form of each instance."""
+    from pyqcm_amd import _lib, qcm
+    C, M, cases = _setup(4, 9)
+    names = ["t", "tp", "U", "mu"] + C.PLAQUETTE_VARIA
+    params = np.array([[v[k] for k in names[:4]] + list(x) for v, x in cases], float)
+    sectors = [C.plaquette_sectors(v) for v, _ in cases]
+    nb, cap, d = len(cases), 2000, 4
+    arr = (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
+    sarr = (ctypes.c_char_p * nb)(*[s.encode() for s in sectors])
+    E0 = np.zeros(nb)
+    Mo = np.zeros(nb, np.int32)
+    e = np.zeros(nb * cap)
+    Q = np.zeros(nb * cap * d)
+    _lib.call("qed_reset_start_vector_state")
+    _lib.call("qed_batch_solve", M.name.encode(), len(names), arr, nb, _lib.dp(params), sarr, 1, cap,
+              _lib.dp(E0), _lib.ip(Mo), _lib.dp(e), _lib.dp(Q))
+    _lib.call("qed_reset_start_vector_state")
+    insts = [M.instance(_values(C, v, x), s) for (v, x), s in zip(cases, sectors)]
+    qcm.batch_solve([I.label for I in insts])
+    zs = 1j * np.array([0.2, 1.0, 4.0])
+    for b, I in enumerate(insts):
+        assert abs(E0[b] - qcm.ground_state_solve(I.label)[0]) <= 1e-12 * abs(E0[b])
+        m = Mo[b]
+        eb = e[b * cap: b * cap + m]
+        Qb = Q[b * cap * d: (b * cap + m) * d].reshape(m, d)
+        G = np.array([(Qb.T / (z - eb)) @ Qb for z in zs])
+        np.testing.assert_allclose(G, qcm.Green_function(zs, 0, I.label), atol=1e-12)
+        I.erase()
