@@ -8,6 +8,13 @@ Prints one JSON line (rank 0, N=1):
   value        whole-job H.v throughput in GB/s of algorithmic bytes
   roofline     achieved GB/s of the H.v kernel(s) from HIP events on the library stream
   cpu_baseline the oracle's H.v (reference loop order, one core) on the same sector
+  cdmft        the second half of the metric: converged CDMFT solves per second on the
+               2x2 plaquette + 8 bath orbitals (16 bath parameters), n seeded points
+               (U ~ U[0.5, 12], mu ~ U[0, U], t' ~ U[-0.5, 0.5]), solved by the batched
+               engine from the fixed cold start and from the learned warm start (the MLP
+               trained on the GPU on the committed training set, BF16 inference), at the
+               same convergence tests and tolerances. A "solve" is one parameter point
+               converged by the CDMFT loop.
 """
 import argparse
 import json
@@ -35,11 +42,18 @@ def main():
     ap.add_argument("--variant", type=int, default=-1, help="H.v kernel variant (-1: library default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--sweep", action="store_true", help="time every H.v variant (stderr)")
+    ap.add_argument("--cdmft", type=int, default=64, help="CDMFT points for the warm-start leg (0: skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 2 GiB five times (for scaling HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the single-GPU H.v path); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     from oracle import oracle as O
     from pyqcm_amd import _lib
@@ -102,6 +116,11 @@ def main():
                          "oracle's restatement of Hamiltonian_Factorized::mult_add, %.1f s" % tc}
         log("cpu baseline: %.2f s per H.v" % tc)
 
+    dx.free()
+    dy.free()
+    del S
+    cdmft = cdmft_leg(args.cdmft) if args.cdmft > 0 else None
+
     out = {
         "metric": "H·v GB/s (16-site ED, % HBM peak); CDMFT solves/s learned warm-start vs plain",
         "value": round(value, 2),
@@ -123,10 +142,80 @@ def main():
             "hv_variant": variant,
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_from_profile()},
         "cpu_baseline": cpu,
+        "cdmft": cdmft,
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration():
+    """A plain copy kernel of known size: 2 GiB read + 2 GiB written per launch."""
+    import torch
+    n = (2 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("copy calibration: 5 launches of %d bytes read + %d bytes written" % (8 * n, 8 * n))
+
+
+def traffic_from_profile():
+    """HBM bytes per H.v launch measured with rocprofv3 counters (profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", "hv_16site_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        if ln.startswith("traffic_bytes_per_launch:"):
+            return float(ln.split(":", 1)[1])
+    return None
+
+
+def cdmft_leg(n):
+    """Plain (cold start) vs learned warm start, batched CDMFT on n seeded points."""
+    from pyqcm_amd import cdmft as C
+    from pyqcm_amd import warmstart as W
+    path = os.path.join(ROOT, "pyqcm_amd", "data", "plaquette_warmstart.npz")
+    d = np.load(path)
+    ok = d["converged"].astype(bool)
+    net = W.MLP(3, 128, 16, seed=0)
+    t0 = time.perf_counter()
+    hist = net.fit(d["features"][ok], d["bath"][ok], epochs=1500, batch=64, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    log("MLP trained on %d points in %.1f s (loss %.3g -> %.3g)" % (ok.sum(), t_train, hist[0], hist[-1]))
+    P = W.plaquette_problem("plaq_bench")
+    pts = W.plaquette_points(n, 20261202)
+    feats = np.array([W.plaquette_features(v) for v in pts])
+    cold = [W.PLAQUETTE_COLD_START] * n
+    t0 = time.perf_counter()
+    warm = list(C.canonical_bath(net.predict(feats, bf16=True)))
+    t_pred = time.perf_counter() - t0
+    res = {}
+    for name, x0 in (("plain", cold), ("warm", warm)):
+        t0 = time.perf_counter()
+        B = C.BatchedCDMFT(P, pts, x0, solver="batched")
+        dt = time.perf_counter() - t0 + (t_pred if name == "warm" else 0.0)
+        good = B.converged & ~B.failed
+        res[name] = {"converged": int(good.sum()), "failed": int((~good).sum()), "seconds": round(dt, 2),
+                     "solves_per_s": round(good.sum() / dt, 4),
+                     "mean_iterations": round(float(B.niter[good].mean()) if good.any() else 0.0, 2),
+                     "impurity_solves": int(B.n_solves.sum())}
+        res[name + "_x"] = np.array(B.x)
+        res[name + "_ok"] = good
+        log("CDMFT %s: %s" % (name, res[name]))
+    both = res["plain_ok"] & res["warm_ok"]
+    dx = np.abs(C.canonical_bath(res["plain_x"][both]) - C.canonical_bath(res["warm_x"][both]))
+    same = dx.max(1) < 1e-2 if both.any() else np.zeros(0, bool)
+    return {
+        "workload": "2x2 plaquette + 8 bath orbitals, %d seeded points, sectors N=8..16 S=0, "
+                    "Legendre (1,10,5,10,5) grid, 32x32 k-grid, accur 1e-4 on parameters" % n,
+        "plain": res["plain"], "warm": res["warm"],
+        "speedup_solves_per_s": round(res["warm"]["solves_per_s"] / max(res["plain"]["solves_per_s"], 1e-12), 3),
+        "same_fixed_point": "%d of %d points converged by both" % (int(same.sum()), int(both.sum())),
+        "mlp": {"train_points": int(ok.sum()), "train_s": round(t_train, 2), "final_loss": float(hist[-1]),
+                "inference": "bf16 v_mfma_f32_16x16x32_bf16"},
+    }
 
 
 if __name__ == "__main__":

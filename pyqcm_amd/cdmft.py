@@ -271,8 +271,14 @@ class CDMFTProblem:
             vals.update(zip(self.varia, np.asarray(x, float)))
             insts.append(self.model.instance(vals, self.sectors_of(v)))
         self.last_stats = qcm.batch_solve([I.label for I in insts], graph=graph)
-        hosts = [self.host(I) for I in insts]
+        hosts = []
         for I in insts:
+            E0 = I.ground_state()[0]
+            if np.isfinite(E0):
+                hosts.append(self.host(I))
+            else:  # the impurity solve failed for this instance
+                n = self.model.n
+                hosts.append(np.full((len(self.grid.wr), n, n), np.nan, complex))
             I.erase()
         return hosts
 
@@ -453,9 +459,17 @@ class BatchedCDMFT:
         t0 = time.perf_counter()
         hosts = self.solver([self.values[i] for i in idx], xs)
         t1 = time.perf_counter()
-        X, F, NF, OK = self.problem.fit(np.asarray(hosts), np.arange(len(idx)), np.array(xs),
-                                        self.initial_step, self.accur_bath, self.accur_dist,
-                                        self.maxfev, self.fit_where)
+        hosts = np.asarray(hosts)
+        good = [j for j in range(len(idx)) if np.all(np.isfinite(hosts[j]))]
+        X = np.array(xs, float)
+        F = np.full(len(idx), np.nan)
+        NF = np.zeros(len(idx), int)
+        OK = np.zeros(len(idx), bool)
+        if good:
+            Xg, Fg, NFg, OKg = self.problem.fit(hosts[good], np.arange(len(good)), X[good],
+                                                self.initial_step, self.accur_bath, self.accur_dist,
+                                                self.maxfev, self.fit_where)
+            X[good], F[good], NF[good], OK[good] = Xg, Fg, NFg, OKg
         self.t_ed += t1 - t0
         self.t_fit += time.perf_counter() - t1
         out = []

@@ -19,6 +19,7 @@
 // block range; block -> job by binary search. Reductions are two-pass and deterministic.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -504,15 +505,28 @@ void lanczos_gs_batch(const std::vector<qed_sector*>& S, const std::vector<doubl
     });
     for (int j = 0; j < nj; ++j)
       if (newly[j] > 0) {
-        if (newly[j] > max_iter)
-          fail("Lanczos procedure exceeded " + std::to_string(max_iter) + " iterations");
+        if (newly[j] > max_iter) {  // converged too late: a failure, as in the scalar solver
+          evec[j].clear();
+          E0[j] = std::nan("");
+          newly[j] = -1;
+        }
         done_at[j] = newly[j];
         active[j] = 0;
         --remaining;
       }
     k = k1;
-    if (remaining > 0 && k >= max_iter)
-      fail("Lanczos procedure exceeded " + std::to_string(max_iter) + " iterations");
+    if (remaining > 0 && k >= max_iter) {
+      // "Lanczos procedure exceeded max_iter iterations" for the sectors still running:
+      // they are reported as failed (E0 = NaN, niter = -1); the others are unaffected
+      for (int j = 0; j < nj; ++j)
+        if (active[j]) {
+          active[j] = 0;
+          done_at[j] = -1;
+          evec[j].clear();
+          E0[j] = std::nan("");
+          --remaining;
+        }
+    }
     QED_HIP(hipMemcpyAsync(dact.p, active.data(), nj * sizeof(int), hipMemcpyHostToDevice, stream()));
   }
   if (exec) QED_HIP(hipGraphExecDestroy(exec));
@@ -528,14 +542,15 @@ void lanczos_gs_batch(const std::vector<qed_sector*>& S, const std::vector<doubl
   for (int j = 0; j < nj; ++j) {
     niter[j] = done_at[j];
     const uint64_t n = S[j]->D;
-    ev_dev.emplace_back(evec[j].size() * 8);
-    QED_HIP(hipMemcpyAsync(ev_dev.back().p, evec[j].data(), evec[j].size() * 8, hipMemcpyHostToDevice, stream()));
+    ev_dev.emplace_back(std::max<size_t>(1, evec[j].size()) * 8);
+    if (!evec[j].empty())
+      QED_HIP(hipMemcpyAsync(ev_dev.back().p, evec[j].data(), evec[j].size() * 8, hipMemcpyHostToDevice, stream()));
     // r = start vector (copy kept in psi), q = 0, psi *= evec[0]
     QED_HIP(hipMemcpyAsync(A[j]->p, psi[j], n * 8, hipMemcpyDeviceToDevice, stream()));
     R.add(RJob{A[j]->as<double>(), Bv[j]->as<double>(), psi[j], alpha + (size_t)j * ld, beta + (size_t)j * ld,
                ev_dev.back().as<double>(), (int)evec[j].size(), n, 0},
           n);
-    PS.add(PsiScale{psi[j], evec[j][0], n, 0}, n);
+    PS.add(PsiScale{psi[j], evec[j].empty() ? 0.0 : evec[j][0], n, 0}, n);
     maxlen = std::max(maxlen, (int)evec[j].size());
   }
   k_fill0<<<F.total, kBT, 0, stream()>>>(dF, nj);
@@ -545,7 +560,7 @@ void lanczos_gs_batch(const std::vector<qed_sector*>& S, const std::vector<doubl
   QED_HIP(hipMemcpyAsync(dact.p, ones.data(), nj * sizeof(int), hipMemcpyHostToDevice, stream()));
   // replay H.v: x = r (A), y = q (B): parity 0 of HB
   for (int i = 1; i < maxlen; ++i) {
-    for (int j = 0; j < nj; ++j) active[j] = i < (int)evec[j].size();
+    for (int j = 0; j < nj; ++j) active[j] = i < (int)evec[j].size() ? 1 : 0;
     QED_HIP(hipMemcpyAsync(dact.p, active.data(), nj * sizeof(int), hipMemcpyHostToDevice, stream()));
     hv_batch(HB, cfg, 0, act);
     k_replay_batch<<<R.total, kBT, 0, stream()>>>(dR, nj, i);
@@ -695,6 +710,7 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
     // ---- host: complete the previous step's band entries and its eigenvalue test, then
     //      take this step's deflation decision
     std::vector<int> proceed(nj, 0);
+    const auto tc0 = std::chrono::steady_clock::now();
     parallel_for(nj, [&](int i) {
       BandJob& b = J[i];
       if (b.done) return;
@@ -751,6 +767,7 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
       b.z = z;
       proceed[i] = 1;
     });
+    if (stats) stats->t_band_checks += std::chrono::duration<double>(std::chrono::steady_clock::now() - tc0).count();
     // ---- phase 2: v[j] /= z; pending block orthogonalized against v[j] ----
     // scale: y = y * (1/z) with no sum (MAxpy with m = 0)
     TA.clear();
@@ -972,6 +989,11 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
         for (int k = 0; k < b.p0; ++k) Q[i][(size_t)c * b.p0 + k] += b.P[(size_t)m * b.p0 + k] * u;
       }
   });
+}
+
+BatchStats& last_batch_stats() {
+  static BatchStats s;
+  return s;
 }
 
 }  // namespace qed

@@ -47,31 +47,47 @@ int guard(F&& f) {
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool owned = true;  // false: a view into memory owned elsewhere
   DevBuf() = default;
   explicit DevBuf(size_t n) { alloc(n); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), owned(o.owned) {
+    o.p = nullptr;
+    o.bytes = 0;
+    o.owned = true;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
       release();
       p = o.p;
       bytes = o.bytes;
+      owned = o.owned;
       o.p = nullptr;
       o.bytes = 0;
+      o.owned = true;
     }
     return *this;
   }
   ~DevBuf() { release(); }
+  static DevBuf view(void* ptr, size_t n) {
+    DevBuf b;
+    b.p = ptr;
+    b.bytes = n;
+    b.owned = false;
+    return b;
+  }
   void alloc(size_t n) {
     release();
     if (n) QED_HIP(hipMalloc(&p, n));
     bytes = n;
+    owned = true;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    owned = true;
   }
   template <class T>
   T* as() const { return static_cast<T*>(p); }

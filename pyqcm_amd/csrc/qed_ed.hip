@@ -4,12 +4,14 @@
 // self-energy, hybridization function and the Potthoff functional. Trivial symmetry group,
 // normal or up/down mixing, real Hilbert space.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -593,7 +595,12 @@ void green_function_solve(Instance& I) {
 // one band Lanczos pass for every (instance, state, spin, particle/hole) Green function
 // sector. Instances that need another method (Davidson, several states per sector,
 // continued fractions) are solved one by one.
+double wall() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 void batch_solve(std::vector<Instance*>& insts, bool graph, int chunk, BatchStats* stats) {
+  double t0 = wall();
   struct GS {
     Instance* I;
     int nup, ndn;
@@ -623,6 +630,9 @@ void batch_solve(std::vector<Instance*>& insts, bool graph, int chunk, BatchStat
       gs.push_back(std::move(g));
     }
   }
+  QED_HIP(hipStreamSynchronize(stream()));
+  if (stats) stats->t_setup = wall() - t0;
+  t0 = wall();
   {
     std::vector<qed_sector*> S;
     std::vector<double*> psi;
@@ -637,15 +647,34 @@ void batch_solve(std::vector<Instance*>& insts, bool graph, int chunk, BatchStat
     double max_gap = -std::log(gd("minimum_weight")) * temperature;
     if (max_gap < MIN_GAP) max_gap = MIN_GAP;
     std::map<Instance*, std::vector<std::unique_ptr<State>>> states;
+    std::set<Instance*> failed;
     for (size_t k = 0; k < gs.size(); ++k) {
+      if (!(nit[k] > 0)) {
+        failed.insert(gs[k].I);
+        continue;
+      }
       std::vector<double> ev{E[k]};
       std::vector<DevBuf> vv;
       vv.push_back(std::move(gs[k].v));
       add_states(*gs[k].I, gs[k].nup, gs[k].ndn, ev, vv, max_gap, states[gs[k].I]);
     }
-    for (Instance* I : batched) finish_states(*I, states[I]);
+    for (Instance* I : batched) {
+      if (failed.count(I)) {
+        // a sector's Lanczos did not converge within max_iter_lanczos: the instance is
+        // reported with E0 = NaN and no states instead of failing the whole batch
+        I->states.clear();
+        I->GS_energy = I->E0 = std::nan("");
+        I->gs_string = "failed";
+        I->gf_solved = true;
+        continue;
+      }
+      finish_states(*I, states[I]);
+    }
   }
   gs.clear();
+  QED_HIP(hipStreamSynchronize(stream()));
+  if (stats) stats->t_gs = wall() - t0;
+  t0 = wall();
   // Green functions
   struct GF {
     Instance* I;
@@ -703,6 +732,9 @@ void batch_solve(std::vector<Instance*>& insts, bool graph, int chunk, BatchStat
     pv.push_back(p);
     which.push_back((int)k);
   }
+  QED_HIP(hipStreamSynchronize(stream()));
+  if (stats) stats->t_gf_setup = wall() - t0;
+  t0 = wall();
   if (!S.empty()) {
     std::vector<std::vector<double>> ev, Q;
     band_lanczos_batch(S, phi, pv, gi("max_iter_BL"), gd("accur_deflation"), gd("accur_band_lanczos"), ev, Q,
@@ -714,6 +746,8 @@ void batch_solve(std::vector<Instance*>& insts, bool graph, int chunk, BatchStat
       q.Q = std::move(Q[u]);
     }
   }
+  if (stats) stats->t_band = wall() - t0;
+  t0 = wall();
   // assemble per (instance, state, spin) exactly as build_qmatrix does
   for (size_t k = 0; k < gf.size(); k += 2) {
     GF& h = gf[k];
@@ -742,6 +776,10 @@ void batch_solve(std::vector<Instance*>& insts, bool graph, int chunk, BatchStat
   for (Instance* I : insts)
     if (I->correlated && !I->cf) I->gf_solved = true;
   QED_HIP(hipStreamSynchronize(stream()));
+  if (stats) {
+    stats->t_assemble = wall() - t0;
+    last_batch_stats() = *stats;
+  }
 }
 
 int spin_index(const Instance& I, int spin_down) {
@@ -987,6 +1025,14 @@ int qed_ed_ground_state_solve(uint64_t label, double* E0, char* buf, int buflen)
 int qed_ed_green_function_solve(uint64_t label) {
   std::lock_guard<std::recursive_mutex> lk(lib_mutex());
   return guard([&] { green_function_solve(inst(label)); });
+}
+
+int qed_batch_last_timings(double* t, int n) {
+  return guard([&] {
+    const BatchStats& s = last_batch_stats();
+    const double v[6] = {s.t_setup, s.t_gs, s.t_gf_setup, s.t_band, s.t_band_checks, s.t_assemble};
+    for (int i = 0; i < n && i < 6; ++i) t[i] = v[i];
+  });
 }
 
 int qed_ed_batch_solve_instances(int n, const uint64_t* labels, int graph, int chunk, int* stats) {

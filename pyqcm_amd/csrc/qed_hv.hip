@@ -15,6 +15,8 @@
 #include <memory>
 #include <mutex>
 #include <random>
+#include <thread>
+#include <atomic>
 
 #include "../../include/qed.h"
 #include "qed_sector.hpp"
@@ -74,6 +76,7 @@ __global__ void k_hop_fill(const uint32_t* bin, uint32_t dim, const Bond* bonds,
     uint32_t j = cns_rank(s ^ mf ^ mt, binom);
     ent[k++] = j | ((uint32_t)(2 * b + neg) << kTargetBits);
   }
+  if (!diag) return;
   double d = 0.0;
   for (int q = 0; q < nd; ++q)
     if (s & (1u << dsite[q])) d += dval[q];
@@ -283,7 +286,7 @@ __global__ __launch_bounds__(BS) void k_hv_ell_batch(const HvJob* __restrict__ j
 
 // Same product, two adjacent columns per thread: 16-byte loads of the down-spin neighbour
 // rows and of the ELL entry pairs (needs an even up dimension).
-template <int BS>
+template <int BS, bool DOWN = true, bool XCDMAP = false>
 __global__ __launch_bounds__(BS) void k_hv_ell2(HvView v, const uint32_t* __restrict__ up_ell,
                                                 uint32_t Wu, const uint32_t* __restrict__ dn_ell,
                                                 uint32_t Wd, const double* __restrict__ x,
@@ -293,7 +296,14 @@ __global__ __launch_bounds__(BS) void k_hv_ell2(HvView v, const uint32_t* __rest
   const uint32_t nvu = v.up_nvals;
   double* vals_s = smem;
   double* xs = smem + ((nvu + 1u) & ~1u);  // 16-byte aligned row
-  const uint32_t r = blockIdx.x;
+  // XCDMAP: workgroup b runs on XCD b % 8; give each XCD one contiguous eighth of the
+  // rows, so that rows and their nearby down-spin neighbours meet in the same L2
+  uint32_t r = blockIdx.x;
+  if (XCDMAP) {
+    const uint32_t per = (v.dd + 7u) / 8u;
+    r = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    if (r >= v.dd) return;
+  }
   const double* xr = x + (uint64_t)r * du;
   const uint32_t dh = du / 2;
   for (uint32_t i = threadIdx.x; i < nvu; i += BS) vals_s[i] = v.up_vals[i];
@@ -322,7 +332,7 @@ __global__ __launch_bounds__(BS) void k_hv_ell2(HvView v, const uint32_t* __rest
       }
     }
     double b0 = 0.0, b1 = 0.0;
-    for (uint32_t g = 0; g < dgroups; g += kG) {
+    for (uint32_t g = 0; DOWN && g < dgroups; g += kG) {
       uint32_t e[kG];
       double2 xv[kG];
 #pragma unroll
@@ -341,6 +351,58 @@ __global__ __launch_bounds__(BS) void k_hv_ell2(HvView v, const uint32_t* __rest
     yo.x += a0 + b0;
     yo.y += a1 + b1;
     yr[c2] = yo;
+  }
+}
+
+// Down-spin term alone, by column chunks of W doubles: Y[r, C] += sum_k t_k X[r'_k, C].
+// Block w runs on XCD w % 8; the XCD's blocks take the chunks congruent to it one after
+// the other, so a chunk of X (dd rows x W columns) is served from that XCD's L2 to all
+// the rows that need it. Each lane owns two adjacent columns (16-byte loads) of one row;
+// eight neighbour loads are in flight per lane. Needs an even up dimension.
+template <int W>
+__global__ __launch_bounds__(256) void k_hv_down_ell_chunks(HvView v, const uint32_t* __restrict__ dn_ell,
+                                                           uint32_t Wd, const double* __restrict__ x,
+                                                           double* __restrict__ y, uint32_t nchunks,
+                                                           uint32_t rows_per_block, uint32_t nrb) {
+  constexpr int LPR = W / 2;      // lanes per row
+  constexpr int RPW = 64 / LPR;   // rows per wave
+  const uint32_t w = blockIdx.x;
+  const uint32_t xcd = w & 7u, kk = w >> 3;
+  const uint32_t chunk = (kk / nrb) * 8u + xcd;
+  const uint32_t rb = kk % nrb;
+  if (chunk >= nchunks) return;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint32_t du = v.du, dd = v.dd;
+  const uint32_t c = chunk * W + 2u * (uint32_t)(lane % LPR);
+  const uint32_t sub = (uint32_t)(lane / LPR);
+  const uint32_t r0 = rb * rows_per_block;
+  const uint32_t r1 = min(r0 + rows_per_block, dd);
+  if (c >= du) return;
+  for (uint32_t r = r0 + (uint32_t)wid * RPW + sub; r < r1; r += 4u * RPW) {
+    const uint32_t dcnt = v.dn_ptr[r + 1] - v.dn_ptr[r];
+    const uint32_t dgroups = (dcnt + kG - 1) / kG * kG;
+    const uint32_t* dl = dn_ell + (uint64_t)r * Wd;
+    double b0 = 0.0, b1 = 0.0;
+    for (uint32_t g = 0; g < dgroups; g += kG) {
+      uint32_t e[kG];
+      double2 xv[kG];
+#pragma unroll
+      for (int q = 0; q < kG; ++q) e[q] = dl[g + q];
+#pragma unroll
+      for (int q = 0; q < kG; ++q)
+        xv[q] = *reinterpret_cast<const double2*>(x + (uint64_t)(e[q] & kTargetMask) * du + c);
+#pragma unroll
+      for (int q = 0; q < kG; ++q) {
+        const double cf = v.dn_vals[e[q] >> kTargetBits];
+        b0 += cf * xv[q].x;
+        b1 += cf * xv[q].y;
+      }
+    }
+    double2* yp = reinterpret_cast<double2*>(y + (uint64_t)r * du + c);
+    double2 yo = *yp;
+    yo.x += b0;
+    yo.y += b1;
+    *yp = yo;
   }
 }
 
@@ -436,6 +498,83 @@ HalfTables build_half_tables(int L, int N, const std::vector<Bond>& bonds,
   return h;
 }
 
+// Structure of one half space for a bond list: ascending basis, CSR hop lists and their
+// ELL copy (up spin column-major, down spin row-major). It does not depend on the
+// amplitudes, so sectors of the same model with other parameter values share it.
+struct HalfStruct {
+  DevBuf bin, ptr, ent, ell;
+  std::vector<uint32_t> bin_h;
+  uint32_t dim = 0, nnz = 0, maxdeg = 0, w = 0, nvals = 0;
+};
+
+std::shared_ptr<HalfStruct> half_struct(int L, int N, const std::vector<Bond>& bonds, bool colmajor) {
+  static std::mutex mu;
+  static std::map<std::vector<int>, std::shared_ptr<HalfStruct>> cache;
+  std::vector<int> key{L, N, colmajor ? 1 : 0};
+  for (const Bond& b : bonds) {
+    key.push_back(b.from);
+    key.push_back(b.to);
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  auto h = std::make_shared<HalfStruct>();
+  const auto& B = binom_host();
+  if (N >= 0 && N <= L) h->dim = B[L * 33 + N];
+  h->nvals = (uint32_t)(2 * bonds.size() + 1);
+  if (h->dim > 0) {
+    if (h->dim > kTargetMask) fail("half basis too large for the packed hop tables");
+    if (2 * bonds.size() > kMaxVals) fail("too many distinct hopping amplitudes in one sector");
+    hipStream_t st = stream();
+    const uint32_t* bd = binom_dev();
+    h->bin.alloc(h->dim * sizeof(uint32_t));
+    k_unrank<<<(h->dim + 255) / 256, 256, 0, st>>>(h->bin.as<uint32_t>(), h->dim, L, N, bd);
+    DevBuf bonds_d = upload(bonds.empty() ? std::vector<Bond>{{0, 0}} : bonds);
+    DevBuf cnt(h->dim * sizeof(uint32_t));
+    k_hop_count<<<(h->dim + 255) / 256, 256, 0, st>>>(h->bin.as<uint32_t>(), h->dim, bonds_d.as<Bond>(),
+                                                      (int)bonds.size(), cnt.as<uint32_t>());
+    std::vector<uint32_t> c(h->dim), ptr(h->dim + 1, 0);
+    h->bin_h.resize(h->dim);
+    QED_HIP(hipMemcpyAsync(c.data(), cnt.p, h->dim * 4, hipMemcpyDeviceToHost, st));
+    QED_HIP(hipMemcpyAsync(h->bin_h.data(), h->bin.p, h->dim * 4, hipMemcpyDeviceToHost, st));
+    QED_HIP(hipStreamSynchronize(st));
+    for (uint32_t i = 0; i < h->dim; ++i) {
+      ptr[i + 1] = ptr[i] + c[i];
+      h->maxdeg = std::max(h->maxdeg, c[i]);
+    }
+    h->nnz = ptr[h->dim];
+    h->ptr = upload(ptr);
+    h->ent.alloc(std::max<uint32_t>(h->nnz, 1) * sizeof(uint32_t));
+    k_hop_fill<<<(h->dim + 255) / 256, 256, 0, st>>>(h->bin.as<uint32_t>(), h->dim, bonds_d.as<Bond>(),
+                                                     (int)bonds.size(), h->ptr.as<uint32_t>(),
+                                                     h->ent.as<uint32_t>(), nullptr, nullptr, 0, nullptr, bd);
+    // ELL copy, padded to multiples of 8 entries with zero-amplitude self hops
+    h->w = (h->maxdeg + 7) / 8 * 8;
+    h->ell.alloc(std::max<uint64_t>((uint64_t)h->w * h->dim, 1) * 4);
+    if (colmajor)
+      k_ell_colmajor<<<(h->dim + 255) / 256, 256, 0, st>>>(h->ptr.as<uint32_t>(), h->ent.as<uint32_t>(), h->dim,
+                                                           h->w, h->nvals - 1, h->ell.as<uint32_t>());
+    else
+      k_ell_rowmajor<<<(h->dim + 255) / 256, 256, 0, st>>>(h->ptr.as<uint32_t>(), h->ent.as<uint32_t>(), h->dim,
+                                                           h->w, h->nvals - 1, h->ell.as<uint32_t>());
+    QED_HIP(hipGetLastError());
+    QED_HIP(hipStreamSynchronize(st));
+  }
+  if (cache.size() > 512) cache.clear();
+  cache[key] = h;
+  return h;
+}
+
+// one-body diagonal of every half state (same order of additions as the device fill)
+void half_diag(const HalfStruct& h, const std::vector<int>& dsite, const std::vector<double>& dval, double* d) {
+  for (uint32_t i = 0; i < h.dim; ++i) {
+    const uint32_t s = h.bin_h[i];
+    double a = 0.0;
+    for (size_t q = 0; q < dsite.size(); ++q)
+      if (s & (1u << dsite[q])) a += dval[q];
+    d[i] = a;
+  }
+}
 
 }  // namespace
 
@@ -483,6 +622,10 @@ static void set_lds_attributes() {
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     (void)hipFuncSetAttribute((const void*)k_hv_ell2<512>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void*)k_hv_ell2<512, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void*)k_hv_ell2<512, true, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   });
 }
 
@@ -492,6 +635,11 @@ static void set_lds_attributes() {
 //   4  one workgroup per row, ELL tables, 8-deep load groups, one column per thread
 //   5  as 4 with two columns per thread and 16-byte loads (default; falls back to 4 when the
 //      up dimension is odd or small, or the row does not fit in LDS)
+//   6  two passes: variant 5 without the down-spin term, then the down-spin term by
+//      XCD-local column chunks served from L2 (falls back to 5 where 5 falls back)
+int g_chunk_w = 32;      // column-chunk width of variant 6 (16, 32 or 64 doubles)
+int g_chunk_rows = 64;   // rows per workgroup of variant 6
+
 void hv(const qed_sector& s, const double* x, double* y, int variant) {
   if (s.D == 0) return;
   set_lds_attributes();
@@ -501,6 +649,35 @@ void hv(const qed_sector& s, const double* x, double* y, int variant) {
   hipStream_t st = stream();
   const uint32_t* ue = s.up_ell.as<uint32_t>();
   const uint32_t* de = s.dn_ell.as<uint32_t>();
+  if (variant == 6) {
+    const size_t lds2 = ((size_t)s.du + s.up_nvals + 1) * sizeof(double);
+    if (s.du % 2 == 0 && s.du >= 4096 && lds2 <= lds_budget()) {
+      k_hv_ell2<512, false><<<s.dd, 512, lds2, st>>>(v, ue, s.up_w, de, s.dn_w, x, y);
+      const uint32_t W = (uint32_t)g_chunk_w, rpb = (uint32_t)g_chunk_rows;
+      const uint32_t nchunks = (s.du + W - 1) / W;
+      const uint32_t nrb = (s.dd + rpb - 1) / rpb;
+      const uint64_t blocks = (uint64_t)((nchunks + 7) / 8) * 8ull * nrb;
+      if (W == 16)
+        k_hv_down_ell_chunks<16><<<(uint32_t)blocks, 256, 0, st>>>(v, de, s.dn_w, x, y, nchunks, rpb, nrb);
+      else if (W == 64)
+        k_hv_down_ell_chunks<64><<<(uint32_t)blocks, 256, 0, st>>>(v, de, s.dn_w, x, y, nchunks, rpb, nrb);
+      else
+        k_hv_down_ell_chunks<32><<<(uint32_t)blocks, 256, 0, st>>>(v, de, s.dn_w, x, y, nchunks, rpb, nrb);
+      QED_HIP(hipGetLastError());
+      return;
+    }
+    variant = 5;
+  }
+  if (variant == 7) {
+    const size_t lds2 = ((size_t)s.du + s.up_nvals + 1) * sizeof(double);
+    if (s.du % 2 == 0 && s.du >= 4096 && lds2 <= lds_budget()) {
+      const uint32_t grid = ((s.dd + 7u) / 8u) * 8u;
+      k_hv_ell2<512, true, true><<<grid, 512, lds2, st>>>(v, ue, s.up_w, de, s.dn_w, x, y);
+      QED_HIP(hipGetLastError());
+      return;
+    }
+    variant = 5;
+  }
   if (variant == 5) {
     const size_t lds2 = ((size_t)s.du + s.up_nvals + 1) * sizeof(double);
     if (s.du % 2 == 0 && s.du >= 4096 && lds2 <= lds_budget()) {
@@ -664,8 +841,129 @@ namespace {
 
 // start-vector state shared by every sector, as the reference keeps one process-wide
 // normal_distribution (src_ed/Hamiltonian/Lanczos.cpp:7)
-std::normal_distribution<double> g_normal(0.0, 1.0);
+// The stream is std::normal_distribution<double>(0, 1) (libstdc++: Marsaglia's polar
+// method, the second deviate of each pair kept for the next call) over a fresh
+// std::default_random_engine(seed) (minstd_rand0) through
+// std::generate_canonical<double, 53> (two engine draws per uniform). It is restated here
+// so that it can be generated in parallel: attempt t of the polar method uses engine
+// draws 4t+1 .. 4t+4, reachable by a jump of the linear congruential engine, so blocks of
+// attempts are counted and then filled independently. The values are identical to the
+// sequential stream.
+struct NormalState {
+  bool saved_available = false;
+  double saved = 0.0;
+};
+NormalState g_normal;
 std::mutex g_normal_mutex;
+
+constexpr uint64_t kLcgA = 16807, kLcgM = 2147483647;
+
+uint64_t lcg_pow(uint64_t k) {  // a^k mod m
+  uint64_t a = kLcgA, r = 1;
+  while (k) {
+    if (k & 1) r = r * a % kLcgM;
+    a = a * a % kLcgM;
+    k >>= 1;
+  }
+  return r;
+}
+
+// std::generate_canonical<double, 53> over minstd_rand0 (min 1, max m - 1): two draws
+double canonical2(uint64_t x1, uint64_t x2) {
+  const long double r = static_cast<long double>(kLcgM - 1) - static_cast<long double>(1) + 1.0L;
+  double sum = 0.0, tmp = 1.0;
+  sum += double(x1 - 1) * tmp;
+  tmp *= r;
+  sum += double(x2 - 1) * tmp;
+  tmp *= r;
+  double ret = sum / tmp;
+  if (ret >= 1.0) ret = std::nextafter(1.0, 0.0);
+  return ret;
+}
+
+// one polar-method attempt starting at engine state st (draws st*a .. st*a^4)
+inline bool polar_attempt(uint64_t& st, double& y_out, double& x_out) {
+  const uint64_t x1 = st * kLcgA % kLcgM, x2 = x1 * kLcgA % kLcgM, x3 = x2 * kLcgA % kLcgM,
+                 x4 = x3 * kLcgA % kLcgM;
+  st = x4;
+  const double x = 2.0 * canonical2(x1, x2) - 1.0;
+  const double y = 2.0 * canonical2(x3, x4) - 1.0;
+  const double r2 = x * x + y * y;
+  if (r2 > 1.0 || r2 == 0.0) return false;
+  const double mult = std::sqrt(-2 * std::log(r2) / r2);
+  x_out = x * mult;
+  y_out = y * mult;
+  return true;
+}
+
+// next n values of the stream (value = 0.0 + (deviate * 1.0 + 0.0), as drawn with mean
+// 0 and standard deviation 1) for a fresh engine seeded with `seed`
+void normal_stream(unsigned seed, uint64_t n, double* out, int threads) {
+  uint64_t s0 = seed % kLcgM;
+  if (s0 == 0) s0 = 1;
+  auto emit = [](double v) { return 0.0 + (v * 1.0 + 0.0); };
+  uint64_t pos = 0;
+  if (n == 0) return;
+  if (g_normal.saved_available) {
+    out[pos++] = emit(g_normal.saved);
+    g_normal.saved_available = false;
+  }
+  const uint64_t need = n - pos;
+  if (need == 0) return;
+  const uint64_t pairs = (need + 1) / 2;
+  const uint64_t K = 1 << 15;  // attempts per block
+  uint64_t nblk = (uint64_t)(pairs * 1.2732395447351628 * 1.02 / K) + 2;  // 4/pi acceptance
+  std::vector<uint64_t> cnt;
+  const int nt = std::max(1, threads);
+  auto run = [&](uint64_t b0, uint64_t b1, auto&& body) {
+    std::vector<std::thread> th;
+    std::atomic<uint64_t> next{b0};
+    for (int t = 0; t < nt; ++t)
+      th.emplace_back([&] {
+        for (uint64_t b = next++; b < b1; b = next++) body(b);
+      });
+    for (auto& t : th) t.join();
+  };
+  // pass 1: accepted attempts per block (extend until there are enough)
+  uint64_t have = 0, counted = 0;
+  while (true) {
+    cnt.resize(nblk, 0);
+    run(counted, nblk, [&](uint64_t b) {
+      uint64_t st = s0 * lcg_pow(4 * K * b) % kLcgM;
+      uint64_t c = 0;
+      double y, x;
+      for (uint64_t t = 0; t < K; ++t) c += polar_attempt(st, y, x);
+      cnt[b] = c;
+    });
+    for (uint64_t b = counted; b < nblk; ++b) have += cnt[b];
+    counted = nblk;
+    if (have >= pairs) break;
+    nblk += nblk / 8 + 2;
+  }
+  // pass 2: block b fills pairs [first[b], first[b] + cnt[b])
+  std::vector<uint64_t> first(nblk + 1, 0);
+  for (uint64_t b = 0; b < nblk; ++b) first[b + 1] = first[b] + cnt[b];
+  uint64_t lastblk = 0;
+  while (first[lastblk + 1] < pairs) ++lastblk;
+  double saved = 0.0;
+  run(0, lastblk + 1, [&](uint64_t b) {
+    uint64_t st = s0 * lcg_pow(4 * K * b) % kLcgM;
+    uint64_t p = first[b];
+    double y, x;
+    for (uint64_t t = 0; t < K && p < pairs; ++t) {
+      if (!polar_attempt(st, y, x)) continue;
+      const uint64_t i = 2 * p;
+      out[pos + i] = emit(y);
+      if (i + 1 < need) out[pos + i + 1] = emit(x);
+      else saved = x;  // odd count: the pair's second deviate stays for the next call
+      ++p;
+    }
+  });
+  if (need % 2 == 1) {
+    g_normal.saved = saved;
+    g_normal.saved_available = true;
+  }
+}
 
 }  // namespace
 
@@ -766,45 +1064,63 @@ int qed_sector_create(int L, int n_up, int n_dn, int n_ops, const int* types, co
       s->hub_sites |= lo;
     }
     if (!s->hubbard_uniform) s->hub_sites = 0;
-    HalfTables hu = build_half_tables(L, n_up, bu, vu, dsu, dvu);
-    HalfTables hd = build_half_tables(L, n_dn, bd, vd, dsd, dvd);
-    s->du = hu.dim;
-    s->dd = hd.dim;
+    auto hu = half_struct(L, n_up, bu, true);
+    auto hd = half_struct(L, n_dn, bd, false);
+    s->keep_up = hu;
+    s->keep_dn = hd;
+    s->du = hu->dim;
+    s->dd = hd->dim;
     s->D = (uint64_t)s->du * s->dd;
-    s->bup = std::move(hu.bin);
-    s->bdn = std::move(hd.bin);
-    s->up_ptr = std::move(hu.ptr);
-    s->dn_ptr = std::move(hd.ptr);
-    s->up_ent = std::move(hu.ent);
-    s->dn_ent = std::move(hd.ent);
-    s->up_vals = std::move(hu.vals);
-    s->dn_vals = std::move(hd.vals);
-    s->diag_up = std::move(hu.diag);
-    s->diag_dn = std::move(hd.diag);
-    s->up_nnz = hu.nnz;
-    s->dn_nnz = hd.nnz;
-    s->up_nvals = hu.nvals;
-    s->dn_nvals = hd.nvals;
-    s->up_maxdeg = hu.maxdeg;
-    s->dn_maxdeg = hd.maxdeg;
-    // ELL copies (entries padded to multiples of 8 with zero-amplitude self hops)
-    s->up_w = (s->up_maxdeg + 7) / 8 * 8;
-    s->dn_w = (s->dn_maxdeg + 7) / 8 * 8;
-    if (s->du && s->dd) {
-      s->up_ell.alloc(std::max<uint64_t>((uint64_t)s->up_w * s->du, 1) * 4);
-      s->dn_ell.alloc(std::max<uint64_t>((uint64_t)s->dn_w * s->dd, 1) * 4);
-      k_ell_colmajor<<<(s->du + 255) / 256, 256, 0, stream()>>>(
-          s->up_ptr.as<uint32_t>(), s->up_ent.as<uint32_t>(), s->du, s->up_w, s->up_nvals - 1,
-          s->up_ell.as<uint32_t>());
-      k_ell_rowmajor<<<(s->dd + 255) / 256, 256, 0, stream()>>>(
-          s->dn_ptr.as<uint32_t>(), s->dn_ent.as<uint32_t>(), s->dd, s->dn_w, s->dn_nvals - 1,
-          s->dn_ell.as<uint32_t>());
-      QED_HIP(hipGetLastError());
+    s->bup = DevBuf::view(hu->bin.p, hu->bin.bytes);
+    s->bdn = DevBuf::view(hd->bin.p, hd->bin.bytes);
+    s->up_ptr = DevBuf::view(hu->ptr.p, hu->ptr.bytes);
+    s->dn_ptr = DevBuf::view(hd->ptr.p, hd->ptr.bytes);
+    s->up_ent = DevBuf::view(hu->ent.p, hu->ent.bytes);
+    s->dn_ent = DevBuf::view(hd->ent.p, hd->ent.bytes);
+    s->up_ell = DevBuf::view(hu->ell.p, hu->ell.bytes);
+    s->dn_ell = DevBuf::view(hd->ell.p, hd->ell.bytes);
+    s->up_nnz = hu->nnz;
+    s->dn_nnz = hd->nnz;
+    s->up_nvals = hu->nvals;
+    s->dn_nvals = hd->nvals;
+    s->up_maxdeg = hu->maxdeg;
+    s->dn_maxdeg = hd->maxdeg;
+    s->up_w = hu->w;
+    s->dn_w = hd->w;
+    // this sector's amplitudes in one allocation: signed hop values (with a trailing
+    // zero for the ELL padding), one-body diagonals, interaction values and masks
+    const size_t nvu = hu->nvals, nvd = hd->nvals;
+    const size_t nint = std::max<size_t>(1, s->int_val.size());
+    std::vector<double> hbuf(nvu + nvd + s->du + s->dd + nint, 0.0);
+    double* vup = hbuf.data();
+    double* vdn = vup + nvu;
+    double* dgu = vdn + nvd;
+    double* dgd = dgu + s->du;
+    double* iv = dgd + s->dd;
+    for (size_t b = 0; b < bu.size(); ++b) {
+      vup[2 * b] = vu[b];
+      vup[2 * b + 1] = -vu[b];
     }
-    s->int_mask_d = upload(s->int_mask.empty() ? std::vector<uint64_t>{0} : s->int_mask);
-    s->int_val_d = upload(s->int_val.empty() ? std::vector<double>{0.0} : s->int_val);
-    s->red.alloc(4096 * sizeof(double) * 8);
-    s->red_out.alloc(64 * sizeof(double));
+    for (size_t b = 0; b < bd.size(); ++b) {
+      vdn[2 * b] = vd[b];
+      vdn[2 * b + 1] = -vd[b];
+    }
+    half_diag(*hu, dsu, dvu, dgu);
+    half_diag(*hd, dsd, dvd, dgd);
+    for (size_t q = 0; q < s->int_val.size(); ++q) iv[q] = s->int_val[q];
+    std::vector<uint64_t> masks(nint, 0);
+    for (size_t q = 0; q < s->int_mask.size(); ++q) masks[q] = s->int_mask[q];
+    const size_t nd = hbuf.size();
+    s->valbuf.alloc((nd + nint) * 8);
+    QED_HIP(hipMemcpyAsync(s->valbuf.p, hbuf.data(), nd * 8, hipMemcpyHostToDevice, stream()));
+    QED_HIP(hipMemcpyAsync(s->valbuf.as<double>() + nd, masks.data(), nint * 8, hipMemcpyHostToDevice, stream()));
+    double* base = s->valbuf.as<double>();
+    s->up_vals = DevBuf::view(base, nvu * 8);
+    s->dn_vals = DevBuf::view(base + nvu, nvd * 8);
+    s->diag_up = DevBuf::view(base + nvu + nvd, (size_t)s->du * 8);
+    s->diag_dn = DevBuf::view(base + nvu + nvd + s->du, (size_t)s->dd * 8);
+    s->int_val_d = DevBuf::view(base + nvu + nvd + s->du + s->dd, nint * 8);
+    s->int_mask_d = DevBuf::view(base + nd, nint * 8);
     QED_HIP(hipStreamSynchronize(stream()));
     *out = s.release();
   });
@@ -827,6 +1143,16 @@ int qed_sector_dims(const qed_sector* s, uint64_t* dim, uint32_t* du, uint32_t* 
 
 int qed_hv(qed_sector* s, const double* x, double* y) {
   return guard([&] { hv(*s, x, y); });
+}
+
+// sets the column-chunk width and rows per workgroup of the two-pass kernel (variant 6)
+int qed_hv_chunk_params(int width, int rows) {
+  return guard([&] {
+    if (width != 16 && width != 32 && width != 64) fail("chunk width must be 16, 32 or 64");
+    if (rows < 4 || rows > 4096) fail("rows per workgroup out of range");
+    g_chunk_w = width;
+    g_chunk_rows = rows;
+  });
 }
 
 int qed_hv_variant(qed_sector* s, int variant, const double* x, double* y) {
@@ -879,8 +1205,8 @@ int qed_start_vector(qed_sector* s, int seed, double* x_dev) {
     std::vector<double> x(s->D);
     {
       std::lock_guard<std::mutex> lk(g_normal_mutex);
-      std::default_random_engine gen((unsigned)seed);
-      for (uint64_t i = 0; i < s->D; ++i) x[i] = 0.0 + g_normal(gen);
+      const unsigned hc = std::thread::hardware_concurrency();
+      normal_stream((unsigned)seed, s->D, x.data(), s->D < (1u << 16) ? 1 : (int)std::min(16u, std::max(1u, hc)));
     }
     double z = 0.0;
     for (double v : x) z += v * v;
@@ -892,10 +1218,34 @@ int qed_start_vector(qed_sector* s, int seed, double* x_dev) {
   });
 }
 
+// n values of the start-vector stream for `seed` after a reset, computed with the
+// standard library's distribution (which = 0) or the parallel restatement (which = 1,
+// `threads` workers); the state is reset before and after.
+int qed_util_normal_stream(int seed, uint64_t n, int which, int threads, double* out) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(g_normal_mutex);
+    g_normal = NormalState();
+    if (which == 0) {
+      std::normal_distribution<double> nd(0.0, 1.0);
+      std::default_random_engine gen((unsigned)seed);
+      // two calls of n/2 values each, so that a carried-over deviate is exercised
+      const uint64_t h = n / 2;
+      for (uint64_t i = 0; i < h; ++i) out[i] = 0.0 + nd(gen);
+      std::default_random_engine gen2((unsigned)seed + 1);
+      for (uint64_t i = h; i < n; ++i) out[i] = 0.0 + nd(gen2);
+    } else {
+      const uint64_t h = n / 2;
+      normal_stream((unsigned)seed, h, out, threads);
+      normal_stream((unsigned)seed + 1, n - h, out + h, threads);
+    }
+    g_normal = NormalState();
+  });
+}
+
 int qed_reset_start_vector_state(void) {
   return guard([] {
     std::lock_guard<std::mutex> lk(g_normal_mutex);
-    g_normal.reset();
+    g_normal = NormalState();
   });
 }
 

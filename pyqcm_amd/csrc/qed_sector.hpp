@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -38,9 +39,10 @@ struct qed_sector {
 
   int hv_variant = -1;  // -1 = automatic
 
-  // scratch for reductions
-  qed::DevBuf red;      // partial sums
-  qed::DevBuf red_out;  // final scalars (device)
+  // the structure tables (bases, hop lists, ELL) are views into cached tables shared by
+  // every sector with the same half-space structure; the amplitudes are this sector's own
+  std::shared_ptr<void> keep_up, keep_dn;
+  qed::DevBuf valbuf;  // owns up_vals, dn_vals, diag_up, diag_dn, int_val_d
 };
 
 namespace qed {
@@ -107,7 +109,11 @@ struct BatchStats {
   HvBatchConfig hv_config;
   int lanczos_iterations = 0;  // lockstep iterations of the ground-state pass
   int band_steps = 0;          // synchronized steps of the band Lanczos pass
+  // wall time (s) of the phases: sectors + start vectors, ground-state Lanczos,
+  // Green-function start vectors, band Lanczos (of which: host eigenvalue tests), assembly
+  double t_setup = 0, t_gs = 0, t_gf_setup = 0, t_band = 0, t_band_checks = 0, t_assemble = 0;
 };
+BatchStats& last_batch_stats();
 
 // Lanczos ground states of many sectors in lockstep; psi[j]: start vector in, ground
 // state out; the stopping test runs every `chunk` iterations (graph: replay a captured

@@ -99,3 +99,71 @@ class MLP:
 def plaquette_features(values):
     """(U, mu, t') of a parameter dict: the inputs of the 2x2+8-bath warm start."""
     return np.array([values["U"], values["mu"], values.get("tp", 0.0)], float)
+
+
+# ------------------------------------------------------------------ 2x2 + 8-bath problem
+
+PLAQUETTE_COLD_START = np.array([-1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0] + [0.5] * 8)
+
+
+def plaquette_points(n, seed):
+    """Seeded model points: U ~ U[0.5, 12], mu ~ U[0, U], t' ~ U[-0.5, 0.5] (t = 1)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        U = rng.uniform(0.5, 12.0)
+        out.append({"t": 1.0, "tp": rng.uniform(-0.5, 0.5), "U": U, "mu": rng.uniform(0.0, U)})
+    return out
+
+
+def plaquette_problem(name="plaq_ws"):
+    from . import cdmft as C
+    M = C.plaquette_8b_model(name)
+    return C.CDMFTProblem(M, C.PLAQUETTE_VARIA, C.plaquette_sectors)
+
+
+def solve_points(problem, points, x0, batch=64, verbose=False):
+    """Plain CDMFT (batched engine) from the starting points x0 (one per point or one for
+    all); returns (x (n x nvar, canonical gauge), converged, niter, solves)."""
+    from . import cdmft as C
+    x0 = np.atleast_2d(x0)
+    if len(x0) == 1:
+        x0 = np.repeat(x0, len(points), 0)
+    X, conv, nit, ns = [], [], [], []
+    for b0 in range(0, len(points), batch):
+        pts = points[b0:b0 + batch]
+        B = C.BatchedCDMFT(problem, pts, list(x0[b0:b0 + batch]), solver="batched", verbose=verbose)
+        X.extend(C.canonical_bath(np.array(B.x)))
+        conv.extend(B.converged & ~B.failed)
+        nit.extend(B.niter)
+        ns.extend(B.n_solves)
+        if verbose:
+            print("points %d..%d: %d converged" % (b0, b0 + len(pts), int((B.converged & ~B.failed).sum())),
+                  flush=True)
+    return np.array(X), np.array(conv), np.array(nit), np.array(ns)
+
+
+def generate_training_set(n, seed, path, batch=64, verbose=True):
+    """Training data for the 2x2+8-bath warm start: n seeded points solved by the plain
+    loop from PLAQUETTE_COLD_START; saved as .npz (features U, mu, t'; converged baths)."""
+    P = plaquette_problem()
+    pts = plaquette_points(n, seed)
+    X, conv, nit, ns = solve_points(P, pts, PLAQUETTE_COLD_START, batch, verbose)
+    feats = np.array([plaquette_features(v) for v in pts])
+    np.savez(path, features=feats, bath=X, converged=conv, iterations=nit, solves=ns, seed=seed)
+    return feats, X, conv
+
+
+if __name__ == "__main__":
+    import argparse
+    import os
+    ap = argparse.ArgumentParser(description="warm-start training data for the 2x2+8-bath CDMFT")
+    ap.add_argument("--points", type=int, default=384)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "data", "plaquette_warmstart.npz"))
+    a = ap.parse_args()
+    from . import _lib as L
+    L.call("qed_init", 0)
+    os.makedirs(os.path.dirname(a.out), exist_ok=True)
+    generate_training_set(a.points, a.seed, a.out)
+    print("wrote", a.out)

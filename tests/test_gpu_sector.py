@@ -206,3 +206,15 @@ def test_band_lanczos_matches_oracle(gpu_lib):
             Go = O.qmatrix_gf(eo, Qo, z)
             Gg = O.qmatrix_gf(eg, Qg, z)
             assert np.abs(Gg - Go).max() < 1e-8
+
+
+@pytest.mark.parametrize("seed,n", [(0, 1), (0, 7), (1, 100001), (42, 2 ** 20 + 3)])
+def test_parallel_start_vector_stream_equals_standard_library(gpu_lib, seed, n):
+    """The start-vector stream restated for parallel generation (polar method attempts
+    located by jumps of the linear congruential engine) against std::normal_distribution
+    itself, bit for bit, including a deviate carried from one call to the next."""
+    a = np.zeros(n)
+    b = np.zeros(n)
+    gpu_lib.call("qed_util_normal_stream", seed, n, 0, 1, gpu_lib.dp(a))
+    gpu_lib.call("qed_util_normal_stream", seed, n, 1, 8, gpu_lib.dp(b))
+    np.testing.assert_array_equal(a, b)
