@@ -748,9 +748,13 @@ void batch_solve(std::vector<Instance*>& insts, bool graph, int chunk, BatchStat
   }
   if (stats) stats->t_band = wall() - t0;
   t0 = wall();
-  // assemble per (instance, state, spin) exactly as build_qmatrix does
+  // assemble per (instance, state, spin) exactly as build_qmatrix does; an instance whose
+  // Lehmann form fails the unitarity check is reported failed (E0 = NaN, no states) instead
+  // of failing the whole batch (the one-instance path raises the reference's error)
+  std::set<Instance*> bad;
   for (size_t k = 0; k < gf.size(); k += 2) {
     GF& h = gf[k];
+    if (bad.count(h.I)) continue;
     QMat parts[2];
     for (int part = 0; part < 2; ++part) {
       GF& g = gf[k + part];
@@ -768,10 +772,22 @@ void batch_solve(std::vector<Instance*>& insts, bool graph, int chunk, BatchStat
     append(tot, parts[0]);
     append(tot, parts[1]);
     tot.L = h.I->model->n_sites;
-    check_norm(tot, gd("accur_Q_matrix"));
+    try {
+      check_norm(tot, gd("accur_Q_matrix"));
+    } catch (const Error&) {
+      bad.insert(h.I);
+      continue;
+    }
     const double sw = std::sqrt(h.st->weight);
     for (double& v : tot.Q) v *= sw;
     h.st->gf[h.spin] = tot;
+  }
+  gf.clear();
+  for (Instance* I : bad) {
+    I->states.clear();
+    I->GS_energy = I->E0 = std::nan("");
+    I->gs_string = "failed";
+    I->gf_solved = true;
   }
   for (Instance* I : insts)
     if (I->correlated && !I->cf) I->gf_solved = true;

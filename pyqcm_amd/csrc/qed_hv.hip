@@ -286,7 +286,7 @@ __global__ __launch_bounds__(BS) void k_hv_ell_batch(const HvJob* __restrict__ j
 
 // Same product, two adjacent columns per thread: 16-byte loads of the down-spin neighbour
 // rows and of the ELL entry pairs (needs an even up dimension).
-template <int BS, bool DOWN = true, bool XCDMAP = false>
+template <int BS, bool DOWN = true>
 __global__ __launch_bounds__(BS) void k_hv_ell2(HvView v, const uint32_t* __restrict__ up_ell,
                                                 uint32_t Wu, const uint32_t* __restrict__ dn_ell,
                                                 uint32_t Wd, const double* __restrict__ x,
@@ -296,14 +296,7 @@ __global__ __launch_bounds__(BS) void k_hv_ell2(HvView v, const uint32_t* __rest
   const uint32_t nvu = v.up_nvals;
   double* vals_s = smem;
   double* xs = smem + ((nvu + 1u) & ~1u);  // 16-byte aligned row
-  // XCDMAP: workgroup b runs on XCD b % 8; give each XCD one contiguous eighth of the
-  // rows, so that rows and their nearby down-spin neighbours meet in the same L2
-  uint32_t r = blockIdx.x;
-  if (XCDMAP) {
-    const uint32_t per = (v.dd + 7u) / 8u;
-    r = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-    if (r >= v.dd) return;
-  }
+  const uint32_t r = blockIdx.x;
   const double* xr = x + (uint64_t)r * du;
   const uint32_t dh = du / 2;
   for (uint32_t i = threadIdx.x; i < nvu; i += BS) vals_s[i] = v.up_vals[i];
@@ -403,6 +396,65 @@ __global__ __launch_bounds__(256) void k_hv_down_ell_chunks(HvView v, const uint
     yo.x += b0;
     yo.y += b1;
     *yp = yo;
+  }
+}
+
+// Hop-only pass over rows of length n with an in-row hop table (column-major ELL, W
+// entries per position): z[r, :] = H_hops x[r, :] (assignment). On the transposed vector
+// (rows = up states, positions = down states) with the down-spin table this is the
+// down-spin term. Two positions per lane; n even.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_row_hops2(uint32_t n, const uint32_t* __restrict__ ell, uint32_t W,
+                                                 const double* __restrict__ vals, uint32_t nv,
+                                                 const double* __restrict__ x, double* __restrict__ z) {
+  extern __shared__ double smem[];
+  double* vals_s = smem;
+  double* xs = smem + ((nv + 1u) & ~1u);
+  const uint32_t r = blockIdx.x;
+  const double* xr = x + (uint64_t)r * n;
+  const uint32_t nh = n / 2;
+  for (uint32_t i = threadIdx.x; i < nv; i += BS) vals_s[i] = vals[i];
+  for (uint32_t c = threadIdx.x; c < nh; c += BS)
+    reinterpret_cast<double2*>(xs)[c] = reinterpret_cast<const double2*>(xr)[c];
+  __syncthreads();
+  double2* zr = reinterpret_cast<double2*>(z + (uint64_t)r * n);
+  for (uint32_t c2 = threadIdx.x; c2 < nh; c2 += BS) {
+    const uint32_t c = 2 * c2;
+    double a0 = 0.0, a1 = 0.0;
+    for (uint32_t g = 0; g < W; g += kG) {
+      uint2 e[kG];
+#pragma unroll
+      for (int q = 0; q < kG; ++q) e[q] = *reinterpret_cast<const uint2*>(ell + (uint64_t)(g + q) * n + c);
+#pragma unroll
+      for (int q = 0; q < kG; ++q) {
+        a0 += vals_s[e[q].x >> kTargetBits] * xs[e[q].x & kTargetMask];
+        a1 += vals_s[e[q].y >> kTargetBits] * xs[e[q].y & kTargetMask];
+      }
+    }
+    zr[c2] = make_double2(a0, a1);
+  }
+}
+
+// out (cols x rows) = transpose of in (rows x cols), or out += it (ADD), by 64 x 64 tiles
+// staged in LDS (padded against bank conflicts)
+template <bool ADD>
+__global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ in, double* __restrict__ out,
+                                                   uint32_t rows, uint32_t cols) {
+  __shared__ double tile[64][65];
+  const uint32_t c0 = blockIdx.x * 64u, r0 = blockIdx.y * 64u;
+  const uint32_t tx = threadIdx.x & 63u, ty = threadIdx.x >> 6;
+  for (uint32_t i = ty; i < 64u; i += 4u) {
+    const uint32_t r = r0 + i, c = c0 + tx;
+    if (r < rows && c < cols) tile[i][tx] = in[(uint64_t)r * cols + c];
+  }
+  __syncthreads();
+  for (uint32_t i = ty; i < 64u; i += 4u) {
+    const uint32_t c = c0 + i, r = r0 + tx;
+    if (c < cols && r < rows) {
+      double* o = out + (uint64_t)c * rows + r;
+      if (ADD) *o += tile[tx][i];
+      else *o = tile[tx][i];
+    }
   }
 }
 
@@ -624,8 +676,10 @@ static void set_lds_attributes() {
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     (void)hipFuncSetAttribute((const void*)k_hv_ell2<512, false>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    (void)hipFuncSetAttribute((const void*)k_hv_ell2<512, true, true>,
+
+    (void)hipFuncSetAttribute((const void*)k_row_hops2<512>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+
   });
 }
 
@@ -637,6 +691,8 @@ static void set_lds_attributes() {
 //      up dimension is odd or small, or the row does not fit in LDS)
 //   6  two passes: variant 5 without the down-spin term, then the down-spin term by
 //      XCD-local column chunks served from L2 (falls back to 5 where 5 falls back)
+//   10 the down-spin term on the transposed vector: y += (up + diag) x by rows,
+//      x^T by LDS tiles, (down hops) x^T by rows, y += its transpose (falls back to 5)
 int g_chunk_w = 32;      // column-chunk width of variant 6 (16, 32 or 64 doubles)
 int g_chunk_rows = 64;   // rows per workgroup of variant 6
 
@@ -668,11 +724,23 @@ void hv(const qed_sector& s, const double* x, double* y, int variant) {
     }
     variant = 5;
   }
-  if (variant == 7) {
-    const size_t lds2 = ((size_t)s.du + s.up_nvals + 1) * sizeof(double);
-    if (s.du % 2 == 0 && s.du >= 4096 && lds2 <= lds_budget()) {
-      const uint32_t grid = ((s.dd + 7u) / 8u) * 8u;
-      k_hv_ell2<512, true, true><<<grid, 512, lds2, st>>>(v, ue, s.up_w, de, s.dn_w, x, y);
+  if (variant == 10) {
+    // up-spin term and diagonal by rows, the down-spin term on the transposed vector:
+    // y += (up + diag) x; xt = x^T; zt = (down hops) xt by rows; y += zt^T
+    const size_t lds_u = ((size_t)s.du + s.up_nvals + 1) * sizeof(double);
+    const size_t lds_d = ((size_t)s.dd + s.dn_nvals + 1) * sizeof(double);
+    if (s.du % 2 == 0 && s.dd % 2 == 0 && s.du >= 4096 && lds_u <= lds_budget() && lds_d <= lds_budget() &&
+        s.dn_ell_cm.p) {
+      if (s.xt.bytes < s.D * 8) s.xt.alloc(s.D * 8);
+      if (s.zt.bytes < s.D * 8) s.zt.alloc(s.D * 8);
+      double* xt = s.xt.as<double>();
+      double* zt = s.zt.as<double>();
+      k_hv_ell2<512, false><<<s.dd, 512, lds_u, st>>>(v, ue, s.up_w, de, s.dn_w, x, y);
+      const dim3 g1((s.du + 63) / 64, (s.dd + 63) / 64), g2((s.dd + 63) / 64, (s.du + 63) / 64);
+      k_transpose<false><<<g1, 256, 0, st>>>(x, xt, s.dd, s.du);
+      k_row_hops2<512><<<s.du, 512, lds_d, st>>>(s.dd, s.dn_ell_cm.as<uint32_t>(), s.dn_w_cm,
+                                                 s.dn_vals.as<double>(), s.dn_nvals, xt, zt);
+      k_transpose<true><<<g2, 256, 0, st>>>(zt, y, s.du, s.dd);
       QED_HIP(hipGetLastError());
       return;
     }
@@ -1066,6 +1134,10 @@ int qed_sector_create(int L, int n_up, int n_dn, int n_ops, const int* types, co
     if (!s->hubbard_uniform) s->hub_sites = 0;
     auto hu = half_struct(L, n_up, bu, true);
     auto hd = half_struct(L, n_dn, bd, false);
+    auto hdc = half_struct(L, n_dn, bd, true);
+    s->keep_dn_cm = hdc;
+    s->dn_ell_cm = DevBuf::view(hdc->ell.p, hdc->ell.bytes);
+    s->dn_w_cm = hdc->w;
     s->keep_up = hu;
     s->keep_dn = hd;
     s->du = hu->dim;

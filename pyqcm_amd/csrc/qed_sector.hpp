@@ -43,6 +43,12 @@ struct qed_sector {
   // every sector with the same half-space structure; the amplitudes are this sector's own
   std::shared_ptr<void> keep_up, keep_dn;
   qed::DevBuf valbuf;  // owns up_vals, dn_vals, diag_up, diag_dn, int_val_d
+  // down-spin hop table in the column-major ELL layout (for the transposed pass), and
+  // the two scratch vectors of that pass (allocated on first use)
+  std::shared_ptr<void> keep_dn_cm;
+  qed::DevBuf dn_ell_cm;
+  uint32_t dn_w_cm = 0;
+  mutable qed::DevBuf xt, zt;
 };
 
 namespace qed {
