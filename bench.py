@@ -43,6 +43,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--sweep", action="store_true", help="time every H.v variant (stderr)")
     ap.add_argument("--cdmft", type=int, default=64, help="CDMFT points for the warm-start leg (0: skip)")
+    ap.add_argument("--cdmft-batch", type=int, default=32, help="CDMFT problems active at a time")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 2 GiB five times (for scaling HBM counters)")
     args = ap.parse_args()
@@ -119,7 +120,7 @@ def main():
     dx.free()
     dy.free()
     del S
-    cdmft = cdmft_leg(args.cdmft) if args.cdmft > 0 else None
+    cdmft = cdmft_leg(args.cdmft, args.cdmft_batch) if args.cdmft > 0 else None
 
     out = {
         "metric": "H·v GB/s (16-site ED, % HBM peak); CDMFT solves/s learned warm-start vs plain",
@@ -172,18 +173,18 @@ def traffic_from_profile():
     return None
 
 
-def cdmft_leg(n):
-    """Plain (cold start) vs learned warm start, batched CDMFT on n seeded points."""
+def cdmft_leg(n, batch):
+    """Plain (cold start) vs learned warm start, batched CDMFT on n seeded points with
+    `batch` problems active at a time (a converged or failed problem is replaced by the
+    next queued one)."""
     from pyqcm_amd import cdmft as C
     from pyqcm_amd import warmstart as W
-    path = os.path.join(ROOT, "pyqcm_amd", "data", "plaquette_warmstart.npz")
-    d = np.load(path)
-    ok = d["converged"].astype(bool)
+    X, Y = W.load_training_set()
     net = W.MLP(3, 128, 16, seed=0)
     t0 = time.perf_counter()
-    hist = net.fit(d["features"][ok], d["bath"][ok], epochs=1500, batch=64, lr=2e-3)
+    hist = net.fit(X, Y, epochs=1500, batch=64, lr=2e-3)
     t_train = time.perf_counter() - t0
-    log("MLP trained on %d points in %.1f s (loss %.3g -> %.3g)" % (ok.sum(), t_train, hist[0], hist[-1]))
+    log("MLP trained on %d points in %.1f s (loss %.3g -> %.3g)" % (len(X), t_train, hist[0], hist[-1]))
     P = W.plaquette_problem("plaq_bench")
     pts = W.plaquette_points(n, 20261202)
     feats = np.array([W.plaquette_features(v) for v in pts])
@@ -194,7 +195,7 @@ def cdmft_leg(n):
     res = {}
     for name, x0 in (("plain", cold), ("warm", warm)):
         t0 = time.perf_counter()
-        B = C.BatchedCDMFT(P, pts, x0, solver="batched")
+        B = C.BatchedCDMFT(P, pts, x0, solver="batched", capacity=batch)
         dt = time.perf_counter() - t0 + (t_pred if name == "warm" else 0.0)
         good = B.converged & ~B.failed
         res[name] = {"converged": int(good.sum()), "failed": int((~good).sum()), "seconds": round(dt, 2),
@@ -208,12 +209,13 @@ def cdmft_leg(n):
     dx = np.abs(C.canonical_bath(res["plain_x"][both]) - C.canonical_bath(res["warm_x"][both]))
     same = dx.max(1) < 1e-2 if both.any() else np.zeros(0, bool)
     return {
-        "workload": "2x2 plaquette + 8 bath orbitals, %d seeded points, sectors N=8..16 S=0, "
-                    "Legendre (1,10,5,10,5) grid, 32x32 k-grid, accur 1e-4 on parameters" % n,
+        "workload": "2x2 plaquette + 8 bath orbitals, %d seeded points (%d active at a time), sectors "
+                    "N=8..16 S=0, Legendre (1,10,5,10,5) grid, 32x32 k-grid, accur 1e-4 on parameters, "
+                    "Broyden maxiter 32" % (n, batch),
         "plain": res["plain"], "warm": res["warm"],
         "speedup_solves_per_s": round(res["warm"]["solves_per_s"] / max(res["plain"]["solves_per_s"], 1e-12), 3),
         "same_fixed_point": "%d of %d points converged by both" % (int(same.sum()), int(both.sum())),
-        "mlp": {"train_points": int(ok.sum()), "train_s": round(t_train, 2), "final_loss": float(hist[-1]),
+        "mlp": {"train_points": int(len(X)), "train_s": round(t_train, 2), "final_loss": float(hist[-1]),
                 "inference": "bf16 v_mfma_f32_16x16x32_bf16"},
     }
 

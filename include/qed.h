@@ -101,6 +101,24 @@ int qed_band_lanczos(qed_sector* s, const double* phi_dev, int p, int M0, int ma
                      double accur_deflation, double accur_band_lanczos, double* evals, double* Q,
                      int cap, int* M);
 
+/* Block Lanczos on p orthonormal start vectors q0_dev (p consecutive sector vectors).
+ * qr != 0: upper-triangular off-diagonal blocks from modified Gram-Schmidt; qr == 0 (the
+ * reference's default): symmetric blocks from the polar decomposition of the residual
+ * block. M0 <= 0 uses 14 p ln(dim), capped by max_iter_BL. On exit M floors: A and B hold
+ * M blocks of p x p (column-major, block j at j p^2); capacity cap floors.
+ * Replaces blockLanczos / blockLanczosSVD (src_ed/Hamiltonian/Lanczos.hpp:426-779). */
+int qed_block_lanczos(qed_sector* s, const double* q0_dev, int p, int M0, int max_iter_BL,
+                      double accur_deflation, double accur_band_lanczos, int qr, double* A, double* B,
+                      int cap, int* M);
+
+/* Matrix continued fraction G(z) = W^T F_0(z) W with F_M = 0 and
+ * F_j = (z - A_j - B_j^T F_{j+1} B_j)^-1 (B_{M-1} unused), added to G (transposed when
+ * transpose != 0) for nz frequencies z (complex interleaved). A, B: M blocks p x p; W: p x q
+ * (column-major); G: nz blocks q x q complex column-major. p, q <= 24.
+ * Replaces matrix_continued_fraction::evaluate (src_ed/matrix_continued_fraction.hpp:139-182). */
+int qed_mcf_eval(int p, int q, int M, const double* A, const double* B, const double* W, int nz,
+                 const double* z, int transpose, double* G);
+
 /* Davidson-Liu for the k lowest states (device vectors psi_dev: k consecutive sector
  * vectors, out). Replaces Davidson (src_ed/Hamiltonian/Davidson.hpp:38-230). */
 int qed_davidson(qed_sector* s, int k, double accur, int max_iter, double* evals, double* psi_dev,

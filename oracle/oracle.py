@@ -482,6 +482,141 @@ def cf_evaluate(a, b, z):
 
 
 # ----------------------------------------------------------------------------------------
+# block Lanczos and the matrix continued fraction (GF_method 'M')
+
+
+def mgs_weight(phi, accur_deflation=1e-7):
+    """Modified Gram-Schmidt of the start block with its upper-triangular factor W
+    (phi[l] = sum_{k<=l} q[k] W[k, l]); stops at the first column whose norm falls below
+    accur_deflation (src_ed/model_instance.hpp:885-905). Returns (q, W[:p_actual, :])."""
+    q = [np.array(v, np.float64, copy=True) for v in phi]
+    p = len(q)
+    W = np.zeros((p, p))
+    pa = p
+    for l in range(p):
+        for k in range(l):
+            z = q[k] @ q[l]
+            W[k, l] = z
+            q[l] = q[l] - z * q[k]
+        nrm = np.linalg.norm(q[l])
+        if nrm < accur_deflation:
+            pa = l
+            break
+        W[l, l] = nrm
+        q[l] = q[l] / nrm
+    return q[:pa], W[:pa, :]
+
+
+def _block_lowest(A, B, j, p):
+    sz = (j + 1) * p
+    T = np.zeros((sz, sz))
+    for jj in range(j + 1):
+        T[jj * p:(jj + 1) * p, jj * p:(jj + 1) * p] = A[jj]
+    for jj in range(j):
+        T[(jj + 1) * p:(jj + 2) * p, jj * p:(jj + 1) * p] = B[jj]
+        T[jj * p:(jj + 1) * p, (jj + 1) * p:(jj + 2) * p] = B[jj].T
+    return np.linalg.eigvalsh(T)[0]
+
+
+def block_lanczos(hv, q0, qr=False, accur_deflation=1e-7, accur_band_lanczos=1e-12, max_iter_BL=600,
+                  M0=None):
+    """Block Lanczos on p start vectors: H Q_j = Q_{j-1} B_{j-1}^T + Q_j A_j + Q_{j+1} B_j.
+    qr=True: B_j upper triangular from modified Gram-Schmidt of the residual block
+    (blockLanczos, src_ed/Hamiltonian/Lanczos.hpp:426-603); qr=False: B_j the symmetric
+    square root of the residual Gram matrix (polar decomposition, blockLanczosSVD,
+    src_ed/Hamiltonian/Lanczos.hpp:604-779). The lowest eigenvalue of the block-tridiagonal
+    projection is tested every p steps from step 2p. Returns the lists (A, B)."""
+    p = len(q0)
+    dim = len(q0[0])
+    if M0 is None:
+        M0 = int(14 * p * np.log(1.0 * dim))  # src_ed/model_instance.hpp:929
+    M0 = min(M0, max_iter_BL)
+    Qc = [np.array(v, np.float64, copy=True) for v in q0]
+    for l in range(p):  # the start block is orthonormalized again
+        for k in range(l):
+            Qc[l] = Qc[l] - (Qc[k] @ Qc[l]) * Qc[k]
+        nrm = np.linalg.norm(Qc[l])
+        if nrm < accur_deflation:
+            raise ValueError("block Lanczos: rank-deficient start block")
+        Qc[l] = Qc[l] / nrm
+    Qp = None
+    A, B = [], []
+    ev_old = 1e12
+    for j in range(M0):
+        Z = [hv(Qc[k]) for k in range(p)]
+        Aj = np.zeros((p, p))
+        for l in range(p):
+            for k in range(l + 1):
+                Aj[k, l] = Aj[l, k] = Qc[k] @ Z[l]
+        A.append(Aj)
+        for l in range(p):
+            for k in range(p):
+                Z[l] = Z[l] - Aj[k, l] * Qc[k]
+        if j > 0:
+            Bp = B[-1]
+            for l in range(p):
+                for k in range(p):
+                    Z[l] = Z[l] - Bp[l, k] * Qp[k]
+        Bj = np.zeros((p, p))
+        singular = False
+        if qr:
+            for l in range(p):
+                for k in range(l):
+                    z = Z[k] @ Z[l]
+                    Bj[k, l] = z
+                    Z[l] = Z[l] - z * Z[k]
+                nrm = np.linalg.norm(Z[l])
+                if nrm < accur_deflation:
+                    for ll in range(l, p):
+                        Z[ll] = np.zeros(dim)
+                    singular = True
+                    break
+                Bj[l, l] = nrm
+                Z[l] = Z[l] / nrm
+            B.append(Bj)
+            Qn = Z
+        else:
+            G = np.array([[Z[k] @ Z[l] for l in range(p)] for k in range(p)])
+            D, V = np.linalg.eigh(G)
+            singular = bool(np.any(D < accur_deflation ** 2))
+            Bj = (V * np.sqrt(np.maximum(D, 0.0))) @ V.T
+            B.append(Bj)
+            if singular:
+                break
+            Binv = (V / np.sqrt(D)) @ V.T
+            Qn = [sum(Binv[k, l] * Z[k] for k in range(p)) for l in range(p)]
+        if (j + 1) >= 2 * p and (j + 1) % p == 0:
+            e0 = _block_lowest(A, B, j, p)
+            delta = abs(e0 - ev_old)
+            ev_old = e0
+            if delta < accur_band_lanczos:
+                break
+        if singular:
+            break
+        Qp, Qc = Qc, Qn
+    return A, B
+
+
+def mcf_shift(A, e0, create):
+    """Energy shift of the diagonal blocks: A - e0 for particles, e0 - A for holes
+    (src_ed/matrix_continued_fraction.hpp:93-115)."""
+    I = np.eye(A[0].shape[0])
+    return [(a - e0 * I) if create else (e0 * I - a) for a in A]
+
+
+def mcf_evaluate(A, B, W, z):
+    """G(z) = W^T F_0 W with F_M = 0, F_j = (z - A_j - B_j^T F_{j+1} B_j)^-1; the last B is
+    the truncation residual and is not used (src_ed/matrix_continued_fraction.hpp:139-182)."""
+    M = len(A)
+    p = A[0].shape[0]
+    F = np.zeros((p, p), complex)
+    for j in range(M - 1, -1, -1):
+        S = B[j].T @ F @ B[j] if j < M - 1 else 0.0
+        F = np.linalg.inv(z * np.eye(p) - A[j] - S)
+    return W.T @ F @ W
+
+
+# ----------------------------------------------------------------------------------------
 # lattice kernels (dense numpy, one (k, w) at a time)
 
 

@@ -414,18 +414,23 @@ class CDMFT:
 
 
 class BatchedCDMFT:
-    """B independent CDMFT problems in lockstep (same model, different parameter values
-    and starting points). Every instance follows the plain loop's semantics; per outer
-    iteration the solves of the active instances and their bath fits are batched.
+    """Many independent CDMFT problems (same model, different parameter values and
+    starting points), each following the plain loop's semantics. Up to `capacity`
+    problems are active at a time; each round, the impurity solves and bath fits of all
+    active problems go to the device together, and a problem that converges or fails
+    leaves the batch and the next queued problem takes its place, so the batch stays full
+    until the queue is empty.
 
     `solver(values_list, x_list)` returns the host functions of the listed instances;
-    the default solves them one after the other through `problem`."""
+    "batched" solves them in one device call (qcm.batch_solve); the default solves them
+    one after the other through `problem`."""
 
     def __init__(self, problem, values_list, x0_list, maxiter=32, depth=2, accur=1e-4,
                  accur_bath=1e-6, accur_dist=1e-8, initial_step=0.1, max_function_eval=1_000_000,
-                 solver=None, verbose=False, fit="device"):
+                 solver=None, verbose=False, fit="device", capacity=None):
         self.problem = problem
         self.fit_where = fit
+        self.capacity = capacity or len(values_list)
         B = len(values_list)
         self.values = [dict(v) for v in values_list]
         self.x = [np.array(x, float) for x in x0_list]
@@ -484,57 +489,61 @@ class BatchedCDMFT:
         return out
 
     def _run(self):
-        """Broyden per instance, advanced in lockstep (pyqcm/__init__.py:2844-2898)."""
+        """Broyden per problem (pyqcm/__init__.py:2844-2898) as a state machine: each round
+        evaluates F at the pending point of every active problem (the start point first),
+        then applies the inverse-Jacobian update, the step and the convergence tests."""
+        from collections import deque
         B = len(self.x)
         n = len(self.x[0])
         Ij = [np.eye(n) for _ in range(B)]
-        idx = list(range(B))
-        xs_new = self._steps(idx, [self.x[i] for i in idx])
-        f0 = [self.x[i] - xs_new[j] for j, i in enumerate(idx)]
-        f = [fi.copy() for fi in f0]
-        x = [self.x[i].copy() for i in idx]
+        f0 = [None] * B
+        dxs = [None] * B
         it = np.zeros(B, int)
-        active = list(range(B))
-        while active:
-            need = []
-            for i in active:
+        started = np.zeros(B, bool)
+        queue = deque(range(B))
+        active = []
+        self.rounds = 0
+        while active or queue:
+            while queue and len(active) < self.capacity:
+                active.append(queue.popleft())
+            pts = [np.copy(self.x[i]) for i in active]
+            new = self._steps(active, pts)
+            self.rounds += 1
+            nxt = []
+            for i, x, xn in zip(active, pts, new):
                 if self.failed[i]:
                     continue
-                dx = -Ij[i] @ f[i]
+                fi = x - xn
+                if not started[i]:
+                    started[i] = True
+                else:
+                    df = fi - f0[i]
+                    dx = dxs[i]
+                    T = np.outer(dx - Ij[i] @ df, dx)
+                    Ij[i] = Ij[i] + T @ Ij[i] / (dx @ (Ij[i] @ df))
+                    it[i] += 1
+                    if it[i] > self.maxiter:
+                        self.failed[i] = True
+                        continue
+                f0[i] = fi.copy()
+                dx = -Ij[i] @ fi
                 if np.any(np.isnan(dx)):
                     self.failed[i] = True
                     continue
-                x[i] = x[i] + dx
-                self.x[i] = x[i]
+                dxs[i] = dx
+                self.x[i] = x + dx
                 if np.linalg.norm(dx) < 1e-6:
-                    self.conv[i].test(np.copy(x[i]))
+                    self.conv[i].test(np.copy(self.x[i]))
                     self.converged[i] = True
                     continue
-                if self.conv[i].test(np.copy(x[i])):
+                if self.conv[i].test(np.copy(self.x[i])):
                     self.converged[i] = True
-                    continue
-                need.append((i, dx))
-            if not need:
-                break
-            ids = [i for i, _ in need]
-            new = self._steps(ids, [x[i] for i in ids])
-            nxt = []
-            for (i, dx), xn in zip(need, new):
-                fi = x[i] - xn
-                df = fi - f0[i]
-                f0[i] = fi.copy()
-                f[i] = fi
-                T = np.outer(dx - Ij[i] @ df, dx)
-                Ij[i] = Ij[i] + T @ Ij[i] / (dx @ (Ij[i] @ df))
-                it[i] += 1
-                if it[i] > self.maxiter:
-                    self.failed[i] = True
                     continue
                 nxt.append(i)
             active = nxt
             if self.verbose:
-                print("batched CDMFT: %d active, %d converged" % (len(active), self.converged.sum()),
-                      flush=True)
+                print("batched CDMFT: %d active, %d queued, %d converged, %d failed"
+                      % (len(active), len(queue), self.converged.sum(), self.failed.sum()), flush=True)
         self.niter = it
 
 

@@ -83,11 +83,22 @@ __global__ __launch_bounds__(kBT) void k_mdot_part(const MDot* __restrict__ d, i
   const MDot& D = d[j];
   const uint64_t i0 = (uint64_t)(b - D.blk0) * kChunk;
   const uint64_t i1 = min(i0 + kChunk, D.n);
-  for (int q = 0; q < D.m; ++q) {
-    double s = 0.0;
-    for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBT) s += D.x[i] * D.y[q][i];
-    s = block_sum(s, red);
-    if (threadIdx.x == 0) part[(size_t)b * kBM + q] = s;
+  const int m = D.m;
+  double acc[kBM];
+#pragma unroll
+  for (int q = 0; q < kBM; ++q) acc[q] = 0.0;
+  for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBT) {  // x read once
+    const double xv = D.x[i];
+#pragma unroll
+    for (int q = 0; q < kBM; ++q)
+      if (q < m) acc[q] += xv * D.y[q][i];
+  }
+#pragma unroll
+  for (int q = 0; q < kBM; ++q) {
+    if (q < m) {  // uniform across the block
+      const double s = block_sum(acc[q], red);
+      if (threadIdx.x == 0) part[(size_t)b * kBM + q] = s;
+    }
   }
 }
 
@@ -121,11 +132,15 @@ __global__ __launch_bounds__(kBT) void k_maxpy(const MAxpy* __restrict__ d, int 
   const MAxpy& D = d[j];
   const uint64_t i0 = (uint64_t)(b - D.blk0) * kChunk;
   const uint64_t i1 = min(i0 + kChunk, D.n);
+  const int m = D.m;
   double c[kBM];
-  for (int q = 0; q < D.m; ++q) c[q] = D.sgn * *D.c[q];
+#pragma unroll
+  for (int q = 0; q < kBM; ++q) c[q] = (q < m) ? D.sgn * *D.c[q] : 0.0;
   for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBT) {
     double v = D.y[i] * D.scale;
-    for (int q = 0; q < D.m; ++q) v += c[q] * D.x[q][i];
+#pragma unroll
+    for (int q = 0; q < kBM; ++q)
+      if (q < m) v += c[q] * D.x[q][i];
     D.y[i] = v;
   }
 }
@@ -146,10 +161,15 @@ __global__ __launch_bounds__(kBT) void k_mrev(const MRev* __restrict__ d, int nj
   const MRev& D = d[j];
   const uint64_t i0 = (uint64_t)(b - D.blk0) * kChunk;
   const uint64_t i1 = min(i0 + kChunk, D.n);
-  for (int q = 0; q < D.m; ++q) {
-    const double c = *D.c[q];
-    double* y = D.y[q];
-    for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBT) y[i] -= c * D.x[i];
+  const int m = D.m;
+  double c[kBM];
+#pragma unroll
+  for (int q = 0; q < kBM; ++q) c[q] = (q < m) ? *D.c[q] : 0.0;
+  for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBT) {  // x read once
+    const double xv = D.x[i];
+#pragma unroll
+    for (int q = 0; q < kBM; ++q)
+      if (q < m) D.y[q][i] -= c[q] * xv;
   }
 }
 
@@ -582,7 +602,8 @@ struct BandJob {
   std::vector<const double*> ph;
   std::vector<double> t, P;
   std::vector<int> defl, I;
-  double ev_old = 1e12;
+  double ev_old = 1e12, ev_old2 = 1e12;
+  std::vector<double> evec;  // lowest eigenvector of the last test (warm start)
   std::vector<std::unique_ptr<DevBuf>> owned;
   std::vector<double*> free_list;
   // device scalars of the step: [0..p0] multi-dot, [p0+1 .. p0+pc] pending-block overlaps,
@@ -729,12 +750,27 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
         b.prev_j = -1;
         if (b.check_pending) {
           b.check_pending = false;
-          std::vector<double> H((size_t)pj * pj), e(pj);
-          for (int c = 0; c < pj; ++c)
-            for (int r = 0; r < pj; ++r) H[(size_t)c * pj + r] = b.T(r, c);
-          sym_eigen(pj, H.data(), e.data(), false);
-          const double test = std::fabs(e[0] - b.ev_old);
-          b.ev_old = e[0];
+          // lowest eigenvalue of the pj x pj projected matrix: shifted inverse iteration on
+          // the band (no deflated vectors) from the previous eigenvector, the shift below
+          // the previous value by more than its last decrease; dense eigensolver otherwise
+          double e0 = 0.0;
+          bool ok = false;
+          if (b.I.empty() && !b.evec.empty() && b.ev_old2 < 1e11) {
+            const double drop = std::fabs(b.ev_old2 - b.ev_old);
+            const double sigma = b.ev_old - std::max(4.0 * drop, 1e-9 * std::max(1.0, std::fabs(b.ev_old)));
+            ok = band_lowest([&](int r, int c) { return b.T(r, c); }, pj, b.p0, sigma, 3, b.evec, e0);
+          }
+          if (!ok) {
+            std::vector<double> H((size_t)pj * pj), e(pj);
+            for (int c = 0; c < pj; ++c)
+              for (int r = 0; r < pj; ++r) H[(size_t)c * pj + r] = b.T(r, c);
+            sym_eigen(pj, H.data(), e.data(), true);
+            e0 = e[0];
+            b.evec.assign(H.begin(), H.begin() + pj);
+          }
+          const double test = std::fabs(e0 - b.ev_old);
+          b.ev_old2 = b.ev_old;
+          b.ev_old = e0;
           if (test < accur_band_lanczos) {
             b.j = pj;  // stop with the pj x pj projected matrix
             b.done = true;

@@ -36,7 +36,8 @@ struct Globals {
                                   {"temperature", 0.0},         {"minimum_weight", 0.01},
                                   {"accur_Davidson", 1e-5},     {"accur_continued_fraction", 0.01},
                                   {"band_lanczos_minimum_gap", 1e-5}};
-  std::map<std::string, bool> b{{"verb_ED", false}, {"no_degenerate_BL", false}, {"continued_fraction", false}};
+  std::map<std::string, bool> b{{"verb_ED", false}, {"no_degenerate_BL", false}, {"continued_fraction", false},
+                              {"block_Lanczos_QR", false}};
   std::map<std::string, char> c{{"GF_method", 'L'}, {"Ground_state_method", 'L'}, {"Hamiltonian_format", 'F'}};
 };
 Globals& G() {
@@ -81,12 +82,21 @@ struct CFrac {
   bool set = false;
 };
 
+// matrix continued fraction of one (spin, hole/particle) part: M floors of pa x pa blocks
+// (energy-shifted A, B) and the pa x q weight W (sqrt(weight) folded in)
+struct MCFrac {
+  int pa = 0, q = 0;
+  std::vector<double> A, B, W;
+  int M() const { return pa ? (int)(A.size() / ((size_t)pa * pa)) : 0; }
+};
+
 struct State {
   int nup = 0, ndn = 0;
   double energy = 0.0, weight = 1.0;
   DevBuf psi;
   QMat gf[2];
   std::vector<CFrac> cfe[2], cfh[2];  // n x n (o1 >= o2), electron / hole parts
+  MCFrac mcf[2][2];                   // [spin][0: holes, 1: particles]
 };
 
 struct Instance {
@@ -96,7 +106,8 @@ struct Instance {
   std::vector<std::pair<int, int>> sectors;  // (N, S)
   int mixing = 0;
   bool correlated = false, gs_solved = false, gf_solved = false;
-  bool cf = false;
+  bool cf = false;   // continued fractions ('F')
+  bool mcf = false;  // matrix continued fractions ('M')
   std::vector<std::unique_ptr<State>> states;
   double GS_energy = 0.0, E0 = 0.0, SEF_bath = 0.0;
   // one-body matrices (index 0 up, 1 down), bath diagonalized
@@ -404,6 +415,70 @@ void build_cf(Instance& I, State& st, int spin) {
   }
 }
 
+// matrix continued fractions (model_instance::build_mcf, src_ed/model_instance.hpp:846-946):
+// start block c_i^(+-)|psi>, its modified Gram-Schmidt factor W (first pa columns that do
+// not deflate), block Lanczos on the orthonormal block, diagonal blocks shifted by E0
+void build_mcf(Instance& I, State& st, int spin) {
+  Model& M = *I.model;
+  const int p = M.n_sites, L = M.L;
+  const double acc = gd("accur_deflation");
+  for (int part = 0; part < 2; ++part) {
+    const int pm = 2 * part - 1;
+    MCFrac& f = st.mcf[spin][part];
+    f = MCFrac();
+    const int tnup = st.nup + (spin == 0 ? pm : 0), tndn = st.ndn + (spin == 1 ? pm : 0);
+    if (!sector_valid(I, tnup, tndn)) continue;
+    SectorPtr T = make_sector(I, tnup, tndn);
+    if (T->D == 0) continue;
+    const uint64_t D = T->D;
+    DevBuf phi((size_t)p * D * 8);
+    double* ph = phi.as<double>();
+    QED_HIP(hipMemsetAsync(ph, 0, (size_t)p * D * 8, stream()));
+    for (int i = 0; i < p; ++i)
+      if (qed_apply_c(L, st.nup, st.ndn, i + spin * L, pm, 1.0, st.psi.as<double>(), ph + (uint64_t)i * D) != 0)
+        fail(qed_last_error());
+    std::vector<double> W((size_t)p * p, 0.0);
+    int pa = p;
+    for (int l = 0; l < p; ++l) {
+      double* xl = ph + (uint64_t)l * D;
+      for (int k = 0; k < l; ++k) {
+        const double* xk = ph + (uint64_t)k * D;
+        const double z = dot(xk, xl, D);
+        W[k + (size_t)p * l] = z;
+        axpy(-z, xk, xl, D);
+      }
+      const double nrm = nrm2(xl, D);
+      if (nrm < acc) {
+        pa = l;
+        break;
+      }
+      W[l + (size_t)p * l] = nrm;
+      scal(1.0 / nrm, xl, D);
+    }
+    if (pa == 0) continue;
+    const double sw = std::sqrt(st.weight);
+    std::vector<std::vector<double>> A, B;
+    const int M0 = (int)(14 * pa * std::log(1.0 * D));
+    block_lanczos(*T, ph, pa, M0, gi("max_iter_BL"), acc, gd("accur_band_lanczos"), gb("block_Lanczos_QR"), A, B);
+    if (A.empty()) continue;
+    f.pa = pa;
+    f.q = p;
+    f.W.assign((size_t)pa * p, 0.0);
+    for (int l = 0; l < p; ++l)
+      for (int k = 0; k < pa; ++k) f.W[k + (size_t)pa * l] = sw * W[k + (size_t)p * l];
+    const size_t pp = (size_t)pa * pa;
+    f.A.resize(A.size() * pp);
+    f.B.resize(A.size() * pp);
+    for (size_t j = 0; j < A.size(); ++j)
+      for (size_t t = 0; t < pp; ++t) {
+        const bool diag = (t % pa) == (t / pa);
+        const double a = A[j][t];
+        f.A[j * pp + t] = (pm == 1) ? a - (diag ? st.energy : 0.0) : (diag ? st.energy : 0.0) - a;
+        f.B[j * pp + t] = B[j][t];
+      }
+  }
+}
+
 std::string sector_name(int N, int S) {
   return "R0:N" + std::to_string(N) + ":S" + std::to_string(S);
 }
@@ -572,17 +647,26 @@ void finish_states(Instance& I, std::vector<std::unique_ptr<State>>& states) {
   set_hopping_matrix(I);
 }
 
+// Green-function format from GF_method (src_ed/model_instance_base.cpp:25-38):
+// 'F' continued fractions, 'M' matrix continued fractions, otherwise the Lehmann
+// Q-matrix; continued_fraction = true turns 'L' into 'F'
+void set_gf_format(Instance& I) {
+  char gf = gc("GF_method");
+  if (gf == 'L' && gb("continued_fraction")) gf = 'F';
+  I.cf = gf == 'F';
+  I.mcf = gf == 'M';
+}
+
 void green_function_solve(Instance& I) {
   if (I.gf_solved) return;
   low_energy_states(I);
   if (!I.correlated) return;
-  I.cf = gc("GF_method") == 'F' || gb("continued_fraction");
+  set_gf_format(I);
   for (auto& s : I.states) {
-    if (I.cf) build_cf(I, *s, 0);
-    else build_qmatrix(I, *s, 0);
-    if (I.mixing & 4) {
-      if (I.cf) build_cf(I, *s, 1);
-      else build_qmatrix(I, *s, 1);
+    for (int spin = 0; spin < ((I.mixing & 4) ? 2 : 1); ++spin) {
+      if (I.cf) build_cf(I, *s, spin);
+      else if (I.mcf) build_mcf(I, *s, spin);
+      else build_qmatrix(I, *s, spin);
     }
   }
   I.gf_solved = true;
@@ -686,8 +770,8 @@ void batch_solve(std::vector<Instance*>& insts, bool graph, int chunk, BatchStat
   std::vector<GF> gf;
   for (Instance* I : insts) {
     if (I->gf_solved || !I->correlated) continue;
-    I->cf = gc("GF_method") == 'F' || gb("continued_fraction");
-    if (I->cf) {
+    set_gf_format(*I);
+    if (I->cf || I->mcf) {
       green_function_solve(*I);
       continue;
     }
@@ -811,7 +895,14 @@ void green_function(Instance& I, int nz, const double* z, int spin_down, double*
   const int d = I.model->n_sites;
   std::fill(out, out + (size_t)2 * nz * d * d, 0.0);
   for (auto& s : I.states) {
-    if (!I.cf || !I.correlated) {
+    if (I.mcf && I.correlated) {
+      // particles added as they are, holes transposed (src_ed/mcf_set.hpp:111-131)
+      for (int part = 1; part >= 0; --part) {
+        const MCFrac& f = s->mcf[sp][part];
+        if (f.M() == 0) continue;
+        mcf_eval(f.pa, f.q, f.M(), f.A.data(), f.B.data(), f.W.data(), nz, z, part == 0, out);
+      }
+    } else if (!I.cf || !I.correlated) {
       const QMat& q = s->gf[sp];
       if (q.M() == 0) continue;
       if (qed_qmatrix_gf(d, q.M(), q.e.data(), q.Q.data(), nz, z, out) != 0) fail(qed_last_error());
@@ -1176,7 +1267,8 @@ int qed_ed_qmatrix(uint64_t label, int spin_down, int* M, double* e, double* Q) 
   return guard([&] {
     Instance& I = inst(label);
     green_function_solve(I);
-    if (I.cf && I.correlated) fail("the Q-matrix is not available with the continued-fraction Green function");
+    if ((I.cf || I.mcf) && I.correlated)
+      fail("the Q-matrix is not available with the continued-fraction Green function");
     const int sp = spin_index(I, spin_down);
     const int d = I.model->n_sites;
     int tot = 0;

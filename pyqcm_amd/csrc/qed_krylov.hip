@@ -519,11 +519,11 @@ __global__ void k_combine(PtrList in, int m, MutPtrList out, int K, const double
                           uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    double v[kMaxMulti];
-    for (int p = 0; p < m; ++p) v[p] = in.p[p][i];
+    // (the inputs of one index are re-read per output from L1/L2 rather than kept in a
+    // register array indexed at run time, which would live in scratch memory)
     for (int k = 0; k < K; ++k) {
       double acc = 0.0;
-      for (int p = 0; p < m; ++p) acc += C[p + m * k] * v[p];
+      for (int p = 0; p < m; ++p) acc += C[p + m * k] * in.p[p][i];
       out.p[k][i] = acc;
     }
   }

@@ -164,6 +164,15 @@ int band_lanczos(qed_sector& s, const double* phi, int p0, int M0, int max_iter_
                  std::vector<double>& Q);
 void lanczos_green(qed_sector& s, const double* psi, double accur_cf, int max_iter,
                    std::vector<double>& a, std::vector<double>& b);
+// block Lanczos on p orthonormal device vectors q0 (p x D); floors A, B (p x p each,
+// column-major): QR blocks if qr, polar (symmetric) blocks otherwise
+int block_lanczos(qed_sector& s, const double* q0, int p, int M0, int max_iter_BL, double accur_deflation,
+                  double accur_band_lanczos, bool qr, std::vector<std::vector<double>>& A,
+                  std::vector<std::vector<double>>& B);
+// G[z] += W^T F_0(z) W (transposed if transpose) for the matrix continued fraction with M
+// floors of pa x pa blocks; W pa x q; host arrays, G q x q complex column-major per z
+void mcf_eval(int pa, int q, int M, const double* A, const double* B, const double* W, int nz, const double* z,
+              bool transpose, double* G);
 void davidson(qed_sector& s, int k, double accur, int max_iter, std::vector<double>& evals,
               double* psi, int* niter);
 std::recursive_mutex& lib_mutex();

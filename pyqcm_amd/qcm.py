@@ -194,7 +194,7 @@ def cluster_averages(label=0):
 
 # ---------------------------------------------------------------- global parameters
 
-_BOOL = {"verb_ED", "no_degenerate_BL", "continued_fraction"}
+_BOOL = {"verb_ED", "no_degenerate_BL", "continued_fraction", "block_Lanczos_QR"}
 
 
 def set_global_parameter(name, value=None):

@@ -11,6 +11,7 @@ prediction, since convergence is still decided by its own tests.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -123,44 +124,62 @@ def plaquette_problem(name="plaq_ws"):
 
 
 def solve_points(problem, points, x0, batch=64, verbose=False):
-    """Plain CDMFT (batched engine) from the starting points x0 (one per point or one for
-    all); returns (x (n x nvar, canonical gauge), converged, niter, solves)."""
+    """Plain CDMFT (batched engine, `batch` problems active at a time) from the starting
+    points x0 (one per point or one for all); returns (x (n x nvar, canonical gauge),
+    converged, niter, solves)."""
     from . import cdmft as C
     x0 = np.atleast_2d(x0)
     if len(x0) == 1:
         x0 = np.repeat(x0, len(points), 0)
-    X, conv, nit, ns = [], [], [], []
-    for b0 in range(0, len(points), batch):
-        pts = points[b0:b0 + batch]
-        B = C.BatchedCDMFT(problem, pts, list(x0[b0:b0 + batch]), solver="batched", verbose=verbose)
-        X.extend(C.canonical_bath(np.array(B.x)))
-        conv.extend(B.converged & ~B.failed)
-        nit.extend(B.niter)
-        ns.extend(B.n_solves)
-        if verbose:
-            print("points %d..%d: %d converged" % (b0, b0 + len(pts), int((B.converged & ~B.failed).sum())),
-                  flush=True)
-    return np.array(X), np.array(conv), np.array(nit), np.array(ns)
+    B = C.BatchedCDMFT(problem, points, list(x0), solver="batched", verbose=verbose, capacity=batch)
+    good = B.converged & ~B.failed
+    if verbose:
+        print("%d points: %d converged" % (len(points), int(good.sum())), flush=True)
+    return C.canonical_bath(np.array(B.x)), good, np.array(B.niter), np.array(B.n_solves)
 
 
-def generate_training_set(n, seed, path, batch=64, verbose=True):
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+def load_training_set(pattern="plaquette_warmstart"):
+    """All committed training files (data/<pattern>*.npz): features and converged baths of
+    the points the plain loop converged."""
+    X, Y = [], []
+    for f in sorted(os.listdir(DATA_DIR)):
+        if f.startswith(pattern) and f.endswith(".npz"):
+            d = np.load(os.path.join(DATA_DIR, f))
+            ok = d["converged"].astype(bool)
+            X.append(d["features"][ok])
+            Y.append(d["bath"][ok])
+    if not X:
+        raise FileNotFoundError("no training data in " + DATA_DIR)
+    return np.concatenate(X), np.concatenate(Y)
+
+
+def generate_training_set(n, seed, path, batch=64, chunk=128, verbose=True):
     """Training data for the 2x2+8-bath warm start: n seeded points solved by the plain
-    loop from PLAQUETTE_COLD_START; saved as .npz (features U, mu, t'; converged baths)."""
+    loop from PLAQUETTE_COLD_START, `chunk` points at a time; the .npz (features U, mu, t';
+    converged baths) is rewritten after every chunk with the points solved so far."""
     P = plaquette_problem()
     pts = plaquette_points(n, seed)
-    X, conv, nit, ns = solve_points(P, pts, PLAQUETTE_COLD_START, batch, verbose)
     feats = np.array([plaquette_features(v) for v in pts])
-    np.savez(path, features=feats, bath=X, converged=conv, iterations=nit, solves=ns, seed=seed)
+    parts = []
+    for c0 in range(0, n, chunk):
+        parts.append(solve_points(P, pts[c0:c0 + chunk], PLAQUETTE_COLD_START, batch, verbose))
+        X, conv, nit, ns = (np.concatenate(a) for a in zip(*parts))
+        k = len(conv)
+        np.savez(path, features=feats[:k], bath=X, converged=conv, iterations=nit, solves=ns, seed=seed)
+        if verbose:
+            print("points 0..%d solved, %d converged, saved to %s" % (k, int(conv.sum()), path), flush=True)
     return feats, X, conv
 
 
 if __name__ == "__main__":
     import argparse
-    import os
     ap = argparse.ArgumentParser(description="warm-start training data for the 2x2+8-bath CDMFT")
     ap.add_argument("--points", type=int, default=384)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "data", "plaquette_warmstart.npz"))
+    ap.add_argument("--out", default=os.path.join(DATA_DIR, "plaquette_warmstart.npz"))
     a = ap.parse_args()
     from . import _lib as L
     L.call("qed_init", 0)

@@ -1,0 +1,133 @@
+"""Block Lanczos and the matrix continued fraction (GF_method 'M') on the GPU, against the
+oracle's restatement and against brute-force Lehmann sums."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import models as TM
+from tests.test_gpu_ed import register, with_sites
+
+pytestmark = pytest.mark.gpu
+
+
+def _device_block(gpu_lib, vecs):
+    from pyqcm_amd._lib import DeviceArray
+    flat = np.ascontiguousarray(np.concatenate(vecs))
+    return DeviceArray.from_host(flat)
+
+
+@pytest.mark.parametrize("qr", [0, 1])
+def test_block_lanczos_vs_oracle(gpu_lib, qr):
+    """8-site chain, particle block c+_i|psi0> (p = 8, D = 3920): floors against the oracle
+    (same recurrence, same stopping test). The first floors agree to round-off; later ones
+    drift as orthogonality is lost in both, so the test compares 8 floors and then the
+    Green functions built from all floors."""
+    from pyqcm_amd.sector import Sector
+    m, vals = TM.chain(8, tp=-0.3, U=3.0, mu=1.1)
+    E0, gs, *_ = O.Sector(m, vals, 4, 4).lanczos_gs()
+    To = O.Sector(m, vals, 5, 4)
+    phi = [O.create_destroy(8, 4, 4, a, 1, gs) for a in range(8)]
+    q, W = O.mgs_weight(phi)
+    A_o, B_o = O.block_lanczos(To.hv, q, qr=bool(qr))
+    S = Sector(8, 5, 4, m.flat(vals))
+    p, cap = len(q), 700
+    dq = _device_block(gpu_lib, q)
+    A = np.zeros(cap * p * p)
+    B = np.zeros(cap * p * p)
+    M = ctypes.c_int()
+    gpu_lib.call("qed_block_lanczos", S.h, dq.ptr, p, 0, 600, 1e-7, 1e-12, qr, gpu_lib.dp(A), gpu_lib.dp(B),
+                 cap, ctypes.byref(M))
+    dq.free()
+    M = M.value
+    print("floors: gpu %d, oracle %d" % (M, len(A_o)))
+    assert abs(M - len(A_o)) <= p
+    A = A[:M * p * p].reshape(M, p, p).transpose(0, 2, 1)  # column-major blocks
+    B = B[:M * p * p].reshape(M, p, p).transpose(0, 2, 1)
+    for j in range(8):
+        np.testing.assert_allclose(A[j], A_o[j], atol=1e-10)
+        np.testing.assert_allclose(B[j], B_o[j], atol=1e-10)
+    zs = 1j * np.array([0.1, 0.5, 2.0])
+    As, Ao = O.mcf_shift(list(A), E0, True), O.mcf_shift(A_o, E0, True)
+    for z in zs:
+        g = O.mcf_evaluate(As, list(B), W, z)
+        go = O.mcf_evaluate(Ao, B_o, W, z)
+        assert np.abs(g - go).max() < 1e-10
+
+
+def test_mcf_eval_kernel_vs_oracle(gpu_lib):
+    """Device evaluation of W^T F_0 W on random blocks (non-symmetric B, non-square W,
+    transposed accumulation) against the oracle's recursion."""
+    rng = np.random.default_rng(3)
+    for pa, q, M in ((3, 5, 7), (8, 8, 40), (13, 16, 25)):
+        A = [0.5 * (x + x.T) for x in rng.standard_normal((M, pa, pa))]
+        B = [np.triu(x) for x in rng.standard_normal((M, pa, pa))]
+        W = rng.standard_normal((pa, q))
+        zs = np.array([0.3j, 1.0 + 0.5j, -2.0 + 0.05j, 4j])
+        Ac = np.ascontiguousarray(np.array(A).transpose(0, 2, 1)).ravel()
+        Bc = np.ascontiguousarray(np.array(B).transpose(0, 2, 1)).ravel()
+        Wc = np.ascontiguousarray(W.T).ravel()
+        for tr in (0, 1):
+            base = rng.standard_normal((len(zs), q, q)) + 1j * rng.standard_normal((len(zs), q, q))
+            G = np.ascontiguousarray(base.transpose(0, 2, 1)).astype(np.complex128)
+            zz = np.ascontiguousarray(zs.astype(np.complex128))
+            gpu_lib.call("qed_mcf_eval", pa, q, M, gpu_lib.dp(Ac), gpu_lib.dp(Bc), gpu_lib.dp(Wc), len(zs),
+                         gpu_lib.dp(zz.view(np.float64)), tr, gpu_lib.dp(G.view(np.float64)))
+            got = G.transpose(0, 2, 1)
+            for i, z in enumerate(zs):
+                ref = O.mcf_evaluate(A, B, W, z)
+                ref = base[i] + (ref.T if tr else ref)
+                err = np.abs(got[i] - ref).max() / max(1.0, np.abs(ref).max())
+                assert err < 1e-11, (pa, q, M, tr, err)
+
+
+def _with_method(qcm, qr):
+    qcm.set_global_parameter("GF_method", "M")
+    qcm.set_global_parameter("block_Lanczos_QR", bool(qr))
+
+
+def _restore(qcm):
+    qcm.set_global_parameter("GF_method", "L")
+    qcm.set_global_parameter("block_Lanczos_QR", False)
+
+
+@pytest.mark.parametrize("qr", [0, 1])
+def test_gf_method_M_exhaustive_matches_bruteforce(gpu_lib, qr):
+    """4-site chain: the block Krylov spaces are exhausted, so the matrix continued
+    fraction is the exact Green function."""
+    from pyqcm_amd import qcm
+    m, vals = TM.chain(4, tp=-0.3, U=3.0, mu=1.2)
+    _with_method(qcm, qr)
+    try:
+        lab = register(with_sites(m, 4), vals, "R0:N4:S0")
+        zs = np.array([0.05j, 0.5j, 2j, 1.0 + 0.1j, -2.0 + 0.3j])
+        G = qcm.Green_function(zs, 0, lab)
+        with pytest.raises(Exception):
+            qcm.qmatrix(lab)
+    finally:
+        _restore(qcm)
+    _, Gref = O.dense_gf(m, vals, 2, 2, [0, 1, 2, 3], zs)
+    np.testing.assert_allclose(G, Gref, atol=1e-11)
+
+
+@pytest.mark.parametrize("qr", [0, 1])
+def test_gf_method_M_matches_band_lanczos(gpu_lib, qr):
+    """3x3 Hubbard (N=9, S=1: spin up and down parts): the matrix continued fraction and
+    the band-Lanczos Lehmann form of the same instance agree to the convergence of both
+    recurrences (stopping test 1e-12 on the lowest eigenvalue: ~1e-8 in G)."""
+    from pyqcm_amd import qcm
+    m, vals = TM.square(3, 3, tp=-0.3, U=8.0, mu=4.0)
+    zs = 1j * np.array([0.05, 0.2, 0.7, 2.0, 6.0])
+    lab_l = register(with_sites(m, 9), vals, "R0:N9:S1")
+    GL = [qcm.Green_function(zs, s, lab_l) for s in (0, 1)]
+    _with_method(qcm, qr)
+    try:
+        lab_m = register(with_sites(m, 9), vals, "R0:N9:S1")
+        GM = [qcm.Green_function(zs, s, lab_m) for s in (0, 1)]
+    finally:
+        _restore(qcm)
+    for s in (0, 1):
+        err = np.abs(GM[s] - GL[s]).max()
+        print("spin %d: max |G_M - G_L| = %.3g" % (s, err))
+        assert err < 5e-8
