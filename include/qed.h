@@ -101,6 +101,14 @@ int qed_band_lanczos(qed_sector* s, const double* phi_dev, int p, int M0, int ma
                      double accur_deflation, double accur_band_lanczos, double* evals, double* Q,
                      int cap, int* M);
 
+/* y_j += H_j x_j for n sectors (device vectors) in one batched launch; no counterpart in
+ * the reference (its sectors are applied one at a time). mode 0: one workgroup per row;
+ * mode 1: the two-pass tiled form for short rows (fails when the rows do not fit its LDS
+ * tiles). The launch is repeated reps times (y accumulates each time); *ms, if not null,
+ * receives the average launch time in milliseconds. */
+int qed_hv_batch(int n, qed_sector* const* s, const double* const* x, double* const* y, int mode, int reps,
+                 double* ms);
+
 /* Block Lanczos on p orthonormal start vectors q0_dev (p consecutive sector vectors).
  * qr != 0: upper-triangular off-diagonal blocks from modified Gram-Schmidt; qr == 0 (the
  * reference's default): symmetric blocks from the polar decomposition of the residual

@@ -48,6 +48,7 @@ SIGNATURES = [
     ("qed_davidson", c_int, [c_void_p, c_int, c_double, c_int, dptr, c_void_p, iptr]),
     ("qed_apply_c", c_int, [c_int, c_int, c_int, c_int, c_int, c_double, c_void_p, c_void_p]),
     ("qed_qmatrix_gf", c_int, [c_int, c_int, dptr, dptr, c_int, dptr, dptr]),
+    ("qed_hv_batch", c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, dptr]),
     ("qed_block_lanczos", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double, c_int, dptr,
                                   dptr, c_int, iptr]),
     ("qed_mcf_eval", c_int, [c_int, c_int, c_int, dptr, dptr, dptr, c_int, dptr, c_int, dptr]),

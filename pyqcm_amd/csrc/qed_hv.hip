@@ -284,6 +284,116 @@ __global__ __launch_bounds__(BS) void k_hv_ell_batch(const HvJob* __restrict__ j
   }
 }
 
+// Two-pass tiled form of the batched product for sectors with short rows (the impurity
+// problems of a CDMFT batch: up to a few thousand states per spin). The row kernel above
+// re-reads the whole up-spin hop table for every row and about twenty neighbour rows per
+// row for the down-spin hops, both from L2, for every row it produces. Here
+//   pass A (row tiles): R consecutive rows of X in LDS; each thread holds the hop-table
+//     entries of one column and applies them to the R rows (table traffic / R), with the
+//     diagonal: y[r, :] += (diag + up hops) x[r, :];
+//   pass B (column tiles): the W-column panel X[:, c0:c0+W] of all rows in LDS; the
+//     down-spin hops are gathers inside it: y[:, c0:c0+W] += (down hops) x[:, c0:c0+W].
+// Each pass streams X once and y once; the re-reads are LDS reads.
+__device__ __forceinline__ int find_job(const HvJob* __restrict__ jobs, int njobs, uint32_t t, bool pass_b) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((pass_b ? jobs[mid].b0 : jobs[mid].a0) <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_hv_tileA_batch(const HvJob* __restrict__ jobs, int njobs,
+                                                        uint32_t ntiles, int parity,
+                                                        const int* __restrict__ active) {
+  extern __shared__ double smem[];
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int j = find_job(jobs, njobs, t, false);
+    const HvJob& J = jobs[j];
+    if (active[j]) {
+      const HvView& v = J.v;
+      const uint32_t du = v.du, dd = v.dd, nvu = v.up_nvals;
+      const uint32_t r0 = (t - J.a0) * R;
+      const uint32_t nr = min((uint32_t)R, dd - r0);
+      double* vals = smem;
+      double* xs = smem + ((nvu + 1u) & ~1u);
+      const double* __restrict__ x = J.x[parity];
+      double* __restrict__ y = J.y[parity];
+      for (uint32_t i = threadIdx.x; i < nvu; i += 256) vals[i] = v.up_vals[i];
+      const double* xr = x + (uint64_t)r0 * du;
+      for (uint32_t i = threadIdx.x; i < nr * du; i += 256) xs[i] = xr[i];
+      __syncthreads();
+      for (uint32_t c = threadIdx.x; c < du; c += 256) {
+        const uint32_t su = v.bup[c];
+        const double dgu = v.diag_up[c];
+        double acc[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+          acc[q] = (q < (int)nr) ? (v.diag_dn[r0 + q] + dgu + interaction(v, su, v.bdn[r0 + q])) * xs[q * du + c]
+                                 : 0.0;
+        for (uint32_t g = 0; g < J.Wu; g += kG) {
+          uint32_t e[kG];
+#pragma unroll
+          for (int k = 0; k < kG; ++k) e[k] = J.up_ell[(uint64_t)(g + k) * du + c];
+#pragma unroll
+          for (int k = 0; k < kG; ++k) {
+            const double cf = vals[e[k] >> kTargetBits];
+            const uint32_t tg = e[k] & kTargetMask;
+#pragma unroll
+            for (int q = 0; q < R; ++q)
+              if (q < (int)nr) acc[q] += cf * xs[q * du + tg];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+          if (q < (int)nr) y[(uint64_t)(r0 + q) * du + c] += acc[q];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_hv_tileB_batch(const HvJob* __restrict__ jobs, int njobs,
+                                                        uint32_t ntiles, int parity,
+                                                        const int* __restrict__ active) {
+  extern __shared__ double smem[];
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int j = find_job(jobs, njobs, t, true);
+    const HvJob& J = jobs[j];
+    if (active[j]) {
+      const HvView& v = J.v;
+      const uint32_t du = v.du, dd = v.dd;
+      const uint32_t c0 = (t - J.b0) * W;
+      const uint32_t nc = min((uint32_t)W, du - c0);
+      const double* __restrict__ x = J.x[parity];
+      double* __restrict__ y = J.y[parity];
+      double* xs = smem;  // dd x W
+      for (uint32_t i = threadIdx.x; i < dd * W; i += 256) {
+        const uint32_t r = i / W, cc = i % W;
+        xs[i] = (cc < nc) ? x[(uint64_t)r * du + c0 + cc] : 0.0;
+      }
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < dd * W; i += 256) {
+        const uint32_t r = i / W, cc = i % W;
+        const uint32_t* dl = J.dn_ell + (uint64_t)r * J.Wd;
+        double acc = 0.0;
+        for (uint32_t g = 0; g < J.Wd; g += kG) {
+          uint32_t e[kG];
+#pragma unroll
+          for (int k = 0; k < kG; ++k) e[k] = dl[g + k];
+#pragma unroll
+          for (int k = 0; k < kG; ++k) acc += v.dn_vals[e[k] >> kTargetBits] * xs[(e[k] & kTargetMask) * W + cc];
+        }
+        if (cc < nc) y[(uint64_t)r * du + c0 + cc] += acc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Same product, two adjacent columns per thread: 16-byte loads of the down-spin neighbour
 // rows and of the ELL entry pairs (needs an even up dimension).
 template <int BS, bool DOWN = true>
@@ -808,8 +918,34 @@ std::map<TuneKey, HvBatchConfig>& tune_cache() {
   return c;
 }
 
+template <int R>
+void launch_tileA(const HvBatch& B, uint32_t grid, int parity, const int* active, hipStream_t st) {
+  k_hv_tileA_batch<R><<<grid, 256, B.lds_a, st>>>(B.jobs, B.njobs, B.ntiles_a, parity, active);
+}
+template <int W>
+void launch_tileB(const HvBatch& B, uint32_t grid, int parity, const int* active, hipStream_t st) {
+  k_hv_tileB_batch<W><<<grid, 256, B.lds_b, st>>>(B.jobs, B.njobs, B.ntiles_b, parity, active);
+}
+
 void launch_batch(const HvBatch& B, const HvBatchConfig& cfg, int parity, const int* active,
                   hipStream_t st) {
+  if (cfg.mode == 1) {
+    const uint32_t ga = cfg.grid ? std::min<uint32_t>(cfg.grid, B.ntiles_a) : B.ntiles_a;
+    const uint32_t gb = cfg.grid ? std::min<uint32_t>(cfg.grid, B.ntiles_b) : B.ntiles_b;
+    switch (B.R) {
+      case 16: launch_tileA<16>(B, ga, parity, active, st); break;
+      case 8: launch_tileA<8>(B, ga, parity, active, st); break;
+      case 4: launch_tileA<4>(B, ga, parity, active, st); break;
+      default: launch_tileA<2>(B, ga, parity, active, st); break;
+    }
+    switch (B.W) {
+      case 32: launch_tileB<32>(B, gb, parity, active, st); break;
+      case 16: launch_tileB<16>(B, gb, parity, active, st); break;
+      case 8: launch_tileB<8>(B, gb, parity, active, st); break;
+      default: launch_tileB<4>(B, gb, parity, active, st); break;
+    }
+    return;
+  }
   const uint32_t grid = cfg.grid ? std::min<uint32_t>(cfg.grid, B.ntiles) : B.ntiles;
   if (cfg.block == 1024)
     k_hv_ell_batch<1024><<<grid, 1024, B.lds, st>>>(B.jobs, B.njobs, B.ntiles, parity, active);
@@ -839,6 +975,32 @@ void hv_batch_prepare(HvBatch& B, const std::vector<const qed_sector*>& S) {
     lds = std::max(lds, ((size_t)s.du + s.up_nvals) * sizeof(double));
   }
   if (lds > lds_budget()) fail("batched H.v: a sector row does not fit in LDS");
+  // tiled two-pass form: R rows (pass A) and W-column panels (pass B) within 64 KB of LDS
+  constexpr size_t kTileLds = 64 * 1024;
+  int R = 16, W = 32;
+  for (const qed_sector* sp : S) {
+    const qed_sector& s = *sp;
+    while (R >= 2 && ((size_t)R * s.du + s.up_nvals + 1) * sizeof(double) > kTileLds) R /= 2;
+    while (W >= 4 && (size_t)W * s.dd * sizeof(double) > kTileLds) W /= 2;
+  }
+  B.tiled = R >= 2 && W >= 4 && !S.empty();
+  B.R = R;
+  B.W = W;
+  B.lds_a = B.lds_b = 0;
+  uint32_t ta = 0, tb = 0;
+  for (size_t j = 0; j < S.size(); ++j) {
+    const qed_sector& s = *S[j];
+    h[j].a0 = ta;
+    h[j].b0 = tb;
+    if (B.tiled) {
+      ta += (s.dd + R - 1) / R;
+      tb += (s.du + W - 1) / W;
+      B.lds_a = std::max(B.lds_a, ((size_t)R * s.du + s.up_nvals + 1) * sizeof(double));
+      B.lds_b = std::max(B.lds_b, (size_t)W * s.dd * sizeof(double));
+    }
+  }
+  B.ntiles_a = ta;
+  B.ntiles_b = tb;
   B.host = h;
   B.njobs = (int)S.size();
   B.ntiles = row;
@@ -872,7 +1034,9 @@ HvBatchConfig hv_batch_tune(const HvBatch& B, const int* active, bool force) {
   const int cus = device_cu_count();
   std::vector<HvBatchConfig> cand;
   for (int bs : {256, 512, 1024})
-    for (int mult : {0, 2, 4, 8}) cand.push_back({bs, mult ? (uint32_t)(cus * mult) : 0u, 0.0f});
+    for (int mult : {0, 2, 4, 8}) cand.push_back({bs, mult ? (uint32_t)(cus * mult) : 0u, 0.0f, 0});
+  if (B.tiled)
+    for (int mult : {0, 4, 8}) cand.push_back({256, mult ? (uint32_t)(cus * mult) : 0u, 0.0f, 1});
   hipEvent_t e0, e1;
   QED_HIP(hipEventCreate(&e0));
   QED_HIP(hipEventCreate(&e1));
@@ -1249,6 +1413,41 @@ int qed_hv_timed(qed_sector* s, int variant, const double* x, double* y, int war
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *ms = (double)t / (reps > 0 ? reps : 1);
+  });
+}
+
+int qed_hv_batch(int n, qed_sector* const* s, const double* const* x, double* const* y, int mode, int reps,
+                 double* ms) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    if (n <= 0) return;
+    std::vector<const qed_sector*> S(s, s + n);
+    HvBatch B;
+    hv_batch_prepare(B, S);
+    if (mode == 1 && !B.tiled) fail("batched H.v: the tiled form needs rows that fit in LDS tiles");
+    for (int j = 0; j < n; ++j) {
+      B.host[j].x[0] = B.host[j].x[1] = x[j];
+      B.host[j].y[0] = B.host[j].y[1] = y[j];
+    }
+    hv_batch_upload(B);
+    std::vector<int> on(n, 1);
+    DevBuf act = upload(on);
+    HvBatchConfig cfg;
+    cfg.block = mode == 1 ? 256 : 512;
+    cfg.mode = mode == 1 ? 1 : 0;
+    if (reps < 1) reps = 1;
+    hipEvent_t e0, e1;
+    QED_HIP(hipEventCreate(&e0));
+    QED_HIP(hipEventCreate(&e1));
+    QED_HIP(hipEventRecord(e0, stream()));
+    for (int r = 0; r < reps; ++r) hv_batch(B, cfg, 0, act.as<int>());
+    QED_HIP(hipEventRecord(e1, stream()));
+    QED_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    QED_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms) *ms = (double)t / reps;
   });
 }
 

@@ -84,6 +84,7 @@ struct HvJob {
   const uint32_t* dn_ell;
   uint32_t Wu, Wd;
   uint32_t row0;  // first tile (down row) of this job in the batch
+  uint32_t a0, b0;  // first tiles of this job in the two passes of the tiled form
   const double* x[2];
   double* y[2];
 };
@@ -96,12 +97,18 @@ struct HvBatch {
   uint32_t ntiles = 0;
   size_t lds = 0;
   std::vector<uint64_t> sig;  // shape signature (autotuner cache key)
+  // two-pass tiled form (short rows): R rows per pass-A tile, W-column panels in pass B
+  bool tiled = false;
+  int R = 0, W = 0;
+  uint32_t ntiles_a = 0, ntiles_b = 0;
+  size_t lds_a = 0, lds_b = 0;
 };
 
 struct HvBatchConfig {
   int block = 256;
   uint32_t grid = 0;  // 0: one workgroup per tile; else a persistent grid of this size
   float ms = 0.0f;
+  int mode = 0;  // 0: one workgroup per row; 1: two-pass tiled form
 };
 
 void hv_batch_prepare(HvBatch& B, const std::vector<const qed_sector*>& S);

@@ -103,3 +103,37 @@ def test_qed_batch_solve_abi_matches_instances(gpu_lib):
         G = np.array([(Qb.T / (z - eb)) @ Qb for z in zs])
         np.testing.assert_allclose(G, qcm.Green_function(zs, 0, I.label), atol=1e-12)
         I.erase()
+
+
+def test_batched_hv_forms_match_single_sector(gpu_lib):
+    """qed_hv_batch in both forms (one workgroup per row; two-pass tiled) against the
+    single-sector product and the oracle, on 12-orbital impurity sectors of uneven shape
+    (up dimensions 495, 792, 924, 220: tile widths that do not divide them)."""
+    from pyqcm_amd._lib import DeviceArray
+    from pyqcm_amd.sector import Sector
+    from oracle import oracle as O
+    from tests import models as TM
+    rng = np.random.default_rng(4)
+    m, vals = TM.plaquette_bath(TM.random_bath_params(rng))
+    shapes = [(4, 6), (5, 7), (6, 6), (3, 8), (7, 4)]
+    S = [Sector(12, nu, nd, m.flat(vals)) for nu, nd in shapes]
+    xs = [rng.standard_normal(s.dim) for s in S]
+    y0 = [rng.standard_normal(s.dim) for s in S]
+    ref = []
+    for (nu, nd), x, y in zip(shapes, xs, y0):
+        ref.append(O.Sector(m, vals, nu, nd).hv(x, y))
+    X = [DeviceArray.from_host(x) for x in xs]
+    for mode in (0, 1):
+        Y = [DeviceArray.from_host(y) for y in y0]
+        arr = lambda L: (ctypes.c_void_p * len(L))(*[a.ptr.value for a in L])
+        hs = (ctypes.c_void_p * len(S))(*[s.h.value for s in S])
+        ms = np.zeros(1)
+        gpu_lib.call("qed_hv_batch", len(S), hs, arr(X), arr(Y), mode, 1, gpu_lib.dp(ms))
+        for k, (yd, r) in enumerate(zip(Y, ref)):
+            got = yd.to_host()
+            err = np.abs(got - r).max() / np.abs(r).max()
+            print("mode %d sector %s: rel err %.2e" % (mode, shapes[k], err))
+            assert err < 1e-13
+            yd.free()
+    for xd in X:
+        xd.free()
