@@ -19,6 +19,8 @@
 // block range; block -> job by binary search. Reductions are two-pass and deterministic.
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
+#include <functional>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -367,29 +369,88 @@ void ensure(DevBuf& b, size_t bytes) {
   if (b.bytes < bytes) b.alloc(bytes);
 }
 
+// Persistent worker pool for the per-step host work of the batched passes (one call per
+// Krylov step: spawning threads per call costs more than the work at small batches).
+class HostPool {
+ public:
+  static HostPool& get() {
+    static HostPool p;
+    return p;
+  }
+  void run(int n, const std::function<void(int)>& f) {
+    std::unique_lock<std::mutex> lk(mu_);
+    idle_.wait(lk, [&] { return active_ == 0; });
+    fn_ = &f;
+    n_ = n;
+    next_ = 0;
+    err_ = nullptr;
+    ++gen_;
+    lk.unlock();
+    wake_.notify_all();
+    drain();
+    lk.lock();
+    idle_.wait(lk, [&] { return active_ == 0; });
+    fn_ = nullptr;
+    if (err_) std::rethrow_exception(err_);
+  }
+  int threads() const { return (int)th_.size() + 1; }
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    wake_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+
+ private:
+  HostPool() {
+    const int nt = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    for (int t = 1; t < nt; ++t) th_.emplace_back([this] { worker(); });
+  }
+  void worker() {
+    uint64_t seen = 0;
+    for (;;) {
+      std::unique_lock<std::mutex> lk(mu_);
+      wake_.wait(lk, [&] { return stop_ || (gen_ != seen && fn_ != nullptr); });
+      if (stop_) return;
+      seen = gen_;
+      ++active_;
+      lk.unlock();
+      drain();
+      lk.lock();
+      if (--active_ == 0) idle_.notify_all();
+    }
+  }
+  void drain() {
+    try {
+      for (int i = next_++; i < n_; i = next_++) (*fn_)(i);
+    } catch (...) {
+      std::lock_guard<std::mutex> lk(err_mu_);
+      if (!err_) err_ = std::current_exception();
+      next_ = n_;
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_, err_mu_;
+  std::condition_variable wake_, idle_;
+  const std::function<void(int)>* fn_ = nullptr;
+  int n_ = 0;
+  std::atomic<int> next_{0};
+  int active_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+  std::exception_ptr err_;
+};
+
 template <class F>
 void parallel_for(int n, F f) {
-  const int nt = std::max(1, std::min<int>(n, (int)std::min<unsigned>(16u, std::thread::hardware_concurrency())));
-  if (nt <= 1 || n < 4) {
+  if (n < 4 || HostPool::get().threads() <= 1) {
     for (int i = 0; i < n; ++i) f(i);
     return;
   }
-  std::vector<std::thread> th;
-  std::atomic<int> next{0};
-  std::exception_ptr err;
-  std::mutex mu;
-  for (int t = 0; t < nt; ++t)
-    th.emplace_back([&] {
-      try {
-        for (int i = next++; i < n; i = next++) f(i);
-      } catch (...) {
-        std::lock_guard<std::mutex> lk(mu);
-        if (!err) err = std::current_exception();
-        next = n;
-      }
-    });
-  for (auto& t : th) t.join();
-  if (err) std::rethrow_exception(err);
+  const std::function<void(int)> g = f;
+  HostPool::get().run(n, g);
 }
 
 }  // namespace
@@ -731,6 +792,7 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
     // ---- host: complete the previous step's band entries and its eigenvalue test, then
     //      take this step's deflation decision
     std::vector<int> proceed(nj, 0);
+    std::atomic<int> n_checks{0}, n_dense{0};
     const auto tc0 = std::chrono::steady_clock::now();
     parallel_for(nj, [&](int i) {
       BandJob& b = J[i];
@@ -751,16 +813,32 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
         if (b.check_pending) {
           b.check_pending = false;
           // lowest eigenvalue of the pj x pj projected matrix: shifted inverse iteration on
-          // the band (no deflated vectors) from the previous eigenvector, the shift below
-          // the previous value by more than its last decrease; dense eigensolver otherwise
+          // the band (the deflated vectors' rows and columns as a border) from the previous
+          // eigenvector, the shift below the previous value by more than its last decrease;
+          // dense eigensolver otherwise
           double e0 = 0.0;
           bool ok = false;
-          if (b.I.empty() && !b.evec.empty() && b.ev_old2 < 1e11) {
+          n_checks++;
+          if (!b.evec.empty() && b.ev_old2 < 1e11) {
+            // a second try with a lower shift and more solves when the first shift was not
+            // below the spectrum or the residual test failed
             const double drop = std::fabs(b.ev_old2 - b.ev_old);
-            const double sigma = b.ev_old - std::max(4.0 * drop, 1e-9 * std::max(1.0, std::fabs(b.ev_old)));
-            ok = band_lowest([&](int r, int c) { return b.T(r, c); }, pj, b.p0, sigma, 3, b.evec, e0);
+            const double scale = std::max(1.0, std::fabs(b.ev_old));
+            // the band part of T is stored in the lower triangle only
+            auto Tb = [&](int r, int c) { return r >= c ? b.T(r, c) : b.T(c, r); };
+            std::vector<int> bd;
+            for (int k : b.I)
+              if (k < pj) bd.push_back(k);
+            std::vector<double> x0 = b.evec;
+            ok = band_lowest(Tb, pj, b.p0, b.ev_old - std::max(4.0 * drop, 1e-9 * scale), 3, b.evec, e0, bd);
+            if (!ok) {
+              b.evec = x0;
+              ok = band_lowest(Tb, pj, b.p0, b.ev_old - std::max(64.0 * drop, 1e-6 * scale), 10, b.evec, e0,
+                               bd);
+            }
           }
           if (!ok) {
+            n_dense++;
             std::vector<double> H((size_t)pj * pj), e(pj);
             for (int c = 0; c < pj; ++c)
               for (int r = 0; r < pj; ++r) H[(size_t)c * pj + r] = b.T(r, c);
@@ -803,7 +881,11 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
       b.z = z;
       proceed[i] = 1;
     });
-    if (stats) stats->t_band_checks += std::chrono::duration<double>(std::chrono::steady_clock::now() - tc0).count();
+    if (stats) {
+      stats->t_band_checks += std::chrono::duration<double>(std::chrono::steady_clock::now() - tc0).count();
+      stats->band_checks += n_checks;
+      stats->band_dense_checks += n_dense;
+    }
     // ---- phase 2: v[j] /= z; pending block orthogonalized against v[j] ----
     // scale: y = y * (1/z) with no sum (MAxpy with m = 0)
     TA.clear();

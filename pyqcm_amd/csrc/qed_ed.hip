@@ -1137,9 +1137,9 @@ int qed_ed_green_function_solve(uint64_t label) {
 int qed_batch_last_timings(double* t, int n) {
   return guard([&] {
     const BatchStats& s = last_batch_stats();
-    const double v[7] = {s.t_setup, s.t_gs, s.t_gf_setup, s.t_band, s.t_band_checks, s.t_assemble,
-                         (double)s.hv_config.mode};
-    for (int i = 0; i < n && i < 7; ++i) t[i] = v[i];
+    const double v[9] = {s.t_setup, s.t_gs, s.t_gf_setup, s.t_band, s.t_band_checks, s.t_assemble,
+                         (double)s.hv_config.mode, (double)s.band_checks, (double)s.band_dense_checks};
+    for (int i = 0; i < n && i < 9; ++i) t[i] = v[i];
   });
 }
 

@@ -323,7 +323,20 @@ __global__ __launch_bounds__(256) void k_hv_tileA_batch(const HvJob* __restrict_
       double* __restrict__ y = J.y[parity];
       for (uint32_t i = threadIdx.x; i < nvu; i += 256) vals[i] = v.up_vals[i];
       const double* xr = x + (uint64_t)r0 * du;
-      for (uint32_t i = threadIdx.x; i < nr * du; i += 256) xs[i] = xr[i];
+      const uint32_t na = nr * du;
+      for (uint32_t i0 = 0; i0 < na; i0 += 256 * kG) {  // kG loads in flight per thread
+        double t[kG];
+#pragma unroll
+        for (int k = 0; k < kG; ++k) {
+          const uint32_t i = i0 + threadIdx.x + 256u * k;
+          t[k] = i < na ? xr[i] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kG; ++k) {
+          const uint32_t i = i0 + threadIdx.x + 256u * k;
+          if (i < na) xs[i] = t[k];
+        }
+      }
       __syncthreads();
       for (uint32_t c = threadIdx.x; c < du; c += 256) {
         const uint32_t su = v.bup[c];
@@ -371,9 +384,20 @@ __global__ __launch_bounds__(256) void k_hv_tileB_batch(const HvJob* __restrict_
       const double* __restrict__ x = J.x[parity];
       double* __restrict__ y = J.y[parity];
       double* xs = smem;  // dd x W
-      for (uint32_t i = threadIdx.x; i < dd * W; i += 256) {
-        const uint32_t r = i / W, cc = i % W;
-        xs[i] = (cc < nc) ? x[(uint64_t)r * du + c0 + cc] : 0.0;
+      const uint32_t nb = dd * W;
+      for (uint32_t i0 = 0; i0 < nb; i0 += 256 * kG) {  // kG loads in flight per thread
+        double t[kG];
+#pragma unroll
+        for (int k = 0; k < kG; ++k) {
+          const uint32_t i = i0 + threadIdx.x + 256u * k;
+          const uint32_t r = i / W, cc = i % W;
+          t[k] = (i < nb && cc < nc) ? x[(uint64_t)r * du + c0 + cc] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kG; ++k) {
+          const uint32_t i = i0 + threadIdx.x + 256u * k;
+          if (i < nb) xs[i] = t[k];
+        }
       }
       __syncthreads();
       for (uint32_t i = threadIdx.x; i < dd * W; i += 256) {
@@ -975,8 +999,8 @@ void hv_batch_prepare(HvBatch& B, const std::vector<const qed_sector*>& S) {
     lds = std::max(lds, ((size_t)s.du + s.up_nvals) * sizeof(double));
   }
   if (lds > lds_budget()) fail("batched H.v: a sector row does not fit in LDS");
-  // tiled two-pass form: R rows (pass A) and W-column panels (pass B) within 64 KB of LDS
-  constexpr size_t kTileLds = 64 * 1024;
+  // tiled two-pass form: R rows (pass A) and W-column panels (pass B) within 32 KB of LDS
+  constexpr size_t kTileLds = 32 * 1024;  // several workgroups per CU (measured: 64 KB tiles are slower)
   int R = 16, W = 32;
   for (const qed_sector* sp : S) {
     const qed_sector& s = *sp;

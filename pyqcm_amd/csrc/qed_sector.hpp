@@ -122,6 +122,8 @@ struct BatchStats {
   HvBatchConfig hv_config;
   int lanczos_iterations = 0;  // lockstep iterations of the ground-state pass
   int band_steps = 0;          // synchronized steps of the band Lanczos pass
+  int band_checks = 0;         // eigenvalue tests of the band pass, of which dense
+  int band_dense_checks = 0;
   // wall time (s) of the phases: sectors + start vectors, ground-state Lanczos,
   // Green-function start vectors, band Lanczos (of which: host eigenvalue tests), assembly
   double t_setup = 0, t_gs = 0, t_gf_setup = 0, t_band = 0, t_band_checks = 0, t_assemble = 0;
