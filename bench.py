@@ -44,6 +44,8 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="time every H.v variant (stderr)")
     ap.add_argument("--cdmft", type=int, default=64, help="CDMFT points for the warm-start leg (0: skip)")
     ap.add_argument("--cdmft-batch", type=int, default=32, help="CDMFT problems active at a time")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched Lanczos steps as direct launches instead of hipGraph replays")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 2 GiB five times (for scaling HBM counters)")
     args = ap.parse_args()
@@ -120,7 +122,7 @@ def main():
     dx.free()
     dy.free()
     del S
-    cdmft = cdmft_leg(args.cdmft, args.cdmft_batch) if args.cdmft > 0 else None
+    cdmft = cdmft_leg(args.cdmft, args.cdmft_batch, not args.no_graph) if args.cdmft > 0 else None
 
     out = {
         "metric": "H·v GB/s (16-site ED, % HBM peak); CDMFT solves/s learned warm-start vs plain",
@@ -173,7 +175,7 @@ def traffic_from_profile():
     return None
 
 
-def cdmft_leg(n, batch):
+def cdmft_leg(n, batch, graph=True):
     """Plain (cold start) vs learned warm start, batched CDMFT on n seeded points with
     `batch` problems active at a time (a converged or failed problem is replaced by the
     next queued one)."""
@@ -186,6 +188,7 @@ def cdmft_leg(n, batch):
     t_train = time.perf_counter() - t0
     log("MLP trained on %d points in %.1f s (loss %.3g -> %.3g)" % (len(X), t_train, hist[0], hist[-1]))
     P = W.plaquette_problem("plaq_bench")
+    P.graph = graph
     pts = W.plaquette_points(n, 20261202)
     feats = np.array([W.plaquette_features(v) for v in pts])
     cold = [W.PLAQUETTE_COLD_START] * n

@@ -244,8 +244,9 @@ class CDMFTProblem:
     cluster suffix '_1', as in pyqcm); `sectors` is the target-sector string, or a function
     of the parameter values returning it."""
 
-    def __init__(self, model, varia, sectors, grid=None, kside=32):
+    def __init__(self, model, varia, sectors, grid=None, kside=32, graph=True):
         self.model = model
+        self.graph = graph  # replay the batched Lanczos steps as hipGraphs
         self.varia = [v[:-2] if v.endswith("_1") else v for v in varia]
         self.sectors = sectors
         self.grid = grid or frequency_grid()
@@ -261,7 +262,7 @@ class CDMFTProblem:
     def host(self, inst):
         return inst.cdmft_host(self.grid.wr, self.kside)
 
-    def solve_batch(self, values_list, x_list, graph=True):
+    def solve_batch(self, values_list, x_list, graph=None):
         """Host functions of many instances: one batched device solve for all of them
         (qcm.batch_solve), then the k-sums per instance."""
         from . import qcm
@@ -270,7 +271,7 @@ class CDMFTProblem:
             vals = dict(v)
             vals.update(zip(self.varia, np.asarray(x, float)))
             insts.append(self.model.instance(vals, self.sectors_of(v)))
-        self.last_stats = qcm.batch_solve([I.label for I in insts], graph=graph)
+        self.last_stats = qcm.batch_solve([I.label for I in insts], graph=self.graph if graph is None else graph)
         hosts = []
         for I in insts:
             E0 = I.ground_state()[0]

@@ -104,16 +104,19 @@ __global__ __launch_bounds__(kBT) void k_mdot_part(const MDot* __restrict__ d, i
   }
 }
 
+// one wavefront per (job, q): lane-strided partial sums, then a fixed-order wave sum
 __global__ void k_mdot_final(const MDot* __restrict__ d, int nj, const double* __restrict__ part) {
-  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  const int id = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
   if (id >= nj * kBM) return;
   const int j = id / kBM, q = id % kBM;
   const MDot& D = d[j];
   if (q >= D.m) return;
   const uint32_t nb = nblocks(D.n);
   double s = 0.0;
-  for (uint32_t b = 0; b < nb; ++b) s += part[(size_t)(D.blk0 + b) * kBM + q];
-  D.out[q] = s;
+  for (uint32_t b = lane; b < nb; b += 64) s += part[(size_t)(D.blk0 + b) * kBM + q];
+  s = wave_sum(s);
+  if (lane == 0) D.out[q] = s;
 }
 
 // ---- multi-axpy: y += sgn * sum_q (*c[q]) x[q]; optionally y *= scale first -------------
@@ -242,16 +245,23 @@ __global__ __launch_bounds__(kBT) void k_lz_dot(const LJob* __restrict__ d, int 
   if (threadIdx.x == 0) part[b] = s;
 }
 
-// alpha[k] = sum of the partials (one thread per job)
-__global__ void k_lz_alpha(const LJob* __restrict__ d, int nj, const int* __restrict__ kdev,
-                           const int* __restrict__ active, const double* __restrict__ part) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nj || !active[j]) return;
-  const LJob& D = d[j];
+// sum of one job's block partials: one wavefront per job, lane-strided, fixed-order wave sum
+__device__ __forceinline__ double job_partial_sum(const LJob& D, const double* __restrict__ part, int lane) {
   const uint32_t nb = nblocks(D.n);
   double s = 0.0;
-  for (uint32_t b = 0; b < nb; ++b) s += part[D.blk0 + b];
-  D.alpha[*kdev] = s;
+  for (uint32_t b = lane; b < nb; b += 64) s += part[D.blk0 + b];
+  return wave_sum(s);
+}
+
+// alpha[k] = sum of the partials
+__global__ void k_lz_alpha(const LJob* __restrict__ d, int nj, const int* __restrict__ kdev,
+                           const int* __restrict__ active, const double* __restrict__ part) {
+  const int j = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (j >= nj || !active[j]) return;
+  const LJob& D = d[j];
+  const double s = job_partial_sum(D, part, lane);
+  if (lane == 0) D.alpha[*kdev] = s;
 }
 
 // q -= alpha[k] r, partials of |q|^2
@@ -283,13 +293,12 @@ __global__ __launch_bounds__(kBT) void k_lz_axpy(const LJob* __restrict__ d, int
 // beta[k] = sqrt(sum of partials)
 __global__ void k_lz_beta(const LJob* __restrict__ d, int nj, const int* __restrict__ kdev,
                           const int* __restrict__ active, const double* __restrict__ part) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
   if (j >= nj || !active[j]) return;
   const LJob& D = d[j];
-  const uint32_t nb = nblocks(D.n);
-  double s = 0.0;
-  for (uint32_t b = 0; b < nb; ++b) s += part[D.blk0 + b];
-  D.beta[*kdev] = std::sqrt(s);
+  const double s = job_partial_sum(D, part, lane);
+  if (lane == 0) D.beta[*kdev] = std::sqrt(s);
 }
 
 __global__ void k_lz_next(int* __restrict__ kdev) {
@@ -522,7 +531,7 @@ void lanczos_gs_batch(const std::vector<qed_sector*>& S, const std::vector<doubl
   const int* act = dact.as<int>();
   int* kd = dk.as<int>();
   double* pd = part.as<double>();
-  const int nb1 = (nj + 255) / 256;
+  const int nb1 = (nj + 3) / 4;  // one wavefront per job
   auto iteration = [&](int parity) {
     k_lz_scal<<<T.total, kBT, 0, stream()>>>(dT, nj, parity, kd, act);
     hv_batch(HB, cfg, parity, act);
@@ -669,9 +678,9 @@ struct BandJob {
   std::vector<double*> free_list;
   // device scalars of the step: [0..p0] multi-dot, [p0+1 .. p0+pc] pending-block overlaps,
   // then the deflated-vector overlaps and T(j, j)
-  DevBuf sc;
-  double* scd = nullptr;
-  std::vector<double> sch;
+  double* scd = nullptr;  // this job's kSc scalars in the batch's device array
+  double* sch = nullptr;  // and their host copy
+  double* hcd = nullptr;  // kBM host-supplied coefficients of phase 4 (device)
   // pending host reads from the previous step
   int prev_j = -1, prev_nk = 0, prev_pc = 0, prev_nI = 0, nk = 0;
   bool check_pending = false;
@@ -730,11 +739,18 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
     b.t.assign((size_t)b.M0 * b.M0, 0.0);
     b.P.assign((size_t)b.p0 * b.M0, 0.0);
     b.defl.assign(b.M0, 0);
-    b.sc.alloc(kSc * sizeof(double));
-    b.scd = b.sc.as<double>();
-    b.sch.assign(kSc, 0.0);
     hv_of[i] = (int)cs.size();
     cs.push_back(S[i]);
+  }
+  // the step scalars of all jobs in one device array (one download per step) and the
+  // host-supplied phase-4 coefficients in another (one upload per step)
+  DevBuf sc_all((size_t)std::max(1, nj) * kSc * sizeof(double));
+  DevBuf hc_all((size_t)std::max(1, nj) * kBM * sizeof(double));
+  std::vector<double> sch_all((size_t)std::max(1, nj) * kSc, 0.0), hch_all((size_t)std::max(1, nj) * kBM, 0.0);
+  for (int i = 0; i < nj; ++i) {
+    J[i].scd = sc_all.as<double>() + (size_t)i * kSc;
+    J[i].sch = sch_all.data() + (size_t)i * kSc;
+    J[i].hcd = hc_all.as<double>() + (size_t)i * kBM;
   }
   HvBatch HB;
   if (!cs.empty()) hv_batch_prepare(HB, cs);
@@ -753,7 +769,7 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
     MDot* d = TD.upload();
     ensure(part, (size_t)TD.total * kBM * sizeof(double));
     k_mdot_part<<<TD.total, kBT, 0, stream()>>>(d, (int)TD.h.size(), part.as<double>());
-    k_mdot_final<<<((int)TD.h.size() * kBM + 255) / 256, 256, 0, stream()>>>(d, (int)TD.h.size(), part.as<double>());
+    k_mdot_final<<<((int)TD.h.size() * kBM + 3) / 4, 256, 0, stream()>>>(d, (int)TD.h.size(), part.as<double>());
   };
   auto run_maxpy = [&]() {
     if (TA.h.empty()) return;
@@ -785,8 +801,8 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
     }
     if (!live) break;
     run_mdot();
-    for (auto& b : J)
-      if (!b.done) QED_HIP(hipMemcpyAsync(b.sch.data(), b.scd, kSc * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    QED_HIP(hipMemcpyAsync(sch_all.data(), sc_all.p, (size_t)nj * kSc * sizeof(double), hipMemcpyDeviceToHost,
+                           stream()));
     QED_HIP(hipStreamSynchronize(stream()));
     ++steps;
     // ---- host: complete the previous step's band entries and its eigenvalue test, then
@@ -797,7 +813,7 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
     parallel_for(nj, [&](int i) {
       BandJob& b = J[i];
       if (b.done) return;
-      const double* s = b.sch.data();
+      const double* s = b.sch;
       if (b.prev_j >= 0) {
         // previous step: [kBM, kBM + nk) pending-block overlaps, [2 kBM, 2 kBM + nI)
         // deflated-vector overlaps, [3 kBM] T(j, j); phase 1 wrote only [0, p0]
@@ -973,6 +989,7 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
     }
     // ---- phase 4: w -= sum_k T(k, j) v[k] over the band; release the oldest vector ----
     TA.clear();
+    bool any_hc = false;
     for (int i = 0; i < nj; ++i) {
       if (!proceed[i]) continue;
       BandJob& b = J[i];
@@ -1011,12 +1028,15 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
       }
       a.m = m;
       if (!b.hcoef.empty()) {
-        double* slot = b.scd + 3 * kBM + 8;
-        QED_HIP(hipMemcpyAsync(slot, b.hcoef.data(), b.hcoef.size() * 8, hipMemcpyHostToDevice, stream()));
-        for (size_t u = 0; u < hslot.size(); ++u) a.c[hslot[u]] = slot + u;
+        std::copy(b.hcoef.begin(), b.hcoef.end(), hch_all.begin() + (size_t)i * kBM);
+        any_hc = true;
+        for (size_t u = 0; u < hslot.size(); ++u) a.c[hslot[u]] = b.hcd + u;
       }
       if (m > 0) TA.add(a, b.n);
     }
+    if (any_hc)
+      QED_HIP(hipMemcpyAsync(hc_all.p, hch_all.data(), (size_t)nj * kBM * sizeof(double), hipMemcpyHostToDevice,
+                             stream()));
     run_maxpy();
     // ---- phase 5: deflated vectors, one at a time (w -= <v_k, w> v_k) ----
     int maxI = 0;

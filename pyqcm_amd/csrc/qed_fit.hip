@@ -112,9 +112,10 @@ struct FitArgs {
 };
 
 // LDS layout (doubles): G (nw n n 2) | sim ((N+1) N) | fsim (N+1) | xbar N | pts (N N) |
-// fpts N | eb nb | tcb n nb | distance scratch (inv, term, part)
+// fpts N | eb nb | tcb n nb | distance scratch (inv, term, part) | E (N nb) | T (N n nb) |
+// w nw | weight nw
 struct Lds {
-  double *G, *sim, *fsim, *xbar, *pts, *fpts, *eb, *tcb, *inv, *term, *part;
+  double *G, *sim, *fsim, *xbar, *pts, *fpts, *eb, *tcb, *inv, *term, *part, *E, *T, *w, *wt;
 };
 
 __device__ Lds carve(double* base, const FitArgs& A) {
@@ -131,6 +132,10 @@ __device__ Lds carve(double* base, const FitArgs& A) {
   L.inv = L.tcb + A.n * A.nb;
   L.term = L.inv + 2 * A.nw * A.nb;
   L.part = L.term + A.nw * A.n * A.n;
+  L.E = L.part + A.nw;
+  L.T = L.E + A.nvar * A.nb;
+  L.w = L.T + (size_t)A.nvar * A.n * A.nb;
+  L.wt = L.w + A.nw;
   return L;
 }
 
@@ -140,16 +145,16 @@ __device__ double eval_point(const FitArgs& A, const Lds& L, const double* x) {
   const int nt = A.n * A.nb;
   for (int m = lane; m < A.nb; m += 64) {
     double acc = 0.0;
-    for (int p = 0; p < A.nvar; ++p) acc = acc + x[p] * A.E[p * A.nb + m];
+    for (int p = 0; p < A.nvar; ++p) acc = acc + x[p] * L.E[p * A.nb + m];
     L.eb[m] = acc;
   }
   for (int q = lane; q < nt; q += 64) {
     double acc = 0.0;
-    for (int p = 0; p < A.nvar; ++p) acc = acc + x[p] * A.T[(size_t)p * nt + q];
+    for (int p = 0; p < A.nvar; ++p) acc = acc + x[p] * L.T[(size_t)p * nt + q];
     L.tcb[q] = acc;
   }
   __syncthreads();
-  return wave_distance(A.n, A.nb, A.nw, A.w, A.weight, L.G, L.eb, L.tcb, A.scale, L.inv, L.term,
+  return wave_distance(A.n, A.nb, A.nw, L.w, L.wt, L.G, L.eb, L.tcb, A.scale, L.inv, L.term,
                        L.part);
 }
 
@@ -202,6 +207,12 @@ __global__ __launch_bounds__(64) void k_nelder_mead(FitArgs A) {
   const Lds L = carve(lds, A);
   const double* Gh = A.Ghost + (size_t)A.host_of[b] * A.nw * A.n * A.n * 2;
   for (int i = lane; i < A.nw * A.n * A.n * 2; i += 64) L.G[i] = Gh[i];
+  for (int i = lane; i < N * A.nb; i += 64) L.E[i] = A.E[i];
+  for (int i = lane; i < N * A.n * A.nb; i += 64) L.T[i] = A.T[i];
+  for (int i = lane; i < A.nw; i += 64) {
+    L.w[i] = A.w[i];
+    L.wt[i] = A.weight[i];
+  }
   const double dim = (double)N;
   const double rho = 1.0, chi = 1.0 + 2.0 / dim, psi = 0.75 - 1.0 / (2.0 * dim),
                sigma = 1.0 - 1.0 / dim;
@@ -388,7 +399,7 @@ int qed_cdmft_fit_batch(int n, int nb, int nw, const double* w, const double* we
       if (host_of[b] < 0 || host_of[b] >= nhost) fail("cdmft fit: host index out of range");
     const int N = nvar;
     size_t lds_d = (size_t)nw * n * n * 2 + (size_t)(N + 1) * N + (N + 1) + N + (size_t)N * N + N +
-                   nb + (size_t)n * nb + wave_scratch(n, nb, nw);
+                   nb + (size_t)n * nb + wave_scratch(n, nb, nw) + (size_t)N * nb + (size_t)N * n * nb + 2 * nw;
     size_t lds = lds_d * sizeof(double);
     if (lds > 64 * 1024) fail("cdmft fit: problem too large for one workgroup's LDS");
     DevBuf dw = upload(w, nw), dwt = upload(weight, nw);
