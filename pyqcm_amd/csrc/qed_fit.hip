@@ -24,8 +24,8 @@
 namespace qed {
 namespace {
 
-// Distance of one parameter set, evaluated by one wavefront (lane = threadIdx.x & 63,
-// all 64 lanes of the wave take part). G: host, nw matrices n x n complex column-major
+// Distance of one parameter set, evaluated by the whole workgroup (one wavefront for the
+// distance kernel, four for the fit). G: host, nw matrices n x n complex column-major
 // (interleaved re, im); eb: nb bath energies; tcb: n x nb couplings column-major.
 // Scratch (per wave): inv 2 nw nb, term nw n n, part nw doubles.
 // Stages: 1/(i w - e_m) per (w, m); |Gamma + G|^2 per matrix entry (a + c n); the
@@ -35,9 +35,9 @@ namespace {
 __device__ double wave_distance(int n, int nb, int nw, const double* w, const double* weight,
                                 const double* G, const double* eb, const double* tcb,
                                 double scale, double* inv, double* term, double* part) {
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x, nth = blockDim.x;
   const int nn = n * n;
-  for (int q = lane; q < nw * nb; q += 64) {
+  for (int q = lane; q < nw * nb; q += nth) {
     const int iw = q / nb, m = q % nb;
     // 1/(i w - e) = (-e - i w)/(e^2 + w^2)
     const double d = eb[m] * eb[m] + w[iw] * w[iw];
@@ -45,7 +45,7 @@ __device__ double wave_distance(int n, int nb, int nw, const double* w, const do
     inv[2 * q + 1] = -w[iw] / d;
   }
   __syncthreads();
-  for (int q = lane; q < nw * nn; q += 64) {
+  for (int q = lane; q < nw * nn; q += nth) {
     const int iw = q / nn, e = q % nn;
     const int a = e % n, c = e / n;
     double gr = G[2 * (size_t)q];
@@ -59,7 +59,7 @@ __device__ double wave_distance(int n, int nb, int nw, const double* w, const do
     term[q] = gr * gr + gi * gi;
   }
   __syncthreads();
-  for (int iw = lane; iw < nw; iw += 64) {
+  for (int iw = lane; iw < nw; iw += nth) {
     double s = 0.0;
     for (int e = 0; e < nn; ++e) s = s + term[iw * nn + e];
     part[iw] = s * weight[iw];
@@ -93,6 +93,8 @@ __global__ __launch_bounds__(64) void k_dist_wave(int n, int nb, int nw, const d
                            scale, inv, term, part);
   if (threadIdx.x == 0) d[b] = v;
 }
+
+constexpr int kFitThreads = 256;  // four wavefronts per fit: the distance stages run 4x wider
 
 struct FitArgs {
   int n, nb, nw, nvar;
@@ -143,12 +145,12 @@ __device__ Lds carve(double* base, const FitArgs& A) {
 __device__ double eval_point(const FitArgs& A, const Lds& L, const double* x) {
   const int lane = threadIdx.x;
   const int nt = A.n * A.nb;
-  for (int m = lane; m < A.nb; m += 64) {
+  for (int m = lane; m < A.nb; m += (int)blockDim.x) {
     double acc = 0.0;
     for (int p = 0; p < A.nvar; ++p) acc = acc + x[p] * L.E[p * A.nb + m];
     L.eb[m] = acc;
   }
-  for (int q = lane; q < nt; q += 64) {
+  for (int q = lane; q < nt; q += (int)blockDim.x) {
     double acc = 0.0;
     for (int p = 0; p < A.nvar; ++p) acc = acc + x[p] * L.T[(size_t)p * nt + q];
     L.tcb[q] = acc;
@@ -161,6 +163,17 @@ __device__ double eval_point(const FitArgs& A, const Lds& L, const double* x) {
 __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
+}
+
+// maximum over the workgroup (every thread gets it); red: blockDim/64 doubles of LDS
+__device__ double block_max(double v, double* red) {
+  v = wave_max(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double m = red[0];
+  for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, red[i]);
+  return m;
 }
 
 // Simplex vertices stay in fixed LDS rows; ord[0..N] lists the rows by increasing value
@@ -198,18 +211,19 @@ __device__ void reinsert_last(const Lds& L, int* ord, int N) {
   __syncthreads();
 }
 
-__global__ __launch_bounds__(64) void k_nelder_mead(FitArgs A) {
+__global__ __launch_bounds__(kFitThreads) void k_nelder_mead(FitArgs A) {
   extern __shared__ double lds[];
   __shared__ int ord[65];
+  __shared__ double red[kFitThreads / 64];
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
   const int N = A.nvar;
   const Lds L = carve(lds, A);
   const double* Gh = A.Ghost + (size_t)A.host_of[b] * A.nw * A.n * A.n * 2;
-  for (int i = lane; i < A.nw * A.n * A.n * 2; i += 64) L.G[i] = Gh[i];
-  for (int i = lane; i < N * A.nb; i += 64) L.E[i] = A.E[i];
-  for (int i = lane; i < N * A.n * A.nb; i += 64) L.T[i] = A.T[i];
-  for (int i = lane; i < A.nw; i += 64) {
+  for (int i = lane; i < A.nw * A.n * A.n * 2; i += kFitThreads) L.G[i] = Gh[i];
+  for (int i = lane; i < N * A.nb; i += kFitThreads) L.E[i] = A.E[i];
+  for (int i = lane; i < N * A.n * A.nb; i += kFitThreads) L.T[i] = A.T[i];
+  for (int i = lane; i < A.nw; i += kFitThreads) {
     L.w[i] = A.w[i];
     L.wt[i] = A.weight[i];
   }
@@ -217,11 +231,11 @@ __global__ __launch_bounds__(64) void k_nelder_mead(FitArgs A) {
   const double rho = 1.0, chi = 1.0 + 2.0 / dim, psi = 0.75 - 1.0 / (2.0 * dim),
                sigma = 1.0 - 1.0 / dim;
   const double* x0 = A.x0 + (size_t)b * N;
-  for (int i = lane; i < (N + 1) * N; i += 64) {
+  for (int i = lane; i < (N + 1) * N; i += kFitThreads) {
     int r = i / N, c = i % N;
     L.sim[i] = x0[c] + ((r == c + 1) ? A.step : 0.0);
   }
-  for (int r = lane; r <= N; r += 64) ord[r] = r;
+  for (int r = lane; r <= N; r += kFitThreads) ord[r] = r;
   __syncthreads();
   for (int r = 0; r <= N; ++r) {
     double fv = eval_point(A, L, L.sim + r * N);
@@ -241,13 +255,13 @@ __global__ __launch_bounds__(64) void k_nelder_mead(FitArgs A) {
     // termination: max |sim[r] - sim[0]| <= xatol and max |f[0] - f[r]| <= fatol
     const double* best = L.sim + ord[0] * N;
     double dx = 0.0, df = 0.0;
-    for (int i = lane; i < N * N; i += 64) {
+    for (int i = lane; i < N * N; i += kFitThreads) {
       int r = i / N + 1, c = i % N;
       dx = fmax(dx, fabs(L.sim[ord[r] * N + c] - best[c]));
     }
-    for (int r = 1 + lane; r <= N; r += 64) df = fmax(df, fabs(L.fsim[ord[0]] - L.fsim[ord[r]]));
-    dx = wave_max(dx);
-    df = wave_max(df);
+    for (int r = 1 + lane; r <= N; r += kFitThreads) df = fmax(df, fabs(L.fsim[ord[0]] - L.fsim[ord[r]]));
+    dx = block_max(dx, red);
+    df = block_max(df, red);
     if (dx <= A.xatol && df <= A.fatol) break;
     // centroid of the N best vertices, summed in rank order
     double* worst = L.sim + ord[N] * N;
@@ -309,7 +323,7 @@ __global__ __launch_bounds__(64) void k_nelder_mead(FitArgs A) {
       if (shrink) {
         // every vertex but the best moves toward it, then all are re-ranked
         const double* bst = L.sim + ord[0] * N;
-        for (int i = lane; i < N * N; i += 64) {
+        for (int i = lane; i < N * N; i += kFitThreads) {
           int r = i / N + 1, c = i % N;
           double* row = L.sim + ord[r] * N;
           row[c] = bst[c] + sigma * (row[c] - bst[c]);
@@ -411,7 +425,7 @@ int qed_cdmft_fit_batch(int n, int nb, int nw, const double* w, const double* we
     FitArgs A{n, nb, nw, nvar, dw.as<double>(), dwt.as<double>(), dg.as<double>(), dh.as<int>(),
               dE.as<double>(), dT.as<double>(), dx0.as<double>(), initial_step, xatol, fatol,
               scale, maxfev, dx.as<double>(), df.as<double>(), dn.as<int>(), ds.as<int>()};
-    k_nelder_mead<<<B, 64, lds, stream()>>>(A);
+    k_nelder_mead<<<B, kFitThreads, lds, stream()>>>(A);
     QED_HIP(hipGetLastError());
     QED_HIP(hipMemcpyAsync(x, dx.p, (size_t)B * N * 8, hipMemcpyDeviceToHost, stream()));
     QED_HIP(hipMemcpyAsync(f, df.p, (size_t)B * 8, hipMemcpyDeviceToHost, stream()));
