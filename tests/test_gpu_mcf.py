@@ -131,3 +131,26 @@ def test_gf_method_M_matches_band_lanczos(gpu_lib, qr):
         err = np.abs(GM[s] - GL[s]).max()
         print("spin %d: max |G_M - G_L| = %.3g" % (s, err))
         assert err < 5e-8
+
+
+@pytest.mark.parametrize("qr", [0, 1])
+def test_gf_method_M_deflated_blocks_match_oracle(gpu_lib, qr):
+    """4-site chain plus a decoupled, doubly occupied fifth site: the particle start block
+    deflates (W is 4 x 5) and the hole recurrence stops after one floor (a decoupled
+    eigenvector makes the first residual block singular, as in the reference). The
+    instance-level G against the oracle's restatement of the same steps."""
+    from pyqcm_amd import qcm
+    from tests.test_oracle import _chain_with_filled_site, _mcf_gf
+    m, vals = _chain_with_filled_site()
+    zs = 1j * np.array([0.1, 0.7, 3.0]) + 0.02
+    _with_method(qcm, qr)
+    try:
+        lab = register(with_sites(m, 5), vals, "R0:N6:S0")
+        G = qcm.Green_function(zs, 0, lab)
+    finally:
+        _restore(qcm)
+    E0, gs, *_ = O.Sector(m, vals, 3, 3).lanczos_gs()
+    Go = _mcf_gf(m, vals, 3, 3, gs, E0, zs, bool(qr))
+    err = np.abs(G - Go).max()
+    print("max |G_gpu - G_oracle| = %.3g" % err)
+    assert err < 1e-10

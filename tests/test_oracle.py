@@ -400,3 +400,37 @@ def test_block_lanczos_p1_is_scalar_lanczos():
         k = min(len(A), len(a)) - 1
         np.testing.assert_allclose([x[0, 0] for x in A[:k]], a[:k], rtol=1e-10, atol=1e-12)
         np.testing.assert_allclose([x[0, 0] for x in B[:k]], np.abs(b[1:k + 1]), rtol=1e-10)
+
+
+def _chain_with_filled_site():
+    """4-site chain plus a fifth, isolated site pushed 10 below: doubly occupied in the
+    N=6 ground state, so c+_4 |psi0> = 0 and the particle block deflates at its last
+    column (a 4 x 5 weight W in the matrix continued fraction)."""
+    m, vals = O.hubbard_cluster(5, [(0, 1), (1, 2), (2, 3)], t=1.0, U=3.0, mu=1.2)
+    m.add("e4", "one-body", [(4, 4, 1.0), (9, 9, 1.0)])
+    vals["e4"] = -10.0
+    return m, vals
+
+
+@pytest.mark.parametrize("qr", [False, True])
+def test_mcf_with_deflated_start_block(qr):
+    """Particle block: W is 4 x 5 and the fraction is exact. Hole block: c_4|psi0> is an
+    eigenvector of H (the site is decoupled), so the first residual block is singular and
+    the recurrence stops after one floor, as the reference's does
+    (src_ed/Hamiltonian/Lanczos.hpp:542-546, 703-712): G is then truncated there."""
+    m, vals = _chain_with_filled_site()
+    E0, gs, *_ = O.Sector(m, vals, 3, 3).lanczos_gs()
+    zs = 1j * np.array([0.1, 0.7, 3.0]) + 0.02
+    for pm in (1, -1):
+        T = O.Sector(m, vals, 3 + pm, 3)
+        phi = [O.create_destroy(5, 3, 3, a, pm, gs) for a in range(5)]
+        q, W = O.mgs_weight(phi)
+        A, B = O.block_lanczos(T.hv, q, qr=qr)
+        if pm == 1:
+            assert len(q) == 4 and W.shape == (4, 5)
+            e, Q = O.dense_qmatrix(T.hv, T.dim, phi)
+            A = O.mcf_shift(A, E0, True)
+            for z in zs:
+                assert np.abs(O.mcf_evaluate(A, B, W, z) - O.qmatrix_gf(e - E0, Q, z)).max() < 1e-11
+        else:
+            assert len(q) == 5 and len(A) == 1
