@@ -6,8 +6,12 @@ traffic 24 D bytes (read x, read y, write y). Inputs are resident in HBM before 
 
 Prints one JSON line (rank 0, N=1):
   value        whole-job H.v throughput in GB/s of algorithmic bytes
-  roofline     achieved GB/s of the H.v kernel(s) from HIP events on the library stream
+  roofline     achieved GB/s of the H.v kernel(s) from HIP events on the library stream,
+               the HBM bytes per launch from the committed counter profile, and the
+               achievable read-modify-write bandwidth measured in the same run (y += a x)
+  config       also the p10 / p50 / p90 of 100 single timed launches
   cpu_baseline the oracle's H.v (reference loop order, one core) on the same sector
+  cpu_parallel the same H.v with the oracle's OpenMP-over-rows form on up to 16 cores
   cdmft        the second half of the metric: converged CDMFT solves per second on the
                2x2 plaquette + 8 bath orbitals (16 bath parameters), n seeded points
                (U ~ U[0.5, 12], mu ~ U[0, U], t' ~ U[-0.5, 0.5]), solved by the batched
@@ -83,7 +87,7 @@ def main():
     import ctypes
     ms = np.zeros(1)
     if args.sweep:
-        for v in (0, 1, 4, 5):
+        for v in (0, 1, 4, 5, 12):
             _lib.call("qed_hv_timed", S.h, v, dx.ptr, dy.ptr, 3, 10, _lib.dp(ms))
             log("variant %d: %.3f ms per H.v -> %.0f GB/s" % (v, ms[0], bytes_per / ms[0] / 1e6))
 
@@ -104,8 +108,18 @@ def main():
     achieved = bytes_per / (ms[0] * 1e-3) / 1e9
     log("H.v: %.3f ms/step wall, %.3f ms kernel -> %.1f GB/s (%.1f%% of 8 TB/s)"
         % (ms_step, ms[0], achieved, 100 * achieved / PEAK_HBM_GBS))
+    # spread of single launches (HIP events around each)
+    one = []
+    for _ in range(100):
+        _lib.call("qed_hv_timed", S.h, variant, dx.ptr, dy.ptr, 0, 1, _lib.dp(ms))
+        one.append(ms[0])
+    p10, p50, p90 = (float(np.percentile(one, q)) for q in (10, 50, 90))
+    log("H.v single launches: p10 %.3f p50 %.3f p90 %.3f ms" % (p10, p50, p90))
+    _lib.call("qed_stream_rmw_timed", dx.ptr, dy.ptr, D - (D % 2), 20, _lib.dp(ms))
+    stream = 24.0 * (D - (D % 2)) / (ms[0] * 1e-3) / 1e9
+    log("stream y += a x (24 B per element): %.0f GB/s" % stream)
 
-    cpu = None
+    cpu = cpu_par = None
     if not args.no_cpu:
         # one full H.v of the same sector through the oracle (reference loop order, 1 core)
         So = O.Sector(model, vals, 8, 8)
@@ -118,6 +132,15 @@ def main():
                "sample": "one full y += H x on the same N=16 Sz=0 sector (D=165,636,900) through the "
                          "oracle's restatement of Hamiltonian_Factorized::mult_add, %.1f s" % tc}
         log("cpu baseline: %.2f s per H.v" % tc)
+        nt = min(16, os.cpu_count() or 1)
+        yc[:] = 0.0
+        t0 = time.perf_counter()
+        O.lib().or_hv_omp(So.h, O._dp(xh), O._dp(yc), nt)
+        tp = time.perf_counter() - t0
+        cpu_par = {"value": round(bytes_per / tp / 1e9, 4), "unit": "GB/s", "cores": nt,
+                   "kind": "port, OpenMP over rows (a parallelization of ours; the reference's loop is serial)",
+                   "sample": "one full y += H x on the same sector, %.2f s" % tp}
+        log("cpu parallel (%d threads): %.2f s per H.v" % (nt, tp))
 
     dx.free()
     dy.free()
@@ -143,10 +166,13 @@ def main():
             "sector_dim": D,
             "bytes_per_step": bytes_per,
             "hv_variant": variant,
+            "hv_ms_p10_p50_p90": [round(p10, 4), round(p50, 4), round(p90, 4)],
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_from_profile()},
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_from_profile(),
+                     "stream_rmw_measured": round(stream, 1), "frac_of_stream": round(achieved / stream, 4)},
         "cpu_baseline": cpu,
+        "cpu_parallel": cpu_par,
         "cdmft": cdmft,
     }
     print(json.dumps(out), flush=True)

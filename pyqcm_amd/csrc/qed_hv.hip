@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <mutex>
 #include <random>
@@ -424,13 +425,14 @@ template <int BS, bool DOWN = true>
 __global__ __launch_bounds__(BS) void k_hv_ell2(HvView v, const uint32_t* __restrict__ up_ell,
                                                 uint32_t Wu, const uint32_t* __restrict__ dn_ell,
                                                 uint32_t Wd, const double* __restrict__ x,
-                                                double* __restrict__ y) {
+                                                double* __restrict__ y,
+                                                const uint32_t* __restrict__ order = nullptr) {
   extern __shared__ double smem[];
   const uint32_t du = v.du;
   const uint32_t nvu = v.up_nvals;
   double* vals_s = smem;
   double* xs = smem + ((nvu + 1u) & ~1u);  // 16-byte aligned row
-  const uint32_t r = blockIdx.x;
+  const uint32_t r = order ? order[blockIdx.x] : blockIdx.x;
   const double* xr = x + (uint64_t)r * du;
   const uint32_t dh = du / 2;
   for (uint32_t i = threadIdx.x; i < nvu; i += BS) vals_s[i] = v.up_vals[i];
@@ -827,6 +829,47 @@ static void set_lds_attributes() {
 //      XCD-local column chunks served from L2 (falls back to 5 where 5 falls back)
 //   10 the down-spin term on the transposed vector: y += (up + diag) x by rows,
 //      x^T by LDS tiles, (down hops) x^T by rows, y += its transpose (falls back to 5)
+// Rows of the down-spin half basis in 2D tiles: a state splits into its low and high
+// halves of orbitals (a, b); rows are grouped by the particle number of b, then visited
+// in T x T tiles of (rank of b, rank of a). Hops inside one half change only a or only b.
+std::vector<uint32_t> tiled_row_order(const qed_sector& s, int T) {
+  std::vector<uint32_t> st(s.dd);
+  QED_HIP(hipMemcpy(st.data(), s.bdn.p, s.dd * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  const int half = s.L / 2;
+  const uint32_t lo_mask = (1u << half) - 1u;
+  std::map<uint32_t, uint32_t> arank, brank;  // ascending order within one particle number
+  std::map<int, std::vector<uint32_t>> as, bs;
+  for (uint32_t v : st) {
+    as[__builtin_popcount(v & lo_mask)].push_back(v & lo_mask);
+    bs[__builtin_popcount(v >> half)].push_back(v >> half);
+  }
+  for (auto& kv : as) {
+    std::sort(kv.second.begin(), kv.second.end());
+    kv.second.erase(std::unique(kv.second.begin(), kv.second.end()), kv.second.end());
+    for (size_t i = 0; i < kv.second.size(); ++i) arank[kv.second[i]] = (uint32_t)i;
+  }
+  for (auto& kv : bs) {
+    std::sort(kv.second.begin(), kv.second.end());
+    kv.second.erase(std::unique(kv.second.begin(), kv.second.end()), kv.second.end());
+    for (size_t i = 0; i < kv.second.size(); ++i) brank[kv.second[i]] = (uint32_t)i;
+  }
+  struct Key {
+    uint32_t nb, bt, at, br, ar, row;
+  };
+  std::vector<Key> k(s.dd);
+  for (uint32_t r = 0; r < s.dd; ++r) {
+    const uint32_t a = st[r] & lo_mask, b = st[r] >> half;
+    const uint32_t ar = arank[a], br = brank[b];
+    k[r] = {(uint32_t)__builtin_popcount(b), br / (uint32_t)T, ar / (uint32_t)T, br, ar, r};
+  }
+  std::sort(k.begin(), k.end(), [](const Key& p, const Key& q) {
+    return std::tie(p.nb, p.bt, p.at, p.br, p.ar) < std::tie(q.nb, q.bt, q.at, q.br, q.ar);
+  });
+  std::vector<uint32_t> ord(s.dd);
+  for (uint32_t i = 0; i < s.dd; ++i) ord[i] = k[i].row;
+  return ord;
+}
+
 int g_chunk_w = 32;      // column-chunk width of variant 6 (16, 32 or 64 doubles)
 int g_chunk_rows = 64;   // rows per workgroup of variant 6
 
@@ -834,7 +877,7 @@ void hv(const qed_sector& s, const double* x, double* y, int variant) {
   if (s.D == 0) return;
   set_lds_attributes();
   if (variant < 0) variant = s.hv_variant;
-  if (variant < 0) variant = 5;
+  if (variant < 0) variant = 12;
   HvView v = view(s);
   hipStream_t st = stream();
   const uint32_t* ue = s.up_ell.as<uint32_t>();
@@ -875,6 +918,23 @@ void hv(const qed_sector& s, const double* x, double* y, int variant) {
       k_row_hops2<512><<<s.du, 512, lds_d, st>>>(s.dd, s.dn_ell_cm.as<uint32_t>(), s.dn_w_cm,
                                                  s.dn_vals.as<double>(), s.dn_nvals, xt, zt);
       k_transpose<true><<<g2, 256, 0, st>>>(zt, y, s.du, s.dd);
+      QED_HIP(hipGetLastError());
+      return;
+    }
+    variant = 5;
+  }
+  if (variant == 12) {
+    // the row kernel visiting the rows in 2D tiles over the two halves of the down-spin
+    // orbitals (neighbour rows of a tile share either half); measured on the 16-site
+    // sector: tile side 16 gives 5.01 ms against 5.19 ms in rank order, identical results
+    const size_t lds2 = ((size_t)s.du + s.up_nvals + 1) * sizeof(double);
+    if (s.du % 2 == 0 && s.du >= 4096 && lds2 <= lds_budget()) {
+      if (!s.row_order.p) {
+        std::vector<uint32_t> ord = tiled_row_order(s, 16);
+        s.row_order.alloc(ord.size() * sizeof(uint32_t));
+        QED_HIP(hipMemcpy(s.row_order.p, ord.data(), ord.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      }
+      k_hv_ell2<512><<<s.dd, 512, lds2, st>>>(v, ue, s.up_w, de, s.dn_w, x, y, s.row_order.as<uint32_t>());
       QED_HIP(hipGetLastError());
       return;
     }

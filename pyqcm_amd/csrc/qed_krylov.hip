@@ -155,6 +155,18 @@ __global__ void k_multi_axpy_rev(MutPtrList ys, const double* __restrict__ c, in
   }
 }
 
+// y += 0.5 x with 16-byte accesses: the read-modify-write stream the H.v is compared with
+__global__ __launch_bounds__(256) void k_stream_rmw(const double2* __restrict__ x, double2* __restrict__ y,
+                                                   uint64_t n2) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n2; i += (uint64_t)gridDim.x * 256) {
+    const double2 a = x[i];
+    double2 b = y[i];
+    b.x += 0.5 * a.x;
+    b.y += 0.5 * a.y;
+    y[i] = b;
+  }
+}
+
 uint32_t stream_blocks(uint64_t n) {
   uint64_t b = (n + 255) / 256;
   return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(b, 1), (uint64_t)device_cu_count() * 16);
@@ -776,6 +788,34 @@ int qed_davidson(qed_sector* s, int k, double accur, int max_iter, double* evals
     std::vector<double> e;
     davidson(*s, k, accur, max_iter, e, psi, niter);
     std::copy(e.begin(), e.end(), evals);
+  });
+}
+
+// Achievable read-modify-write bandwidth: reps launches of y += 0.5 x over n doubles (n
+// even, 16-byte aligned vectors), timed with events; *ms = milliseconds per launch.
+int qed_stream_rmw_timed(const double* x, double* y, uint64_t n, int reps, double* ms) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    if (n % 2) fail("stream: n must be even");
+    const uint64_t n2 = n / 2;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n2 + 255) / 256, (uint64_t)device_cu_count() * 64);
+    auto run = [&] {
+      k_stream_rmw<<<grid, 256, 0, stream()>>>(reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y), n2);
+    };
+    run();
+    hipEvent_t e0, e1;
+    QED_HIP(hipEventCreate(&e0));
+    QED_HIP(hipEventCreate(&e1));
+    QED_HIP(hipEventRecord(e0, stream()));
+    for (int r = 0; r < std::max(1, reps); ++r) run();
+    QED_HIP(hipEventRecord(e1, stream()));
+    QED_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    QED_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    QED_HIP(hipGetLastError());
+    *ms = (double)t / std::max(1, reps);
   });
 }
 

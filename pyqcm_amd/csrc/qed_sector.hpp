@@ -38,6 +38,8 @@ struct qed_sector {
   uint32_t up_w = 0, dn_w = 0;
 
   int hv_variant = -1;  // -1 = automatic
+  // row visiting order of the tiled-order row kernel (built on its first use)
+  mutable qed::DevBuf row_order;
 
   // the structure tables (bases, hop lists, ELL) are views into cached tables shared by
   // every sector with the same half-space structure; the amplitudes are this sector's own
