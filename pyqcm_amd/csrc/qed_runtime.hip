@@ -61,7 +61,9 @@ static void do_init(int device) {
 void ensure_init() {
   std::call_once(g_once, [] { do_init(0); });
   if (!g_ready) fail(g_init_err);
-  hipSetDevice(g_device);
+  // the calling thread may not have made the library's device current yet
+  const hipError_t e = hipSetDevice(g_device);
+  if (e != hipSuccess) fail(std::string("hipSetDevice: ") + hipGetErrorString(e));
 }
 
 hipStream_t stream() { return g_stream; }
