@@ -5,7 +5,7 @@ A small MLP (n_in -> H -> H -> n_out, tanh hidden layers) maps model parameters 
 the device (qed_mlp_train: fused FP32 forward/backward kernels and Adam) on solutions of
 the plain loop, evaluated with BF16 matrix-core inference (qed_mlp_forward_bf16), and
 replaces the starting point x0 of the CDMFT iteration (pyqcm/cdmft.py:300-302). The plain
-loop then runs from that point unchanged: it is both the fallback and the judge of the
+loop then runs from that point unchanged: it is both the fallback and the arbiter of the
 prediction, since convergence is still decided by its own tests.
 """
 from __future__ import annotations
