@@ -16,7 +16,7 @@ from oracle import oracle as O
 from tests import models as TM
 
 
-# ---------------------------------------------------------------- basis and rank (a1-a3)
+# ---------------------------------------------------------------- basis and rank
 
 @pytest.mark.parametrize("L", [1, 2, 5, 8, 12])
 def test_halfbasis_exhaustive(L):
@@ -36,7 +36,7 @@ def test_halfbasis_sizes():
     assert len(O.halfbasis(18, 9)) == 48620
 
 
-# ---------------------------------------------------------------- H.v vs brute force (a5-a9)
+# ---------------------------------------------------------------- H.v vs brute force
 
 SMALL_MODELS = {
     "chain4": lambda: TM.chain(4),
@@ -84,7 +84,7 @@ def test_hv_atomic_and_free_limits():
     np.testing.assert_allclose(d, expect, atol=1e-13)
 
 
-# ---------------------------------------------------------------- ground state (a10-a11)
+# ---------------------------------------------------------------- ground state
 
 def test_e0_reference_chain4(golden):
     """examples/1D_4/GS.tsv row 1: U=4, mu=2, t=1 (V, V0m at 1e-9: negligible)."""
@@ -145,7 +145,7 @@ def test_e0_reference_chain8(golden):
     assert abs(E0 + 4 * sum(math.cos(k * math.pi / 9) for k in range(1, 5))) < 1e-8
 
 
-# ---------------------------------------------------------------- start vector (a10)
+# ---------------------------------------------------------------- start vector
 
 def _minstd(seed):
     x = seed % 2147483647
@@ -201,7 +201,7 @@ def test_start_vector_matches_libstdcxx_stream():
     np.testing.assert_allclose(v2, r2 / np.linalg.norm(r2), rtol=1e-14, atol=1e-16)
 
 
-# ---------------------------------------------------------------- tridiagonal (a11)
+# ---------------------------------------------------------------- tridiagonal
 
 def test_dstev_matches_numpy():
     rng = np.random.default_rng(0)
@@ -212,7 +212,7 @@ def test_dstev_matches_numpy():
     np.testing.assert_allclose(T @ Z[0], w[0] * Z[0], atol=1e-11)
 
 
-# ---------------------------------------------------------------- create / destroy (a14)
+# ---------------------------------------------------------------- create / destroy
 
 @pytest.mark.parametrize("nup,ndn", [(2, 2), (1, 3), (3, 2)])
 def test_create_destroy_matches_jordan_wigner(nup, ndn):
@@ -234,7 +234,7 @@ def test_create_destroy_matches_jordan_wigner(nup, ndn):
             np.testing.assert_allclose(y, ref, atol=1e-14)
 
 
-# ---------------------------------------------------------------- Green functions (a15-a18)
+# ---------------------------------------------------------------- Green functions
 
 def _phi(m, vals, nup, ndn, gs, pm, orbitals):
     return [O.create_destroy(m.L, nup, ndn, a, pm, gs) for a in orbitals]
@@ -292,7 +292,7 @@ def test_continued_fraction_matches_lehmann():
         np.testing.assert_allclose(G, Gref[:, o, o], atol=1e-10)
 
 
-# ---------------------------------------------------------------- lattice kernels (a21-a23)
+# ---------------------------------------------------------------- lattice kernels
 
 def test_cpt_identity_and_sef():
     rng = np.random.default_rng(5)
@@ -308,7 +308,7 @@ def test_cpt_identity_and_sef():
     assert abs(s - 2 * np.log(abs(np.linalg.det(np.eye(n) - V @ Gc)))) < 1e-12
 
 
-# ---------------------------------------------------------------- warm-start MLP (a26)
+# ---------------------------------------------------------------- warm-start MLP
 
 def test_mlp_oracle_gradient_matches_finite_differences():
     """The oracle's backpropagation against central differences of its own loss."""
@@ -340,7 +340,7 @@ def test_mlp_oracle_adam_first_step_is_signed_lr():
     np.testing.assert_allclose(p, -0.01 * np.sign(g), rtol=1e-4)
 
 
-# ---------------------------------------------------------------- block Lanczos / MCF (a20)
+# ---------------------------------------------------------------- block Lanczos / MCF
 
 def _mcf_gf(m, vals, nup, ndn, gs, E0, zs, qr):
     n = m.L if not hasattr(m, "n_sites") else m.n_sites
