@@ -24,7 +24,7 @@ def chain_bonds():
 
 
 @pytest.fixture(scope="module")
-def c5(gpu_lib):
+def chain18(gpu_lib):
     from pyqcm_amd.sector import Sector
     model, vals = O.hubbard_cluster(L, chain_bonds(), t=1.0, U=4.0, mu=2.0)
     S = Sector(L, NE, NE, model.flat(vals))
@@ -32,14 +32,14 @@ def c5(gpu_lib):
     return model, vals, S
 
 
-def test_c5_hv_product_vector_closed_form(gpu_lib, c5):
+def test_18site_hv_product_vector_closed_form(gpu_lib, chain18):
     from pyqcm_amd._lib import DeviceArray
-    model, vals, S = c5
+    model, vals, S = chain18
     n = 48620
     rng = np.random.default_rng(18)
     u = rng.standard_normal(n)
     w = rng.standard_normal(n)
-    print("C5: filling the product vector (%.1f GB)" % (n * n * 8 / 1e9), flush=True)
+    print("18 sites: filling the product vector (%.1f GB)" % (n * n * 8 / 1e9), flush=True)
     X = DeviceArray(n * n)
     chunk = 1024
     for r0 in range(0, n, chunk):
@@ -53,7 +53,7 @@ def test_c5_hv_product_vector_closed_form(gpu_lib, c5):
     Z = DeviceArray(n * n)
     gpu_lib.call("qed_hv_timed", S.h, -1, X.ptr, Z.ptr, 1, 3, gpu_lib.dp(ms))
     Z.free()
-    print("C5: H.v %.1f ms (%.0f GB/s of 24 D bytes)" % (ms[0], 24.0 * n * n / ms[0] / 1e6), flush=True)
+    print("18 sites: H.v %.1f ms (%.0f GB/s of 24 D bytes)" % (ms[0], 24.0 * n * n / ms[0] / 1e6), flush=True)
     # closed form from one-spin oracle sectors
     Hup = O.Sector(model, vals, NE, 0)
     Hdn = O.Sector(model, vals, 0, NE)
@@ -70,15 +70,15 @@ def test_c5_hv_product_vector_closed_form(gpu_lib, c5):
     r, c = idx[:, 0], idx[:, 1]
     ref = u[r] * hw[c] + hu[r] * w[c] + 4.0 * popc(b[r] & b[c]) * u[r] * w[c]
     err = np.abs(got - ref).max() / np.abs(ref).max()
-    print("C5: sampled relative error %.2e" % err, flush=True)
+    print("18 sites: sampled relative error %.2e" % err, flush=True)
     assert err < 1e-12
     X.free()
     Y.free()
 
 
-def test_c5_lanczos_ground_state(gpu_lib, c5):
+def test_18site_lanczos_ground_state(gpu_lib, chain18):
     from pyqcm_amd._lib import DeviceArray
-    model, vals, S = c5
+    model, vals, S = chain18
     D = S.dim
     psi = DeviceArray(D)
     gpu_lib.call("qed_start_vector", S.h, 0, psi.ptr)
@@ -95,7 +95,7 @@ def test_c5_lanczos_ground_state(gpu_lib, c5):
     nn, ph, yy = dots
     e = E0[0]
     res2 = yy - 2 * e * ph + e * e * nn
-    print("C5: E0 = %.10f after %d iterations, <psi|psi> = %.15f, Rayleigh %.10f, |r|^2 = %.2e"
+    print("18 sites: E0 = %.10f after %d iterations, <psi|psi> = %.15f, Rayleigh %.10f, |r|^2 = %.2e"
           % (e, nit.value, nn, ph / nn, res2), flush=True)
     assert abs(nn - 1.0) < 1e-10
     assert abs(ph / nn - e) < 1e-9 * abs(e)
