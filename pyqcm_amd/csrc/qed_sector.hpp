@@ -148,9 +148,6 @@ void tridiag_lowest(const std::vector<double>& a, const std::vector<double>& b, 
 
 // y += H x on the library stream (device vectors). variant -1 chooses automatically.
 void hv(const qed_sector& s, const double* x, double* y, int variant = -1);
-// y = alpha*y + H x, with <x, y_out> accumulated into dot_partial when non-null:
-// the fused Lanczos pass.
-void hv_fused(const qed_sector& s, const double* x, double* y, double beta_y, double* dot_out);
 
 // BLAS-1 on the library stream. dot/nrm2 return host scalars (synchronizing).
 double dot(const double* x, const double* y, uint64_t n);
