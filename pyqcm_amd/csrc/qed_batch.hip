@@ -46,10 +46,11 @@ __host__ __device__ inline uint32_t nblocks(uint64_t n) { return (uint32_t)((n +
 
 template <class D>
 __device__ __forceinline__ int find_job(const D* __restrict__ d, int nj, uint32_t b) {
+  const uint32_t* b0 = reinterpret_cast<const uint32_t*>(d + nj);  // compact first blocks
   int lo = 0, hi = nj - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (d[mid].blk0 <= b) lo = mid;
+    if (b0[mid] <= b) lo = mid;
     else hi = mid - 1;
   }
   return lo;
@@ -366,12 +367,22 @@ struct Table {
     total += nblocks(n);
     h.push_back(v);
   }
+  // uploads the descriptors followed by their first blocks as a compact uint32 array
+  // (read by find_job: the search touches a few cache lines instead of the descriptors)
   D* upload() {
-    const size_t bytes = std::max<size_t>(1, h.size()) * sizeof(D);
+    const size_t n = h.size();
+    const size_t bytes = std::max<size_t>(1, n) * (sizeof(D) + sizeof(uint32_t));
     if (d.bytes < bytes) d.alloc(bytes * 2);
-    if (!h.empty()) QED_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(D), hipMemcpyHostToDevice, stream()));
+    if (n) {
+      stage.resize(n * sizeof(D) + n * sizeof(uint32_t));
+      std::memcpy(stage.data(), h.data(), n * sizeof(D));
+      uint32_t* b0 = reinterpret_cast<uint32_t*>(stage.data() + n * sizeof(D));
+      for (size_t j = 0; j < n; ++j) b0[j] = h[j].blk0;
+      QED_HIP(hipMemcpyAsync(d.p, stage.data(), stage.size(), hipMemcpyHostToDevice, stream()));
+    }
     return d.as<D>();
   }
+  std::vector<unsigned char> stage;
 };
 
 void ensure(DevBuf& b, size_t bytes) {

@@ -267,15 +267,11 @@ __global__ __launch_bounds__(BS) void k_hv_ell(HvView v, const uint32_t* __restr
 template <int BS>
 __global__ __launch_bounds__(BS) void k_hv_ell_batch(const HvJob* __restrict__ jobs, int njobs,
                                                      uint32_t ntiles, int parity,
-                                                     const int* __restrict__ active) {
+                                                     const int* __restrict__ active,
+                                                     const uint32_t* __restrict__ tile_job) {
   extern __shared__ double smem[];
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {  // last job with row0 <= t
-      const int mid = (lo + hi + 1) >> 1;
-      if (jobs[mid].row0 <= t) lo = mid;
-      else hi = mid - 1;
-    }
+    const int lo = (int)tile_job[t];  // job owning tile t (row0[lo] <= t < row0[lo + 1])
     const HvJob& J = jobs[lo];
     if (active[lo]) {
       ell_row<BS, true>(J.v, J.up_ell, J.Wu, J.dn_ell, J.Wd, J.x[parity], J.y[parity], t - J.row0,
@@ -1031,12 +1027,13 @@ void launch_batch(const HvBatch& B, const HvBatchConfig& cfg, int parity, const 
     return;
   }
   const uint32_t grid = cfg.grid ? std::min<uint32_t>(cfg.grid, B.ntiles) : B.ntiles;
+  const uint32_t* tj = B.tile_job.as<uint32_t>();
   if (cfg.block == 1024)
-    k_hv_ell_batch<1024><<<grid, 1024, B.lds, st>>>(B.jobs, B.njobs, B.ntiles, parity, active);
+    k_hv_ell_batch<1024><<<grid, 1024, B.lds, st>>>(B.jobs, B.njobs, B.ntiles, parity, active, tj);
   else if (cfg.block == 512)
-    k_hv_ell_batch<512><<<grid, 512, B.lds, st>>>(B.jobs, B.njobs, B.ntiles, parity, active);
+    k_hv_ell_batch<512><<<grid, 512, B.lds, st>>>(B.jobs, B.njobs, B.ntiles, parity, active, tj);
   else
-    k_hv_ell_batch<256><<<grid, 256, B.lds, st>>>(B.jobs, B.njobs, B.ntiles, parity, active);
+    k_hv_ell_batch<256><<<grid, 256, B.lds, st>>>(B.jobs, B.njobs, B.ntiles, parity, active, tj);
 }
 }  // namespace
 
@@ -1089,8 +1086,13 @@ void hv_batch_prepare(HvBatch& B, const std::vector<const qed_sector*>& S) {
   B.njobs = (int)S.size();
   B.ntiles = row;
   B.lds = lds;
-  B.dev.alloc(std::max<size_t>(1, h.size()) * sizeof(HvJob) * 2);
+  B.dev.alloc(std::max<size_t>(1, h.size()) * sizeof(HvJob));
   B.jobs = B.dev.as<HvJob>();
+  // tile -> job map of the row kernel (the job mix is fixed for the batch's lifetime)
+  std::vector<uint32_t> tj(std::max<uint32_t>(1, row), 0);
+  for (size_t j = 0; j < S.size(); ++j)
+    for (uint32_t r = 0; r < S[j]->dd; ++r) tj[h[j].row0 + r] = (uint32_t)j;
+  B.tile_job = upload(tj);
   B.sig.clear();
   for (const qed_sector* s : S) {
     B.sig.push_back(s->D);

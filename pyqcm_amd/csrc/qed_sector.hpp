@@ -94,6 +94,7 @@ struct HvJob {
 struct HvBatch {
   std::vector<HvJob> host;  // fill x / y, then hv_batch_upload
   DevBuf dev;
+  DevBuf tile_job;  // job index of every row tile (row kernel)
   HvJob* jobs = nullptr;
   int njobs = 0;
   uint32_t ntiles = 0;
