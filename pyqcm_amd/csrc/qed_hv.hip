@@ -1093,11 +1093,27 @@ void hv_batch_prepare(HvBatch& B, const std::vector<const qed_sector*>& S) {
   for (size_t j = 0; j < S.size(); ++j)
     for (uint32_t r = 0; r < S[j]->dd; ++r) tj[h[j].row0 + r] = (uint32_t)j;
   B.tile_job = upload(tj);
-  B.sig.clear();
+  // autotuner key: the shape class of the batch (the launch shape that wins depends on
+  // the row lengths and on the amount of work, not on the exact sector mix, so batches
+  // of one CDMFT run share their tuning): largest half dimensions and table widths, the
+  // tiling, and the total rows and amplitudes to within a factor of two
+  uint64_t tot = 0;
+  uint32_t mdu = 0, mdd = 0, mw = 0;
   for (const qed_sector* s : S) {
-    B.sig.push_back(s->D);
-    B.sig.push_back(s->up_w | ((uint64_t)s->dn_w << 32));
+    tot += s->D;
+    mdu = std::max(mdu, s->du);
+    mdd = std::max(mdd, s->dd);
+    mw = std::max(mw, std::max(s->up_w, s->dn_w));
   }
+  auto log2u = [](uint64_t v) {
+    int b = 0;
+    while (v > 1) {
+      v >>= 1;
+      ++b;
+    }
+    return (uint64_t)b;
+  };
+  B.sig.assign({mdu, mdd, mw, (uint64_t)B.R, (uint64_t)B.W, log2u(row), log2u(tot)});
 }
 
 void hv_batch_upload(HvBatch& B) {
