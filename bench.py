@@ -18,7 +18,8 @@ Prints one JSON line (rank 0, N=1):
                engine from the fixed cold start and from the learned warm start (the MLP
                trained on the GPU on the committed training set, BF16 inference), at the
                same convergence tests and tolerances. A "solve" is one parameter point
-               converged by the CDMFT loop.
+               converged by the CDMFT loop. --cdmft-jacobian adds a leg that also starts
+               Broyden from the inverse Jacobian of the nearest training run.
 """
 import argparse
 import json
@@ -48,6 +49,8 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="time every H.v variant (stderr)")
     ap.add_argument("--cdmft", type=int, default=64, help="CDMFT points for the warm-start leg (0: skip)")
     ap.add_argument("--cdmft-batch", type=int, default=32, help="CDMFT problems active at a time")
+    ap.add_argument("--cdmft-jacobian", action="store_true",
+                    help="add a warm leg that also starts Broyden from the nearest training run's inverse Jacobian")
     ap.add_argument("--no-graph", action="store_true",
                     help="batched Lanczos steps as direct launches instead of hipGraph replays")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -145,7 +148,7 @@ def main():
     dx.free()
     dy.free()
     del S
-    cdmft = cdmft_leg(args.cdmft, args.cdmft_batch, not args.no_graph) if args.cdmft > 0 else None
+    cdmft = cdmft_leg(args.cdmft, args.cdmft_batch, not args.no_graph, args.cdmft_jacobian) if args.cdmft > 0 else None
 
     out = {
         "metric": "H·v GB/s (16-site ED, % HBM peak); CDMFT solves/s learned warm-start vs plain",
@@ -201,10 +204,12 @@ def traffic_from_profile():
     return None
 
 
-def cdmft_leg(n, batch, graph=True):
+def cdmft_leg(n, batch, graph=True, jacobian=False):
     """Plain (cold start) vs learned warm start, batched CDMFT on n seeded points with
     `batch` problems active at a time (a converged or failed problem is replaced by the
-    next queued one)."""
+    next queued one). With `jacobian`, a third leg also starts each Broyden iteration from
+    the inverse Jacobian that the nearest training run reached (the reference's CDMFT
+    `alpha` accepts "the inverse Jacobian from a previous run")."""
     from pyqcm_amd import cdmft as C
     from pyqcm_amd import warmstart as W
     X, Y = W.load_training_set()
@@ -221,11 +226,22 @@ def cdmft_leg(n, batch, graph=True):
     t0 = time.perf_counter()
     warm = list(C.canonical_bath(net.predict(feats, bf16=True)))
     t_pred = time.perf_counter() - t0
-    res = {}
-    for name, x0 in (("plain", cold), ("warm", warm)):
+    legs = [("plain", cold, None), ("warm", warm, None)]
+    FJ, t_pred_j = None, 0.0
+    if jacobian:
+        FJ, JJ = W.load_jacobians()
+        if FJ is None:
+            raise SystemExit("no training file holds inverse Jacobians")
         t0 = time.perf_counter()
-        B = C.BatchedCDMFT(P, pts, x0, solver="batched", capacity=batch)
-        dt = time.perf_counter() - t0 + (t_pred if name == "warm" else 0.0)
+        alphas = W.nearest_jacobians(feats, FJ, JJ, X.std(0))
+        t_pred_j = time.perf_counter() - t0
+        legs.append(("warm_jacobian", warm, alphas))
+    res = {}
+    for name, x0, alpha in legs:
+        t0 = time.perf_counter()
+        B = C.BatchedCDMFT(P, pts, x0, solver="batched", capacity=batch, alpha_list=alpha)
+        dt = time.perf_counter() - t0 + (t_pred if name != "plain" else 0.0) + \
+            (t_pred_j if alpha is not None else 0.0)
         good = B.converged & ~B.failed
         res[name] = {"converged": int(good.sum()), "failed": int((~good).sum()), "seconds": round(dt, 2),
                      "solves_per_s": round(good.sum() / dt, 4),
@@ -244,6 +260,9 @@ def cdmft_leg(n, batch, graph=True):
         "plain": res["plain"], "warm": res["warm"],
         "speedup_solves_per_s": round(res["warm"]["solves_per_s"] / max(res["plain"]["solves_per_s"], 1e-12), 3),
         "same_fixed_point": "%d of %d points converged by both" % (int(same.sum()), int(both.sum())),
+        **({"warm_jacobian": res["warm_jacobian"], "jacobian_runs": int(len(FJ)),
+            "speedup_jacobian": round(res["warm_jacobian"]["solves_per_s"] /
+                                      max(res["plain"]["solves_per_s"], 1e-12), 3)} if jacobian else {}),
         "mlp": {"train_points": int(len(X)), "train_s": round(t_train, 2), "final_loss": float(hist[-1]),
                 "inference": "bf16 v_mfma_f32_16x16x32_bf16"},
     }

@@ -428,8 +428,13 @@ class BatchedCDMFT:
 
     def __init__(self, problem, values_list, x0_list, maxiter=32, depth=2, accur=1e-4,
                  accur_bath=1e-6, accur_dist=1e-8, initial_step=0.1, max_function_eval=1_000_000,
-                 solver=None, verbose=False, fit="device", capacity=None):
+                 solver=None, verbose=False, fit="device", capacity=None, alpha_list=None):
+        """alpha_list: per problem, the Broyden inverse-Jacobian start (the reference's CDMFT
+        `alpha`: a float gives (1 + alpha) times the unit matrix, a matrix is used as the
+        inverse Jacobian, e.g. from a previous run; pyqcm/cdmft.py:171, 414-421). The
+        inverse Jacobians reached are kept in self.alpha."""
         self.problem = problem
+        self.alpha_in = alpha_list
         self.fit_where = fit
         self.capacity = capacity or len(values_list)
         B = len(values_list)
@@ -496,7 +501,19 @@ class BatchedCDMFT:
         from collections import deque
         B = len(self.x)
         n = len(self.x[0])
-        Ij = [np.eye(n) for _ in range(B)]
+        Ij = []
+        for i in range(B):
+            a = 0.0 if self.alpha_in is None else self.alpha_in[i]
+            if isinstance(a, tuple):
+                raise TypeError("alpha must be a float or an inverse-Jacobian matrix")
+            if np.isscalar(a):
+                Ij.append((1.0 + a) * np.eye(n))
+                continue
+            a = np.array(a, float, copy=True)
+            if a.shape != (n, n):
+                raise ValueError("the initial Jacobian has the wrong dimensions")
+            Ij.append(a)
+        self.alpha = Ij
         f0 = [None] * B
         dxs = [None] * B
         it = np.zeros(B, int)
@@ -599,6 +616,26 @@ def random_plaquette_instances(n, seed):
         x = np.concatenate([rng.uniform(-3, 3, 8), rng.uniform(0.1, 1.0, 8)])
         out.append((vals, x))
     return out
+
+
+def canonical_map(x):
+    """The signed permutation P with canonical_bath(x) = P x for one 16-parameter point
+    (sign flips of the tb, then the swaps of each pair); a Broyden inverse Jacobian J
+    reached at x reads P J P^T in the canonical coordinates."""
+    x = np.asarray(x, float)
+    n = len(x)
+    S = np.eye(n)
+    for k in range(8, 16):
+        if x[k] < 0:
+            S[k, k] = -1.0
+    perm = np.arange(n)
+    for p in range(4):
+        a, b = 2 * p, 2 * p + 1
+        if x[a] > x[b]:
+            perm[[a, b]] = perm[[b, a]]
+            perm[[8 + a, 8 + b]] = perm[[8 + b, 8 + a]]
+    Pm = np.eye(n)[perm]
+    return Pm @ S
 
 
 def canonical_bath(x):

@@ -123,19 +123,24 @@ def plaquette_problem(name="plaq_ws"):
     return C.CDMFTProblem(M, C.PLAQUETTE_VARIA, C.plaquette_sectors)
 
 
-def solve_points(problem, points, x0, batch=64, verbose=False):
+def solve_points(problem, points, x0, batch=64, verbose=False, alpha=None):
     """Plain CDMFT (batched engine, `batch` problems active at a time) from the starting
-    points x0 (one per point or one for all); returns (x (n x nvar, canonical gauge),
-    converged, niter, solves)."""
+    points x0 (one per point or one for all) and, optionally, Broyden inverse-Jacobian
+    starts `alpha` (one per point); returns (x (n x nvar, canonical gauge), converged,
+    niter, solves, inverse Jacobians reached, in the canonical coordinates of x))."""
     from . import cdmft as C
     x0 = np.atleast_2d(x0)
     if len(x0) == 1:
         x0 = np.repeat(x0, len(points), 0)
-    B = C.BatchedCDMFT(problem, points, list(x0), solver="batched", verbose=verbose, capacity=batch)
+    B = C.BatchedCDMFT(problem, points, list(x0), solver="batched", verbose=verbose, capacity=batch,
+                       alpha_list=alpha)
     good = B.converged & ~B.failed
     if verbose:
         print("%d points: %d converged" % (len(points), int(good.sum())), flush=True)
-    return C.canonical_bath(np.array(B.x)), good, np.array(B.niter), np.array(B.n_solves)
+    Xr = np.array(B.x)
+    maps = [C.canonical_map(x) for x in Xr]
+    ij = np.array([Pm @ a @ Pm.T for Pm, a in zip(maps, B.alpha)])
+    return C.canonical_bath(Xr), good, np.array(B.niter), np.array(B.n_solves), ij
 
 
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
@@ -156,6 +161,30 @@ def load_training_set(pattern="plaquette_warmstart"):
     return np.concatenate(X), np.concatenate(Y)
 
 
+def load_jacobians(pattern="plaquette_warmstart"):
+    """Features and Broyden inverse Jacobians (canonical coordinates) of the converged
+    points of the training files that hold them; (None, None) if none does."""
+    F, J = [], []
+    for f in sorted(os.listdir(DATA_DIR)):
+        if f.startswith(pattern) and f.endswith(".npz"):
+            d = np.load(os.path.join(DATA_DIR, f))
+            if "inverse_jacobian" not in d.files:
+                continue
+            ok = d["converged"].astype(bool)
+            F.append(d["features"][ok])
+            J.append(d["inverse_jacobian"][ok])
+    if not F:
+        return None, None
+    return np.concatenate(F), np.concatenate(J)
+
+
+def nearest_jacobians(feats, F, J, scale):
+    """For each feature row, the inverse Jacobian of the nearest training point (features
+    divided by `scale`)."""
+    d = ((feats[:, None, :] - F[None, :, :]) / scale) ** 2
+    return [J[k] for k in d.sum(-1).argmin(1)]
+
+
 def generate_training_set(n, seed, path, batch=64, chunk=128, verbose=True):
     """Training data for the 2x2+8-bath warm start: n seeded points solved by the plain
     loop from PLAQUETTE_COLD_START, `chunk` points at a time; the .npz (features U, mu, t';
@@ -166,9 +195,10 @@ def generate_training_set(n, seed, path, batch=64, chunk=128, verbose=True):
     parts = []
     for c0 in range(0, n, chunk):
         parts.append(solve_points(P, pts[c0:c0 + chunk], PLAQUETTE_COLD_START, batch, verbose))
-        X, conv, nit, ns = (np.concatenate(a) for a in zip(*parts))
+        X, conv, nit, ns, ij = (np.concatenate(a) for a in zip(*parts))
         k = len(conv)
-        np.savez(path, features=feats[:k], bath=X, converged=conv, iterations=nit, solves=ns, seed=seed)
+        np.savez(path, features=feats[:k], bath=X, converged=conv, iterations=nit, solves=ns, seed=seed,
+                 inverse_jacobian=ij)
         if verbose:
             print("points 0..%d solved, %d converged, saved to %s" % (k, int(conv.sum()), path), flush=True)
     return feats, X, conv

@@ -131,3 +131,28 @@ def test_canonical_bath_is_gauge_invariant():
     y[[0, 1]] = y[[1, 0]]
     y[[8, 9]] = y[[9, 8]]
     np.testing.assert_array_equal(C.canonical_bath(x), C.canonical_bath(y))
+
+
+def test_canonical_map_is_the_signed_permutation_of_canonical_bath():
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        x = np.concatenate([rng.uniform(-3, 3, 8), rng.uniform(-1, 1, 8)])
+        Pm = C.canonical_map(x)
+        np.testing.assert_array_equal(Pm @ x, C.canonical_bath(x))
+        np.testing.assert_array_equal(Pm @ Pm.T, np.eye(16))
+
+
+def test_broyden_matrix_start_is_the_inverse_jacobian():
+    """For a linear residual F(x) = A x - b, starting from the exact inverse Jacobian A^-1
+    the first step lands on the root and the next step is zero; a scalar start alpha gives
+    (1 + alpha) times the unit matrix, as in the reference's broyden()."""
+    A = np.array([[3.0, 0.5, 0.0], [0.2, 2.5, 0.3], [0.0, 0.4, 2.0]])
+    b = np.array([1.0, -0.5, 0.25])
+    F = lambda x: A @ x - b
+    x, it, J = C.broyden(F, np.zeros(3), np.linalg.inv(A), xtol=1e-12, maxiter=5)
+    assert it == 1
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), atol=1e-14)
+    x2, it2, _ = C.broyden(F, np.zeros(3), 0.5, xtol=1e-12, maxiter=200)
+    x3, it3, _ = C.broyden(F, np.zeros(3), 1.5 * np.eye(3), xtol=1e-12, maxiter=200)
+    np.testing.assert_array_equal(x2, x3)
+    assert it2 == it3
