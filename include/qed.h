@@ -240,6 +240,18 @@ int qed_ed_dim_gf(uint64_t label, int* d);
  * order of names, average and variance. */
 int qed_ed_cluster_averages(uint64_t label, int nop, const char* const* names, double* ave,
                             double* var);
+/* ED::write_instance_hdf5 (src_ed/qcm_ED.hpp:256-261, qcm_ED.cpp:373-389): solves the Green
+ * function if needed and writes the instance to group `group` of the HDF5 file `filename`
+ * (created if absent; an existing group of that name is replaced), in the reference's
+ * layout (src_ed/model_instance.hpp:1339-1377, src_ed/state.hpp:46-69). libhdf5 is loaded
+ * on first use (QED_HDF5_LIB overrides its path). */
+int qed_ed_write_instance_hdf5(const char* filename, uint64_t label, const char* group);
+/* ED::read_instance_hdf5 (src_ed/qcm_ED.hpp:263-266, qcm_ED.cpp:392-403): replaces the
+ * states of instance `label` by those of the group (energies, weights, Green-function
+ * representation), warns on stdout when a stored parameter differs from the instance's
+ * by more than 1e-8, and marks the instance solved. The read instance holds no
+ * ground-state vectors: cluster averages then fail. */
+int qed_ed_read_instance_hdf5(const char* filename, uint64_t label, const char* group);
 /* Global parameters (src_util/global_parameter.cpp:12-91): int/double/bool/char maps. */
 int qed_ed_set_global_int(const char* name, int v);
 int qed_ed_set_global_double(const char* name, double v);

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/qed.h"
+#include "qed_h5.hpp"
 #include "qed_linalg.hpp"
 #include "qed_sector.hpp"
 
@@ -108,6 +109,7 @@ struct Instance {
   bool correlated = false, gs_solved = false, gf_solved = false;
   bool cf = false;   // continued fractions ('F')
   bool mcf = false;  // matrix continued fractions ('M')
+  bool from_file = false;  // states read from an instance file: Green functions only, no vectors
   std::vector<std::unique_ptr<State>> states;
   double GS_energy = 0.0, E0 = 0.0, SEF_bath = 0.0;
   // one-body matrices (index 0 up, 1 down), bath diagonalized
@@ -968,6 +970,237 @@ void hybridization(Instance& I, int nz, const double* z, int spin_down, double* 
   }
 }
 
+// ---- instance files -------------------------------------------------------------------
+// The reference's HDF5 layout (model_instance::write_hdf5 / read_hdf5,
+// src_ed/model_instance.hpp:1339-1421; state::write_hdf5, src_ed/state.hpp:46-115;
+// Q_matrix / Q_matrix_set, src_ed/Q_matrix.hpp:320-345, src_ed/Q_matrix_set.hpp:183-210;
+// continued_fraction_set, src_ed/continued_fraction_set.cpp:136-176; mcf_set and its
+// matrix continued fractions, src_ed/mcf_set.hpp:162-197, matrix_continued_fraction.hpp:
+// 511-547; primitives src_util/hdf5_io.cpp), for the trivial symmetry group (one block).
+
+const char* gf_format_name(const Instance& I) {
+  if (I.correlated && I.mcf) return "mcf";
+  if (I.correlated && I.cf) return "cf";
+  return "bl";
+}
+
+void write_mcf(const h5::Node& g, const MCFrac& f) {
+  const int M = f.M(), pa = f.pa;
+  const size_t pp = (size_t)pa * pa;
+  h5::write_attr(g, "p", pa);
+  h5::write_attr(g, "floors", M);
+  for (int j = 0; j < M; ++j) {
+    h5::Node a = h5::create_group(g, "A_" + std::to_string(j));
+    h5::write_mat(a, "data", pa, pa, f.A.data() + j * pp);
+  }
+  for (int j = 0; j < (int)(f.B.size() / std::max<size_t>(pp, 1)); ++j) {
+    h5::Node b = h5::create_group(g, "B_" + std::to_string(j));
+    h5::write_mat(b, "data", pa, pa, f.B.data() + j * pp);
+  }
+  h5::Node w = h5::create_group(g, "W");
+  h5::write_mat_complex(w, "data", pa, pa ? f.q : 0, f.W.data(), nullptr);
+}
+
+MCFrac read_mcf(const h5::Node& g) {
+  MCFrac f;
+  f.pa = h5::read_attr_int(g, "p");
+  const int M = h5::read_attr_int(g, "floors");
+  int r, c;
+  for (int j = 0; j < M; ++j) {
+    auto a = h5::read_mat_real(h5::open_group(g, "A_" + std::to_string(j)), "data", r, c);
+    if (r != f.pa || c != f.pa) fail("matrix continued fraction: block A_" + std::to_string(j) + " is not p x p");
+    f.A.insert(f.A.end(), a.begin(), a.end());
+  }
+  for (int j = 0; j < M; ++j) {
+    auto b = h5::read_mat_real(h5::open_group(g, "B_" + std::to_string(j)), "data", r, c);
+    if (r != f.pa || c != f.pa) fail("matrix continued fraction: block B_" + std::to_string(j) + " is not p x p");
+    f.B.insert(f.B.end(), b.begin(), b.end());
+  }
+  f.W = h5::read_mat_real(h5::open_group(g, "W"), "data", r, c);
+  if (M > 0 && r != f.pa) fail("matrix continued fraction: W has " + std::to_string(r) + " rows for p = " +
+                               std::to_string(f.pa));
+  f.q = M > 0 ? c : 0;
+  if (M == 0) f = MCFrac();
+  return f;
+}
+
+void write_cf(const h5::Node& g, const std::vector<CFrac>& e, const std::vector<CFrac>& h, int d) {
+  h5::write_attr(g, "is_complex", 0);
+  for (int a = 0; a < d; ++a)
+    for (int b = 0; b <= a; ++b) {
+      h5::Node s = h5::create_group(g, "r0_a" + std::to_string(a) + "_b" + std::to_string(b));
+      for (int k = 0; k < 2; ++k) {
+        const std::vector<CFrac>& v = k ? h : e;
+        static const CFrac none;
+        const CFrac& f = v.size() == (size_t)d * d ? v[a + (size_t)b * d] : none;
+        h5::Node x = h5::create_group(s, k ? "h" : "e");
+        h5::write_vec(x, "a", f.set ? f.a : std::vector<double>());
+        h5::write_vec(x, "b", f.set ? f.b : std::vector<double>());
+      }
+    }
+}
+
+void read_cf(const h5::Node& g, std::vector<CFrac>& e, std::vector<CFrac>& h, int d) {
+  if (h5::read_attr_int(g, "is_complex") != 0) fail("complex continued fractions: this solver has a real Hilbert space");
+  e.assign((size_t)d * d, CFrac());
+  h.assign((size_t)d * d, CFrac());
+  for (int a = 0; a < d; ++a)
+    for (int b = 0; b <= a; ++b) {
+      h5::Node s = h5::open_group(g, "r0_a" + std::to_string(a) + "_b" + std::to_string(b));
+      for (int k = 0; k < 2; ++k) {
+        h5::Node x = h5::open_group(s, k ? "h" : "e");
+        CFrac f;
+        f.a = h5::read_vec(x, "a");
+        f.b = h5::read_vec(x, "b");
+        if (f.b.size() < f.a.size()) fail("continued fraction r0_a" + std::to_string(a) + "_b" + std::to_string(b) +
+                                          ": fewer b than a coefficients");
+        f.set = !f.a.empty();
+        (k ? h : e)[a + (size_t)b * d] = f;
+      }
+    }
+}
+
+void write_gf(const h5::Node& g, const Instance& I, const State& s, int spin, const std::string& fmt) {
+  const int d = I.model->n_sites;
+  if (fmt == "bl") {
+    const QMat& q = s.gf[spin];
+    h5::write_attr(g, "nblocks", 1);
+    h5::Node b = h5::create_group(g, "block_0");
+    h5::write_attr(b, "L", q.L);
+    h5::write_attr(b, "M", q.M());
+    h5::write_vec(b, "e", q.e);
+    h5::write_mat(b, "v", q.L, q.M(), q.Q.data());
+  } else if (fmt == "cf") {
+    write_cf(g, s.cfe[spin], s.cfh[spin], d);
+  } else {
+    h5::write_attr(g, "nblocks", 1);
+    h5::Node b = h5::create_group(g, "block_0");
+    h5::Node e = h5::create_group(b, "e");
+    write_mcf(e, s.mcf[spin][1]);
+    h5::Node h = h5::create_group(b, "h");
+    write_mcf(h, s.mcf[spin][0]);
+  }
+}
+
+void read_gf(const h5::Node& g, const Instance& I, State& s, int spin, const std::string& fmt) {
+  const int d = I.model->n_sites;
+  if (fmt == "cf") {
+    read_cf(g, s.cfe[spin], s.cfh[spin], d);
+    return;
+  }
+  if (h5::read_attr_int(g, "nblocks") != 1)
+    fail("the Green function has " + std::to_string(h5::read_attr_int(g, "nblocks")) +
+         " symmetry blocks; this solver has the trivial group (one block)");
+  h5::Node b = h5::open_group(g, "block_0");
+  if (fmt == "bl") {
+    QMat q;
+    q.L = h5::read_attr_int(b, "L");
+    const int M = h5::read_attr_int(b, "M");
+    q.e = h5::read_vec(b, "e");
+    int r, c;
+    q.Q = h5::read_mat_real(b, "v", r, c);
+    if ((int)q.e.size() != M || r != q.L || c != M) fail("Q matrix: e and v do not match L and M");
+    if (M > 0 && q.L != d) fail("Q matrix: L = " + std::to_string(q.L) + " for a cluster of " + std::to_string(d));
+    s.gf[spin] = q;
+  } else {
+    s.mcf[spin][1] = read_mcf(h5::open_group(b, "e"));
+    s.mcf[spin][0] = read_mcf(h5::open_group(b, "h"));
+    for (auto& f : s.mcf[spin])
+      if (f.M() > 0 && f.q != d) fail("matrix continued fraction: W has " + std::to_string(f.q) + " columns for a cluster of " +
+                                      std::to_string(d));
+  }
+}
+
+void write_instance_file(Instance& I, const std::string& path, const std::string& group) {
+  green_function_solve(I);
+  const std::string fmt = gf_format_name(I);
+  try {
+    h5::Node f = h5::open_or_create(path);
+    if (h5::exists(f, group)) h5::unlink(f, group);
+    h5::Node g = h5::create_group(f, group);
+    h5::write_attr(g, "GS_energy", I.GS_energy);
+    h5::write_attr(g, "GS_sectors", I.gs_string);
+    h5::write_attr(g, "GF_format", fmt);
+    h5::write_attr(g, "mixing", I.mixing);
+    h5::write_attr(g, "complex_HS", 0);
+    {
+      h5::Node pg = h5::create_group(g, "params");
+      for (auto& kv : I.value) h5::write_attr(pg, kv.first, kv.second);
+    }
+    h5::write_attr(g, "nstates", (int)I.states.size());
+    for (size_t k = 0; k < I.states.size(); ++k) {
+      const State& s = *I.states[k];
+      h5::Node sg = h5::create_group(g, "state_" + std::to_string(k));
+      h5::write_attr(sg, "sec", sector_name(s.nup + s.ndn, s.nup - s.ndn));
+      h5::write_attr(sg, "energy", s.energy);
+      h5::write_attr(sg, "weight", s.weight);
+      h5::write_attr(sg, "gf_format", fmt);
+      write_gf(h5::create_group(sg, "gf"), I, s, 0, fmt);
+      if (I.mixing & 4) write_gf(h5::create_group(sg, "gf_down"), I, s, 1, fmt);
+    }
+  } catch (const std::exception& e) {
+    fail(std::string("HDF5 error while writing instance: ") + e.what());
+  }
+}
+
+std::pair<int, int> sector_numbers(const std::string& s);
+
+void read_instance_file(Instance& I, const std::string& path, const std::string& group) {
+  std::vector<std::unique_ptr<State>> states;
+  double gse = 0.0;
+  std::string fmt, gs;
+  int mixing = 0;
+  try {
+    h5::Node f = h5::open_read(path);
+    h5::Node g = h5::open_group(f, group);
+    gse = h5::read_attr_double(g, "GS_energy");
+    fmt = h5::read_attr_string(g, "GF_format");
+    mixing = h5::read_attr_int(g, "mixing");
+    gs = h5::attr_exists(g, "GS_sectors") ? h5::read_attr_string(g, "GS_sectors") : std::string();
+    {
+      h5::Node pg = h5::open_group(g, "params");
+      for (auto& kv : I.value)
+        if (h5::attr_exists(pg, kv.first)) {
+          const double stored = h5::read_attr_double(pg, kv.first);
+          if (std::abs(stored - kv.second) > MIDDLE_VALUE)
+            std::printf("WARNING : The value of %s from the HDF5 solution (%g) differs from the expected value (%g).\n",
+                        kv.first.c_str(), stored, kv.second);
+          std::fflush(stdout);
+        }
+    }
+    const int n = h5::read_attr_int(g, "nstates");
+    for (int k = 0; k < n; ++k) {
+      h5::Node sg = h5::open_group(g, "state_" + std::to_string(k));
+      auto st = std::make_unique<State>();
+      const auto ns = sector_numbers(h5::read_attr_string(sg, "sec"));
+      st->nup = (ns.first + ns.second) / 2;
+      st->ndn = (ns.first - ns.second) / 2;
+      st->energy = h5::read_attr_double(sg, "energy");
+      st->weight = h5::read_attr_double(sg, "weight");
+      std::string sf = h5::read_attr_string(sg, "gf_format");
+      if (sf != "bl" && sf != "mcf") sf = "cf";
+      if (h5::exists(sg, "gf")) read_gf(h5::open_group(sg, "gf"), I, *st, 0, sf);
+      if (h5::exists(sg, "gf_down")) read_gf(h5::open_group(sg, "gf_down"), I, *st, 1, sf);
+      states.push_back(std::move(st));
+    }
+  } catch (const std::exception& e) {
+    fail(std::string("HDF5 error while reading instance: ") + e.what());
+  }
+  // the instance changes only once the whole group has been read
+  I.states.swap(states);
+  I.GS_energy = gse;
+  I.mixing = mixing;
+  I.cf = fmt == "cf" || (fmt != "bl" && fmt != "mcf");
+  I.mcf = fmt == "mcf";
+  I.correlated = true;
+  I.gs_solved = I.gf_solved = true;
+  I.from_file = true;
+  I.E0 = 0.0;
+  for (auto& s : I.states) I.E0 += s->weight * s->energy;
+  I.gs_string = gs;
+  set_hopping_matrix(I);  // one-body matrices and the bath term for the file's mixing
+}
+
 }  // namespace ed
 }  // namespace qed
 
@@ -1010,6 +1243,8 @@ std::pair<int, int> parse_sector(const std::string& s) {
   return {N, S};
 }
 }  // namespace
+
+std::pair<int, int> qed::ed::sector_numbers(const std::string& s) { return parse_sector(s); }
 
 extern "C" {
 
@@ -1316,6 +1551,7 @@ int qed_ed_cluster_averages(uint64_t label, int nop, const char* const* names, d
     Instance& I = inst(label);
     low_energy_states(I);
     if (!I.correlated) fail("cluster averages of an uncorrelated instance come from the Green function");
+    if (I.from_file) fail("cluster averages need the ground-state vectors, which an instance read from a file does not hold");
     for (int k = 0; k < nop; ++k) {
       std::string nm(names[k]);
       if (!I.value.count(nm)) fail("operator " + nm + " is not activated in instance " + std::to_string(label));
@@ -1339,6 +1575,24 @@ int qed_ed_cluster_averages(uint64_t label, int nop, const char* const* names, d
       ave[k] = a;
       var[k] = v;
     }
+  });
+}
+
+int qed_ed_write_instance_hdf5(const char* filename, uint64_t label, const char* group) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    auto it = instances().find(label);
+    if (it == instances().end()) fail("The label " + std::to_string(label) + " is out of range.");
+    write_instance_file(*it->second, filename, group);
+  });
+}
+
+int qed_ed_read_instance_hdf5(const char* filename, uint64_t label, const char* group) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    auto it = instances().find(label);
+    if (it == instances().end()) fail("The label " + std::to_string(label) + " is out of range.");
+    read_instance_file(*it->second, filename, group);
   });
 }
 

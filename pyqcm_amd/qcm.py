@@ -192,6 +192,19 @@ def cluster_averages(label=0):
     return {n: (float(a), float(v)) for n, a, v in zip(names, ave, var)}
 
 
+def write_instance_to_hdf5(filename, group, label=0):
+    """Writes the solved instance to group `group` of an HDF5 file (created if absent; the
+    group is replaced if present), in the reference's layout
+    (src_python/qcm_ED_wrap.hpp:364-369, src_ed/qcm_ED.cpp:373-389)."""
+    call("qed_ed_write_instance_hdf5", _b(str(filename)), int(label), _b(group))
+
+
+def read_instance_from_hdf5(filename, group, label=0):
+    """Replaces the solution of instance `label` by the one stored in group `group`
+    (src_python/qcm_ED_wrap.hpp:371-376, src_ed/qcm_ED.cpp:392-403)."""
+    call("qed_ed_read_instance_hdf5", _b(str(filename)), int(label), _b(group))
+
+
 # ---------------------------------------------------------------- global parameters
 
 _BOOL = {"verb_ED", "no_degenerate_BL", "continued_fraction", "block_Lanczos_QR"}
