@@ -240,6 +240,17 @@ int qed_ed_dim_gf(uint64_t label, int* d);
  * order of names, average and variance. */
 int qed_ed_cluster_averages(uint64_t label, int nop, const char* const* names, double* ave,
                             double* var);
+/* ED::susceptibility (src_ed/qcm_ED.cpp:307-315, model_instance.hpp:1024-1058): dynamic
+ * susceptibility of operator `op` at nz complex frequencies z (interleaved re, im), the
+ * Boltzmann-weighted sum over low-energy states of <psi|O (w - H + E)^-1 O|psi>, with the
+ * Lehmann form of O|psi> from band Lanczos (full diagonalization below max_dim_full).
+ * chi: nz complex values, interleaved. */
+int qed_ed_susceptibility(uint64_t label, const char* op, int nz, const double* z, double* chi);
+/* ED::susceptibility_poles (src_ed/qcm_ED.cpp:319-326, model_instance.hpp:1066-1113): the
+ * poles (> 1e-8) and residues (> 1e-6) of the same sum, poles closer than 1e-4 merged, as
+ * (pole, residue) pairs in poles[2k], poles[2k+1]; at most cap pairs are written and *n is
+ * their total number. */
+int qed_ed_susceptibility_poles(uint64_t label, const char* op, int cap, int* n, double* poles);
 /* ED::write_instance_hdf5 (src_ed/qcm_ED.hpp:256-261, qcm_ED.cpp:373-389): solves the Green
  * function if needed and writes the instance to group `group` of the HDF5 file `filename`
  * (created if absent; an existing group of that name is replaced), in the reference's

@@ -970,6 +970,37 @@ void hybridization(Instance& I, int nz, const double* z, int spin_down, double* 
   }
 }
 
+// ---- dynamic susceptibility (model_instance::susceptibility / susceptibility_poles,
+// src_ed/model_instance.hpp:1024-1113) ----------------------------------------------------
+// For each low-energy state: b = O|psi> with the operator's own Hamiltonian-space form
+// (coefficient 1), the Lehmann form of b in the state's sector (band Lanczos, or full
+// diagonalization below max_dim_full) and the poles shifted by the state's energy.
+struct SusceptibilityPart {
+  double weight;
+  QMat q;
+};
+
+std::vector<SusceptibilityPart> susceptibility_parts(Instance& I, const std::string& op) {
+  if (!I.model->idx.count(op)) fail("Operator " + op + " is not defined in the model.");
+  if (!I.value.count(op)) fail("operator " + op + " is not activated in instance " + std::to_string(I.label));
+  low_energy_states(I);
+  if (I.from_file) fail("the susceptibility needs the ground-state vectors, which an instance read from a file does not hold");
+  std::vector<SusceptibilityPart> out;
+  for (auto& st : I.states) {
+    if (!st->psi.p) fail("the susceptibility needs the ground-state vectors of the instance");
+    SectorPtr H = make_sector(I, st->nup, st->ndn);
+    if (H->D == 0) continue;
+    SectorPtr O = make_sector(I, st->nup, st->ndn, &op);
+    DevBuf b(H->D * 8);
+    QED_HIP(hipMemsetAsync(b.p, 0, H->D * 8, stream()));
+    hv(*O, st->psi.as<double>(), b.as<double>());
+    QMat q = sector_qmatrix(*H, b.as<double>(), 1);
+    for (double& e : q.e) e -= st->energy;
+    out.push_back({st->weight, std::move(q)});
+  }
+  return out;
+}
+
 // ---- instance files -------------------------------------------------------------------
 // The reference's HDF5 layout (model_instance::write_hdf5 / read_hdf5,
 // src_ed/model_instance.hpp:1339-1421; state::write_hdf5, src_ed/state.hpp:46-115;
@@ -1574,6 +1605,53 @@ int qed_ed_cluster_averages(uint64_t label, int nop, const char* const* names, d
       }
       ave[k] = a;
       var[k] = v;
+    }
+  });
+}
+
+int qed_ed_susceptibility(uint64_t label, const char* op, int nz, const double* z, double* chi) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    auto parts = susceptibility_parts(inst(label), op);
+    for (int j = 0; j < nz; ++j) {
+      const cd w(z[2 * j], z[2 * j + 1]);
+      cd c = 0.0;
+      for (auto& p : parts) {
+        cd g = 0.0;  // Q_matrix::Green_function for one start vector
+        for (int i = 0; i < p.q.M(); ++i) g += p.q.Q[i] * p.q.Q[i] / (w - p.q.e[i]);
+        c += p.weight * g;
+      }
+      chi[2 * j] = c.real();
+      chi[2 * j + 1] = c.imag();
+    }
+  });
+}
+
+int qed_ed_susceptibility_poles(uint64_t label, const char* op, int cap, int* n, double* poles) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    auto parts = susceptibility_parts(inst(label), op);
+    std::vector<std::pair<double, double>> chi, merged;
+    for (auto& p : parts)
+      for (int i = 0; i < p.q.M(); ++i) {
+        if (p.q.e[i] < 1e-8) continue;
+        const double r = std::abs(p.q.Q[i]) * std::abs(p.q.Q[i]) * p.weight;
+        if (r > 1e-6) chi.push_back({p.q.e[i], r});
+      }
+    // consolidation: a pole within 1e-4 of a kept one adds its residue to it
+    for (auto& c : chi) {
+      size_t j = 0;
+      for (; j < merged.size(); ++j)
+        if (std::fabs(merged[j].first - c.first) < 1e-4) {
+          merged[j].second += c.second;
+          break;
+        }
+      if (j == merged.size()) merged.push_back(c);
+    }
+    *n = (int)merged.size();
+    for (int k = 0; k < std::min(cap, *n); ++k) {
+      poles[2 * k] = merged[k].first;
+      poles[2 * k + 1] = merged[k].second;
     }
   });
 }

@@ -192,6 +192,28 @@ def cluster_averages(label=0):
     return {n: (float(a), float(v)) for n, a, v in zip(names, ave, var)}
 
 
+def susceptibility(op, w, label=0):
+    """Dynamic susceptibility of operator `op` at the complex frequencies w
+    (src_python/qcm_ED_wrap.hpp:327-333, src_ed/model_instance.hpp:1024-1058)."""
+    w = np.ascontiguousarray(np.atleast_1d(np.asarray(w, dtype=np.complex128)))
+    out = np.zeros(len(w), np.complex128)
+    call("qed_ed_susceptibility", int(label), _b(op), len(w), dp(w.view(np.float64)), dp(out.view(np.float64)))
+    return out
+
+
+def susceptibility_poles(op, label=0):
+    """Lehmann form of the dynamic susceptibility: an (n, 2) array of (pole, residue) rows
+    (src_python/qcm_ED_wrap.hpp:318-325, src_ed/model_instance.hpp:1066-1113)."""
+    n = ctypes.c_int()
+    cap = 1024
+    while True:
+        out = np.zeros(2 * cap)
+        call("qed_ed_susceptibility_poles", int(label), _b(op), cap, ctypes.byref(n), dp(out))
+        if n.value <= cap:
+            return out[:2 * n.value].reshape(n.value, 2).copy()
+        cap = n.value
+
+
 def write_instance_to_hdf5(filename, group, label=0):
     """Writes the solved instance to group `group` of an HDF5 file (created if absent; the
     group is replaced if present), in the reference's layout
