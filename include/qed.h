@@ -240,6 +240,25 @@ int qed_ed_dim_gf(uint64_t label, int* d);
  * order of names, average and variance. */
 int qed_ed_cluster_averages(uint64_t label, int nop, const char* const* names, double* ave,
                             double* var);
+/* ED::Green_function_average (src_ed/qcm_ED.cpp:240-248, model_instance.hpp:643-654): the
+ * d x d matrix M(b, a) = <c^dagger_a c_b> integrated from the Green-function representation
+ * of the solved instance (Lehmann: poles below zero; continued fractions: the hole b_0;
+ * matrix continued fractions: (W_h^T W_h)^T), column-major complex, interleaved. */
+int qed_ed_green_function_average(uint64_t label, int spin_down, double* M);
+/* ED::Green_function_density (src_ed/qcm_ED.cpp:251-257, model_instance.hpp:661-675): the
+ * trace of that matrix per cluster site (both spins). Solves the Green function first. */
+int qed_ed_green_function_density(uint64_t label, double* density);
+/* ED::fidelity (src_ed/qcm_ED.cpp:360-368, model_instance.hpp:1261-1274): sum over the
+ * pairs of low-energy states in the same sector of |<psi1|psi2>|^2 w1 w2. */
+int qed_ed_fidelity(uint64_t label1, uint64_t label2, double* fidelity);
+/* ED::tr_sigma_inf (src_ed/qcm_ED.hpp:247-252, model_instance.hpp:1277-1312): trace of the
+ * self-energy at z = i/eps, eps = 1e-5, 2e-5, 3e-5, extrapolated to eps = 0 by polynomial
+ * interpolation. */
+int qed_ed_tr_sigma_inf(uint64_t label, double* value);
+/* ED::hybridization (src_ed/qcm_ED.hpp:287-294, model_instance.hpp:1455-1466): the diagonal
+ * of the bath one-body matrix (n_bath values) and the cluster-bath matrix, n_sites x n_bath
+ * column-major. Fails without bath orbitals. */
+int qed_ed_hybridization_lehmann(uint64_t label, int spin_down, double* E, double* T);
 /* ED::susceptibility (src_ed/qcm_ED.cpp:307-315, model_instance.hpp:1024-1058): dynamic
  * susceptibility of operator `op` at nz complex frequencies z (interleaved re, im), the
  * Boltzmann-weighted sum over low-energy states of <psi|O (w - H + E)^-1 O|psi>, with the

@@ -970,6 +970,60 @@ void hybridization(Instance& I, int nz, const double* z, int spin_down, double* 
   }
 }
 
+// ---- integrated Green function: the matrix of <c^dagger_j c_i> from the Green-function
+// representation (model_instance::Green_function_average, src_ed/model_instance.hpp:
+// 643-654; Q_matrix::integrated_Green_function, Q_matrix.hpp:161-171: poles below zero;
+// continued_fraction_set::integrated_Green_function, continued_fraction_set.cpp:104-125:
+// b_0 of the hole fractions; mcf_set::integrated_Green_function, mcf_set.hpp:146-158:
+// (W_h^T W_h)^T). M(b, a) accumulates over the states, d x d column-major.
+std::vector<cd> gf_average(Instance& I, int sp) {
+  const int d = I.model->n_sites;
+  std::vector<cd> M((size_t)d * d, 0.0);
+  auto at = [&](int r, int c) -> cd& { return M[r + (size_t)c * d]; };
+  for (auto& s : I.states) {
+    if (I.mcf && I.correlated) {
+      const MCFrac& h = s->mcf[sp][0];
+      if (h.M() == 0 || h.pa == 0) continue;
+      for (int a = 0; a < h.q; ++a)
+        for (int b = 0; b < h.q; ++b) {
+          double w = 0.0;
+          for (int k = 0; k < h.pa; ++k) w += h.W[k + (size_t)h.pa * a] * h.W[k + (size_t)h.pa * b];
+          at(b, a) += w;  // (W^T W)^T
+        }
+    } else if (I.cf && I.correlated) {
+      const auto& h = s->cfh[sp];
+      if (h.size() != (size_t)d * d) continue;
+      auto b0 = [&](int o1, int o2) { return h[o1 + (size_t)o2 * d].set ? h[o1 + (size_t)o2 * d].b[0] : 0.0; };
+      for (int o = 0; o < d; ++o) at(o, o) += b0(o, o);
+      for (int o1 = 0; o1 < d; ++o1)
+        for (int o2 = 0; o2 < o1; ++o2) {
+          const cd t = 0.5 * (b0(o1, o2) - at(o1, o1) - at(o2, o2));
+          at(o1, o2) += t;
+          at(o2, o1) += t;
+        }
+    } else {
+      const QMat& q = s->gf[sp];
+      for (int i = 0; i < q.M(); ++i) {
+        if (q.e[i] >= 0.0) continue;
+        for (int a = 0; a < q.L; ++a)
+          for (int b = 0; b < q.L; ++b) at(b, a) += q.Q[a + (size_t)i * q.L] * q.Q[b + (size_t)i * q.L];
+      }
+    }
+  }
+  return M;
+}
+
+// Neville's polynomial interpolation of (xa, ya), evaluated at x (Numerical Recipes
+// polint; the reference's polynomial_fit, src_ed/model_instance.cpp:184-222)
+double neville(const std::vector<double>& xa, const std::vector<double>& ya, double x) {
+  const size_t n = xa.size();
+  std::vector<double> P(ya);
+  for (size_t m = 1; m < n; ++m)
+    for (size_t i = 0; i + m < n; ++i)
+      P[i] = ((x - xa[i + m]) * P[i] + (xa[i] - x) * P[i + 1]) / (xa[i] - xa[i + m]);
+  return P[0];
+}
+
 // ---- dynamic susceptibility (model_instance::susceptibility / susceptibility_poles,
 // src_ed/model_instance.hpp:1024-1113) ----------------------------------------------------
 // For each low-energy state: b = O|psi> with the operator's own Hamiltonian-space form
@@ -1606,6 +1660,98 @@ int qed_ed_cluster_averages(uint64_t label, int nop, const char* const* names, d
       ave[k] = a;
       var[k] = v;
     }
+  });
+}
+
+int qed_ed_green_function_average(uint64_t label, int spin_down, double* M) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    Instance& I = inst(label);
+    green_function_solve(I);
+    const auto m = gf_average(I, spin_index(I, spin_down));
+    for (size_t q = 0; q < m.size(); ++q) {
+      M[2 * q] = m[q].real();
+      M[2 * q + 1] = m[q].imag();
+    }
+  });
+}
+
+int qed_ed_green_function_density(uint64_t label, double* density) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    Instance& I = inst(label);
+    green_function_solve(I);
+    const int d = I.model->n_sites;
+    double n = 0.0;
+    const auto m = gf_average(I, 0);
+    for (int i = 0; i < d; ++i) n += m[i + (size_t)i * d].real();
+    if (I.mixing & 4) {
+      const auto md = gf_average(I, 1);
+      for (int i = 0; i < d; ++i) n += md[i + (size_t)i * d].real();
+    }
+    n /= d;
+    if (I.mixing == 0) n *= 2.0;
+    *density = n;
+  });
+}
+
+int qed_ed_fidelity(uint64_t label1, uint64_t label2, double* fidelity) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    Instance& A = inst(label1);
+    Instance& B = inst(label2);
+    low_energy_states(A);
+    low_energy_states(B);
+    double z = 0.0;
+    for (auto& s1 : A.states)
+      for (auto& s2 : B.states) {
+        if (s1->nup != s2->nup || s1->ndn != s2->ndn) continue;
+        if (!s1->psi.p || !s2->psi.p) fail("the fidelity needs the ground-state vectors of both instances");
+        if (A.model->L != B.model->L) fail("the two instances have different Hilbert spaces");
+        auto binom = [](int n, int k) {
+          uint64_t c = 1;
+          for (int i = 1; i <= k; ++i) c = c * (uint64_t)(n - k + i) / (uint64_t)i;
+          return c;
+        };
+        const uint64_t D = binom(A.model->L, s1->nup) * binom(A.model->L, s1->ndn);
+        const double zz = dot(s1->psi.as<double>(), s2->psi.as<double>(), D);
+        z += zz * zz * s1->weight * s2->weight;
+      }
+    *fidelity = z;
+  });
+}
+
+int qed_ed_tr_sigma_inf(uint64_t label, double* value) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    Instance& I = inst(label);
+    const int d = I.model->n_sites;
+    std::vector<double> eps(3), S(3, 0.0), z(6), sig((size_t)2 * 3 * d * d);
+    for (int i = 0; i < 3; ++i) {
+      eps[i] = (i + 1) * 1.0e-5;
+      z[2 * i] = 0.0;
+      z[2 * i + 1] = 1.0 / eps[i];
+    }
+    for (int spin = 0; spin < ((I.mixing & 4) ? 2 : 1); ++spin) {
+      if (qed_ed_self_energy(label, 3, z.data(), spin, sig.data()) != 0) fail(qed_last_error());
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < d; ++j) S[i] += sig[2 * ((size_t)i * d * d + j + (size_t)j * d)];
+    }
+    if (I.mixing == 0)
+      for (double& v : S) v *= 2.0;
+    *value = neville(eps, S, 0.0);
+  });
+}
+
+int qed_ed_hybridization_lehmann(uint64_t label, int spin_down, double* E, double* T) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    Instance& I = inst(label);
+    const int nb = I.model->n_bath, ns = I.model->n_sites;
+    if (nb == 0) fail("System has no bath. Hybridization not defined.");
+    const int sp = (spin_down && (I.mixing & 4)) ? 1 : 0;
+    for (int i = 0; i < nb; ++i) E[i] = I.tb[sp][i + (size_t)i * nb];
+    std::copy(I.tcb[sp].begin(), I.tcb[sp].begin() + (size_t)ns * nb, T);
   });
 }
 

@@ -192,6 +192,51 @@ def cluster_averages(label=0):
     return {n: (float(a), float(v)) for n, a, v in zip(names, ave, var)}
 
 
+def Green_function_average(label=0, spin_down=0):
+    """<c^dagger_i c_j> from the Green-function representation, d x d
+    (src_python/qcm_ED_wrap.hpp:44-50, src_ed/model_instance.hpp:643-654)."""
+    d = Green_function_dimensionC(label)
+    out = np.zeros(2 * d * d)
+    call("qed_ed_green_function_average", int(label), int(spin_down), dp(out))
+    # column-major M, returned with the reference binding's row-major reading of it
+    return out.view(np.complex128).reshape(d, d).copy()
+
+
+def Green_function_density(label=0):
+    """Cluster density from the trace of the Green function average
+    (src_python/qcm_ED_wrap.hpp:52-55, src_ed/model_instance.hpp:661-675)."""
+    out = np.zeros(1)
+    call("qed_ed_green_function_density", int(label), dp(out))
+    return float(out[0])
+
+
+def fidelity(label1, label2):
+    """Overlap of the low-energy states of two instances (src_python/qcm_ED_wrap.hpp:378-381)."""
+    out = np.zeros(1)
+    call("qed_ed_fidelity", int(label1), int(label2), dp(out))
+    return float(out[0])
+
+
+def tr_sigma_inf(label=0):
+    """Trace of the self-energy extrapolated to infinite frequency
+    (src_ed/model_instance.hpp:1277-1312)."""
+    out = np.zeros(1)
+    call("qed_ed_tr_sigma_inf", int(label), dp(out))
+    return float(out[0])
+
+
+def hybridization(label=0, spin_down=0):
+    """Lehmann form of the hybridization function as the reference binding returns it:
+    (bath diagonal, the column-major n_sites x n_bath cluster-bath matrix read row-major
+    into shape (n_sites, n_bath)) (src_python/qcm_ED_wrap.hpp:351-362)."""
+    d = Green_function_dimensionC(label)
+    nb = model_sizeC(_instance_model[int(label)])[1]
+    E = np.zeros(max(nb, 1))
+    T = np.zeros(max(d * nb, 1))
+    call("qed_ed_hybridization_lehmann", int(label), int(spin_down), dp(E), dp(T))
+    return E[:nb].copy(), T[:d * nb].astype(np.complex128).reshape(d, nb)
+
+
 def susceptibility(op, w, label=0):
     """Dynamic susceptibility of operator `op` at the complex frequencies w
     (src_python/qcm_ED_wrap.hpp:327-333, src_ed/model_instance.hpp:1024-1058)."""

@@ -1,0 +1,100 @@
+"""Green-function averages and the remaining instance queries of namespace ED:
+Green_function_average / Green_function_density (model_instance.hpp:643-675), fidelity
+(:1261-1274), tr_sigma_inf (:1277-1312) and the Lehmann form of the hybridization
+(:1455-1466), against brute force and closed forms."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import models as TM
+from tests.test_gpu_ed import register, with_sites
+
+pytestmark = pytest.mark.gpu
+
+
+def _plaquette4():
+    """2x2 plaquette + 4 bath orbitals (L = 8) with a particle-hole symmetric bath at
+    mu = U/2: half filling (N = 8, S = 0) is the ground state of every particle number, so
+    the Lehmann form's poles below zero are exactly the hole poles. (The Lehmann average
+    integrates every pole below zero, as the reference's does; for a sector that is not
+    the ground state of all particle numbers that includes electron poles.)"""
+    p = {"U": 4.0, "mu": 2.0, "eb1": 0.6, "tb1": 0.5, "eb2": -0.6, "tb2": 0.5, "eb3": 1.1, "tb3": 0.4,
+         "eb4": -1.1, "tb4": 0.4}
+    return TM.plaquette_bath(p, n_bath=4)
+
+
+def _bruteforce_average(m, vals, nup, ndn, n_sites):
+    """<c^dagger_a c_b> (spin up) from the oracle's ground state: the Gram matrix of c_a|psi>."""
+    E0, gs, *_ = O.Sector(m, vals, nup, ndn).lanczos_gs()
+    phi = np.array([O.create_destroy(m.L, nup, ndn, a, -1, gs) for a in range(n_sites)])
+    return E0, gs, phi @ phi.T
+
+
+@pytest.mark.parametrize("method", ["L", "F", "M"])
+def test_green_function_average_and_density(gpu_lib, method):
+    """2x2 plaquette + 4 bath orbitals, N = 8, S = 0 (D = 4,900): the integrated Green
+    function in each representation is the one-body density matrix of the ground state
+    (the hole spectral weight is complete in all three), and the density its trace."""
+    from pyqcm_amd import qcm
+    m, vals = _plaquette4()
+    qcm.set_global_parameter("GF_method", method)
+    try:
+        lab = register(with_sites(m, 4), vals, "R0:N8:S0")
+        M = qcm.Green_function_average(lab)
+        Md = qcm.Green_function_average(lab, 1)
+        n = qcm.Green_function_density(lab)
+    finally:
+        qcm.set_global_parameter("GF_method", "L")
+    _, _, ref = _bruteforce_average(m, vals, 4, 4, 4)
+    err = np.abs(M - ref).max()
+    print("%s: max |M - <c+c>| = %.3g, density %.12f" % (method, err, n))
+    assert err < 1e-7
+    np.testing.assert_array_equal(M, Md)  # mixing 0: one spin
+    assert abs(n - 2 * np.trace(ref) / 4) < 1e-7
+
+
+def test_fidelity(gpu_lib):
+    """An instance with itself: 1. Two instances at U = 4 and U = 4.5: |<psi1|psi2>|^2
+    from the oracle's ground states."""
+    from pyqcm_amd import qcm
+    m, vals = _plaquette4()
+    a = register(with_sites(m, 4), vals, "R0:N8:S0")
+    v2 = dict(vals, U=vals["U"] + 0.5)
+    b = register(with_sites(m, 4), v2, "R0:N8:S0")
+    assert abs(qcm.fidelity(a, a) - 1.0) < 1e-12
+    f = qcm.fidelity(a, b)
+    _, g1, _ = _bruteforce_average(m, vals, 4, 4, 4)
+    _, g2, _ = _bruteforce_average(m, v2, 4, 4, 4)
+    ref = float(np.dot(g1, g2)) ** 2
+    print("fidelity %.12f, oracle %.12f" % (f, ref))
+    assert abs(f - ref) < 1e-7
+
+
+def test_tr_sigma_inf_is_the_hartree_term(gpu_lib):
+    """Sigma(z -> infinity) of the Hubbard interaction is U <n_i, other spin> on each
+    site: the trace over both spins is 2 U sum_i <n_i up> (mixing 0)."""
+    from pyqcm_amd import qcm
+    m, vals = _plaquette4()
+    lab = register(with_sites(m, 4), vals, "R0:N8:S0")
+    t = qcm.tr_sigma_inf(lab)
+    _, _, ref = _bruteforce_average(m, vals, 4, 4, 4)
+    hartree = 2 * vals["U"] * np.trace(ref)
+    print("tr_sigma_inf %.10f, Hartree %.10f" % (t, hartree))
+    assert abs(t - hartree) < 1e-6 * max(1.0, abs(hartree))
+
+
+def test_hybridization_lehmann_form(gpu_lib):
+    """The bath energies and the cluster-bath matrix of the model's parameters, in the
+    reference binding's layout; a model without bath raises the reference's message."""
+    from pyqcm_amd import qcm
+    m, vals = _plaquette4()
+    lab = register(with_sites(m, 4), vals, "R0:N8:S0")
+    E, T = qcm.hybridization(lab)
+    np.testing.assert_allclose(E, [vals["eb%d" % x] for x in range(1, 5)], atol=1e-15)
+    tcb = T.reshape(-1).reshape(4, 4).T.real  # the column-major n_sites x n_bath matrix
+    ref = np.array([[vals["tb%d" % x] * TM.PLAQ_SEQ[x][i] for x in range(1, 5)] for i in range(4)])
+    np.testing.assert_allclose(tcb, ref, atol=1e-15)
+    mc, vc = TM.chain(4)
+    lc = register(with_sites(mc, 4), vc, "R0:N4:S0")
+    with pytest.raises(Exception, match="System has no bath. Hybridization not defined."):
+        qcm.hybridization(lc)
