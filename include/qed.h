@@ -236,6 +236,9 @@ int qed_ed_potthoff_functional(uint64_t label, double* omega);
 /* ED::model_size; dims of the Green function. */
 int qed_ed_model_size(const char* model, int* n_sites, int* n_bath);
 int qed_ed_dim_gf(uint64_t label, int* d);
+/* ED::mixing (src_ed/qcm_ED.cpp:162-165): the Hilbert-space mixing of the instance (0
+ * normal, 4 spin up and down apart). */
+int qed_ed_mixing(uint64_t label, int* mixing);
 /* ED::cluster_averages (model_instance.hpp:148-175): for each active operator, in the
  * order of names, average and variance. */
 int qed_ed_cluster_averages(uint64_t label, int nop, const char* const* names, double* ave,

@@ -1625,6 +1625,11 @@ int qed_ed_model_size(const char* model, int* n_sites, int* n_bath) {
   });
 }
 
+int qed_ed_mixing(uint64_t label, int* mixing) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] { *mixing = inst(label).mixing; });
+}
+
 int qed_ed_dim_gf(uint64_t label, int* d) {
   std::lock_guard<std::recursive_mutex> lk(lib_mutex());
   return guard([&] { *d = inst(label).model->n_sites; });

@@ -21,6 +21,7 @@ from ._lib import call, dp, ip
 _instance_values: dict[int, dict] = {}
 _instance_model: dict[int, str] = {}
 _model_sizes: dict[str, tuple] = {}
+_model_ops: dict[str, dict] = {}  # model -> {operator: (type, [(r, c, v)] 0-based, as stored)}
 
 
 def _b(s):
@@ -66,6 +67,41 @@ def new_operator(model, name, type, elements):
     c = np.array(c, np.int32)
     v = np.array(v, np.float64)
     call("qed_ed_new_operator", _b(model), _b(name), _b(type), len(v), ip(r), ip(c), dp(v))
+    _model_ops.setdefault(model, {})[name] = (type, [(int(a), int(b), float(x)) for a, b, x in zip(r, c, v)])
+
+
+def new_operator_complex(model, name, type, elements):
+    """ED::new_operator with complex elements (src_python/qcm_ED_wrap.hpp:219-261). The
+    solver has a real Hilbert space: elements with a nonzero imaginary part are refused."""
+    els = []
+    for i, el in enumerate(elements):
+        if len(el) != 3:
+            raise RuntimeError("element %d of argument 4 of 'new_operator' should be a 3-tuple" % i)
+        z = complex(el[2])
+        if z.imag != 0.0:
+            raise RuntimeError("operator %s has complex matrix elements: this solver has a real Hilbert space" % name)
+        els.append((el[0], el[1], z.real))
+    new_operator(model, name, type, els)
+
+
+def matrix_elements(model, op):
+    """(type, [(r, c, v)]) of an operator, 0-based and as stored (anomalous elements
+    halved) (src_python/qcm_ED_wrap.hpp:121-130)."""
+    if model not in _model_ops:
+        raise RuntimeError("model %s does not exist!" % model)
+    if op not in _model_ops[model]:
+        raise RuntimeError("operator %s does not exist in model %s" % (op, model))
+    t, els = _model_ops[model][op]
+    return t, list(els)
+
+
+def print_models():
+    """Prints the registered models and their operators (src_python/qcm_ED_wrap.hpp:275-276)."""
+    for name in sorted(_model_ops):
+        ns, nb = _model_sizes.get(name, (0, 0))
+        print("model %s: %d sites, %d bath orbitals" % (name, ns, nb))
+        for op, (t, els) in _model_ops[name].items():
+            print("  %s (%s): %d elements" % (op, t, len(els)))
 
 
 def new_model_instanceC(name, values, sectors, label=0):
@@ -76,6 +112,39 @@ def new_model_instanceC(name, values, sectors, label=0):
     call("qed_ed_new_model_instance", _b(name), len(names), arr, dp(vals), _b(sectors), int(label))
     _instance_values[int(label)] = dict(values)
     _instance_model[int(label)] = name
+
+
+def parametersC(label=0):
+    """(dict of parameters, model name) of an instance (src_python/qcm_ED_wrap.hpp:263-273)."""
+    if int(label) not in _instance_values:
+        raise RuntimeError("The label %d is out of range." % int(label))
+    return dict(sorted(_instance_values[int(label)].items())), _instance_model[int(label)]
+
+
+def mixingC(label=0):
+    """The Hilbert-space mixing of an instance (src_python/qcm_ED_wrap.hpp:132-135)."""
+    m = ctypes.c_int()
+    call("qed_ed_mixing", int(label), ctypes.byref(m))
+    return m.value
+
+
+def ED_complex_HS(label=0):
+    """0: the Hilbert space of this solver is real (src_python/qcm_ED_wrap.hpp:27-30)."""
+    return 0
+
+
+def interactions(label=0):
+    """Density-density interaction elements (r, c, value x parameter) of the instance, in
+    the order of the parameter names (model_instance.hpp:1528-1541)."""
+    if int(label) not in _instance_values:
+        raise RuntimeError("The label %d is out of range." % int(label))
+    ops = _model_ops.get(_instance_model[int(label)], {})
+    out = []
+    for name, val in sorted(_instance_values[int(label)].items()):
+        t, els = ops.get(name, (None, []))
+        if t == "interaction":
+            out.extend((r, c, v * float(val)) for r, c, v in els)
+    return out
 
 
 def erase_model_instance(label=0):

@@ -98,3 +98,27 @@ def test_hybridization_lehmann_form(gpu_lib):
     lc = register(with_sites(mc, 4), vc, "R0:N4:S0")
     with pytest.raises(Exception, match="System has no bath. Hybridization not defined."):
         qcm.hybridization(lc)
+
+
+def test_model_and_instance_queries(gpu_lib):
+    """matrix_elements, parametersC, interactions, mixingC, ED_complex_HS and the refusal of
+    complex operator elements (src_python/qcm_ED_wrap.hpp:27-30,108-135,219-273)."""
+    from pyqcm_amd import qcm
+    m, vals = TM.chain(4, U=4.0, mu=2.0)
+    lab = register(with_sites(m, 4), vals, "R0:N5:S1")
+    name = qcm._instance_model[lab]
+    t, els = qcm.matrix_elements(name, "U")
+    assert t == "interaction" and els == [(i, i + 4, 1.0) for i in range(4)]
+    p, mname = qcm.parametersC(lab)
+    assert mname == name and p == dict(sorted(vals.items()))
+    assert qcm.interactions(lab) == [(i, i + 4, 4.0) for i in range(4)]
+    qcm.ground_state_solve(lab)
+    assert qcm.mixingC(lab) == 4  # N = 5, S = 1: spin up and down apart
+    assert qcm.ED_complex_HS(lab) == 0
+    qcm.new_model("cplx_ops", 2, 0)  # operators are added before the first instance
+    qcm.new_operator_complex("cplx_ops", "h_real", "one-body", [(1, 2, 0.5 + 0.0j)])
+    assert qcm.matrix_elements("cplx_ops", "h_real") == ("one-body", [(0, 1, 0.5)])
+    with pytest.raises(RuntimeError, match="complex matrix elements"):
+        qcm.new_operator_complex("cplx_ops", "h_cplx", "one-body", [(1, 2, 0.5j)])
+    with pytest.raises(RuntimeError, match="The label 424242 is out of range."):
+        qcm.parametersC(424242)
