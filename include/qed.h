@@ -240,7 +240,9 @@ int qed_ed_dim_gf(uint64_t label, int* d);
  * normal, 4 spin up and down apart). */
 int qed_ed_mixing(uint64_t label, int* mixing);
 /* ED::cluster_averages (model_instance.hpp:148-175): for each active operator, in the
- * order of names, average and variance. */
+ * order of names, average and variance. For an uncorrelated instance or one read from a
+ * file they come from the Green function average: one-body operators on the cluster only,
+ * variance NaN (NaN and NaN for the other operators). */
 int qed_ed_cluster_averages(uint64_t label, int nop, const char* const* names, double* ave,
                             double* var);
 /* ED::Green_function_average (src_ed/qcm_ED.cpp:240-248, model_instance.hpp:643-654): the
@@ -262,6 +264,11 @@ int qed_ed_tr_sigma_inf(uint64_t label, double* value);
  * of the bath one-body matrix (n_bath values) and the cluster-bath matrix, n_sites x n_bath
  * column-major. Fails without bath orbitals. */
 int qed_ed_hybridization_lehmann(uint64_t label, int spin_down, double* E, double* T);
+/* ED::density_matrix (src_ed/qcm_ED.cpp:205-216): for the factorized Hamiltonian format
+ * the reference returns an empty matrix and basis (model_instance::
+ * density_matrix_factorized, src_ed/model_instance.hpp:1629-1633); so does this solver:
+ * *dim = 0. */
+int qed_ed_density_matrix(uint64_t label, int nsites, const int* sites, int* dim);
 /* ED::susceptibility (src_ed/qcm_ED.cpp:307-315, model_instance.hpp:1024-1058): dynamic
  * susceptibility of operator `op` at nz complex frequencies z (interleaved re, im), the
  * Boltzmann-weighted sum over low-energy states of <psi|O (w - H + E)^-1 O|psi>, with the
@@ -283,7 +290,7 @@ int qed_ed_write_instance_hdf5(const char* filename, uint64_t label, const char*
  * states of instance `label` by those of the group (energies, weights, Green-function
  * representation), warns on stdout when a stored parameter differs from the instance's
  * by more than 1e-8, and marks the instance solved. The read instance holds no
- * ground-state vectors: cluster averages then fail. */
+ * ground-state vectors: its cluster averages come from the Green function average. */
 int qed_ed_read_instance_hdf5(const char* filename, uint64_t label, const char* group);
 /* Global parameters (src_util/global_parameter.cpp:12-91): int/double/bool/char maps. */
 int qed_ed_set_global_int(const char* name, int v);

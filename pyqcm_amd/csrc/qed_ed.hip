@@ -1640,8 +1640,40 @@ int qed_ed_cluster_averages(uint64_t label, int nop, const char* const* names, d
   return guard([&] {
     Instance& I = inst(label);
     low_energy_states(I);
-    if (!I.correlated) fail("cluster averages of an uncorrelated instance come from the Green function");
-    if (I.from_file) fail("cluster averages need the ground-state vectors, which an instance read from a file does not hold");
+    if (!I.correlated || I.from_file) {
+      // from the Green function average (model_instance::cluster_averages,
+      // src_ed/model_instance.hpp:152-163; one_body_operator::average_from_GF,
+      // src_ed/Operators/one_body_operator.hpp:420-441): one-body operators on the cluster
+      // only, no variance
+      green_function_solve(I);
+      const int d = I.model->n_sites, L = I.model->L;
+      const int nsp = (I.mixing & 4) ? 2 : 1;
+      std::vector<std::vector<cd>> M;
+      for (int sp = 0; sp < nsp; ++sp) M.push_back(gf_average(I, sp));
+      for (int k = 0; k < nop; ++k) {
+        std::string nm(names[k]);
+        if (!I.value.count(nm)) fail("operator " + nm + " is not activated in instance " + std::to_string(label));
+        const Op& op = I.model->ops[I.model->idx.at(nm)];
+        ave[k] = var[k] = std::nan("");
+        if (op.interaction || op.target != 1) continue;
+        double a = 0.0;
+        for (int sp = 0; sp < nsp; ++sp)
+          for (size_t e = 0; e < op.v.size(); ++e) {
+            int r = op.r[e], c = op.c[e];
+            if ((r >= L) != (sp == 1) || (c >= L) != (sp == 1)) continue;
+            r -= sp * L;
+            c -= sp * L;
+            if (r >= d || c >= d) continue;
+            a += op.v[e] * M[sp][r + (size_t)c * d].real();
+            if (r != c) a += op.v[e] * M[sp][c + (size_t)r * d].real();
+          }
+        if (I.mixing == 0) a *= 2.0;
+        double norm = 1.0 / d;
+        if (nm == "mu") norm *= -1.0;
+        ave[k] = a * norm;
+      }
+      return;
+    }
     for (int k = 0; k < nop; ++k) {
       std::string nm(names[k]);
       if (!I.value.count(nm)) fail("operator " + nm + " is not activated in instance " + std::to_string(label));
@@ -1757,6 +1789,16 @@ int qed_ed_hybridization_lehmann(uint64_t label, int spin_down, double* E, doubl
     const int sp = (spin_down && (I.mixing & 4)) ? 1 : 0;
     for (int i = 0; i < nb; ++i) E[i] = I.tb[sp][i + (size_t)i * nb];
     std::copy(I.tcb[sp].begin(), I.tcb[sp].begin() + (size_t)ns * nb, T);
+  });
+}
+
+int qed_ed_density_matrix(uint64_t label, int nsites, const int* sites, int* dim) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    inst(label);
+    (void)nsites;
+    (void)sites;
+    *dim = 0;  // the reference's factorized-format form is empty
   });
 }
 

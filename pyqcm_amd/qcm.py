@@ -306,6 +306,15 @@ def hybridization(label=0, spin_down=0):
     return E[:nb].copy(), T[:d * nb].astype(np.complex128).reshape(d, nb)
 
 
+def density_matrix(sites, label=0):
+    """(density matrix, basis) of the subsystem `sites`; empty for the factorized format,
+    as the reference's (src_python/qcm_ED_wrap.hpp:32-42, src_ed/model_instance.hpp:1629-1633)."""
+    s = np.ascontiguousarray(sites, np.int32)
+    dim = ctypes.c_int()
+    call("qed_ed_density_matrix", int(label), len(s), ip(s), ctypes.byref(dim))
+    return np.zeros((0, 0), np.complex128), np.zeros(0, np.uint64)
+
+
 def susceptibility(op, w, label=0):
     """Dynamic susceptibility of operator `op` at the complex frequencies w
     (src_python/qcm_ED_wrap.hpp:327-333, src_ed/model_instance.hpp:1024-1058)."""

@@ -122,3 +122,50 @@ def test_model_and_instance_queries(gpu_lib):
         qcm.new_operator_complex("cplx_ops", "h_cplx", "one-body", [(1, 2, 0.5j)])
     with pytest.raises(RuntimeError, match="The label 424242 is out of range."):
         qcm.parametersC(424242)
+
+
+def _one_body_average(m, op, M, n_sites):
+    """<O> for a cluster one-body operator from the spin-up density matrix M(b, a) =
+    <c+_a c_b>, both spins (normal mixing), times the reference's norm (1/n_sites, -1 for mu)."""
+    a = 0.0
+    for r, c, v in m.operators[op].elements:
+        if r >= m.L or c >= m.L:
+            continue
+        a += v * M[r, c] + (v * M[c, r] if r != c else 0.0)
+    return 2 * a / n_sites * (-1.0 if op == "mu" else 1.0)
+
+
+def test_cluster_averages_from_green_function(gpu_lib, tmp_path):
+    """An instance read from a file and an uncorrelated instance take their averages from
+    the Green function average (one-body cluster operators; NaN otherwise): equal to the
+    averages of the solved instance's ground state, and to the Slater determinant at U = 0."""
+    from pyqcm_amd import qcm
+    m, vals = TM.chain(4, U=4.0, mu=2.0)
+    a = register(with_sites(m, 4), vals, "R0:N4:S0")
+    ref = qcm.cluster_averages(a)
+    path = str(tmp_path / "avg.h5")
+    qcm.write_instance_to_hdf5(path, "cluster_0", a)
+    b = register(with_sites(m, 4), vals, "R0:N4:S0")
+    qcm.read_instance_from_hdf5(path, "cluster_0", b)
+    got = qcm.cluster_averages(b)
+    for op in ("t", "mu"):
+        assert abs(got[op][0] - ref[op][0]) < 1e-9, op
+        assert np.isnan(got[op][1])
+    assert np.isnan(got["U"][0]) and np.isnan(got["U"][1])
+    # U = 0: one-body solve, the grand-canonical Slater determinant (levels below zero)
+    v0 = dict(vals, U=0.0)
+    c = register(with_sites(m, 4), v0, "R0:N4:S0")
+    got0 = qcm.cluster_averages(c)
+    h = np.zeros((4, 4))
+    for op in ("t", "mu"):
+        for r, cc, v in m.operators[op].elements:
+            if r < 4 and cc < 4:
+                h[r, cc] += v * v0[op]
+                if r != cc:
+                    h[cc, r] += v * v0[op]
+    e, U = np.linalg.eigh(h)
+    occ = U[:, e < 0]
+    Mslater = occ @ occ.T
+    for op in ("t", "mu"):
+        assert abs(got0[op][0] - _one_body_average(m, op, Mslater, 4)) < 1e-12, op
+    assert qcm.density_matrix([0, 1], a)[0].shape == (0, 0)

@@ -169,7 +169,8 @@ def test_layout_and_values_match_reference_writers(gpu_lib, tmp_path, method):
 def test_overwrite_parameter_warning_and_errors(gpu_lib, tmp_path, capfd):
     """Writing a group twice replaces it; reading into an instance whose parameters differ
     prints the reference's warning; a missing file, a missing group and an unknown label
-    raise with the reference's messages; a read instance has no ground-state vectors."""
+    raise with the reference's messages; a read instance has no ground-state vectors, so its
+    averages come from the Green function and the susceptibility is refused."""
     from pyqcm_amd import qcm
     m, vals = TM.chain(4, U=4.0, mu=2.0)
     a = register(with_sites(m, 4), vals, "R0:N4:S0")
@@ -190,5 +191,7 @@ def test_overwrite_parameter_warning_and_errors(gpu_lib, tmp_path, capfd):
         qcm.read_instance_from_hdf5(path, "cluster_7", b)
     with pytest.raises(Exception, match="The label 987654 is out of range."):
         qcm.write_instance_to_hdf5(path, "cluster_0", 987654)
+    ave = qcm.cluster_averages(b)  # from the Green function average: no interaction average
+    assert np.isnan(ave["U"][0]) and not np.isnan(ave["mu"][0])
     with pytest.raises(Exception, match="ground-state vectors"):
-        qcm.cluster_averages(b)
+        qcm.susceptibility("mu", [1j], b)
