@@ -264,6 +264,11 @@ int qed_ed_tr_sigma_inf(uint64_t label, double* value);
  * of the bath one-body matrix (n_bath values) and the cluster-bath matrix, n_sites x n_bath
  * column-major. Fails without bath orbitals. */
 int qed_ed_hybridization_lehmann(uint64_t label, int spin_down, double* E, double* T);
+/* ED::print_wavefunction (src_ed/qcm_ED.cpp:431-440): text of the low-energy states (a
+ * header per state; the amplitudes and basis states when the sector dimension is at most
+ * max_dim_print). *needed = the length of the text + 1; up to buflen - 1 characters are
+ * copied into buf. */
+int qed_ed_print_wavefunction(uint64_t label, char* buf, int buflen, int* needed);
 /* ED::density_matrix (src_ed/qcm_ED.cpp:205-216): for the factorized Hamiltonian format
  * the reference returns an empty matrix and basis (model_instance::
  * density_matrix_factorized, src_ed/model_instance.hpp:1629-1633); so does this solver:

@@ -30,7 +30,7 @@ using cd = std::complex<double>;
 struct Globals {
   std::map<std::string, int> i{{"seed", 0},           {"max_iter_lanczos", 1000}, {"max_iter_BL", 600},
                                {"max_iter_CF", 400},  {"max_dim_full", 256},      {"n_states", 1},
-                               {"kgrid_side", 32},    {"GF_lookup_depth", 64}};
+                               {"kgrid_side", 32},    {"GF_lookup_depth", 64},   {"max_dim_print", 64}};
   std::map<std::string, double> d{{"accur_lanczos", 1e-12},     {"accur_band_lanczos", 1e-12},
                                   {"accur_deflation", 1e-7},    {"accur_Q_matrix", 1e-5},
                                   {"Qmatrix_wtol", 1e-5},       {"Qmatrix_vtol", 1e-10},
@@ -1789,6 +1789,50 @@ int qed_ed_hybridization_lehmann(uint64_t label, int spin_down, double* E, doubl
     const int sp = (spin_down && (I.mixing & 4)) ? 1 : 0;
     for (int i = 0; i < nb; ++i) E[i] = I.tb[sp][i + (size_t)i * nb];
     std::copy(I.tcb[sp].begin(), I.tcb[sp].begin() + (size_t)ns * nb, T);
+  });
+}
+
+int qed_ed_print_wavefunction(uint64_t label, char* buf, int buflen, int* needed) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    // model_instance::print_wavefunction, src_ed/model_instance.hpp:1472-1485, and
+    // state::write_wavefunction, src_ed/state.hpp:117-131: per state a header line, then,
+    // when the sector dimension is at most max_dim_print, |psi_i|^2, psi_i and the basis
+    // state written as the down-spin then the up-spin occupations, lowest orbital first
+    // (ED_factorized_basis::print_state, src_ed/ED_basis.cpp:167-170; PrintBinaryDouble,
+    // src_ed/binary_state.cpp:104-112)
+    Instance& I = inst(label);
+    low_energy_states(I);
+    const int L = I.model->L;
+    std::ostringstream o;
+    for (auto& st : I.states) {
+      o << "state\n" << sector_name(st->nup + st->ndn, st->nup - st->ndn) << '\t' << st->energy << '\t' << st->weight
+        << std::endl;
+      if (!st->psi.p) continue;
+      SectorPtr H = make_sector(I, st->nup, st->ndn);
+      if (H->D > (uint64_t)gi("max_dim_print")) continue;
+      std::vector<double> psi(H->D);
+      std::vector<uint32_t> bu(H->du), bd(H->dd);
+      QED_HIP(hipMemcpy(psi.data(), st->psi.p, H->D * 8, hipMemcpyDeviceToHost));
+      QED_HIP(hipMemcpy(bu.data(), H->bup.p, H->du * 4, hipMemcpyDeviceToHost));
+      QED_HIP(hipMemcpy(bd.data(), H->bdn.p, H->dd * 4, hipMemcpyDeviceToHost));
+      auto bits = [&](uint32_t v) {
+        std::string t(L, '0');
+        for (int k = 0; k < L; ++k) t[k] = (v >> k) & 1u ? '1' : '0';
+        return t;
+      };
+      for (uint64_t i = 0; i < H->D; ++i) {
+        const uint32_t up = bu[i % H->du], dn = bd[i / H->du];
+        o << std::abs(psi[i]) * std::abs(psi[i]) << '\t' << psi[i] << '\t' << '|' << bits(dn) << ',' << bits(up) << '>'
+          << '\n';
+      }
+    }
+    const std::string out = o.str();
+    *needed = (int)out.size() + 1;
+    if (buf && buflen > 0) {
+      std::strncpy(buf, out.c_str(), buflen - 1);
+      buf[buflen - 1] = 0;
+    }
   });
 }
 

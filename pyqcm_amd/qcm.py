@@ -306,6 +306,16 @@ def hybridization(label=0, spin_down=0):
     return E[:nb].copy(), T[:d * nb].astype(np.complex128).reshape(d, nb)
 
 
+def print_wavefunction(label=0):
+    """Text of the low-energy states: header, and amplitudes with basis states when the
+    sector is at most max_dim_print (src_python/qcm_ED_wrap.hpp:335-337)."""
+    n = ctypes.c_int()
+    call("qed_ed_print_wavefunction", int(label), None, 0, ctypes.byref(n))
+    buf = ctypes.create_string_buffer(n.value)
+    call("qed_ed_print_wavefunction", int(label), buf, n.value, ctypes.byref(n))
+    return buf.value.decode()
+
+
 def density_matrix(sites, label=0):
     """(density matrix, basis) of the subsystem `sites`; empty for the factorized format,
     as the reference's (src_python/qcm_ED_wrap.hpp:32-42, src_ed/model_instance.hpp:1629-1633)."""

@@ -169,3 +169,38 @@ def test_cluster_averages_from_green_function(gpu_lib, tmp_path):
     for op in ("t", "mu"):
         assert abs(got0[op][0] - _one_body_average(m, op, Mslater, 4)) < 1e-12, op
     assert qcm.density_matrix([0, 1], a)[0].shape == (0, 0)
+
+
+def test_print_wavefunction(gpu_lib):
+    """4-site chain, N = 4, S = 0 (D = 36 <= max_dim_print): the header, and per basis
+    state |psi|^2, psi and |down,up> with the lowest orbital first, against the oracle's
+    ground state (printed with six significant digits) and its half bases in rank order.
+    Above max_dim_print only the header is printed."""
+    from pyqcm_amd import qcm
+    m, vals = TM.chain(4, U=4.0, mu=2.0)
+    lab = register(with_sites(m, 4), vals, "R0:N4:S0")
+    text = qcm.print_wavefunction(lab)
+    lines = text.strip("\n").split("\n")
+    assert lines[0] == "state"
+    sec, e, w = lines[1].split("\t")
+    E0, gs, *_ = O.Sector(m, vals, 2, 2).lanczos_gs()
+    assert sec == "R0:N4:S0" and abs(float(e) - E0) < 1e-5 * abs(E0) and float(w) == 1.0
+    hb = O.halfbasis(4, 2)
+    assert len(lines) == 2 + 36
+    sign = None
+    for i, ln in enumerate(lines[2:]):
+        p2, amp, st = ln.split("\t")
+        up, dn = hb[i % len(hb)], hb[i // len(hb)]
+        bits = lambda v: "".join("1" if (v >> k) & 1 else "0" for k in range(4))
+        assert st == "|%s,%s>" % (bits(dn), bits(up))
+        assert abs(float(p2) - gs[i] ** 2) < 1e-5
+        if abs(gs[i]) > 1e-3:
+            s = np.sign(float(amp) * gs[i])
+            sign = s if sign is None else sign
+            assert s == sign and abs(abs(float(amp)) - abs(gs[i])) < 1e-5
+    qcm.set_global_parameter("max_dim_print", 16)
+    try:
+        lab2 = register(with_sites(m, 4), vals, "R0:N4:S0")
+        assert len(qcm.print_wavefunction(lab2).strip("\n").split("\n")) == 2
+    finally:
+        qcm.set_global_parameter("max_dim_print", 64)
