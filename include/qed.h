@@ -269,6 +269,12 @@ int qed_ed_hybridization_lehmann(uint64_t label, int spin_down, double* E, doubl
  * max_dim_print). *needed = the length of the text + 1; up to buflen - 1 characters are
  * copied into buf. */
 int qed_ed_print_wavefunction(uint64_t label, char* buf, int buflen, int* needed);
+/* ED::get_combined_mcf (src_ed/qcm_ED.hpp:312-340, qcm_ED.cpp:455-540): the combined matrix
+ * continued fraction of the first low-energy state (GF_method 'M' or 'L' with combine_mcf
+ * set): block size *p, *M floors, W of *p x *q; A and B receive M p x p blocks and W the
+ * weight, column-major (pass NULL to query the sizes). */
+int qed_ed_get_combined_mcf(uint64_t label, int spin_down, int* p, int* M, int* q, double* A, double* B,
+                            double* W);
 /* ED::density_matrix (src_ed/qcm_ED.cpp:205-216): for the factorized Hamiltonian format
  * the reference returns an empty matrix and basis (model_instance::
  * density_matrix_factorized, src_ed/model_instance.hpp:1629-1633); so does this solver:

@@ -81,6 +81,7 @@ SIGNATURES = [
     ("qed_ed_mixing", c_int, [c_u64, P(c_int)]),
     ("qed_ed_density_matrix", c_int, [c_u64, c_int, iptr, P(c_int)]),
     ("qed_ed_print_wavefunction", c_int, [c_u64, c_char_p, c_int, P(c_int)]),
+    ("qed_ed_get_combined_mcf", c_int, [c_u64, c_int, P(c_int), P(c_int), P(c_int), dptr, dptr, dptr]),
     ("qed_ed_green_function_average", c_int, [c_u64, c_int, dptr]),
     ("qed_ed_green_function_density", c_int, [c_u64, dptr]),
     ("qed_ed_fidelity", c_int, [c_u64, c_u64, dptr]),

@@ -245,7 +245,14 @@ struct BlockTridiag {
 int block_lanczos(qed_sector& s, const double* q0, int p, int M0, int max_iter_BL, double accur_deflation,
                   double accur_band_lanczos, bool qr, std::vector<std::vector<double>>& A,
                   std::vector<std::vector<double>>& B) {
-  const uint64_t n = s.D;
+  return block_lanczos_op(
+      s.D, [&](const double* x, double* y) { hv(s, x, y); }, q0, p, M0, max_iter_BL, accur_deflation,
+      accur_band_lanczos, qr, A, B);
+}
+
+int block_lanczos_op(uint64_t n, const BlockOperator& op, const double* q0, int p, int M0, int max_iter_BL,
+                     double accur_deflation, double accur_band_lanczos, bool qr, std::vector<std::vector<double>>& A,
+                     std::vector<std::vector<double>>& B) {
   A.clear();
   B.clear();
   if (n == 0 || p == 0) return 0;
@@ -273,7 +280,7 @@ int block_lanczos(qed_sector& s, const double* q0, int p, int M0, int max_iter_B
   for (int j = 0; j < M0; ++j) {
     for (int k = 0; k < p; ++k) {
       QED_HIP(hipMemsetAsync(Z[k], 0, n * sizeof(double), stream()));
-      hv(s, Qc[k], Z[k]);
+      op(Qc[k], Z[k]);
     }
     block_gram(Qc, Z, p, n, G);  // G(k, l) = <Q_k, Z_l>
     std::vector<double> Aj((size_t)p * p);
@@ -380,6 +387,36 @@ int block_lanczos(qed_sector& s, const double* q0, int p, int M0, int max_iter_B
 }
 
 namespace {
+
+// y += diag(e) x
+__global__ void k_diag_axpy(uint64_t n, const double* __restrict__ e, const double* __restrict__ x,
+                            double* __restrict__ y) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += e[i] * x[i];
+}
+
+// y += T x for the direct sum of block-tridiagonal matrices: part k has floors [f0_k, f1_k)
+// of p x p blocks A_j (diagonal) and B_j (sub-diagonal B_{j-1}, super-diagonal B_j^T),
+// column-major; floors of one part are consecutive in A, B and in x (one thread per row)
+__global__ void k_blocktri_axpy(int p, int nf, const int* __restrict__ part_end, const double* __restrict__ A,
+                                const double* __restrict__ B, const double* __restrict__ x, double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf * p) return;
+  const int j = i / p, r = i % p;
+  int f0 = 0, f1 = part_end[0];
+  if (j >= f1) {
+    f0 = f1;
+    f1 = part_end[1];
+  }
+  const size_t pp = (size_t)p * p;
+  double acc = 0.0;
+  for (int c = 0; c < p; ++c) acc += A[j * pp + r + (size_t)p * c] * x[(size_t)j * p + c];
+  if (j > f0)
+    for (int c = 0; c < p; ++c) acc += B[(j - 1) * pp + r + (size_t)p * c] * x[(size_t)(j - 1) * p + c];
+  if (j + 1 < f1)
+    for (int c = 0; c < p; ++c) acc += B[j * pp + c + (size_t)p * r] * x[(size_t)(j + 1) * p + c];
+  y[i] += acc;
+}
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -519,6 +556,57 @@ __global__ __launch_bounds__(RB) void k_mcf_eval(int pa, int q, int M, const dou
 
 // G[z] += W^T F_0(z) W (or its transpose) for nz frequencies; A, B: M floors pa x pa
 // (host), W: pa x q (host), G: host, q x q complex column-major per frequency
+// Block Lanczos on diag(e) (dimension n) from p start vectors given on the host (p x n,
+// vector-major): the matrix continued fraction of a Lehmann form (Q_matrix_to_mcf,
+// src_ed/matrix_continued_fraction.hpp:275-314)
+int block_lanczos_diag(const std::vector<double>& e, const std::vector<double>& phi, int p, int M0, int max_iter_BL,
+                       double accur_deflation, double accur_band_lanczos, bool qr, std::vector<std::vector<double>>& A,
+                       std::vector<std::vector<double>>& B) {
+  const uint64_t n = e.size();
+  DevBuf de(n * 8), dq((size_t)p * n * 8);
+  QED_HIP(hipMemcpyAsync(de.p, e.data(), n * 8, hipMemcpyHostToDevice, stream()));
+  QED_HIP(hipMemcpyAsync(dq.p, phi.data(), (size_t)p * n * 8, hipMemcpyHostToDevice, stream()));
+  const double* ed = de.as<double>();
+  return block_lanczos_op(
+      n,
+      [&](const double* x, double* y) {
+        k_diag_axpy<<<(unsigned)((n + 255) / 256), 256, 0, stream()>>>(n, ed, x, y);
+        QED_HIP(hipGetLastError());
+      },
+      dq.as<double>(), p, M0, max_iter_BL, accur_deflation, accur_band_lanczos, qr, A, B);
+}
+
+// Block Lanczos on the direct sum of two block-tridiagonal matrices (floors Me and Mh of
+// p x p blocks, column-major, consecutive) from p start vectors given on the host (p x N,
+// N = (Me + Mh) p): the combined matrix continued fraction (combine_via_lanczos,
+// src_ed/matrix_continued_fraction.hpp:442-492)
+int block_lanczos_blocktri(int p, int Me, int Mh, const std::vector<double>& Ablk, const std::vector<double>& Bblk,
+                           const std::vector<double>& phi, int M0, int max_iter_BL, double accur_deflation,
+                           double accur_band_lanczos, bool qr, std::vector<std::vector<double>>& A,
+                           std::vector<std::vector<double>>& B) {
+  const int nf = Me + Mh;
+  const uint64_t n = (uint64_t)nf * p;
+  const size_t pp = (size_t)p * p;
+  if (Ablk.size() != nf * pp || Bblk.size() != nf * pp || phi.size() != (size_t)p * n)
+    fail("combined block Lanczos: block arrays do not match the floors");
+  DevBuf dA(nf * pp * 8), dB(nf * pp * 8), dq((size_t)p * n * 8), dend(2 * sizeof(int));
+  const int ends[2] = {Me, nf};
+  QED_HIP(hipMemcpyAsync(dA.p, Ablk.data(), nf * pp * 8, hipMemcpyHostToDevice, stream()));
+  QED_HIP(hipMemcpyAsync(dB.p, Bblk.data(), nf * pp * 8, hipMemcpyHostToDevice, stream()));
+  QED_HIP(hipMemcpyAsync(dq.p, phi.data(), (size_t)p * n * 8, hipMemcpyHostToDevice, stream()));
+  QED_HIP(hipMemcpyAsync(dend.p, ends, sizeof ends, hipMemcpyHostToDevice, stream()));
+  const double* a = dA.as<double>();
+  const double* b = dB.as<double>();
+  const int* pe = dend.as<int>();
+  return block_lanczos_op(
+      n,
+      [&](const double* x, double* y) {
+        k_blocktri_axpy<<<(unsigned)((n + 255) / 256), 256, 0, stream()>>>(p, nf, pe, a, b, x, y);
+        QED_HIP(hipGetLastError());
+      },
+      dq.as<double>(), p, M0, max_iter_BL, accur_deflation, accur_band_lanczos, qr, A, B);
+}
+
 void mcf_eval(int pa, int q, int M, const double* A, const double* B, const double* W, int nz, const double* z,
               bool transpose, double* G) {
   if (M == 0 || nz == 0 || pa == 0) return;

@@ -38,7 +38,7 @@ struct Globals {
                                   {"accur_Davidson", 1e-5},     {"accur_continued_fraction", 0.01},
                                   {"band_lanczos_minimum_gap", 1e-5}};
   std::map<std::string, bool> b{{"verb_ED", false}, {"no_degenerate_BL", false}, {"continued_fraction", false},
-                              {"block_Lanczos_QR", false}};
+                              {"block_Lanczos_QR", false}, {"combine_mcf", false}};
   std::map<std::string, char> c{{"GF_method", 'L'}, {"Ground_state_method", 'L'}, {"Hamiltonian_format", 'F'}};
 };
 Globals& G() {
@@ -98,6 +98,7 @@ struct State {
   QMat gf[2];
   std::vector<CFrac> cfe[2], cfh[2];  // n x n (o1 >= o2), electron / hole parts
   MCFrac mcf[2][2];                   // [spin][0: holes, 1: particles]
+  MCFrac mcf_comb[2];                 // combined fraction (combine_mcf), evaluated alone
 };
 
 struct Instance {
@@ -109,6 +110,7 @@ struct Instance {
   bool correlated = false, gs_solved = false, gf_solved = false;
   bool cf = false;   // continued fractions ('F')
   bool mcf = false;  // matrix continued fractions ('M')
+  bool q_to_mcf = false;  // Lehmann form converted to one matrix continued fraction
   bool from_file = false;  // states read from an instance file: Green functions only, no vectors
   std::vector<std::unique_ptr<State>> states;
   double GS_energy = 0.0, E0 = 0.0, SEF_bath = 0.0;
@@ -657,6 +659,89 @@ void set_gf_format(Instance& I) {
   if (gf == 'L' && gb("continued_fraction")) gf = 'F';
   I.cf = gf == 'F';
   I.mcf = gf == 'M';
+  I.q_to_mcf = gf == 'L' && gb("combine_mcf");  // GF_format_Q_to_MCF, model_instance_base.cpp:29-33
+}
+
+// MGS factor W (p x p, upper triangle, column-major) of p host vectors of length n
+std::vector<double> mgs_factor(std::vector<double> q, int p, size_t n) {
+  std::vector<double> W((size_t)p * p, 0.0);
+  for (int l = 0; l < p; ++l) {
+    double* ql = q.data() + (size_t)l * n;
+    for (int k = 0; k < l; ++k) {
+      const double* qk = q.data() + (size_t)k * n;
+      double z = 0.0;
+      for (size_t i = 0; i < n; ++i) z += qk[i] * ql[i];
+      W[k + (size_t)p * l] = z;
+      for (size_t i = 0; i < n; ++i) ql[i] -= z * qk[i];
+    }
+    double nrm = 0.0;
+    for (size_t i = 0; i < n; ++i) nrm += ql[i] * ql[i];
+    nrm = std::sqrt(nrm);
+    W[l + (size_t)p * l] = nrm;
+    for (size_t i = 0; i < n; ++i) ql[i] /= nrm;
+  }
+  return W;
+}
+
+MCFrac mcf_from_blocks(int p, const std::vector<std::vector<double>>& A, const std::vector<std::vector<double>>& B,
+                       const std::vector<double>& W) {
+  MCFrac f;
+  if (A.empty()) return f;
+  f.pa = p;
+  f.q = p;
+  for (auto& a : A) f.A.insert(f.A.end(), a.begin(), a.end());
+  for (auto& b : B) f.B.insert(f.B.end(), b.begin(), b.end());
+  f.B.resize(f.A.size(), 0.0);
+  f.W = W;
+  return f;
+}
+
+// combine_mcf with GF_method 'M' (mcf_set::build_combined, src_ed/mcf_set.hpp:79-91;
+// combine_via_lanczos, src_ed/matrix_continued_fraction.hpp:442-492): block Lanczos on the
+// direct sum of the particle and hole block-tridiagonal matrices, start block W_e on the
+// first particle floor and W_h on the first hole floor, M0 = Me + Mh
+void build_combined(State& st, int spin) {
+  const MCFrac& e = st.mcf[spin][1];
+  const MCFrac& h = st.mcf[spin][0];
+  st.mcf_comb[spin] = MCFrac();
+  if (e.M() == 0 || h.M() == 0) return;
+  if (e.pa != h.pa || e.pa != e.q || h.pa != h.q) return;  // a deflated start block: kept apart
+  const int p = e.pa, Me = e.M(), Mh = h.M();
+  const size_t pp = (size_t)p * p, n = (size_t)(Me + Mh) * p;
+  std::vector<double> A(e.A), B(e.B);
+  A.insert(A.end(), h.A.begin(), h.A.end());
+  B.resize((size_t)Me * pp, 0.0);
+  B.insert(B.end(), h.B.begin(), h.B.end());
+  B.resize((size_t)(Me + Mh) * pp, 0.0);
+  std::vector<double> phi((size_t)p * n, 0.0);
+  for (int i = 0; i < p; ++i)
+    for (int k = 0; k < p; ++k) {
+      phi[(size_t)i * n + k] = e.W[k + (size_t)p * i];
+      phi[(size_t)i * n + (size_t)Me * p + k] = h.W[k + (size_t)p * i];
+    }
+  const std::vector<double> W = mgs_factor(phi, p, n);
+  std::vector<std::vector<double>> An, Bn;
+  block_lanczos_blocktri(p, Me, Mh, A, B, phi, Me + Mh, gi("max_iter_BL"), gd("accur_deflation"),
+                         gd("accur_band_lanczos"), gb("block_Lanczos_QR"), An, Bn);
+  st.mcf_comb[spin] = mcf_from_blocks(p, An, Bn, W);
+}
+
+// combine_mcf with GF_method 'L' (build_mcf_from_qmatrix, src_ed/model_instance.hpp:
+// 967-990; Q_matrix_to_mcf, src_ed/matrix_continued_fraction.hpp:275-314): block Lanczos on
+// diag(e) of the Lehmann form from the rows of Q, M0 = M
+void qmatrix_to_mcf(State& st, int spin) {
+  const QMat& q = st.gf[spin];
+  st.mcf_comb[spin] = MCFrac();
+  const int L = q.L, M = q.M();
+  if (M == 0 || L == 0) return;
+  std::vector<double> phi((size_t)L * M);
+  for (int a = 0; a < L; ++a)
+    for (int k = 0; k < M; ++k) phi[(size_t)a * M + k] = q.Q[a + (size_t)L * k];
+  const std::vector<double> W = mgs_factor(phi, L, M);
+  std::vector<std::vector<double>> An, Bn;
+  block_lanczos_diag(q.e, phi, L, M, gi("max_iter_BL"), gd("accur_deflation"), gd("accur_band_lanczos"),
+                     gb("block_Lanczos_QR"), An, Bn);
+  st.mcf_comb[spin] = mcf_from_blocks(L, An, Bn, W);
 }
 
 void green_function_solve(Instance& I) {
@@ -666,9 +751,15 @@ void green_function_solve(Instance& I) {
   set_gf_format(I);
   for (auto& s : I.states) {
     for (int spin = 0; spin < ((I.mixing & 4) ? 2 : 1); ++spin) {
-      if (I.cf) build_cf(I, *s, spin);
-      else if (I.mcf) build_mcf(I, *s, spin);
-      else build_qmatrix(I, *s, spin);
+      if (I.cf) {
+        build_cf(I, *s, spin);
+      } else if (I.mcf) {
+        build_mcf(I, *s, spin);
+        if (gb("combine_mcf")) build_combined(*s, spin);
+      } else {
+        build_qmatrix(I, *s, spin);
+        if (I.q_to_mcf) qmatrix_to_mcf(*s, spin);
+      }
     }
   }
   I.gf_solved = true;
@@ -773,7 +864,7 @@ void batch_solve(std::vector<Instance*>& insts, bool graph, int chunk, BatchStat
   for (Instance* I : insts) {
     if (I->gf_solved || !I->correlated) continue;
     set_gf_format(*I);
-    if (I->cf || I->mcf) {
+    if (I->cf || I->mcf || I->q_to_mcf) {
       green_function_solve(*I);
       continue;
     }
@@ -897,7 +988,11 @@ void green_function(Instance& I, int nz, const double* z, int spin_down, double*
   const int d = I.model->n_sites;
   std::fill(out, out + (size_t)2 * nz * d * d, 0.0);
   for (auto& s : I.states) {
-    if (I.mcf && I.correlated) {
+    if ((I.mcf || I.q_to_mcf) && I.correlated && s->mcf_comb[sp].M() > 0) {
+      // one combined fraction (mcf_set::Green_function with combined[r])
+      const MCFrac& f = s->mcf_comb[sp];
+      mcf_eval(f.pa, f.q, f.M(), f.A.data(), f.B.data(), f.W.data(), nz, z, 0, out);
+    } else if (I.mcf && I.correlated) {
       // particles added as they are, holes transposed (src_ed/mcf_set.hpp:111-131)
       for (int part = 1; part >= 0; --part) {
         const MCFrac& f = s->mcf[sp][part];
@@ -1277,6 +1372,7 @@ void read_instance_file(Instance& I, const std::string& path, const std::string&
   I.mixing = mixing;
   I.cf = fmt == "cf" || (fmt != "bl" && fmt != "mcf");
   I.mcf = fmt == "mcf";
+  I.q_to_mcf = false;  // a file holds the particle and hole parts, not a combined fraction
   I.correlated = true;
   I.gs_solved = I.gf_solved = true;
   I.from_file = true;
@@ -1588,7 +1684,7 @@ int qed_ed_qmatrix(uint64_t label, int spin_down, int* M, double* e, double* Q) 
   return guard([&] {
     Instance& I = inst(label);
     green_function_solve(I);
-    if ((I.cf || I.mcf) && I.correlated)
+    if ((I.cf || I.mcf || I.q_to_mcf) && I.correlated)
       fail("the Q-matrix is not available with the continued-fraction Green function");
     const int sp = spin_index(I, spin_down);
     const int d = I.model->n_sites;
@@ -1833,6 +1929,31 @@ int qed_ed_print_wavefunction(uint64_t label, char* buf, int buflen, int* needed
       std::strncpy(buf, out.c_str(), buflen - 1);
       buf[buflen - 1] = 0;
     }
+  });
+}
+
+int qed_ed_get_combined_mcf(uint64_t label, int spin_down, int* p, int* M, int* q, double* A, double* B,
+                            double* W) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    // ED::get_combined_mcf, src_ed/qcm_ED.cpp:455-540: the first state's combined fraction
+    Instance& I = inst(label);
+    if (!I.gf_solved) set_gf_format(I);
+    if (!I.mcf && !I.q_to_mcf)
+      fail("get_combined_mcf: the instance at label " + std::to_string(label) +
+           " does not use the MCF format (GF_method must be 'M', or 'L' with combine_mcf=True).");
+    if (!gb("combine_mcf"))
+      fail("get_combined_mcf: the combined MCF is not built unless the global parameter 'combine_mcf' is set to True.");
+    green_function_solve(I);
+    if (I.states.empty()) fail("get_combined_mcf: no valid ground state found.");
+    const MCFrac& f = I.states.front()->mcf_comb[spin_index(I, spin_down)];
+    if (f.M() == 0) fail("get_combined_mcf: combined MCF is empty. Set combine_mcf=True before solving.");
+    *p = f.pa;
+    *M = f.M();
+    *q = f.q;
+    if (A) std::copy(f.A.begin(), f.A.end(), A);
+    if (B) std::copy(f.B.begin(), f.B.end(), B);
+    if (W) std::copy(f.W.begin(), f.W.end(), W);
   });
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <vector>
@@ -178,6 +179,18 @@ void lanczos_green(qed_sector& s, const double* psi, double accur_cf, int max_it
 int block_lanczos(qed_sector& s, const double* q0, int p, int M0, int max_iter_BL, double accur_deflation,
                   double accur_band_lanczos, bool qr, std::vector<std::vector<double>>& A,
                   std::vector<std::vector<double>>& B);
+// y += Op x on device vectors of the operator's dimension
+using BlockOperator = std::function<void(const double*, double*)>;
+int block_lanczos_op(uint64_t n, const BlockOperator& op, const double* q0, int p, int M0, int max_iter_BL,
+                     double accur_deflation, double accur_band_lanczos, bool qr, std::vector<std::vector<double>>& A,
+                     std::vector<std::vector<double>>& B);
+int block_lanczos_diag(const std::vector<double>& e, const std::vector<double>& phi, int p, int M0, int max_iter_BL,
+                       double accur_deflation, double accur_band_lanczos, bool qr, std::vector<std::vector<double>>& A,
+                       std::vector<std::vector<double>>& B);
+int block_lanczos_blocktri(int p, int Me, int Mh, const std::vector<double>& Ablk, const std::vector<double>& Bblk,
+                           const std::vector<double>& phi, int M0, int max_iter_BL, double accur_deflation,
+                           double accur_band_lanczos, bool qr, std::vector<std::vector<double>>& A,
+                           std::vector<std::vector<double>>& B);
 // G[z] += W^T F_0(z) W (transposed if transpose) for the matrix continued fraction with M
 // floors of pa x pa blocks; W pa x q; host arrays, G q x q complex column-major per z
 void mcf_eval(int pa, int q, int M, const double* A, const double* B, const double* W, int nz, const double* z,

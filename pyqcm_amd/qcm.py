@@ -306,6 +306,24 @@ def hybridization(label=0, spin_down=0):
     return E[:nb].copy(), T[:d * nb].astype(np.complex128).reshape(d, nb)
 
 
+def combined_mcf(label=0, k=None):
+    """(W, [A_j], [B_j]) of the combined matrix continued fraction of the cluster Green
+    function, G(z) = W^T F_0(z) W (src_python/qcm_wrap.hpp:502-522). The lattice form (k
+    given) is not provided."""
+    if k is not None:
+        raise RuntimeError("combined_mcf at a lattice wavevector is not provided by this solver")
+    p, M, q = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    call("qed_ed_get_combined_mcf", int(label), 0, ctypes.byref(p), ctypes.byref(M), ctypes.byref(q), None, None, None)
+    pp = p.value * p.value
+    A = np.zeros(max(M.value * pp, 1))
+    B = np.zeros(max(M.value * pp, 1))
+    W = np.zeros(max(p.value * q.value, 1))
+    call("qed_ed_get_combined_mcf", int(label), 0, ctypes.byref(p), ctypes.byref(M), ctypes.byref(q), dp(A), dp(B), dp(W))
+    blk = lambda X, j: X[j * pp:(j + 1) * pp].reshape(p.value, p.value).T.astype(np.complex128)
+    Wm = W[:p.value * q.value].reshape(q.value, p.value).T.astype(np.complex128)
+    return Wm, [blk(A, j) for j in range(M.value)], [blk(B, j) for j in range(M.value)]
+
+
 def print_wavefunction(label=0):
     """Text of the low-energy states: header, and amplitudes with basis states when the
     sector is at most max_dim_print (src_python/qcm_ED_wrap.hpp:335-337)."""
@@ -362,7 +380,7 @@ def read_instance_from_hdf5(filename, group, label=0):
 
 # ---------------------------------------------------------------- global parameters
 
-_BOOL = {"verb_ED", "no_degenerate_BL", "continued_fraction", "block_Lanczos_QR"}
+_BOOL = {"verb_ED", "no_degenerate_BL", "continued_fraction", "block_Lanczos_QR", "combine_mcf"}
 
 
 def set_global_parameter(name, value=None):

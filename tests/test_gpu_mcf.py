@@ -154,3 +154,88 @@ def test_gf_method_M_deflated_blocks_match_oracle(gpu_lib, qr):
     err = np.abs(G - Go).max()
     print("max |G_gpu - G_oracle| = %.3g" % err)
     assert err < 1e-10
+
+
+def _combine(qcm, method, qr, on=True):
+    qcm.set_global_parameter("GF_method", method)
+    qcm.set_global_parameter("block_Lanczos_QR", bool(qr))
+    qcm.set_global_parameter("combine_mcf", bool(on))
+
+
+def _uncombine(qcm):
+    qcm.set_global_parameter("GF_method", "L")
+    qcm.set_global_parameter("block_Lanczos_QR", False)
+    qcm.set_global_parameter("combine_mcf", False)
+
+
+@pytest.mark.parametrize("qr", [0, 1])
+def test_combine_mcf_exhaustive_matches_bruteforce(gpu_lib, qr):
+    """4-site chain, GF_method 'M' with combine_mcf: the particle and hole fractions are
+    exact (exhausted Krylov spaces) and so is their combination by block Lanczos on the
+    direct sum (combine_via_lanczos); the exported (W, A, B) evaluates to the same G."""
+    from pyqcm_amd import qcm
+    m, vals = TM.chain(4, tp=-0.3, U=3.0, mu=1.2)
+    zs = np.array([0.05j, 0.5j, 2j, 1.0 + 0.1j, -2.0 + 0.3j])
+    _combine(qcm, "M", qr)
+    try:
+        lab = register(with_sites(m, 4), vals, "R0:N4:S0")
+        G = qcm.Green_function(zs, 0, lab)
+        W, A, B = qcm.combined_mcf(lab)
+    finally:
+        _uncombine(qcm)
+    _, Gref = O.dense_gf(m, vals, 2, 2, [0, 1, 2, 3], zs)
+    print("combined MCF: %d floors of %d x %d, max |G - G_exact| = %.3g"
+          % (len(A), A[0].shape[0], A[0].shape[1], np.abs(G - Gref).max()))
+    np.testing.assert_allclose(G, Gref, atol=1e-10)
+    for i, z in enumerate(zs):
+        g = O.mcf_evaluate([a.real for a in A], [b.real for b in B], W.real, z)
+        np.testing.assert_allclose(g, G[i], atol=1e-12)
+
+
+@pytest.mark.parametrize("qr", [0, 1])
+def test_combine_mcf_from_qmatrix_matches_lehmann(gpu_lib, qr):
+    """3x3 Hubbard, N = 9, S = 1, GF_method 'L' with combine_mcf (Q_matrix_to_mcf): the
+    Lehmann form rewritten as one matrix continued fraction by block Lanczos on diag(e).
+    The recurrence stops once the lowest eigenvalue of its projection has converged, as
+    the reference's does, so the rewriting is truncated: against the oracle's restatement
+    of the same steps on the same Lehmann form to 1e-8, against the Lehmann G to 1e-5."""
+    from pyqcm_amd import qcm
+    m, vals = TM.square(3, 3, tp=-0.3, U=8.0, mu=4.0)
+    zs = 1j * np.array([0.05, 0.2, 0.7, 2.0, 6.0]) + 0.1
+    lab_l = register(with_sites(m, 9), vals, "R0:N9:S1")
+    GL = [qcm.Green_function(zs, s, lab_l) for s in (0, 1)]
+    _combine(qcm, "L", qr)
+    try:
+        lab_c = register(with_sites(m, 9), vals, "R0:N9:S1")
+        GC = [qcm.Green_function(zs, s, lab_c) for s in (0, 1)]
+        W, A, B = qcm.combined_mcf(lab_c)
+        with pytest.raises(Exception, match="not available"):
+            qcm.qmatrix(lab_c)
+    finally:
+        _uncombine(qcm)
+    for s in (0, 1):
+        err = np.abs(GC[s] - GL[s]).max()
+        e, Q = qcm.qmatrix(lab_l, s)
+        q, W = O.mgs_weight([Q[:, a].real for a in range(9)])
+        A_o, B_o = O.block_lanczos(lambda x: e * x, q, qr=bool(qr), M0=len(e))
+        Go = np.array([O.mcf_evaluate(A_o, B_o, W, z) for z in zs])
+        err_o = np.abs(GC[s] - Go).max()
+        print("spin %d: %d floors (oracle %d), max |G_combined - G_L| = %.3g, vs oracle %.3g"
+              % (s, len(A), len(A_o), err, err_o))
+        assert err_o < 1e-8
+        assert err < 1e-5
+
+
+def test_combined_mcf_errors(gpu_lib):
+    from pyqcm_amd import qcm
+    m, vals = TM.chain(4, U=3.0, mu=1.5)
+    lab = register(with_sites(m, 4), vals, "R0:N4:S0")
+    with pytest.raises(Exception, match="does not use the MCF format"):
+        qcm.combined_mcf(lab)
+    qcm.set_global_parameter("GF_method", "M")
+    try:
+        lab2 = register(with_sites(m, 4), vals, "R0:N4:S0")
+        with pytest.raises(Exception, match="is not built unless the global parameter 'combine_mcf'"):
+            qcm.combined_mcf(lab2)
+    finally:
+        _uncombine(qcm)
