@@ -1905,8 +1905,13 @@ int qed_ed_print_wavefunction(uint64_t label, char* buf, int buflen, int* needed
       o << "state\n" << sector_name(st->nup + st->ndn, st->nup - st->ndn) << '\t' << st->energy << '\t' << st->weight
         << std::endl;
       if (!st->psi.p) continue;
+      auto binom = [](int n, int k) {
+        uint64_t c = 1;
+        for (int i = 1; i <= k; ++i) c = c * (uint64_t)(n - k + i) / (uint64_t)i;
+        return c;
+      };
+      if (binom(L, st->nup) * binom(L, st->ndn) > (uint64_t)gi("max_dim_print")) continue;
       SectorPtr H = make_sector(I, st->nup, st->ndn);
-      if (H->D > (uint64_t)gi("max_dim_print")) continue;
       std::vector<double> psi(H->D);
       std::vector<uint32_t> bu(H->du), bd(H->dd);
       QED_HIP(hipMemcpy(psi.data(), st->psi.p, H->D * 8, hipMemcpyDeviceToHost));
