@@ -12,6 +12,8 @@ Prints one JSON line (rank 0, N=1):
   config       also the p10 / p50 / p90 of 100 single timed launches
   cpu_baseline the oracle's H.v (reference loop order, one core) on the same sector
   cpu_parallel the same H.v with the oracle's OpenMP-over-rows form on up to 16 cores
+  cpt          the CPT k-sum of the 3x4 cluster (128^2 k-points x 512 frequencies), device
+               time and FP64 rate on 41.5 kflop per (k, w)
   cdmft        the second half of the metric: converged CDMFT solves per second on the
                2x2 plaquette + 8 bath orbitals (16 bath parameters), n seeded points
                (U ~ U[0.5, 12], mu ~ U[0, U], t' ~ U[-0.5, 0.5]), solved by the batched
@@ -49,6 +51,7 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="time every H.v variant (stderr)")
     ap.add_argument("--cdmft", type=int, default=64, help="CDMFT points for the warm-start leg (0: skip)")
     ap.add_argument("--cdmft-batch", type=int, default=32, help="CDMFT problems active at a time")
+    ap.add_argument("--cpt", type=int, default=1, help="time the CPT k-sum on the 3x4 cluster (0: skip)")
     ap.add_argument("--cdmft-jacobian", action="store_true",
                     help="add a warm leg that also starts Broyden from the nearest training run's inverse Jacobian")
     ap.add_argument("--no-graph", action="store_true",
@@ -148,6 +151,7 @@ def main():
     dx.free()
     dy.free()
     del S
+    cpt = cpt_leg() if args.cpt else None
     cdmft = cdmft_leg(args.cdmft, args.cdmft_batch, not args.no_graph, args.cdmft_jacobian) if args.cdmft > 0 else None
 
     out = {
@@ -176,6 +180,7 @@ def main():
                      "stream_rmw_measured": round(stream, 1), "frac_of_stream": round(achieved / stream, 4)},
         "cpu_baseline": cpu,
         "cpu_parallel": cpu_par,
+        "cpt": cpt,
         "cdmft": cdmft,
     }
     print(json.dumps(out), flush=True)
@@ -202,6 +207,53 @@ def traffic_from_profile():
         if ln.startswith("traffic_bytes_per_launch:"):
             return float(ln.split(":", 1)[1])
     return None
+
+
+PEAK_FP64_TFLOPS = 78.6
+CPT_FLOP_PER_POINT = 41.5e3  # two complex 12 x 12 products and one LU inverse per (k, w)
+
+
+def cpt_leg(side=128, nw=512, reps=3):
+    """CPT spectral trace on the 3x4 Hubbard cluster (t=1, U=8, mu=4, sector N=12 S=0,
+    D = 853,776): G_cpt = G_c (1 - V(k) G_c)^-1 over a side^2 k-grid (k_i = (i+0.5)/side
+    in the superdual basis) and nw frequencies w = linspace(-10, 10) + 0.1i, one device
+    launch; device time from HIP events, algorithmic flops 41.5 kflop per (k, w)."""
+    from pyqcm_amd import _lib
+    from pyqcm_amd import lattice as LT
+    T = LT.square_tiling(3, 4)
+    M = LT.LatticeModel("cpt_bench", T)
+    vals = {"t": 1.0, "U": 8.0, "mu": 4.0}
+    inst = M.instance(vals, "R0:N12:S0")
+    t0 = time.perf_counter()
+    E0, sec = inst.ground_state()
+    z = np.linspace(-10.0, 10.0, nw) + 0.1j
+    _, Geff = inst._geff(z)
+    t_ed = time.perf_counter() - t0
+    nv, mats = T.fourier_terms(vals)
+    V0 = inst.v0()
+    k = T.kgrid(side)
+    LT.cpt_batch(Geff, V0, nv, mats, k[:1024], mode="trace")  # warm-up
+    walls, devs = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        LT.cpt_batch(Geff, V0, nv, mats, k, mode="trace")
+        walls.append(time.perf_counter() - t0)
+        ms = np.zeros(1)
+        _lib.call("qed_ksum_last_ms", _lib.dp(ms))
+        devs.append(float(ms[0]))
+    dev = float(np.median(devs))
+    flops = CPT_FLOP_PER_POINT * nw * len(k)
+    tf = flops / (dev * 1e-3) / 1e12
+    inst.erase()
+    log("CPT 3x4: ED + G_c at %d frequencies %.2f s; k-sum over %d x %d points: %.2f ms device, %.2f ms wall "
+        "-> %.2f TFLOP/s" % (nw, t_ed, nw, len(k), dev, float(np.median(walls)) * 1e3, tf))
+    return {"workload": "3x4 Hubbard cluster t=1 U=8 mu=4, sector N=12 S=0 (D=853,776), CPT trace of "
+                        "G_c (1 - V(k) G_c)^-1 on a %dx%d k-grid x %d frequencies" % (side, side, nw),
+            "E0": round(E0, 10), "points": nw * len(k), "ksum_ms_device": round(dev, 3),
+            "ksum_ms_wall": round(float(np.median(walls)) * 1e3, 3), "ed_and_gc_s": round(t_ed, 3),
+            "roofline": {"bound": "fp64", "achieved": round(tf, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(tf / PEAK_FP64_TFLOPS, 4),
+                         "flop_per_point": CPT_FLOP_PER_POINT}}
 
 
 def cdmft_leg(n, batch, graph=True, jacobian=False):

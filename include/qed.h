@@ -165,6 +165,10 @@ int qed_cf_eval(int n, const int* off, const int* depth, const double* a, const 
 int qed_cpt_batch(int n, int nw, const double* Gc, const double* V0, int nterm, const double* Vt,
                   const double* nv, int nk, const double* k, int mode, double* out);
 
+/* Device time in ms of the k-sum kernels of the last qed_cpt_batch call (HIP events on the
+ * library stream); no counterpart in the reference, for measurement. */
+int qed_ksum_last_ms(double* ms);
+
 /* VCA: out[w] = (1/nk) sum_k log|det(V(k) Gc(w) - 1)| per frequency, same V(k) model.
  * Replaces SEF_integrand_k summed by k_integral_grid
  * (src_qcm/lattice_model_instance.cpp:624-643, src_util/integrate.cpp:177-239). */

@@ -238,9 +238,21 @@ __global__ __launch_bounds__(256) void k_ksum(KsumArgs A) {
         for (int i = 0; i < N; ++i)
           if (i == j) d = col[i];
         double tr = d.re, ti = d.im;
-        for (int o = 1; o < N; o <<= 1) {
-          tr += __shfl_xor(tr, o, 64);
-          ti += __shfl_xor(ti, o, 64);
+        if constexpr ((N & (N - 1)) == 0) {
+          for (int o = 1; o < N; o <<= 1) {
+            tr += __shfl_xor(tr, o, 64);
+            ti += __shfl_xor(ti, o, 64);
+          }
+        } else {
+          // groups of N lanes do not align with xor butterflies: gather the diagonal
+          double sr = 0.0, si = 0.0;
+#pragma unroll
+          for (int i = 0; i < N; ++i) {
+            sr += __shfl(tr, base + i, 64);
+            si += __shfl(ti, base + i, 64);
+          }
+          tr = sr;
+          ti = si;
         }
         if (j == 0) {
           A.out[2 * ((size_t)w * A.nk + kk)] = tr;
@@ -298,9 +310,8 @@ void ksum_dispatch(KsumArgs A, int ngroups) {
     case 2: launch_ksum<2>(A, ngroups); break;
     case 4: launch_ksum<4>(A, ngroups); break;
     case 5: case 6: case 7: case 8: launch_ksum<8>(A, ngroups); break;
-    case 9: case 10: case 11: case 12: case 13: case 14: case 15: case 16:
-      launch_ksum<16>(A, ngroups);
-      break;
+    case 12: launch_ksum<12>(A, ngroups); break;  // 5 groups of 12 lanes per wave
+    case 16: launch_ksum<16>(A, ngroups); break;
     default: fail("lattice k-sum: cluster Green function dimension above 16 is not supported");
   }
   QED_HIP(hipGetLastError());
@@ -330,6 +341,7 @@ int padded_dim(int n) {
   if (n <= 2) return n;
   if (n <= 4) return 4;
   if (n <= 8) return 8;
+  if (n <= 12) return 12;
   if (n <= 16) return 16;
   fail("lattice k-sum: cluster Green function dimension above 16 is not supported");
 }
@@ -357,6 +369,32 @@ std::vector<double> inverse_gc(const double* Gc, int nw, int n, int N, std::vect
   }
   return out;
 }
+
+// device time of the last CPT k-sum (HIP events on the library stream)
+double& last_ksum_ms() {
+  static double ms = 0.0;
+  return ms;
+}
+
+struct KsumTimer {
+  hipEvent_t a = nullptr, b = nullptr;
+  KsumTimer() {
+    QED_HIP(hipEventCreate(&a));
+    QED_HIP(hipEventCreate(&b));
+    QED_HIP(hipEventRecord(a, stream()));
+  }
+  void stop() {
+    QED_HIP(hipEventRecord(b, stream()));
+    QED_HIP(hipEventSynchronize(b));
+    float ms = 0.0f;
+    QED_HIP(hipEventElapsedTime(&ms, a, b));
+    last_ksum_ms() = ms;
+  }
+  ~KsumTimer() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
 
 }  // namespace
 
@@ -396,6 +434,11 @@ int qed_cf_eval(int n, const int* off, const int* depth, const double* a, const 
   });
 }
 
+int qed_ksum_last_ms(double* ms) {
+  *ms = last_ksum_ms();
+  return 0;
+}
+
 int qed_cpt_batch(int n, int nw, const double* Gc, const double* V0, int nterm, const double* Vt,
                   const double* nv, int nk, const double* k, int mode, double* out) {
   return guard([&] {
@@ -430,6 +473,7 @@ int qed_cpt_batch(int n, int nw, const double* Gc, const double* V0, int nterm, 
     if (mode == 2) dpart.alloc((size_t)ngroups * N * N * 2 * 8);
     A.out = dout.as<double>();
     A.part = dpart.as<double>();
+    KsumTimer timer;
     ksum_dispatch(A, ngroups);
     if (mode == 2) {
       int len = N * N * 2;
@@ -437,6 +481,7 @@ int qed_cpt_batch(int n, int nw, const double* Gc, const double* V0, int nterm, 
                                                                  dout.as<double>());
       QED_HIP(hipGetLastError());
     }
+    timer.stop();
     std::vector<double> h(out_len);
     QED_HIP(hipMemcpyAsync(h.data(), dout.p, out_len * 8, hipMemcpyDeviceToHost, stream()));
     QED_HIP(hipStreamSynchronize(stream()));

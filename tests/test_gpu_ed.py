@@ -246,7 +246,7 @@ def _vk(V0, Vt, nv, kk):
     return V0 + sum(Vt[t] * np.exp(2j * np.pi * (kk @ nv[t])) for t in range(len(nv)))
 
 
-@pytest.mark.parametrize("n", [2, 3, 4, 6, 12, 16])
+@pytest.mark.parametrize("n", [2, 3, 4, 6, 10, 12, 16])
 def test_cpt_batch_modes(gpu_lib, n):
     from pyqcm_amd import _lib
     Gc, V0, Vt, nv, k = _lattice_case(n, n)
