@@ -8,6 +8,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -118,6 +119,8 @@ struct Instance {
   std::vector<double> tc[2], tb[2], tcb[2];  // tc n x n, tb nb x nb, tcb n x nb (column-major)
   std::vector<double> eb[2], tcb_d[2];       // diagonal bath energies and rotated hybridization
   std::string gs_string;
+  // recently evaluated G(z) per spin, most recent first (model_instance::look_up_table)
+  std::deque<std::pair<std::complex<double>, std::vector<double>>> gf_cache[2];
 };
 
 std::map<std::string, std::unique_ptr<Model>>& models() {
@@ -982,9 +985,45 @@ int spin_index(const Instance& I, int spin_down) {
 }
 
 // G(z) for nz frequencies, d x d complex column-major each
+void green_function_eval(Instance& I, int nz, const double* z, int sp, double* out);
+
+// G(z) for nz frequencies, d x d complex column-major each. Frequencies within 1e-8 of one
+// evaluated before are served from a per-instance table of the last GF_lookup_depth
+// evaluations, as the reference's look-up (model_instance::Green_function,
+// src_ed/model_instance.hpp:566-582,624-632: the oldest matching entry wins)
 void green_function(Instance& I, int nz, const double* z, int spin_down, double* out) {
   green_function_solve(I);
   const int sp = spin_index(I, spin_down);
+  const int d = I.model->n_sites;
+  const size_t dd2 = (size_t)2 * d * d;
+  auto& cache = I.gf_cache[sp];
+  const size_t depth = (size_t)std::max(0, gi("GF_lookup_depth"));
+  std::vector<int> miss;
+  for (int iz = 0; iz < nz; ++iz) {
+    const cd w(z[2 * iz], z[2 * iz + 1]);
+    const std::vector<double>* hit = nullptr;
+    for (auto& e : cache)
+      if (std::abs(w - e.first) < MIDDLE_VALUE) hit = &e.second;
+    if (hit) std::copy(hit->begin(), hit->end(), out + iz * dd2);
+    else miss.push_back(iz);
+  }
+  if (miss.empty()) return;
+  std::vector<double> zm(2 * miss.size()), gm(miss.size() * dd2);
+  for (size_t m = 0; m < miss.size(); ++m) {
+    zm[2 * m] = z[2 * miss[m]];
+    zm[2 * m + 1] = z[2 * miss[m] + 1];
+  }
+  green_function_eval(I, (int)miss.size(), zm.data(), sp, gm.data());
+  for (size_t m = 0; m < miss.size(); ++m) {
+    std::copy(gm.begin() + m * dd2, gm.begin() + (m + 1) * dd2, out + miss[m] * dd2);
+    if (depth == 0) continue;
+    cache.emplace_front(cd(zm[2 * m], zm[2 * m + 1]),
+                        std::vector<double>(gm.begin() + m * dd2, gm.begin() + (m + 1) * dd2));
+    if (cache.size() > depth) cache.pop_back();
+  }
+}
+
+void green_function_eval(Instance& I, int nz, const double* z, int sp, double* out) {
   const int d = I.model->n_sites;
   std::fill(out, out + (size_t)2 * nz * d * d, 0.0);
   for (auto& s : I.states) {
@@ -1368,6 +1407,8 @@ void read_instance_file(Instance& I, const std::string& path, const std::string&
   }
   // the instance changes only once the whole group has been read
   I.states.swap(states);
+  I.gf_cache[0].clear();
+  I.gf_cache[1].clear();
   I.GS_energy = gse;
   I.mixing = mixing;
   I.cf = fmt == "cf" || (fmt != "bl" && fmt != "mcf");

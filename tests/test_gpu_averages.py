@@ -204,3 +204,28 @@ def test_print_wavefunction(gpu_lib):
         assert len(qcm.print_wavefunction(lab2).strip("\n").split("\n")) == 2
     finally:
         qcm.set_global_parameter("max_dim_print", 64)
+
+
+def test_green_function_lookup_table(gpu_lib):
+    """Frequencies within 1e-8 of one already evaluated come from the instance's table of
+    recent evaluations, as the reference's look-up (model_instance.hpp:566-582,624-632);
+    GF_lookup_depth = 0 turns the table off."""
+    from pyqcm_amd import qcm
+    m, vals = TM.chain(4, U=4.0, mu=2.0)
+    z = np.array([0.3j, 1.1j, 0.5 + 0.2j])
+    a = register(with_sites(m, 4), vals, "R0:N4:S0")
+    G1 = qcm.Green_function(z, 0, a)
+    np.testing.assert_array_equal(qcm.Green_function(z + 1e-10, 0, a), G1)
+    G_mixed = qcm.Green_function(np.array([2.0j, z[1] + 1e-10]), 0, a)
+    np.testing.assert_array_equal(G_mixed[1], G1[1])
+    qcm.set_global_parameter("GF_lookup_depth", 0)
+    try:
+        b = register(with_sites(m, 4), vals, "R0:N4:S0")
+        G2 = qcm.Green_function(z, 0, b)
+        G3 = qcm.Green_function(z + 1e-10, 0, b)
+    finally:
+        qcm.set_global_parameter("GF_lookup_depth", 64)
+    assert not np.array_equal(G2, G3)
+    np.testing.assert_allclose(G2, G3, atol=1e-8)
+    np.testing.assert_allclose(G2, G1, atol=1e-10)
+    np.testing.assert_allclose(G_mixed[0], qcm.Green_function(np.array([2.0j]), 0, b)[0], atol=1e-10)
