@@ -52,6 +52,7 @@ def main():
     ap.add_argument("--cdmft", type=int, default=64, help="CDMFT points for the warm-start leg (0: skip)")
     ap.add_argument("--cdmft-batch", type=int, default=32, help="CDMFT problems active at a time")
     ap.add_argument("--cpt", type=int, default=1, help="time the CPT k-sum on the 3x4 cluster (0: skip)")
+    ap.add_argument("--mlp-ensemble", type=int, default=1, help="warm start: mean prediction of this many MLPs")
     ap.add_argument("--cdmft-jacobian", action="store_true",
                     help="add a warm leg that also starts Broyden from the nearest training run's inverse Jacobian")
     ap.add_argument("--no-graph", action="store_true",
@@ -152,7 +153,7 @@ def main():
     dy.free()
     del S
     cpt = cpt_leg() if args.cpt else None
-    cdmft = cdmft_leg(args.cdmft, args.cdmft_batch, not args.no_graph, args.cdmft_jacobian) if args.cdmft > 0 else None
+    cdmft = cdmft_leg(args.cdmft, args.cdmft_batch, not args.no_graph, args.cdmft_jacobian, args.mlp_ensemble) if args.cdmft > 0 else None
 
     out = {
         "metric": "H·v GB/s (16-site ED, % HBM peak); CDMFT solves/s learned warm-start vs plain",
@@ -256,7 +257,7 @@ def cpt_leg(side=128, nw=512, reps=3):
                          "flop_per_point": CPT_FLOP_PER_POINT}}
 
 
-def cdmft_leg(n, batch, graph=True, jacobian=False):
+def cdmft_leg(n, batch, graph=True, jacobian=False, ensemble=1):
     """Plain (cold start) vs learned warm start, batched CDMFT on n seeded points with
     `batch` problems active at a time (a converged or failed problem is replaced by the
     next queued one). With `jacobian`, a third leg also starts each Broyden iteration from
@@ -265,9 +266,9 @@ def cdmft_leg(n, batch, graph=True, jacobian=False):
     from pyqcm_amd import cdmft as C
     from pyqcm_amd import warmstart as W
     X, Y = W.load_training_set()
-    net = W.MLP(3, 128, 16, seed=0)
+    nets = [W.MLP(3, 128, 16, seed=s) for s in range(max(1, ensemble))]
     t0 = time.perf_counter()
-    hist = net.fit(X, Y, epochs=1500, batch=64, lr=2e-3)
+    hist = [net.fit(X, Y, epochs=1500, batch=64, lr=2e-3) for net in nets][0]
     t_train = time.perf_counter() - t0
     log("MLP trained on %d points in %.1f s (loss %.3g -> %.3g)" % (len(X), t_train, hist[0], hist[-1]))
     P = W.plaquette_problem("plaq_bench")
@@ -276,7 +277,7 @@ def cdmft_leg(n, batch, graph=True, jacobian=False):
     feats = np.array([W.plaquette_features(v) for v in pts])
     cold = [W.PLAQUETTE_COLD_START] * n
     t0 = time.perf_counter()
-    warm = list(C.canonical_bath(net.predict(feats, bf16=True)))
+    warm = list(C.canonical_bath(np.mean([net.predict(feats, bf16=True) for net in nets], axis=0)))
     t_pred = time.perf_counter() - t0
     legs = [("plain", cold, None), ("warm", warm, None)]
     FJ, t_pred_j = None, 0.0
@@ -315,7 +316,8 @@ def cdmft_leg(n, batch, graph=True, jacobian=False):
         **({"warm_jacobian": res["warm_jacobian"], "jacobian_runs": int(len(FJ)),
             "speedup_jacobian": round(res["warm_jacobian"]["solves_per_s"] /
                                       max(res["plain"]["solves_per_s"], 1e-12), 3)} if jacobian else {}),
-        "mlp": {"train_points": int(len(X)), "train_s": round(t_train, 2), "final_loss": float(hist[-1]),
+        "mlp": {"train_points": int(len(X)), "networks": len(nets), "train_s": round(t_train, 2),
+                "final_loss": float(hist[-1]),
                 "inference": "bf16 v_mfma_f32_16x16x32_bf16"},
     }
 
