@@ -61,7 +61,7 @@ def main():
                     help="only run a plain device copy of 2 GiB five times (for scaling HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (the single-GPU H.v path); "
+        print("bench.py: the solver runs on one GPU only (the single-GPU H.v path); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 

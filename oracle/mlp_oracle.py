@@ -2,7 +2,7 @@
 
 numpy restatement (float64) of the warm-start MLP of pyqcm_amd/csrc/qed_mlp.hip: forward
 pass, mean-squared-error loss and its gradient by backpropagation, and one Adam step.
-The network has no counterpart in the reference (it is new in this build); the
+The network has no counterpart in the reference (it is new in this solver); the
 restatement is pinned by central finite differences of the loss (tests/test_oracle.py),
 which a wrong transpose, a dropped tanh' factor or a wrong mean normalization fails.
 

@@ -1,5 +1,5 @@
 """bench.py command-line contract on the CPU: more than one GPU is refused with a message
-(this build measures the single-GPU path)."""
+(the benchmark measures the single-GPU path)."""
 import os
 import subprocess
 import sys
