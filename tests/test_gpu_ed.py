@@ -295,3 +295,32 @@ def test_cdmft_distance_batch(gpu_lib):
     for b in range(B):
         ref = O.cdmft_distance(w, wt, Gh, eb[b], tcb[b])
         assert abs(d[b] - ref) < 1e-10 * abs(ref)
+
+
+# ------------------------------------------------------------------ edge sectors
+
+def test_empty_and_full_sectors(gpu_lib):
+    """The vacuum (N = 0, dimension 1): E0 = 0 and G(z) = (z - h)^-1 with h the one-body
+    matrix (particles only). The filled band (N = 2L, dimension 1): E0 = U L - 2 L mu.
+    A sector with more particles than orbitals is refused."""
+    from pyqcm_amd import qcm
+    m, vals = TM.chain(4, U=4.0, mu=1.5)
+    lab = register(with_sites(m, 4), vals, "R0:N0:S0")
+    E0, sec = qcm.ground_state_solve(lab)
+    assert E0 == 0.0 and sec.startswith("R0:N0:S0")
+    zs = np.array([0.3j, 1.0 + 0.2j, -2.0 + 0.5j])
+    G = qcm.Green_function(zs, 0, lab)
+    h = np.zeros((4, 4))
+    for op in ("t", "mu"):
+        for r, c, v in m.operators[op].elements:
+            if r < 4 and c < 4:
+                h[r, c] += v * vals[op]
+                if r != c:
+                    h[c, r] += v * vals[op]
+    for i, z in enumerate(zs):
+        np.testing.assert_allclose(G[i], np.linalg.inv(z * np.eye(4) - h), atol=1e-12)
+    full = register(with_sites(m, 4), vals, "R0:N8:S0")
+    E0f, _ = qcm.ground_state_solve(full)
+    assert abs(E0f - (4.0 * 4 - 2 * 4 * 1.5)) < 1e-12
+    with pytest.raises(Exception):
+        qcm.ground_state_solve(register(with_sites(m, 4), vals, "R0:N10:S0"))
