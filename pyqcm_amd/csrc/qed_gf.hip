@@ -203,8 +203,12 @@ __device__ __forceinline__ double gj_invert(cplx* col, int base, int j, bool wan
   return logdet;
 }
 
+// At least two wavefronts per SIMD: the fully unrolled elimination of a 12- or 16-wide
+// matrix otherwise takes over 256 registers and runs one wavefront per SIMD (measured on
+// the 3x4 cluster: 30.7 ms at one, 22.4 ms at two with a few spilled registers, 25.3 ms at
+// three with more spills)
 template <int N>
-__global__ __launch_bounds__(256) void k_ksum(KsumArgs A) {
+__global__ __launch_bounds__(256, 2) void k_ksum(KsumArgs A) {
   constexpr int GPW = 64 / N;  // groups per wave
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane / N, j = lane % N;
