@@ -874,6 +874,11 @@ void hv(const qed_sector& s, const double* x, double* y, int variant) {
   set_lds_attributes();
   if (variant < 0) variant = s.hv_variant;
   if (variant < 0) variant = 12;
+  if (variant >= 20 && variant <= 22) {
+    // split-halves passes (20 both; 21 row pass only, 22 tile pass only: timing splits)
+    if (hv_split(s, x, y, variant == 20 ? 3 : variant == 21 ? 1 : 2)) return;
+    variant = 12;
+  }
   HvView v = view(s);
   hipStream_t st = stream();
   const uint32_t* ue = s.up_ell.as<uint32_t>();
@@ -1364,6 +1369,9 @@ int qed_sector_create(int L, int n_up, int n_dn, int n_ops, const int* types, co
           if (r == c) {
             (dn ? dsd : dsu).push_back(rs);
             (dn ? dvd : dvu).push_back(v * z);
+            auto& ds = dn ? s->dsite_dn : s->dsite_up;
+            ds.resize(L, 0.0);
+            ds[rs] += v * z;
           } else {
             auto& B = dn ? bd : bu;
             auto& V = dn ? vd : vu;
@@ -1371,6 +1379,9 @@ int qed_sector_create(int L, int n_up, int n_dn, int n_ops, const int* types, co
             V.push_back(v * z);
             B.push_back({rs, cs});
             V.push_back(v * z);
+            (dn ? s->pd_i : s->pu_i).push_back(rs);
+            (dn ? s->pd_j : s->pu_j).push_back(cs);
+            (dn ? s->pd_t : s->pu_t).push_back(v * z);
           }
         } else if (types[k] == 1) {
           int a = std::min(r, c), b = std::max(r, c);

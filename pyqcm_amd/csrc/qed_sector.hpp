@@ -52,6 +52,13 @@ struct qed_sector {
   qed::DevBuf dn_ell_cm;
   uint32_t dn_w_cm = 0;
   mutable qed::DevBuf xt, zt;
+
+  // one-body terms per spin for the split-halves H.v: hop elements (i, j, value) as read
+  // from the operators, and the one-body diagonal per orbital
+  std::vector<int> pu_i, pu_j, pd_i, pd_j;
+  std::vector<double> pu_t, pd_t;
+  std::vector<double> dsite_up, dsite_dn;
+  mutable std::shared_ptr<void> split;  // plan and device tables, built on first use
 };
 
 namespace qed {
@@ -150,6 +157,10 @@ void tridiag_lowest(const std::vector<double>& a, const std::vector<double>& b, 
 
 // y += H x on the library stream (device vectors). variant -1 chooses automatically.
 void hv(const qed_sector& s, const double* x, double* y, int variant = -1);
+// the split-halves passes (passes: 1 row pass, 2 tile pass, 3 both); false when the
+// sector's structure does not suit them (nothing launched)
+bool hv_split(const qed_sector& s, const double* x, double* y, int passes = 3);
+std::string split_status(const qed_sector& s);
 
 // BLAS-1 on the library stream. dot/nrm2 return host scalars (synchronizing).
 double dot(const double* x, const double* y, uint64_t n);

@@ -57,7 +57,7 @@ HV_MODELS = {
 }
 
 
-@pytest.mark.parametrize("variant", [0, 1, 4, 5])
+@pytest.mark.parametrize("variant", [0, 1, 4, 5, 20])
 @pytest.mark.parametrize("name", sorted(HV_MODELS))
 def test_hv_matches_oracle(gpu_lib, name, variant):
     make, sectors = HV_MODELS[name]
@@ -88,7 +88,7 @@ def test_hv_16_sites_half_filling(gpu_lib):
     from pyqcm_amd._lib import DeviceArray
     dx = DeviceArray.from_host(x)
     outs = {}
-    for variant in (0, 4, 5, 12):
+    for variant in (0, 4, 5, 12, 20):
         dy = DeviceArray(Sg.dim).zeros()
         Sg.hv_dev(dx, dy, variant)
         got = dy.to_host()
