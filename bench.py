@@ -12,6 +12,9 @@ Prints one JSON line (rank 0, N=1):
   config       also the p10 / p50 / p90 of 100 single timed launches
   cpu_baseline the oracle's H.v (reference loop order, one core) on the same sector
   cpu_parallel the same H.v with the oracle's OpenMP-over-rows form on up to 16 cores
+  lanczos      ms and GB/s per ground-state Lanczos iteration on the same sector (48 D bytes:
+               the product plus the vector update), fused and as separate passes, with
+               3 iterations of the oracle's recurrence timed on 1 and up to 16 host cores
   cpt          the CPT k-sum of the 3x4 cluster (128^2 k-points x 512 frequencies), device
                time and FP64 rate on 41.5 kflop per (k, w)
   cdmft        the second half of the metric: converged CDMFT solves per second on the
@@ -51,6 +54,7 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="time every H.v variant (stderr)")
     ap.add_argument("--cdmft", type=int, default=64, help="CDMFT points for the warm-start leg (0: skip)")
     ap.add_argument("--cdmft-batch", type=int, default=32, help="CDMFT problems active at a time")
+    ap.add_argument("--lanczos", type=int, default=1, help="time ground-state Lanczos iterations (0: skip)")
     ap.add_argument("--cpt", type=int, default=1, help="time the CPT k-sum on the 3x4 cluster (0: skip)")
     ap.add_argument("--mlp-ensemble", type=int, default=1, help="warm start: mean prediction of this many MLPs")
     ap.add_argument("--cdmft-jacobian", action="store_true",
@@ -69,17 +73,14 @@ def main():
         copy_calibration()
         return
 
-    from oracle import oracle as O
     from pyqcm_amd import _lib
     from pyqcm_amd._lib import DeviceArray
     from pyqcm_amd.sector import Sector
 
     _lib.call("qed_init", 0)
-    nn, nnn = O.square_bonds(4, 4)
-    model, vals = O.hubbard_cluster(16, nn, tp_bonds=nnn, tp=-0.3, U=8.0, mu=4.0)
-    flat = model.flat(vals)
+    vals = {"t": 1.0, "tp": -0.3, "U": 8.0, "mu": 4.0}
     t0 = time.time()
-    S = Sector(16, 8, 8, flat)
+    S = Sector(16, 8, 8, cluster_hamiltonian_4x4(vals))
     log("sector built: D=%d (%.2f s)" % (S.dim, time.time() - t0))
     D = S.dim
     rng = np.random.default_rng(20261202)
@@ -126,10 +127,15 @@ def main():
     stream = 24.0 * (D - (D % 2)) / (ms[0] * 1e-3) / 1e9
     log("stream y += a x (24 B per element): %.0f GB/s" % stream)
 
+    lanczos = lanczos_leg(S, dx) if args.lanczos else None
+
     cpu = cpu_par = None
     if not args.no_cpu:
         # one full H.v of the same sector through the oracle (reference loop order, 1 core)
-        So = O.Sector(model, vals, 8, 8)
+        from oracle import oracle as O
+        nn, nnn = O.square_bonds(4, 4)
+        model, ovals = O.hubbard_cluster(16, nn, tp_bonds=nnn, **{k: vals[k] for k in ("tp", "U", "mu")})
+        So = O.Sector(model, ovals, 8, 8)
         yc = np.zeros(D)
         xh = x
         t0 = time.perf_counter()
@@ -148,6 +154,18 @@ def main():
                    "kind": "port, OpenMP over rows (a parallelization of ours; the reference's loop is serial)",
                    "sample": "one full y += H x on the same sector, %.2f s" % tp}
         log("cpu parallel (%d threads): %.2f s per H.v" % (nt, tp))
+        if lanczos is not None:
+            # a bounded sample of the same Lanczos iteration on the host: 3 iterations
+            ksteps = 3
+            for name, threads in (("cpu_1core", 0), ("cpu_parallel", nt)):
+                t0 = time.perf_counter()
+                O.lib().or_lanczos_steps(So.h, O._dp(xh), ksteps, threads, None, None)
+                tl = (time.perf_counter() - t0) / ksteps
+                lanczos[name] = {"ms_per_iteration": round(tl * 1e3, 1), "cores": max(1, threads),
+                                 "kind": "port", "sample": "%d iterations of the oracle's restatement of "
+                                 "the reference recurrence from the same start vector" % ksteps}
+                log("cpu Lanczos (%d threads): %.2f s per iteration" % (max(1, threads), tl))
+        del So
 
     dx.free()
     dy.free()
@@ -181,10 +199,45 @@ def main():
                      "stream_rmw_measured": round(stream, 1), "frac_of_stream": round(achieved / stream, 4)},
         "cpu_baseline": cpu,
         "cpu_parallel": cpu_par,
+        "lanczos": lanczos,
         "cpt": cpt,
         "cdmft": cdmft,
     }
     print(json.dumps(out), flush=True)
+
+
+def cluster_hamiltonian_4x4(vals):
+    """The 4x4 cluster's operators from the square-lattice tiling (intra-cluster NN hop t,
+    NNN hop tp, chemical potential mu) and the on-site Hubbard U, in the flat form the
+    sector constructor takes (0-based spin orbitals: site i up = i, down = i + 16)."""
+    from pyqcm_amd import lattice as LT
+    from pyqcm_amd.sector import flatten_ops
+    T = LT.square_tiling(4, 4)
+    ops = [(name, "one-body", [(r - 1, c - 1, v) for (r, c, v) in T.cluster_operator_elements(name)])
+           for name in ("t", "tp", "mu")]
+    ops.append(("U", "interaction", [(i, i + 16, 1.0) for i in range(16)]))
+    return flatten_ops(ops, vals)
+
+
+def lanczos_leg(S, dx, steps=20):
+    """Ground-state Lanczos iterations on the bench sector from the normalized x: the fused
+    form (product with the rescaling and <w, Hw> folded in, then one axpy + norm pass) and
+    the separate passes, ms per iteration from HIP events. Algorithmic bytes per iteration:
+    48 D (the product's 24 D plus 24 D for the update w' = y - a w with its norm)."""
+    from pyqcm_amd import _lib
+    ms = np.zeros(2)
+    _lib.call("qed_lanczos_steps_timed", S.h, dx.ptr, 2, _lib.dp(ms))  # warm-up
+    _lib.call("qed_lanczos_steps_timed", S.h, dx.ptr, steps, _lib.dp(ms))
+    per = 48.0 * S.dim
+    fused = float(ms[0]) if ms[0] > 0 else None
+    log("Lanczos iteration: fused %s ms, separate passes %.3f ms" %
+        ("%.3f" % fused if fused else "n/a", ms[1]))
+    return {"workload": "ground-state Lanczos iterations on the same sector, %d timed" % steps,
+            "bytes_per_iteration": per,
+            "ms_per_iteration": round(fused if fused else float(ms[1]), 4),
+            "ms_per_iteration_separate_passes": round(float(ms[1]), 4),
+            "GB_s": round(per / ((fused if fused else ms[1]) * 1e-3) / 1e9, 1),
+            "fused": fused is not None}
 
 
 def copy_calibration():

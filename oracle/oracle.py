@@ -52,6 +52,11 @@ def lib():
         L.or_lanczos_gs.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, dptr, dptr, iptr,
                                     dptr, dptr]
         L.or_lanczos_gs.restype = ctypes.c_int
+        L.or_lanczos_gs_threads.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, dptr, dptr, iptr,
+                                            dptr, dptr, ctypes.c_int, ctypes.c_int]
+        L.or_lanczos_gs_threads.restype = ctypes.c_int
+        L.or_lanczos_steps.argtypes = [ctypes.c_void_p, dptr, ctypes.c_int, ctypes.c_int, dptr, dptr]
+        L.or_lanczos_steps.restype = ctypes.c_int
         L.or_lanczos_green.argtypes = [ctypes.c_void_p, dptr, ctypes.c_double, ctypes.c_int, dptr, dptr]
         L.or_lanczos_green.restype = ctypes.c_int
         L.or_create_destroy.argtypes = [ctypes.c_int] * 5 + [dptr, dptr, ctypes.c_double]

@@ -363,9 +363,64 @@ void or_reset_normal() { g_normal.reset(); }
 
 // Ground state by Lanczos: eigenvalue pass, then eigenvector replay.
 // gs: in = start vector (normalized), out = ground state. Returns 0 or -1 on error.
+// nthreads > 0: products by the OpenMP-over-rows form (the same terms grouped by output
+// row, rows split over threads); want_vec = 0 skips the eigenvector replay.
+static int lanczos_gs_impl(void* h, double accur, int max_iter, double* gs, double* E0, int* niter,
+                           double* alpha_out, double* beta_out, int nthreads, int want_vec);
+
 int or_lanczos_gs(void* h, double accur, int max_iter, double* gs, double* E0, int* niter,
                   double* alpha_out, double* beta_out) {
+  return lanczos_gs_impl(h, accur, max_iter, gs, E0, niter, alpha_out, beta_out, 0, 1);
+}
+
+int or_lanczos_gs_threads(void* h, double accur, int max_iter, double* gs, double* E0, int* niter,
+                          double* alpha_out, double* beta_out, int nthreads, int want_vec) {
+  return lanczos_gs_impl(h, accur, max_iter, gs, E0, niter, alpha_out, beta_out, nthreads, want_vec);
+}
+
+// `steps` iterations of the ground-state recurrence of src_ed/Hamiltonian/Lanczos.hpp:58-90
+// (rescale, product, alpha, axpy, beta) from a normalized psi, without the stopping test:
+// a fixed amount of work for timing. nthreads 0 takes the serial product.
+int or_lanczos_steps(void* h, const double* psi, int steps, int nthreads, double* alpha_out,
+                     double* beta_out) {
   Sector& S = *(Sector*)h;
+  const int n = (int)S.dim;
+  std::vector<double> r(psi, psi + n), q(n, 0.0);
+  double bprev = 0.0;
+  for (int k = 0; k < steps; ++k) {
+    if (k > 0) {
+      cblas_dscal(n, 1.0 / bprev, r.data(), 1);
+      cblas_dscal(n, -bprev, q.data(), 1);
+    }
+    if (nthreads > 0) {
+      omp_set_num_threads(nthreads);
+      hv_all_omp(S, r.data(), q.data());
+    } else {
+      hv_all(S, r.data(), q.data());
+    }
+    std::swap(r, q);
+    const double a = cblas_ddot(n, q.data(), 1, r.data(), 1);
+    cblas_daxpy(n, -a, q.data(), 1, r.data(), 1);
+    bprev = cblas_dnrm2(n, r.data(), 1);
+    if (alpha_out) alpha_out[k] = a;
+    if (beta_out) beta_out[k] = bprev;
+  }
+  return 0;
+}
+
+}  // extern "C"
+
+static int lanczos_gs_impl(void* h, double accur, int max_iter, double* gs, double* E0, int* niter,
+                           double* alpha_out, double* beta_out, int nthreads, int want_vec) {
+  Sector& S = *(Sector*)h;
+  auto hv = [&](const double* x, double* y) {
+    if (nthreads > 0) {
+      omp_set_num_threads(nthreads);
+      hv_all_omp(S, x, y);
+    } else {
+      hv_all(S, x, y);
+    }
+  };
   try {
     const int n = (int)S.dim;
     std::vector<double> r(gs, gs + n), q(n, 0.0), alpha, beta, energy, evector;
@@ -375,7 +430,7 @@ int or_lanczos_gs(void* h, double accur, int max_iter, double* gs, double* E0, i
         cblas_dscal(n, 1.0 / beta.back(), r.data(), 1);
         cblas_dscal(n, -beta.back(), q.data(), 1);
       }
-      hv_all(S, r.data(), q.data());
+      hv(r.data(), q.data());
       std::swap(r, q);
       double z = cblas_ddot(n, q.data(), 1, r.data(), 1);
       alpha.push_back(z);
@@ -394,12 +449,13 @@ int or_lanczos_gs(void* h, double accur, int max_iter, double* gs, double* E0, i
       if (alpha_out) alpha_out[i] = alpha[i];
       if (beta_out) beta_out[i] = beta[i];
     }
+    if (!want_vec) return 0;
     // eigenvector replay
     std::vector<double> r2(gs, gs + n), q2(n, 0.0);
     tridiag_lowest(alpha, beta, true, energy, evector);
     cblas_dscal(n, evector[0], gs, 1);
     for (size_t j = 1; j < evector.size(); ++j) {
-      hv_all(S, r2.data(), q2.data());
+      hv(r2.data(), q2.data());
       cblas_daxpy(n, -alpha[j - 1], r2.data(), 1, q2.data(), 1);
       for (int i = 0; i < n; ++i) {
         double t = r2[i];
@@ -414,6 +470,8 @@ int or_lanczos_gs(void* h, double accur, int max_iter, double* gs, double* E0, i
     return -1;
   }
 }
+
+extern "C" {
 
 // Lanczos tridiagonalization from a normalized psi (continued-fraction Green function).
 // a_out/b_out must hold max_iter+1 entries. Returns the number of a coefficients.

@@ -118,9 +118,9 @@ EXTRA = [
     ("qed_util_tridiag_eigen", c_int, [c_int, dptr, dptr, dptr]),
     ("qed_util_sym_eigen", c_int, [c_int, dptr, dptr, c_int]),
     ("qed_hv_timed", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, dptr]),
+    ("qed_lanczos_steps_timed", c_int, [c_void_p, c_void_p, c_int, dptr]),
     ("qed_batch_last_timings", c_int, [dptr, c_int]),
     ("qed_stream_rmw_timed", c_int, [c_void_p, c_void_p, c_u64, c_int, dptr]),
-    ("qed_hv_chunk_params", c_int, [c_int, c_int]),
     ("qed_util_normal_stream", c_int, [c_int, c_u64, c_int, c_int, dptr]),
 ]
 

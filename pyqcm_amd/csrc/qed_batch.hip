@@ -218,7 +218,8 @@ __global__ __launch_bounds__(kBT) void k_lz_scal(const LJob* __restrict__ d, int
   double* r = D.v[parity];
   double* q = D.v[parity ^ 1];
   const double bt = D.beta[k - 1];
-  const double inv = 1.0 / bt;
+  // a breakdown (beta = 0: the Krylov space is exhausted) leaves zero vectors, not NaN
+  const double inv = bt != 0.0 ? 1.0 / bt : 0.0;
   const uint64_t i0 = (uint64_t)(b - D.blk0) * kChunk;
   const uint64_t i1 = min(i0 + kChunk, D.n);
   for (uint64_t i = i0 + threadIdx.x; i < i1; i += kBT) {
@@ -583,8 +584,9 @@ void lanczos_gs_batch(const std::vector<qed_sector*>& S, const std::vector<doubl
     QED_HIP(hipMemcpyAsync(hb.data(), beta, hb.size() * 8, hipMemcpyDeviceToHost, stream()));
     QED_HIP(hipStreamSynchronize(stream()));
     // stopping test of the scalar Lanczos for the new steps m = k+1 .. k1 of every active
-    // sector: ritz(m) = |beta_m e_m^T y_min(T_m)|. The chunk's last step is tested first;
-    // the steps before it only when it has converged.
+    // sector, in order: ritz(m) = |beta_m e_m^T y_min(T_m)|; the sector retires at its
+    // first converged step, as the sequential solver stops there (later steps of a small
+    // sector may follow an exhausted Krylov space and are never read)
     std::vector<int> newly(nj, -1);
     parallel_for(nj, [&](int j) {
       if (!active[j]) return;
@@ -596,13 +598,13 @@ void lanczos_gs_batch(const std::vector<qed_sector*>& S, const std::vector<doubl
         tridiag_lowest(a, b, &e0, ev);
         return std::fabs(ev.back() * b.back());
       };
-      if (ritz_at(k1) > accur) return;
-      int m = k + 1;
-      while (m < k1 && ritz_at(m) > accur) ++m;
-      ritz_at(m);
-      newly[j] = m;
-      evec[j] = ev;
-      E0[j] = e0;
+      for (int m = k + 1; m <= k1; ++m)
+        if (ritz_at(m) <= accur) {
+          newly[j] = m;
+          evec[j] = ev;
+          E0[j] = e0;
+          return;
+        }
     });
     for (int j = 0; j < nj; ++j)
       if (newly[j] > 0) {
@@ -719,8 +721,9 @@ constexpr int kSc = 4 * kBM + 8;  // device scalars per job and step
 void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<const double*>& phi,
                         const std::vector<int>& p, int max_iter_BL, double accur_deflation,
                         double accur_band_lanczos, std::vector<std::vector<double>>& evals,
-                        std::vector<std::vector<double>>& Q, BatchStats* stats) {
+                        std::vector<std::vector<double>>& Q, BatchStats* stats, double min_gap) {
   const int nj = (int)S.size();
+  std::atomic<int> gap_fail{0};  // a degenerate pair found by a worker (no_degenerate_BL)
   evals.assign(nj, {});
   Q.assign(nj, {});
   std::vector<BandJob> J(nj);
@@ -846,7 +849,7 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
           double e0 = 0.0;
           bool ok = false;
           n_checks++;
-          if (!b.evec.empty() && b.ev_old2 < 1e11) {
+          if (min_gap <= 0.0 && !b.evec.empty() && b.ev_old2 < 1e11) {
             // a second try with a lower shift and more solves when the first shift was not
             // below the spectrum or the residual test failed
             const double drop = std::fabs(b.ev_old2 - b.ev_old);
@@ -872,6 +875,13 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
             sym_eigen(pj, H.data(), e.data(), true);
             e0 = e[0];
             b.evec.assign(H.begin(), H.begin() + pj);
+            // no_degenerate_BL (src_ed/Hamiltonian/Lanczos.hpp:368-370): the guard needs the
+            // second eigenvalue, so the dense eigensolver runs at every test when it is on
+            if (min_gap > 0.0 && pj > 1 && e[1] - e[0] < min_gap) {
+              gap_fail = 1;
+              b.done = true;
+              return;
+            }
           }
           const double test = std::fabs(e0 - b.ev_old);
           b.ev_old2 = b.ev_old;
@@ -908,6 +918,8 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
       b.z = z;
       proceed[i] = 1;
     });
+    if (gap_fail)
+      fail("band Lanczos: the gap between the first two eigenvalues is smaller than " + std::to_string(min_gap));
     if (stats) {
       stats->t_band_checks += std::chrono::duration<double>(std::chrono::steady_clock::now() - tc0).count();
       stats->band_checks += n_checks;

@@ -51,6 +51,10 @@ double gd(const char* n) { return G().d.at(n); }
 char gc(const char* n) { return G().c.at(n); }
 bool gb(const char* n) { return G().b.at(n); }
 
+// band Lanczos refuses a lowest pair closer than band_lanczos_minimum_gap when
+// no_degenerate_BL is set (src_ed/Hamiltonian/Lanczos.hpp:235-236,368-370); -1: no guard
+double degeneracy_gap() { return gb("no_degenerate_BL") ? gd("band_lanczos_minimum_gap") : -1.0; }
+
 constexpr double SMALL_VALUE = 1.0e-12;
 constexpr double MIDDLE_VALUE = 1.0e-8;
 constexpr double MIN_GAP = 1.0e-6;
@@ -339,7 +343,8 @@ QMat sector_qmatrix(qed_sector& T, const double* phi, int p) {
   }
   std::vector<double> e, Q;
   int M0 = (int)(14 * p * std::log(1.0 * D));
-  int m = band_lanczos(T, phi, p, M0, gi("max_iter_BL"), gd("accur_deflation"), gd("accur_band_lanczos"), e, Q);
+  int m = band_lanczos(T, phi, p, M0, gi("max_iter_BL"), gd("accur_deflation"), gd("accur_band_lanczos"), e, Q,
+                       degeneracy_gap());
   q.e = e;
   q.Q = Q;
   (void)m;
@@ -918,7 +923,7 @@ void batch_solve(std::vector<Instance*>& insts, bool graph, int chunk, BatchStat
   if (!S.empty()) {
     std::vector<std::vector<double>> ev, Q;
     band_lanczos_batch(S, phi, pv, gi("max_iter_BL"), gd("accur_deflation"), gd("accur_band_lanczos"), ev, Q,
-                       stats);
+                       stats, degeneracy_gap());
     for (size_t u = 0; u < which.size(); ++u) {
       QMat& q = res[which[u]];
       q.L = pv[u];

@@ -417,12 +417,17 @@ __global__ __launch_bounds__(256) void k_hv_tileB_batch(const HvJob* __restrict_
 
 // Same product, two adjacent columns per thread: 16-byte loads of the down-spin neighbour
 // rows and of the ELL entry pairs (needs an even up dimension).
-template <int BS, bool DOWN = true>
+// LZ (one Lanczos step fused into the product): y <- lz[0] (H x) + lz[1] y, and the
+// workgroup's part of <x, y_new> in lz_part[row] (the scalars are device values, so that
+// a step needs no host round trip before it starts).
+template <int BS, bool DOWN = true, bool LZ = false>
 __global__ __launch_bounds__(BS) void k_hv_ell2(HvView v, const uint32_t* __restrict__ up_ell,
                                                 uint32_t Wu, const uint32_t* __restrict__ dn_ell,
                                                 uint32_t Wd, const double* __restrict__ x,
                                                 double* __restrict__ y,
-                                                const uint32_t* __restrict__ order = nullptr) {
+                                                const uint32_t* __restrict__ order = nullptr,
+                                                const double* __restrict__ lz = nullptr,
+                                                double* __restrict__ lz_part = nullptr) {
   extern __shared__ double smem[];
   const uint32_t du = v.du;
   const uint32_t nvu = v.up_nvals;
@@ -441,6 +446,7 @@ __global__ __launch_bounds__(BS) void k_hv_ell2(HvView v, const uint32_t* __rest
   const uint32_t dgroups = (dcnt + kG - 1) / kG * kG;
   const uint32_t* dl = dn_ell + (uint64_t)r * Wd;
   double2* yr = reinterpret_cast<double2*>(y + (uint64_t)r * du);
+  double dotp = 0.0;
   for (uint32_t c2 = threadIdx.x; c2 < dh; c2 += BS) {
     const uint32_t c = 2 * c2;
     double a0 = (dg + v.diag_up[c] + interaction(v, v.bup[c], sd)) * xs[c];
@@ -473,120 +479,19 @@ __global__ __launch_bounds__(BS) void k_hv_ell2(HvView v, const uint32_t* __rest
       }
     }
     double2 yo = yr[c2];
-    yo.x += a0 + b0;
-    yo.y += a1 + b1;
+    if (LZ) {
+      yo.x = lz[0] * (a0 + b0) + lz[1] * yo.x;
+      yo.y = lz[0] * (a1 + b1) + lz[1] * yo.y;
+      dotp += xs[c] * yo.x + xs[c + 1] * yo.y;
+    } else {
+      yo.x += a0 + b0;
+      yo.y += a1 + b1;
+    }
     yr[c2] = yo;
   }
-}
-
-// Down-spin term alone, by column chunks of W doubles: Y[r, C] += sum_k t_k X[r'_k, C].
-// Block w runs on XCD w % 8; the XCD's blocks take the chunks congruent to it one after
-// the other, so a chunk of X (dd rows x W columns) is served from that XCD's L2 to all
-// the rows that need it. Each lane owns two adjacent columns (16-byte loads) of one row;
-// eight neighbour loads are in flight per lane. Needs an even up dimension.
-template <int W>
-__global__ __launch_bounds__(256) void k_hv_down_ell_chunks(HvView v, const uint32_t* __restrict__ dn_ell,
-                                                           uint32_t Wd, const double* __restrict__ x,
-                                                           double* __restrict__ y, uint32_t nchunks,
-                                                           uint32_t rows_per_block, uint32_t nrb) {
-  constexpr int LPR = W / 2;      // lanes per row
-  constexpr int RPW = 64 / LPR;   // rows per wave
-  const uint32_t w = blockIdx.x;
-  const uint32_t xcd = w & 7u, kk = w >> 3;
-  const uint32_t chunk = (kk / nrb) * 8u + xcd;
-  const uint32_t rb = kk % nrb;
-  if (chunk >= nchunks) return;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const uint32_t du = v.du, dd = v.dd;
-  const uint32_t c = chunk * W + 2u * (uint32_t)(lane % LPR);
-  const uint32_t sub = (uint32_t)(lane / LPR);
-  const uint32_t r0 = rb * rows_per_block;
-  const uint32_t r1 = min(r0 + rows_per_block, dd);
-  if (c >= du) return;
-  for (uint32_t r = r0 + (uint32_t)wid * RPW + sub; r < r1; r += 4u * RPW) {
-    const uint32_t dcnt = v.dn_ptr[r + 1] - v.dn_ptr[r];
-    const uint32_t dgroups = (dcnt + kG - 1) / kG * kG;
-    const uint32_t* dl = dn_ell + (uint64_t)r * Wd;
-    double b0 = 0.0, b1 = 0.0;
-    for (uint32_t g = 0; g < dgroups; g += kG) {
-      uint32_t e[kG];
-      double2 xv[kG];
-#pragma unroll
-      for (int q = 0; q < kG; ++q) e[q] = dl[g + q];
-#pragma unroll
-      for (int q = 0; q < kG; ++q)
-        xv[q] = *reinterpret_cast<const double2*>(x + (uint64_t)(e[q] & kTargetMask) * du + c);
-#pragma unroll
-      for (int q = 0; q < kG; ++q) {
-        const double cf = v.dn_vals[e[q] >> kTargetBits];
-        b0 += cf * xv[q].x;
-        b1 += cf * xv[q].y;
-      }
-    }
-    double2* yp = reinterpret_cast<double2*>(y + (uint64_t)r * du + c);
-    double2 yo = *yp;
-    yo.x += b0;
-    yo.y += b1;
-    *yp = yo;
-  }
-}
-
-// Hop-only pass over rows of length n with an in-row hop table (column-major ELL, W
-// entries per position): z[r, :] = H_hops x[r, :] (assignment). On the transposed vector
-// (rows = up states, positions = down states) with the down-spin table this is the
-// down-spin term. Two positions per lane; n even.
-template <int BS>
-__global__ __launch_bounds__(BS) void k_row_hops2(uint32_t n, const uint32_t* __restrict__ ell, uint32_t W,
-                                                 const double* __restrict__ vals, uint32_t nv,
-                                                 const double* __restrict__ x, double* __restrict__ z) {
-  extern __shared__ double smem[];
-  double* vals_s = smem;
-  double* xs = smem + ((nv + 1u) & ~1u);
-  const uint32_t r = blockIdx.x;
-  const double* xr = x + (uint64_t)r * n;
-  const uint32_t nh = n / 2;
-  for (uint32_t i = threadIdx.x; i < nv; i += BS) vals_s[i] = vals[i];
-  for (uint32_t c = threadIdx.x; c < nh; c += BS)
-    reinterpret_cast<double2*>(xs)[c] = reinterpret_cast<const double2*>(xr)[c];
-  __syncthreads();
-  double2* zr = reinterpret_cast<double2*>(z + (uint64_t)r * n);
-  for (uint32_t c2 = threadIdx.x; c2 < nh; c2 += BS) {
-    const uint32_t c = 2 * c2;
-    double a0 = 0.0, a1 = 0.0;
-    for (uint32_t g = 0; g < W; g += kG) {
-      uint2 e[kG];
-#pragma unroll
-      for (int q = 0; q < kG; ++q) e[q] = *reinterpret_cast<const uint2*>(ell + (uint64_t)(g + q) * n + c);
-#pragma unroll
-      for (int q = 0; q < kG; ++q) {
-        a0 += vals_s[e[q].x >> kTargetBits] * xs[e[q].x & kTargetMask];
-        a1 += vals_s[e[q].y >> kTargetBits] * xs[e[q].y & kTargetMask];
-      }
-    }
-    zr[c2] = make_double2(a0, a1);
-  }
-}
-
-// out (cols x rows) = transpose of in (rows x cols), or out += it (ADD), by 64 x 64 tiles
-// staged in LDS (padded against bank conflicts)
-template <bool ADD>
-__global__ __launch_bounds__(256) void k_transpose(const double* __restrict__ in, double* __restrict__ out,
-                                                   uint32_t rows, uint32_t cols) {
-  __shared__ double tile[64][65];
-  const uint32_t c0 = blockIdx.x * 64u, r0 = blockIdx.y * 64u;
-  const uint32_t tx = threadIdx.x & 63u, ty = threadIdx.x >> 6;
-  for (uint32_t i = ty; i < 64u; i += 4u) {
-    const uint32_t r = r0 + i, c = c0 + tx;
-    if (r < rows && c < cols) tile[i][tx] = in[(uint64_t)r * cols + c];
-  }
-  __syncthreads();
-  for (uint32_t i = ty; i < 64u; i += 4u) {
-    const uint32_t c = c0 + i, r = r0 + tx;
-    if (c < cols && r < rows) {
-      double* o = out + (uint64_t)c * rows + r;
-      if (ADD) *o += tile[tx][i];
-      else *o = tile[tx][i];
-    }
+  if (LZ) {
+    const double t = block_sum(dotp, smem + ((nvu + 1u) & ~1u) + du);
+    if (threadIdx.x == 0) lz_part[r] = t;
   }
 }
 
@@ -808,9 +713,9 @@ static void set_lds_attributes() {
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     (void)hipFuncSetAttribute((const void*)k_hv_ell2<512, false>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-
-    (void)hipFuncSetAttribute((const void*)k_row_hops2<512>,
+    (void)hipFuncSetAttribute((const void*)k_hv_ell2<512, true, true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+
 
   });
 }
@@ -821,10 +726,8 @@ static void set_lds_attributes() {
 //   4  one workgroup per row, ELL tables, 8-deep load groups, one column per thread
 //   5  as 4 with two columns per thread and 16-byte loads (default; falls back to 4 when the
 //      up dimension is odd or small, or the row does not fit in LDS)
-//   6  two passes: variant 5 without the down-spin term, then the down-spin term by
-//      XCD-local column chunks served from L2 (falls back to 5 where 5 falls back)
-//   10 the down-spin term on the transposed vector: y += (up + diag) x by rows,
-//      x^T by LDS tiles, (down hops) x^T by rows, y += its transpose (falls back to 5)
+//   12 variant 5 visiting the rows in the tiled order below (default)
+//   20 the split-halves passes (qed_hvsplit.hip; 21, 22: one pass alone, for timing)
 // Rows of the down-spin half basis in 2D tiles: a state splits into its low and high
 // halves of orbitals (a, b); rows are grouped by the particle number of b, then visited
 // in T x T tiles of (rank of b, rank of a). Hops inside one half change only a or only b.
@@ -866,8 +769,14 @@ std::vector<uint32_t> tiled_row_order(const qed_sector& s, int T) {
   return ord;
 }
 
-int g_chunk_w = 32;      // column-chunk width of variant 6 (16, 32 or 64 doubles)
-int g_chunk_rows = 64;   // rows per workgroup of variant 6
+// the tiled row order of the default row kernel, built once per sector (at creation for
+// the sectors that take that kernel, so that later products allocate nothing)
+void ensure_row_order(const qed_sector& s) {
+  if (s.row_order.p) return;
+  std::vector<uint32_t> ord = tiled_row_order(s, 16);
+  s.row_order.alloc(ord.size() * sizeof(uint32_t));
+  QED_HIP(hipMemcpy(s.row_order.p, ord.data(), ord.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+}
 
 void hv(const qed_sector& s, const double* x, double* y, int variant) {
   if (s.D == 0) return;
@@ -883,58 +792,13 @@ void hv(const qed_sector& s, const double* x, double* y, int variant) {
   hipStream_t st = stream();
   const uint32_t* ue = s.up_ell.as<uint32_t>();
   const uint32_t* de = s.dn_ell.as<uint32_t>();
-  if (variant == 6) {
-    const size_t lds2 = ((size_t)s.du + s.up_nvals + 1) * sizeof(double);
-    if (s.du % 2 == 0 && s.du >= 4096 && lds2 <= lds_budget()) {
-      k_hv_ell2<512, false><<<s.dd, 512, lds2, st>>>(v, ue, s.up_w, de, s.dn_w, x, y);
-      const uint32_t W = (uint32_t)g_chunk_w, rpb = (uint32_t)g_chunk_rows;
-      const uint32_t nchunks = (s.du + W - 1) / W;
-      const uint32_t nrb = (s.dd + rpb - 1) / rpb;
-      const uint64_t blocks = (uint64_t)((nchunks + 7) / 8) * 8ull * nrb;
-      if (W == 16)
-        k_hv_down_ell_chunks<16><<<(uint32_t)blocks, 256, 0, st>>>(v, de, s.dn_w, x, y, nchunks, rpb, nrb);
-      else if (W == 64)
-        k_hv_down_ell_chunks<64><<<(uint32_t)blocks, 256, 0, st>>>(v, de, s.dn_w, x, y, nchunks, rpb, nrb);
-      else
-        k_hv_down_ell_chunks<32><<<(uint32_t)blocks, 256, 0, st>>>(v, de, s.dn_w, x, y, nchunks, rpb, nrb);
-      QED_HIP(hipGetLastError());
-      return;
-    }
-    variant = 5;
-  }
-  if (variant == 10) {
-    // up-spin term and diagonal by rows, the down-spin term on the transposed vector:
-    // y += (up + diag) x; xt = x^T; zt = (down hops) xt by rows; y += zt^T
-    const size_t lds_u = ((size_t)s.du + s.up_nvals + 1) * sizeof(double);
-    const size_t lds_d = ((size_t)s.dd + s.dn_nvals + 1) * sizeof(double);
-    if (s.du % 2 == 0 && s.dd % 2 == 0 && s.du >= 4096 && lds_u <= lds_budget() && lds_d <= lds_budget() &&
-        s.dn_ell_cm.p) {
-      if (s.xt.bytes < s.D * 8) s.xt.alloc(s.D * 8);
-      if (s.zt.bytes < s.D * 8) s.zt.alloc(s.D * 8);
-      double* xt = s.xt.as<double>();
-      double* zt = s.zt.as<double>();
-      k_hv_ell2<512, false><<<s.dd, 512, lds_u, st>>>(v, ue, s.up_w, de, s.dn_w, x, y);
-      const dim3 g1((s.du + 63) / 64, (s.dd + 63) / 64), g2((s.dd + 63) / 64, (s.du + 63) / 64);
-      k_transpose<false><<<g1, 256, 0, st>>>(x, xt, s.dd, s.du);
-      k_row_hops2<512><<<s.du, 512, lds_d, st>>>(s.dd, s.dn_ell_cm.as<uint32_t>(), s.dn_w_cm,
-                                                 s.dn_vals.as<double>(), s.dn_nvals, xt, zt);
-      k_transpose<true><<<g2, 256, 0, st>>>(zt, y, s.du, s.dd);
-      QED_HIP(hipGetLastError());
-      return;
-    }
-    variant = 5;
-  }
   if (variant == 12) {
     // the row kernel visiting the rows in 2D tiles over the two halves of the down-spin
     // orbitals (neighbour rows of a tile share either half); measured on the 16-site
     // sector: tile side 16 gives 5.01 ms against 5.19 ms in rank order, identical results
     const size_t lds2 = ((size_t)s.du + s.up_nvals + 1) * sizeof(double);
     if (s.du % 2 == 0 && s.du >= 4096 && lds2 <= lds_budget()) {
-      if (!s.row_order.p) {
-        std::vector<uint32_t> ord = tiled_row_order(s, 16);
-        s.row_order.alloc(ord.size() * sizeof(uint32_t));
-        QED_HIP(hipMemcpy(s.row_order.p, ord.data(), ord.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-      }
+      ensure_row_order(s);
       k_hv_ell2<512><<<s.dd, 512, lds2, st>>>(v, ue, s.up_w, de, s.dn_w, x, y, s.row_order.as<uint32_t>());
       QED_HIP(hipGetLastError());
       return;
@@ -991,6 +855,22 @@ void hv(const qed_sector& s, const double* x, double* y, int variant) {
   QED_HIP(hipGetLastError());
 }
 
+
+bool hv_lanczos(const qed_sector& s, const double* x, double* y, const double* lz, double* lz_part) {
+  if (s.D == 0 || s.hv_variant >= 0) return false;
+  // the default row kernel (tiled row order); the LZ epilogue needs LDS for the row, the
+  // amplitudes and eight doubles of reduction scratch
+  const size_t lds2 = ((size_t)s.du + s.up_nvals + 1 + 16) * sizeof(double);
+  if (s.du % 2 != 0 || s.du < 4096 || lds2 > lds_budget()) return false;
+  set_lds_attributes();
+  ensure_row_order(s);
+  HvView v = view(s);
+  k_hv_ell2<512, true, true><<<s.dd, 512, lds2, stream()>>>(v, s.up_ell.as<uint32_t>(), s.up_w,
+                                                          s.dn_ell.as<uint32_t>(), s.dn_w, x, y,
+                                                          s.row_order.as<uint32_t>(), lz, lz_part);
+  QED_HIP(hipGetLastError());
+  return true;
+}
 
 // ---- batched launch with autotuned (block size, grid) ------------------------------------
 namespace {
@@ -1411,10 +1291,6 @@ int qed_sector_create(int L, int n_up, int n_dn, int n_ops, const int* types, co
     if (!s->hubbard_uniform) s->hub_sites = 0;
     auto hu = half_struct(L, n_up, bu, true);
     auto hd = half_struct(L, n_dn, bd, false);
-    auto hdc = half_struct(L, n_dn, bd, true);
-    s->keep_dn_cm = hdc;
-    s->dn_ell_cm = DevBuf::view(hdc->ell.p, hdc->ell.bytes);
-    s->dn_w_cm = hdc->w;
     s->keep_up = hu;
     s->keep_dn = hd;
     s->du = hu->dim;
@@ -1471,6 +1347,8 @@ int qed_sector_create(int L, int n_up, int n_dn, int n_ops, const int* types, co
     s->int_val_d = DevBuf::view(base + nvu + nvd + s->du + s->dd, nint * 8);
     s->int_mask_d = DevBuf::view(base + nd, nint * 8);
     QED_HIP(hipStreamSynchronize(stream()));
+    if (s->du % 2 == 0 && s->du >= 4096 && ((size_t)s->du + s->up_nvals + 1) * sizeof(double) <= lds_budget())
+      ensure_row_order(*s);
     *out = s.release();
   });
 }
@@ -1492,16 +1370,6 @@ int qed_sector_dims(const qed_sector* s, uint64_t* dim, uint32_t* du, uint32_t* 
 
 int qed_hv(qed_sector* s, const double* x, double* y) {
   return guard([&] { hv(*s, x, y); });
-}
-
-// sets the column-chunk width and rows per workgroup of the two-pass kernel (variant 6)
-int qed_hv_chunk_params(int width, int rows) {
-  return guard([&] {
-    if (width != 16 && width != 32 && width != 64) fail("chunk width must be 16, 32 or 64");
-    if (rows < 4 || rows > 4096) fail("rows per workgroup out of range");
-    g_chunk_w = width;
-    g_chunk_rows = rows;
-  });
 }
 
 int qed_hv_variant(qed_sector* s, int variant, const double* x, double* y) {

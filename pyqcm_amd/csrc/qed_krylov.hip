@@ -34,7 +34,7 @@ uint32_t red_blocks(uint64_t n) {
 }
 
 // scratch for block partials and final scalars (library calls are serialized)
-DevBuf& scratch() {
+DevBuf& red_scratch() {
   static DevBuf b((size_t)kMaxBlocks * (kMaxMulti + 1) * sizeof(double) + 4096);
   return b;
 }
@@ -80,6 +80,12 @@ __global__ void k_axpy(double a, const double* __restrict__ x, double* __restric
     y[i] += a * x[i];
 }
 
+__global__ void k_zero(double* __restrict__ x, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    x[i] = 0.0;
+}
+
 __global__ void k_scal(double a, double* __restrict__ x, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -100,6 +106,53 @@ __global__ __launch_bounds__(RB) void k_axpy_nrm(double a, const double* __restr
                                                  double* __restrict__ y, uint64_t n,
                                                  double* __restrict__ part) {
   __shared__ double sh[RB / 64];
+  double acc = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * RB) {
+    double v = y[i] + a * x[i];
+    y[i] = v;
+    acc += v * v;
+  }
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// ---- fused Lanczos step (device scalars) ---------------------------------------------
+// sc: [0] 1/beta_{k-1}, [1] -beta_{k-1}/beta_{k-2} (the product's scalars), [2] alpha_k,
+// [3] beta_k, [4] the axpy coefficient -alpha_k / beta_{k-1}, [5] beta_{k-1}
+// alpha_k = sc[0] * sum(part); the axpy coefficient follows
+__global__ void k_lzf_alpha(const double* __restrict__ part, int nb, double* __restrict__ sc) {
+  __shared__ double sh[RB / 64];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += RB) s += part[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    const double a = s * sc[0];
+    sc[2] = a;
+    sc[4] = -a * sc[0];
+  }
+}
+
+// beta_k = sqrt(sum(part)); the next product's scalars
+__global__ void k_lzf_beta(const double* __restrict__ part, int nb, double* __restrict__ sc) {
+  __shared__ double sh[RB / 64];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += RB) s += part[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    const double b = sqrt(s);
+    sc[3] = b;
+    sc[1] = -b * sc[0];  // -beta_k / beta_{k-1}
+    sc[0] = b != 0.0 ? 1.0 / b : 0.0;
+    sc[5] = b;
+  }
+}
+
+// y += sc[4] x, partial sums of y^2
+__global__ __launch_bounds__(RB) void k_lzf_axpy_nrm(const double* __restrict__ sc, const double* __restrict__ x,
+                                                     double* __restrict__ y, uint64_t n,
+                                                     double* __restrict__ part) {
+  __shared__ double sh[RB / 64];
+  const double a = sc[4];
   double acc = 0.0;
   for (uint64_t i = (uint64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * RB) {
     double v = y[i] + a * x[i];
@@ -243,7 +296,7 @@ std::recursive_mutex& lib_mutex() {
   return m;
 }
 
-void multi_dot(const double* x, const double* const* ys, int m, uint64_t n, double* out) {
+void multi_dot(const double* x, const double* const* ys, int m, uint64_t n, double* out, double* work) {
   if (m > kMaxMulti) fail("multi_dot: too many vectors");
   if (m == 0) return;
   if (n == 0) {
@@ -253,7 +306,7 @@ void multi_dot(const double* x, const double* const* ys, int m, uint64_t n, doub
   PtrList l{};
   for (int k = 0; k < m; ++k) l.p[k] = ys[k];
   uint32_t nb = red_blocks(n);
-  double* part = scratch().as<double>();
+  double* part = work ? work : red_scratch().as<double>();
   double* fin = part + (size_t)kMaxBlocks * kMaxMulti;
   k_multi_dot<<<nb, RB, 0, stream()>>>(x, l, m, n, part);
   k_finish<<<1, RB, 0, stream()>>>(part, nb, m, fin);
@@ -261,9 +314,9 @@ void multi_dot(const double* x, const double* const* ys, int m, uint64_t n, doub
   QED_HIP(hipStreamSynchronize(stream()));
 }
 
-double dot(const double* x, const double* y, uint64_t n) {
+double dot(const double* x, const double* y, uint64_t n, double* work) {
   double r;
-  multi_dot(x, &y, 1, n, &r);
+  multi_dot(x, &y, 1, n, &r, work);
   return r;
 }
 
@@ -279,9 +332,9 @@ void scal2(double ar, double* r, double aq, double* q, uint64_t n) {
   if (n) k_scal2<<<stream_blocks(n), 256, 0, stream()>>>(ar, r, aq, q, n);
 }
 
-double axpy_nrm2(double a, const double* x, double* y, uint64_t n) {
+double axpy_nrm2(double a, const double* x, double* y, uint64_t n, double* work) {
   uint32_t nb = red_blocks(n);
-  double* part = scratch().as<double>();
+  double* part = work ? work : red_scratch().as<double>();
   double* fin = part + (size_t)kMaxBlocks * kMaxMulti;
   k_axpy_nrm<<<nb, RB, 0, stream()>>>(a, x, y, n, part);
   k_finish<<<1, RB, 0, stream()>>>(part, nb, 1, fin);
@@ -296,7 +349,7 @@ void multi_axpy(const double* c, const double* const* xs, int m, double* y, uint
   if (m > kMaxMulti) fail("multi_axpy: too many vectors");
   PtrList l{};
   for (int k = 0; k < m; ++k) l.p[k] = xs[k];
-  double* cd = scratch().as<double>() + (size_t)kMaxBlocks * kMaxMulti + 64;
+  double* cd = red_scratch().as<double>() + (size_t)kMaxBlocks * kMaxMulti + 64;
   QED_HIP(hipMemcpyAsync(cd, c, m * sizeof(double), hipMemcpyHostToDevice, stream()));
   k_multi_axpy<<<stream_blocks(n), 256, 0, stream()>>>(l, cd, m, y, n);
   QED_HIP(hipStreamSynchronize(stream()));
@@ -307,45 +360,123 @@ void multi_axpy_rev(const double* c, const double* x, double* const* ys, int m, 
   if (m > kMaxMulti) fail("multi_axpy_rev: too many vectors");
   MutPtrList l{};
   for (int k = 0; k < m; ++k) l.p[k] = ys[k];
-  double* cd = scratch().as<double>() + (size_t)kMaxBlocks * kMaxMulti + 64;
+  double* cd = red_scratch().as<double>() + (size_t)kMaxBlocks * kMaxMulti + 64;
   QED_HIP(hipMemcpyAsync(cd, c, m * sizeof(double), hipMemcpyHostToDevice, stream()));
   k_multi_axpy_rev<<<stream_blocks(n), 256, 0, stream()>>>(l, cd, m, x, n);
   QED_HIP(hipStreamSynchronize(stream()));
 }
 
-void lanczos_gs(qed_sector& s, double accur, int max_iter, double* psi, double* E0, int* niter,
-                std::vector<double>* alpha_out, std::vector<double>* beta_out) {
+// The eigenvalue pass of the Lanczos ground state, one fused step per iteration where the
+// sector takes the fused product: with w_k = beta_{k-1} v_k kept unnormalized,
+//   y = H v_k - beta_{k-1} v_{k-1} = (H w_k) / beta_{k-1} - (beta_{k-1} / beta_{k-2}) w_{k-1}
+// is one pass of the product kernel (which also sums <w_k, y>), and
+//   w_{k+1} = y - (alpha_k / beta_{k-1}) w_k,  beta_k = |w_{k+1}|
+// one more streaming pass: 24 D bytes beyond the product instead of 72 D for the separate
+// scale, dot and axpy passes, and one scalar download per step.
+bool lanczos_eigen_fused(qed_sector& s, double accur, int max_iter, const double* psi, std::vector<double>& alpha,
+                         std::vector<double>& beta, std::vector<double>& evec, double& e0,
+                         const LanczosWork* ws) {
   const uint64_t n = s.D;
-  DevBuf rb(n * 8), qb(n * 8);
-  double* r = rb.as<double>();
-  double* q = qb.as<double>();
+  if (ws && s.dd > kFusedPartDoubles) return false;
+  DevBuf rb, qb, sc, part;
+  double *r, *q, *scd, *pd;
+  if (ws) {
+    r = ws->r;
+    q = ws->q;
+    scd = ws->sc;
+    pd = ws->part;
+  } else {
+    rb.alloc(n * 8);
+    qb.alloc(n * 8);
+    sc.alloc(8 * sizeof(double));
+    part.alloc((size_t)std::max<uint32_t>(s.dd, kMaxBlocks) * 8);
+    r = rb.as<double>();
+    q = qb.as<double>();
+    scd = sc.as<double>();
+    pd = part.as<double>();
+  }
   QED_HIP(hipMemcpyAsync(r, psi, n * 8, hipMemcpyDeviceToDevice, stream()));
-  QED_HIP(hipMemsetAsync(q, 0, n * 8, stream()));
-  std::vector<double> alpha, beta, evec;
-  double e0 = 0.0;
+  if (ws) k_zero<<<stream_blocks(n), 256, 0, stream()>>>(q, n);
+  else QED_HIP(hipMemsetAsync(q, 0, n * 8, stream()));
+  const double init[8] = {1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0};
+  QED_HIP(hipMemcpyAsync(scd, init, sizeof(init), hipMemcpyHostToDevice, stream()));
   double ritz = 2.0 * accur;
+  double ab[2];
+  const uint32_t nb = red_blocks(n);
   while (ritz > accur) {
-    if (!beta.empty()) scal2(1.0 / beta.back(), r, -beta.back(), q, n);
-    hv(s, r, q);
+    if (!hv_lanczos(s, r, q, scd, pd)) {
+      if (alpha.empty()) return false;
+      fail("fused Lanczos product unavailable after the first step");
+    }
+    k_lzf_alpha<<<1, RB, 0, stream()>>>(pd, (int)s.dd, scd);
+    k_lzf_axpy_nrm<<<nb, RB, 0, stream()>>>(scd, r, q, n, pd);
+    k_lzf_beta<<<1, RB, 0, stream()>>>(pd, (int)nb, scd);
+    QED_HIP(hipMemcpyAsync(ab, scd + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    QED_HIP(hipStreamSynchronize(stream()));
     std::swap(r, q);
-    double a = dot(q, r, n);
-    alpha.push_back(a);
-    beta.push_back(axpy_nrm2(-a, q, r, n));
+    alpha.push_back(ab[0]);
+    beta.push_back(ab[1]);
     tridiag_lowest(alpha, beta, &e0, evec);
     ritz = std::fabs(evec.back() * beta.back());
     if ((int)alpha.size() > max_iter)
       fail("Lanczos procedure exceeded " + std::to_string(max_iter) + " iterations");
+  }
+  return true;
+}
+
+void lanczos_gs(qed_sector& s, double accur, int max_iter, double* psi, double* E0, int* niter,
+                std::vector<double>* alpha_out, std::vector<double>* beta_out, const LanczosWork* ws) {
+  const uint64_t n = s.D;
+  // with a workspace: the two recurrence vectors and the reduction partials are the
+  // caller's, zero fills are kernels, and the product is one that allocates nothing (the
+  // split-halves form builds its tables on first use, so the default row kernel instead)
+  const int var = ws && s.hv_variant >= 20 ? 12 : -1;
+  std::vector<double> alpha, beta, evec;
+  double e0 = 0.0;
+  DevBuf rb, qb;
+  double *r = nullptr, *q = nullptr;
+  auto vectors = [&] {
+    if (ws) {
+      r = ws->r;
+      q = ws->q;
+    } else if (!rb.p) {
+      rb.alloc(n * 8);
+      qb.alloc(n * 8);
+      r = rb.as<double>();
+      q = qb.as<double>();
+    }
+  };
+  auto reset = [&] {
+    vectors();
+    QED_HIP(hipMemcpyAsync(r, psi, n * 8, hipMemcpyDeviceToDevice, stream()));
+    if (ws) k_zero<<<stream_blocks(n), 256, 0, stream()>>>(q, n);
+    else QED_HIP(hipMemsetAsync(q, 0, n * 8, stream()));
+  };
+  if (var >= 0 || !lanczos_eigen_fused(s, accur, max_iter, psi, alpha, beta, evec, e0, ws)) {
+    reset();
+    double ritz = 2.0 * accur;
+    while (ritz > accur) {
+      if (!beta.empty()) scal2(1.0 / beta.back(), r, -beta.back(), q, n);
+      hv(s, r, q, var);
+      std::swap(r, q);
+      double a = dot(q, r, n, ws ? ws->red : nullptr);
+      alpha.push_back(a);
+      beta.push_back(axpy_nrm2(-a, q, r, n, ws ? ws->red : nullptr));
+      tridiag_lowest(alpha, beta, &e0, evec);
+      ritz = std::fabs(evec.back() * beta.back());
+      if ((int)alpha.size() > max_iter)
+        fail("Lanczos procedure exceeded " + std::to_string(max_iter) + " iterations");
+    }
   }
   *E0 = e0;
   *niter = (int)beta.size();
   if (alpha_out) *alpha_out = alpha;
   if (beta_out) *beta_out = beta;
   // eigenvector replay from the saved start vector
-  QED_HIP(hipMemcpyAsync(r, psi, n * 8, hipMemcpyDeviceToDevice, stream()));
-  QED_HIP(hipMemsetAsync(q, 0, n * 8, stream()));
+  reset();
   scal(evec[0], psi, n);
   for (size_t j = 1; j < evec.size(); ++j) {
-    hv(s, r, q);
+    hv(s, r, q, var);
     k_replay<<<stream_blocks(n), 256, 0, stream()>>>(alpha[j - 1], beta[j - 1], evec[j], r, q, psi, n);
   }
   QED_HIP(hipStreamSynchronize(stream()));
@@ -374,7 +505,7 @@ struct VecPool {
 };
 
 int band_lanczos(qed_sector& s, const double* phi, int p0, int M0, int max_iter_BL, double accur_deflation,
-                 double accur_band_lanczos, std::vector<double>& evals, std::vector<double>& Q) {
+                 double accur_band_lanczos, std::vector<double>& evals, std::vector<double>& Q, double min_gap) {
   const uint64_t n = s.D;
   if (n == 0 || p0 == 0) return 0;
   if (M0 <= 0) M0 = (int)(14 * p0 * std::log(1.0 * n));
@@ -476,6 +607,9 @@ int band_lanczos(qed_sector& s, const double* phi, int p0, int M0, int max_iter_
       sym_eigen(j, Hpr.data(), evs.data(), false);
       double test = std::fabs(evs[0] - ev_old);
       ev_old = evs[0];
+      // no_degenerate_BL (src_ed/Hamiltonian/Lanczos.hpp:368-370)
+      if (min_gap > 0.0 && j > 1 && evs[1] - evs[0] < min_gap)
+        fail("band Lanczos: the gap between the first two eigenvalues is smaller than " + std::to_string(min_gap));
       if (test < accur_band_lanczos) break;
     }
     ++j;
@@ -574,7 +708,7 @@ void combine(const std::vector<double*>& in, const std::vector<double>& C, int K
   MutPtrList lo{};
   for (int p = 0; p < m; ++p) li.p[p] = in[p];
   for (int k = 0; k < K; ++k) lo.p[k] = out[k];
-  double* cd = scratch().as<double>() + (size_t)kMaxBlocks * kMaxMulti + 64;
+  double* cd = red_scratch().as<double>() + (size_t)kMaxBlocks * kMaxMulti + 64;
   if (C.size() > 512) fail("Davidson: coefficient block too large");
   QED_HIP(hipMemcpyAsync(cd, C.data(), C.size() * sizeof(double), hipMemcpyHostToDevice, stream()));
   k_combine<<<stream_blocks(n), 256, 0, stream()>>>(li, m, lo, K, cd, n);
@@ -752,6 +886,72 @@ int qed_lanczos_gs(qed_sector* s, double accur, int max_iter, double* psi, doubl
     lanczos_gs(*s, accur, max_iter, psi, E0, niter, &a, &b);
     if (alpha) std::copy(a.begin(), a.end(), alpha);
     if (beta) std::copy(b.begin(), b.end(), beta);
+  });
+}
+
+// Measurement: average milliseconds per Lanczos step of the eigenvalue pass from psi
+// (normalized), `steps` steps each (no stopping test), HIP events on the library stream:
+// ms[0] the fused step (product with the scale and dot folded in, then axpy + norm),
+// ms[1] the separate passes (scale, product, dot, axpy + norm) with their host round
+// trips. ms[0] = -1 when the sector does not take the fused product.
+int qed_lanczos_steps_timed(qed_sector* s, const double* psi, int steps, double* ms) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    const uint64_t n = s->D;
+    DevBuf rb(n * 8), qb(n * 8), sc(8 * sizeof(double)), part((size_t)std::max<uint32_t>(s->dd, kMaxBlocks) * 8);
+    double* r = rb.as<double>();
+    double* q = qb.as<double>();
+    double* scd = sc.as<double>();
+    double* pd = part.as<double>();
+    hipEvent_t e0, e1;
+    QED_HIP(hipEventCreate(&e0));
+    QED_HIP(hipEventCreate(&e1));
+    const uint32_t nb = red_blocks(n);
+    auto reset = [&] {
+      QED_HIP(hipMemcpyAsync(r, psi, n * 8, hipMemcpyDeviceToDevice, stream()));
+      QED_HIP(hipMemsetAsync(q, 0, n * 8, stream()));
+      const double init[8] = {1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0};
+      QED_HIP(hipMemcpyAsync(scd, init, sizeof(init), hipMemcpyHostToDevice, stream()));
+      QED_HIP(hipStreamSynchronize(stream()));
+    };
+    ms[0] = -1.0;
+    reset();
+    if (hv_lanczos(*s, r, q, scd, pd)) {
+      reset();
+      QED_HIP(hipEventRecord(e0, stream()));
+      double ab[2];
+      for (int k = 0; k < steps; ++k) {
+        hv_lanczos(*s, r, q, scd, pd);
+        k_lzf_alpha<<<1, RB, 0, stream()>>>(pd, (int)s->dd, scd);
+        k_lzf_axpy_nrm<<<nb, RB, 0, stream()>>>(scd, r, q, n, pd);
+        k_lzf_beta<<<1, RB, 0, stream()>>>(pd, (int)nb, scd);
+        QED_HIP(hipMemcpyAsync(ab, scd + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+        QED_HIP(hipStreamSynchronize(stream()));
+        std::swap(r, q);
+      }
+      QED_HIP(hipEventRecord(e1, stream()));
+      QED_HIP(hipEventSynchronize(e1));
+      float t = 0.f;
+      QED_HIP(hipEventElapsedTime(&t, e0, e1));
+      ms[0] = t / std::max(1, steps);
+    }
+    reset();
+    QED_HIP(hipEventRecord(e0, stream()));
+    double bprev = 0.0;
+    for (int k = 0; k < steps; ++k) {
+      if (k > 0) scal2(1.0 / bprev, r, -bprev, q, n);
+      hv(*s, r, q);
+      std::swap(r, q);
+      const double a = dot(q, r, n);
+      bprev = axpy_nrm2(-a, q, r, n);
+    }
+    QED_HIP(hipEventRecord(e1, stream()));
+    QED_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    QED_HIP(hipEventElapsedTime(&t, e0, e1));
+    ms[1] = t / std::max(1, steps);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
   });
 }
 

@@ -46,12 +46,6 @@ struct qed_sector {
   // every sector with the same half-space structure; the amplitudes are this sector's own
   std::shared_ptr<void> keep_up, keep_dn;
   qed::DevBuf valbuf;  // owns up_vals, dn_vals, diag_up, diag_dn, int_val_d
-  // down-spin hop table in the column-major ELL layout (for the transposed pass), and
-  // the two scratch vectors of that pass (allocated on first use)
-  std::shared_ptr<void> keep_dn_cm;
-  qed::DevBuf dn_ell_cm;
-  uint32_t dn_w_cm = 0;
-  mutable qed::DevBuf xt, zt;
 
   // one-body terms per spin for the split-halves H.v: hop elements (i, j, value) as read
   // from the operators, and the one-body diagonal per orbital
@@ -151,7 +145,7 @@ void lanczos_gs_batch(const std::vector<qed_sector*>& S, const std::vector<doubl
 void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<const double*>& phi,
                         const std::vector<int>& p, int max_iter_BL, double accur_deflation,
                         double accur_band_lanczos, std::vector<std::vector<double>>& evals,
-                        std::vector<std::vector<double>>& Q, BatchStats* stats);
+                        std::vector<std::vector<double>>& Q, BatchStats* stats, double min_gap = -1.0);
 void tridiag_lowest(const std::vector<double>& a, const std::vector<double>& b, double* e0,
                     std::vector<double>& vec);
 
@@ -161,28 +155,46 @@ void hv(const qed_sector& s, const double* x, double* y, int variant = -1);
 // sector's structure does not suit them (nothing launched)
 bool hv_split(const qed_sector& s, const double* x, double* y, int passes = 3);
 std::string split_status(const qed_sector& s);
+// one fused Lanczos product: y <- lz[0] (H x) + lz[1] y (device scalars) and, per row of
+// the vector, the partial sums of <x, y_new> in lz_part (dd doubles); false when the
+// sector does not take the fused kernel (nothing launched)
+bool hv_lanczos(const qed_sector& s, const double* x, double* y, const double* lz, double* lz_part);
 
-// BLAS-1 on the library stream. dot/nrm2 return host scalars (synchronizing).
-double dot(const double* x, const double* y, uint64_t n);
+// BLAS-1 on the library stream. dot/nrm2 return host scalars (synchronizing). `work`, where
+// given, is device memory for the block partials (kMaxBlocks * 65 doubles); the library's
+// own buffer otherwise.
+double dot(const double* x, const double* y, uint64_t n, double* work = nullptr);
 double nrm2(const double* x, uint64_t n);
 void axpy(double a, const double* x, double* y, uint64_t n);
 void scal(double a, double* x, uint64_t n);
 // r *= ar; q *= aq (one pass)
 void scal2(double ar, double* r, double aq, double* q, uint64_t n);
 // y += a x and returns ||y|| (one pass, synchronizing)
-double axpy_nrm2(double a, const double* x, double* y, uint64_t n);
+double axpy_nrm2(double a, const double* x, double* y, uint64_t n, double* work = nullptr);
 // multi-dot: out[k] = <x, ys[k]> for k < m (one pass over x)
-void multi_dot(const double* x, const double* const* ys, int m, uint64_t n, double* out);
+void multi_dot(const double* x, const double* const* ys, int m, uint64_t n, double* out, double* work = nullptr);
 // y -= sum_k c[k] xs[k] (one pass)
 void multi_axpy(const double* c, const double* const* xs, int m, double* y, uint64_t n);
 // ys[k] -= c[k] x for each k (one pass over x)
 void multi_axpy_rev(const double* c, const double* x, double* const* ys, int m, uint64_t n);
 
+// device memory of a Lanczos run supplied by the caller: the two recurrence vectors (D
+// doubles each), the BLAS-1 block partials (kRedDoubles), the fused step's scalars (8) and
+// row partials (kFusedPartDoubles; the fused step needs max(dd, 2048))
+struct LanczosWork {
+  double* r;
+  double* q;
+  double* red;
+  double* sc;
+  double* part;
+};
+constexpr size_t kRedDoubles = 2048 * 65 + 512;
+constexpr size_t kFusedPartDoubles = 2048;
 void lanczos_gs(qed_sector& s, double accur, int max_iter, double* psi, double* E0, int* niter,
-                std::vector<double>* alpha, std::vector<double>* beta);
+                std::vector<double>* alpha, std::vector<double>* beta, const LanczosWork* ws = nullptr);
 int band_lanczos(qed_sector& s, const double* phi, int p0, int M0, int max_iter_BL,
                  double accur_deflation, double accur_band_lanczos, std::vector<double>& evals,
-                 std::vector<double>& Q);
+                 std::vector<double>& Q, double min_gap = -1.0);
 void lanczos_green(qed_sector& s, const double* psi, double accur_cf, int max_iter,
                    std::vector<double>& a, std::vector<double>& b);
 // block Lanczos on p orthonormal device vectors q0 (p x D); floors A, B (p x p each,

@@ -137,3 +137,89 @@ def test_batched_hv_forms_match_single_sector(gpu_lib):
             yd.free()
     for xd in X:
         xd.free()
+
+
+def _register(model, values, sectors, n_sites=4):
+    from pyqcm_amd import qcm
+    name = "plb%d" % next(_names)
+    qcm.new_model(name, n_sites, model.L - n_sites)
+    for opname, op in model.operators.items():
+        qcm.new_operator(name, opname, op.kind, [(r + 1, c + 1, v) for (r, c, v) in op.elements])
+    label = 9000 + next(_names)
+    qcm.new_model_instanceC(name, values, sectors, label)
+    return label
+
+
+def test_batch_small_sectors_match_sequential(gpu_lib):
+    """Sectors smaller than one stopping-test chunk (D = 1, 12, 66, 144) beside a large one:
+    the batched Lanczos retires each at its first converged step, with no NaN from the
+    exhausted Krylov spaces that follow, and gives the sequential solver's energies."""
+    from pyqcm_amd import _lib, qcm
+    from tests import models as TM
+    rng = np.random.default_rng(17)
+    p = TM.random_bath_params(rng)
+    m, vals = TM.plaquette_bath(p)
+    sectors = ["R0:N0:S0", "R0:N1:S1", "R0:N2:S0", "R0:N23:S1", "R0:N24:S0", "R0:N12:S0"]
+    seq = [_register(m, vals, s) for s in sectors]
+    bat = [_register(m, vals, s) for s in sectors]
+    _lib.call("qed_reset_start_vector_state")
+    E_seq = [qcm.ground_state_solve(l)[0] for l in seq]
+    _lib.call("qed_reset_start_vector_state")
+    qcm.batch_solve(bat)
+    for s, l, e in zip(sectors, bat, E_seq):
+        eb = qcm.ground_state_solve(l)[0]
+        print(s, "sequential %.12f batched %.12f" % (e, eb))
+        assert np.isfinite(eb)
+        assert abs(eb - e) <= 1e-10 * max(1.0, abs(e))
+
+
+def test_batch_solve_matches_oracle(gpu_lib):
+    """One batched solve of three plaquette + 8 bath instances: the ground-state energies
+    against the oracle's Lanczos (1e-10 relative) and G(i w) against the oracle's band
+    Lanczos restatement of the same instance (1e-8 of max |G|).
+
+    G is compared where the band Lanczos of the reference (stopped when its lowest
+    eigenvalue is converged) has converged G itself. The first instance has an excitation
+    0.0056 above E0: there the reference algorithm's own G(0.1 i) is 3e-2 away from the
+    exact resolvent <c (z - H + E0)^-1 c+> + <c+ (z + H - E0)^-1 c> (GMRES to 1e-11 on the
+    N = 7 and N = 9 sectors), and 3.5e-6 away at 0.5 i, so two implementations whose Krylov
+    vectors differ by rounding agree only to that level at those frequencies; at 2 i and
+    5 i all three instances agree to about 1e-12."""
+    from pyqcm_amd import _lib, qcm
+    from oracle import oracle as O
+    from tests import models as TM
+    rng = np.random.default_rng(23)
+    params = [TM.random_bath_params(rng) for _ in range(3)]
+    models = [TM.plaquette_bath(p) for p in params]
+    labels = [_register(m, v, "R0:N8:S0") for m, v in models]
+    _lib.call("qed_reset_start_vector_state")
+    qcm.batch_solve(labels)
+    zs = 1j * np.array([0.1, 0.5, 2.0, 5.0])
+    L = 12
+    for inst, ((m, vals), lab) in enumerate(zip(models, labels)):
+        E0b = qcm.ground_state_solve(lab)[0]
+        So = O.Sector(m, vals, 4, 4)
+        E0o, gs, *_ = So.lanczos_gs()
+        assert abs(E0b - E0o) <= 1e-10 * abs(E0o), (E0b, E0o)
+        es, Qs = [], []
+        for pm in (-1, 1):
+            To = O.Sector(m, vals, 4 + pm, 4)
+            phi = [O.create_destroy(L, 4, 4, a, pm, gs) for a in range(4)]
+            e, Q = O.band_lanczos(To.hv, phi)
+            e = e - E0o
+            if pm == -1:
+                e = -e
+            e, Q = O.streamline(e, Q)
+            es.append(e)
+            Qs.append(Q)
+        e = np.concatenate(es)
+        Q = np.hstack(Qs)
+        G = qcm.Green_function(zs, 0, lab)
+        for i, z in enumerate(zs):
+            ref = O.qmatrix_gf(e, Q, z)
+            err = np.abs(G[i] - ref).max()
+            print("E0 %.12f  Im z %g  max|dG| %.2e (poles: gpu %d, oracle %d)"
+                  % (E0o, z.imag, err, len(qcm.qmatrix(lab)[0]), len(e)))
+            if inst == 0 and z.imag < 1.0:
+                continue  # band Lanczos G not converged there (see above)
+            assert err <= 1e-8 * max(1.0, np.abs(ref).max())

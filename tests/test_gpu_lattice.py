@@ -160,9 +160,36 @@ def test_cdmft_chain2_4b_reaches_reference_fixed_point(gpu_lib, golden):
     run = C.CDMFT(P, vals, [1.0, -1.0, 0.5, 0.5])
     ref = np.array([float(row[k]) for k in ("eb1_1", "eb2_1", "tb1_1", "tb2_1")])
     print("converged", run.x, "reference", ref, "iterations", run.niter, "distance", run.dist)
-    np.testing.assert_allclose(run.x, ref, atol=2e-3)
+    np.testing.assert_allclose(run.x, ref, atol=2e-4)
+    # run.dist (6.0e-5 here) is the distance of the last fit, against a host that the
+    # final step of the loop moved; the reference recorded 1.20e-4 at its own stopping
+    # point. The distance function itself is compared at the recorded baths in
+    # test_cdmft_distance_at_reference_baths.
     inst = P.solve(vals, ref)
     assert abs(inst.ground_state()[0] - float(row["E0_1"])) < 2e-7
+
+
+def test_cdmft_distance_at_reference_baths(gpu_lib, golden):
+    """The distance function (weights, frequency grid, the hybridization of the bath, the
+    normalisation) against the values recorded along the mu scan of
+    examples/1D_2_4b/cdmft.tsv: at each converged bath x, d(x; host(x)) over the recorded
+    distance is one constant for all rows. The file was written by pyqcm v1.0.7 (its
+    version column), whose recorded distance is twice the current definition
+    (src_qcm/CPT.cpp:519-555: factor 2 at mixing 0, times the frequency step, over dim^2),
+    which this package follows: the ratio is 1/2 to the three digits the file keeps."""
+    from pyqcm_amd import cdmft as C
+    rows = golden["chain2_4b_cdmft"]["rows"]
+    M = chain2_4b_model()
+    P = C.CDMFTProblem(M, ["eb1_1", "eb2_1", "tb1_1", "tb2_1"], "R0:N6:S0",
+                       grid=C.frequency_grid("matsubara", (2, 50)))
+    ratio = []
+    for row in rows[::10]:
+        vals = {"t": float(row["t"]), "mu": float(row["mu"]), "U": float(row["U"])}
+        x = np.array([float(row[k]) for k in ("eb1_1", "eb2_1", "tb1_1", "tb2_1")])
+        d = float(P.distance(P.host(P.solve(vals, x)), x[None, :])[0])
+        ratio.append(d / float(row["distance"]))
+        print("mu", row["mu"], "recorded", row["distance"], "here %.4e" % d, "ratio %.4f" % ratio[-1])
+    assert all(0.497 < r < 0.507 for r in ratio), ratio
 
 
 def test_batched_cdmft_equals_plain_loop(gpu_lib):
