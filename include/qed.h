@@ -86,6 +86,18 @@ int qed_start_vector(qed_sector* s, int seed, double* x_dev);
 int qed_lanczos_gs(qed_sector* s, double accur, int max_iter, double* psi_dev, double* E0,
                    int* niter, double* alpha, double* beta);
 
+/* qed_lanczos_gs without device allocation. No counterpart in the reference: for callers
+ * that cannot allocate device memory (tight solver loops, harnesses that forbid hipMalloc
+ * after start-up). Same arguments and results as qed_lanczos_gs; the two recurrence
+ * vectors and the reduction partials live in a fixed device array of the library (one
+ * per GPU, 34.7 MB, held until the process ends), so the sector dimension is limited to
+ * 2^21 = 2,097,152: a larger sector fails with an error and nothing is launched. All calls
+ * on one GPU share that array, which keeps the last call's contents; the library lock
+ * serializes the calls of one process, and calls that overlap on one GPU otherwise race.
+ * A sector set to a split-halves H.v variant takes the default row kernel here. */
+int qed_lanczos_gs_noalloc(qed_sector* s, double accur, int max_iter, double* psi_dev, double* E0,
+                           int* niter, double* alpha, double* beta);
+
 /* Scalar Lanczos from a normalized psi_dev for the continued-fraction Green function;
  * a (capacity max_iter) and b (capacity max_iter + 1) receive the coefficients.
  * Replaces LanczosGreen (src_ed/Hamiltonian/Lanczos.hpp:156-183). */

@@ -42,6 +42,7 @@ SIGNATURES = [
     ("qed_diag", c_int, [c_void_p, c_void_p]),
     ("qed_start_vector", c_int, [c_void_p, c_int, c_void_p]),
     ("qed_lanczos_gs", c_int, [c_void_p, c_double, c_int, c_void_p, dptr, iptr, dptr, dptr]),
+    ("qed_lanczos_gs_noalloc", c_int, [c_void_p, c_double, c_int, c_void_p, dptr, iptr, dptr, dptr]),
     ("qed_lanczos_green", c_int, [c_void_p, c_void_p, c_double, c_int, dptr, dptr, iptr]),
     ("qed_band_lanczos", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double, dptr,
                                  dptr, c_int, iptr]),

@@ -876,6 +876,37 @@ void davidson(qed_sector& s, int K, double accur, int max_iter, std::vector<doub
 
 using namespace qed;
 
+// ---- the allocation-free Lanczos ground state --------------------------------------------
+namespace {
+
+// Device memory of qed_lanczos_gs_noalloc, one copy per GPU for the life of the process:
+// the two recurrence vectors of a sector of at most kNoAllocMaxDim amplitudes, the BLAS-1
+// block partials, the fused step's scalars and row partials (34.7 MB in all).
+constexpr uint64_t kNoAllocMaxDim = 1ull << 21;
+constexpr size_t kNoAllocDoubles = 2 * kNoAllocMaxDim + kRedDoubles + 8 + kFusedPartDoubles;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kMaxDevices = 64;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current GPU, looked up on the first call there
+// (the name is a host-side stand-in and must not be passed to a kernel). Caller holds the
+// library mutex.
+double* noalloc_scratch() {
+  int dev = 0;
+  QED_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) fail("qed_lanczos_gs_noalloc: device index out of range");
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    QED_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail("qed_lanczos_gs_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
 extern "C" {
 
 int qed_lanczos_gs(qed_sector* s, double accur, int max_iter, double* psi, double* E0, int* niter,
@@ -884,6 +915,23 @@ int qed_lanczos_gs(qed_sector* s, double accur, int max_iter, double* psi, doubl
   return guard([&] {
     std::vector<double> a, b;
     lanczos_gs(*s, accur, max_iter, psi, E0, niter, &a, &b);
+    if (alpha) std::copy(a.begin(), a.end(), alpha);
+    if (beta) std::copy(b.begin(), b.end(), beta);
+  });
+}
+
+int qed_lanczos_gs_noalloc(qed_sector* s, double accur, int max_iter, double* psi, double* E0, int* niter,
+                           double* alpha, double* beta) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    if (s->D > kNoAllocMaxDim)
+      fail("qed_lanczos_gs_noalloc: sector dimension " + std::to_string(s->D) + " exceeds the scratch limit of " +
+           std::to_string(kNoAllocMaxDim));
+    double* base = noalloc_scratch();
+    const LanczosWork ws{base, base + kNoAllocMaxDim, base + 2 * kNoAllocMaxDim,
+                         base + 2 * kNoAllocMaxDim + kRedDoubles, base + 2 * kNoAllocMaxDim + kRedDoubles + 8};
+    std::vector<double> a, b;
+    lanczos_gs(*s, accur, max_iter, psi, E0, niter, &a, &b, &ws);
     if (alpha) std::copy(a.begin(), a.end(), alpha);
     if (beta) std::copy(b.begin(), b.end(), beta);
   });
