@@ -54,6 +54,11 @@ bool gb(const char* n) { return G().b.at(n); }
 // band Lanczos refuses a lowest pair closer than band_lanczos_minimum_gap when
 // no_degenerate_BL is set (src_ed/Hamiltonian/Lanczos.hpp:235-236,368-370); -1: no guard
 double degeneracy_gap() { return gb("no_degenerate_BL") ? gd("band_lanczos_minimum_gap") : -1.0; }
+}  // namespace ed
+
+bool verbose_ed() { return ed::gb("verb_ED"); }
+
+namespace ed {
 
 constexpr double SMALL_VALUE = 1.0e-12;
 constexpr double MIDDLE_VALUE = 1.0e-8;
@@ -579,7 +584,8 @@ void low_energy_states(Instance& I) {
       if (method == 'D') {
         DevBuf all((size_t)n_states * H->D * 8);
         int nit = 0;
-        davidson(*H, n_states, gd("accur_Davidson"), gi("max_iter_lanczos"), evals, all.as<double>(), &nit);
+        davidson(*H, n_states, gd("accur_Davidson"), gi("max_iter_lanczos"), evals, all.as<double>(), &nit,
+                 gb("verb_ED"));
         for (int k = 0; k < n_states; ++k) {
           DevBuf v(H->D * 8);
           QED_HIP(hipMemcpyAsync(v.p, all.as<double>() + (uint64_t)k * H->D, H->D * 8, hipMemcpyDeviceToDevice,

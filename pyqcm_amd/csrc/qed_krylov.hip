@@ -661,179 +661,240 @@ void lanczos_green(qed_sector& s, const double* psi_in, double accur_cf, int max
 namespace {
 
 // out_k[i] = sum_p C[p + m k] in_p[i], k < K (outputs distinct from inputs)
-__global__ void k_combine(PtrList in, int m, MutPtrList out, int K, const double* __restrict__ C,
-                          uint64_t n) {
+constexpr int kDavMax = 24;  // basis vectors of the Davidson subspace (3 K, K <= 8)
+
+// out[k][i] = sum_p C[p + m k] in[p][i] for k < nout, with every input of index i read
+// before any output of it is written: the outputs may be some of the inputs
+__global__ __launch_bounds__(256) void k_dav_transform(PtrList in, int m, MutPtrList out, int nout,
+                                                       const double* __restrict__ C, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    // (the inputs of one index are re-read per output from L1/L2 rather than kept in a
-    // register array indexed at run time, which would live in scratch memory)
-    for (int k = 0; k < K; ++k) {
+    double v[kDavMax];
+#pragma unroll
+    for (int p = 0; p < kDavMax; ++p) v[p] = p < m ? in.p[p][i] : 0.0;
+    for (int k = 0; k < nout; ++k) {
       double acc = 0.0;
-      for (int p = 0; p < m; ++p) acc += C[p + m * k] * in.p[p][i];
+#pragma unroll
+      for (int p = 0; p < kDavMax; ++p)
+        if (p < m) acc += C[p + m * k] * v[p];
       out.p[k][i] = acc;
     }
   }
 }
 
-// c = r / (mu - d + tiny) with d the diagonal of H as the factorized format reports it
-__global__ void k_precondition(HvView v, const double* __restrict__ r, double mu, double tiny,
-                               double* __restrict__ c, uint64_t n, const double* __restrict__ d) {
-  for (uint64_t I = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n;
-       I += (uint64_t)gridDim.x * blockDim.x)
-    c[I] = r[I] / (-d[I] + mu + tiny);
+// the diagonal of H as the factorized format reports it (the interaction part)
+__device__ __forceinline__ double diag_interaction(const HvView& v, uint64_t I) {
+  const uint32_t iu = (uint32_t)(I % v.du), id = (uint32_t)(I / v.du);
+  const uint32_t su = v.bup[iu], sd = v.bdn[id];
+  if (v.hubbard_uniform) return v.hub_U * (double)__popc(su & sd & v.hub_sites);
+  double e = 0.0;
+  const uint64_t w = (uint64_t)sd | ((uint64_t)su << 32);
+  for (int k = 0; k < v.n_int; ++k)
+    if ((w & v.int_mask[k]) == v.int_mask[k]) e += v.int_val[k];
+  return e;
 }
 
-__global__ void k_diag_only_interaction(HvView v, double* d, uint64_t D) {
-  for (uint64_t I = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; I < D;
+// correction vector of one unconverged root: c = (H x - mu x) / (mu - d + tiny)
+__global__ void k_dav_correction(HvView v, const double* __restrict__ hx, const double* __restrict__ x, double mu,
+                                 double tiny, double* __restrict__ c, uint64_t n) {
+  for (uint64_t I = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n;
+       I += (uint64_t)gridDim.x * blockDim.x)
+    c[I] = (hx[I] - mu * x[I]) / (-diag_interaction(v, I) + mu + tiny);
+}
+
+// the order of find_lowest (src_util/vector_num.hpp:242-264): ascending values, and among
+// equal values the later index first
+__device__ __forceinline__ bool dav_before(double va, uint64_t ia, double vb, uint64_t ib) {
+  return va < vb || (va == vb && ia > ib);
+}
+
+// block partials of the first diagonal entry after (pv, pi) in that order (all entries
+// when have_prev is 0); a block with no candidate reports +inf
+__global__ __launch_bounds__(256) void k_dav_next_lowest(HvView v, uint64_t n, int have_prev, double pv, uint64_t pi,
+                                                         double* __restrict__ part_v, uint64_t* __restrict__ part_i) {
+  __shared__ double sv[256];
+  __shared__ uint64_t si[256];
+  double bv = INFINITY;
+  uint64_t bi = 0;
+  for (uint64_t I = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n;
        I += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t iu = (uint32_t)(I % v.du), id = (uint32_t)(I / v.du);
-    uint32_t su = v.bup[iu], sd = v.bdn[id];
-    double e = 0.0;
-    if (v.hubbard_uniform) {
-      e = v.hub_U * (double)__popc(su & sd & v.hub_sites);
-    } else {
-      uint64_t w = (uint64_t)sd | ((uint64_t)su << 32);
-      for (int k = 0; k < v.n_int; ++k)
-        if ((w & v.int_mask[k]) == v.int_mask[k]) e += v.int_val[k];
+    const double d = diag_interaction(v, I);
+    if (have_prev && !dav_before(pv, pi, d, I)) continue;
+    if (dav_before(d, I, bv, bi)) {
+      bv = d;
+      bi = I;
     }
-    d[I] = e;
+  }
+  sv[threadIdx.x] = bv;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w && dav_before(sv[threadIdx.x + w], si[threadIdx.x + w], sv[threadIdx.x], si[threadIdx.x])) {
+      sv[threadIdx.x] = sv[threadIdx.x + w];
+      si[threadIdx.x] = si[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part_v[blockIdx.x] = sv[0];
+    part_i[blockIdx.x] = si[0];
   }
 }
 
-void combine(const std::vector<double*>& in, const std::vector<double>& C, int K,
-             const std::vector<double*>& out, uint64_t n) {
-  const int m = (int)in.size();
-  if (m > kMaxMulti || K > kMaxMulti) fail("Davidson: subspace too large");
+// the K lowest diagonal entries in find_lowest's order, one device pass per entry
+std::vector<uint64_t> lowest_diagonal(const qed_sector& s, int K) {
+  const uint64_t n = s.D;
+  const uint32_t nb = (uint32_t)std::min<uint64_t>(kMaxBlocks, std::max<uint64_t>(1, (n + 255) / 256));
+  DevBuf pv(nb * sizeof(double)), pi(nb * sizeof(uint64_t));
+  std::vector<double> hv_(nb);
+  std::vector<uint64_t> hi_(nb);
+  std::vector<uint64_t> out;
+  double lv = 0.0;
+  uint64_t li = 0;
+  for (int k = 0; k < K; ++k) {
+    k_dav_next_lowest<<<nb, 256, 0, stream()>>>(view(s), n, k > 0, lv, li, pv.as<double>(), pi.as<uint64_t>());
+    QED_HIP(hipMemcpyAsync(hv_.data(), pv.p, nb * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    QED_HIP(hipMemcpyAsync(hi_.data(), pi.p, nb * sizeof(uint64_t), hipMemcpyDeviceToHost, stream()));
+    QED_HIP(hipStreamSynchronize(stream()));
+    double bv = INFINITY;
+    uint64_t bi = 0;
+    for (uint32_t b = 0; b < nb; ++b)
+      if (std::isfinite(hv_[b]) && (bv == INFINITY || hv_[b] < bv || (hv_[b] == bv && hi_[b] > bi))) {
+        bv = hv_[b];
+        bi = hi_[b];
+      }
+    if (!std::isfinite(bv)) fail("Davidson: fewer diagonal entries than requested states");
+    out.push_back(bi);
+    lv = bv;
+    li = bi;
+  }
+  return out;
+}
+
+// out = C-weighted combinations of `in` (see k_dav_transform); C is m x nout column-major
+void dav_transform(const std::vector<double*>& in, const std::vector<double>& C, const std::vector<double*>& out,
+                   uint64_t n) {
+  const int m = (int)in.size(), nout = (int)out.size();
+  if (m > kDavMax || nout > kDavMax) fail("Davidson: subspace too large");
   PtrList li{};
   MutPtrList lo{};
   for (int p = 0; p < m; ++p) li.p[p] = in[p];
-  for (int k = 0; k < K; ++k) lo.p[k] = out[k];
+  for (int k = 0; k < nout; ++k) lo.p[k] = out[k];
   double* cd = red_scratch().as<double>() + (size_t)kMaxBlocks * kMaxMulti + 64;
-  if (C.size() > 512) fail("Davidson: coefficient block too large");
   QED_HIP(hipMemcpyAsync(cd, C.data(), C.size() * sizeof(double), hipMemcpyHostToDevice, stream()));
-  k_combine<<<stream_blocks(n), 256, 0, stream()>>>(li, m, lo, K, cd, n);
+  k_dav_transform<<<stream_blocks(n), 256, 0, stream()>>>(li, m, lo, nout, cd, n);
   QED_HIP(hipStreamSynchronize(stream()));
-}
-
-// find_lowest (src_util/vector_num.hpp:242-264): insertion into a sorted list of nm
-// entries, later indices placed before earlier equal values
-std::vector<size_t> find_lowest(const std::vector<double>& x, size_t nm) {
-  std::vector<double> minv(nm);
-  std::vector<size_t> idx(nm, 0);
-  for (size_t i = 0; i < nm; ++i) minv[i] = 1.0e12 + 1.0 * i;
-  for (size_t i = 0; i < x.size(); ++i) {
-    int j;
-    for (j = (int)nm - 1; j >= 0; --j)
-      if (x[i] > minv[j]) break;
-    ++j;
-    if (j < (int)nm) {
-      for (int k = (int)nm - 1; k > j; --k) {
-        minv[k] = minv[k - 1];
-        idx[k] = idx[k - 1];
-      }
-      minv[j] = x[i];
-      idx[j] = i;
-    }
-  }
-  return idx;
 }
 
 }  // namespace
 
-void davidson(qed_sector& s, int K, double accur, int max_iter, std::vector<double>& val,
-              double* xout, int* niter) {
+// Davidson-Liu for the K lowest states, following src_ed/Hamiltonian/Davidson.hpp:38-230
+// (start vectors on the K lowest diagonal entries, at most 3K basis vectors, the new basis
+// made of the Ritz vectors, the basis vectors of the unconverged roots and their
+// preconditioned residuals, Gram-Schmidt), with a memory layout for large sectors: the
+// products H b are not kept (each is formed in a scratch vector, reduced against the basis
+// and dropped), the residuals H x - mu x are formed from a fresh product of the Ritz vector,
+// and the basis is transformed in place. Device memory: the K output vectors (the first K
+// basis slots) plus 2K further slots and two scratch vectors, 3K + 2 vectors in all (14 for
+// K = 4) instead of the reference's ~9K; the diagonal is never stored.
+void davidson(qed_sector& s, int K, double accur, int max_iter, std::vector<double>& val, double* xout,
+              int* niter, bool verb) {
   const uint64_t n = s.D;
   const double tiny = 1e-14;
   if ((uint64_t)K > n) K = (int)n;
+  if (K < 1) fail("Davidson: no state requested");
+  const int J = (int)std::min<uint64_t>(3 * (uint64_t)K, n);
+  if (J > kDavMax) fail("Davidson: at most " + std::to_string(kDavMax / 3) + " states");
   val.assign(K, 0.0);
-  VecPool pool(n);
-  double* d = pool.get();
-  k_diag_only_interaction<<<stream_blocks(n), 256, 0, stream()>>>(view(s), d, n);
-  std::vector<double> dh(n);
-  QED_HIP(hipMemcpyAsync(dh.data(), d, n * 8, hipMemcpyDeviceToHost, stream()));
-  QED_HIP(hipStreamSynchronize(stream()));
-  std::vector<size_t> low = find_lowest(dh, (size_t)K);
-  dh.clear();
-  dh.shrink_to_fit();
-  std::vector<double*> b, h, x(K);
+  {
+    size_t fr = 0, tot = 0;
+    QED_HIP(hipMemGetInfo(&fr, &tot));
+    const double need = (double)(J - K + 2) * n * 8;
+    if (need > (double)fr)
+      fail("Davidson: " + std::to_string(J - K + 2) + " work vectors of the sector need " +
+           std::to_string(need / 1e9) + " GB of device memory, " + std::to_string(fr / 1e9) + " GB free");
+  }
+  std::vector<DevBuf> own;
+  std::vector<double*> slot(J);
+  for (int k = 0; k < K; ++k) slot[k] = xout + (uint64_t)k * n;
+  for (int k = K; k < J; ++k) {
+    own.emplace_back(n * 8);
+    slot[k] = own.back().as<double>();
+  }
+  DevBuf tb(n * 8), ub(n * 8);
+  double* t = tb.as<double>();
+  double* u = ub.as<double>();
+  const uint32_t sb = stream_blocks(n);
+
+  if (verb)
+    printf("\nDavidson method for %d vectors in sector (%d, %d), dim = %llu\n", K, s.nup, s.ndn,
+           (unsigned long long)n);
+  const std::vector<uint64_t> low = lowest_diagonal(s, K);
+  const double one = 1.0;
   for (int k = 0; k < K; ++k) {
-    double* bk = pool.get();
-    QED_HIP(hipMemsetAsync(bk, 0, n * 8, stream()));
-    const double one = 1.0;
-    QED_HIP(hipMemcpyAsync(bk + low[k], &one, 8, hipMemcpyHostToDevice, stream()));
-    double* hk = pool.get();
-    QED_HIP(hipMemsetAsync(hk, 0, n * 8, stream()));
-    hv(s, bk, hk);
-    b.push_back(bk);
-    h.push_back(hk);
-    x[k] = pool.get();
+    k_zero<<<sb, 256, 0, stream()>>>(slot[k], n);
+    QED_HIP(hipMemcpyAsync(slot[k] + low[k], &one, 8, hipMemcpyHostToDevice, stream()));
+  }
+  int m = K;
+  // bhb(i, j) = (H b_j)[low_i]
+  std::vector<double> bhb((size_t)m * m);
+  for (int j = 0; j < m; ++j) {
+    k_zero<<<sb, 256, 0, stream()>>>(u, n);
+    hv(s, slot[j], u);
+    for (int i = 0; i < m; ++i)
+      QED_HIP(hipMemcpyAsync(&bhb[(size_t)j * m + i], u + low[i], 8, hipMemcpyDeviceToHost, stream()));
   }
   QED_HIP(hipStreamSynchronize(stream()));
-  int m = K;
-  std::vector<double> bhb((size_t)m * m);
-  for (int i = 0; i < m; ++i)
-    for (int j = 0; j < m; ++j)
-      QED_HIP(hipMemcpy(&bhb[(size_t)j * m + i], h[j] + low[i], 8, hipMemcpyDeviceToHost));
+
   int iter;
-  std::vector<double*> r(K, nullptr);
   for (iter = 0; iter < max_iter; ++iter) {
     std::vector<double> a(bhb), mu(m);
-    sym_eigen(m, a.data(), mu.data(), true);  // columns of a are eigenvectors
-    std::vector<double> C((size_t)m * K);
+    sym_eigen(m, a.data(), mu.data(), true);  // column k of a: eigenvector k
+    std::vector<double*> basis(slot.begin(), slot.begin() + m);
+    // residual norms |H x_k - mu_k x_k| with x_k = sum_p a_pk b_p formed in t
+    std::vector<int> unconv;
+    double res_last = 0.0;
+    for (int k = 0; k < K; ++k) {
+      std::vector<double> ck(a.begin() + (size_t)k * m, a.begin() + (size_t)(k + 1) * m);
+      dav_transform(basis, ck, {t}, n);
+      k_zero<<<sb, 256, 0, stream()>>>(u, n);
+      hv(s, t, u);
+      res_last = axpy_nrm2(-mu[k], t, u, n);
+      if (res_last > accur) unconv.push_back(k);
+    }
+    if (verb && iter % 10 == 0) {
+      printf("--> iteration %d, evalue = %.12g,\tresidual = %.12g\n", iter, mu[0], res_last);
+      fflush(stdout);
+    }
+    for (int k = 0; k < K; ++k) val[k] = mu[k];
+    const int Q = (int)unconv.size();
+    // the basis in place: slots 0..K-1 <- Ritz vectors, then the unconverged roots' own
+    // basis vectors
+    std::vector<double> C((size_t)m * (K + Q), 0.0);
     for (int k = 0; k < K; ++k)
       for (int p = 0; p < m; ++p) C[p + (size_t)m * k] = a[(size_t)k * m + p];
-    combine(b, C, K, x, n);
-    std::vector<double*> rr(K);
-    for (int k = 0; k < K; ++k) rr[k] = pool.get();
-    combine(h, C, K, rr, n);
-    std::vector<int> unconv;
-    for (int k = 0; k < K; ++k) {
-      double res = axpy_nrm2(-mu[k], x[k], rr[k], n);
-      if (res > accur) unconv.push_back(k);
+    for (int q = 0; q < Q; ++q) C[unconv[q] + (size_t)m * (K + q)] = 1.0;
+    dav_transform(basis, C, std::vector<double*>(slot.begin(), slot.begin() + K + Q), n);
+    if (Q == 0) break;
+    // preconditioned residuals of the unconverged roots
+    for (int q = 0; q < Q; ++q) {
+      const int k = unconv[q];
+      k_zero<<<sb, 256, 0, stream()>>>(u, n);
+      hv(s, slot[k], u);
+      k_dav_correction<<<sb, 256, 0, stream()>>>(view(s), u, slot[k], mu[k], tiny, slot[K + Q + q], n);
     }
-    const int Q = (int)unconv.size();
-    if (Q == 0) {
-      for (int k = 0; k < K; ++k) {
-        val[k] = mu[k];
-        pool.put(rr[k]);
-      }
-      break;
-    }
-    // new subspace: Ritz vectors, previous basis vectors of unconverged roots, corrections
-    std::vector<double*> nb;
-    for (int k = 0; k < K; ++k) {
-      double* t = pool.get();
-      QED_HIP(hipMemcpyAsync(t, x[k], n * 8, hipMemcpyDeviceToDevice, stream()));
-      nb.push_back(t);
-    }
-    std::vector<bool> keep_b(b.size(), false);
-    for (int k : unconv) {
-      nb.push_back(b[k]);
-      keep_b[k] = true;
-    }
-    for (int k : unconv) {
-      k_precondition<<<stream_blocks(n), 256, 0, stream()>>>(view(s), rr[k], mu[k], tiny, rr[k], n, d);
-      nb.push_back(rr[k]);
-      rr[k] = nullptr;
-    }
-    for (int k = 0; k < K; ++k) pool.put(rr[k]);
-    for (size_t p = 0; p < b.size(); ++p)
-      if (!keep_b[p]) pool.put(b[p]);
-    for (double* hp : h) pool.put(hp);
-    h.clear();
     // Gram-Schmidt in the reference's order (vectors below tiny are removed)
-    b = nb;
+    std::vector<double*> b(slot.begin(), slot.begin() + K + 2 * Q);
     for (size_t i = 0; i < b.size(); ++i) {
-      double z = nrm2(b[i], n);
+      const double z = nrm2(b[i], n);
       if (z < tiny) {
-        pool.put(b[i]);
-        b[i] = b.back();
+        std::swap(b[i], b.back());
         b.pop_back();
         --i;
         continue;
       }
       scal(1.0 / z, b[i], n);
-      int cnt = (int)b.size() - (int)(i + 1);
+      const int cnt = (int)b.size() - (int)(i + 1);
       if (cnt > 0) {
         std::vector<const double*> ys(b.begin() + i + 1, b.end());
         std::vector<double> zz(cnt);
@@ -842,33 +903,43 @@ void davidson(qed_sector& s, int K, double accur, int max_iter, std::vector<doub
         multi_axpy_rev(zz.data(), b[i], ym.data(), cnt, n);
       }
     }
-    if ((int)b.size() == K) {
-      for (int k = 0; k < K; ++k) val[k] = mu[k];
-      break;
+    // keep the slots in basis order (a removed vector's slot moves to the end)
+    {
+      std::vector<double*> rest;
+      for (double* p : slot)
+        if (std::find(b.begin(), b.end(), p) == b.end()) rest.push_back(p);
+      for (size_t i = 0; i < b.size(); ++i) slot[i] = b[i];
+      for (size_t i = 0; i < rest.size(); ++i) slot[b.size() + i] = rest[i];
     }
+    if ((int)b.size() == K) break;
     m = (int)b.size();
-    for (int p = 0; p < m; ++p) {
-      double* hp = pool.get();
-      QED_HIP(hipMemsetAsync(hp, 0, n * 8, stream()));
-      hv(s, b[p], hp);
-      h.push_back(hp);
-    }
+    // bhb(i, k) = <b_i, H b_k> for i >= k (the reference's pairing), one product at a time
     bhb.assign((size_t)m * m, 0.0);
-    for (int i = 0; i < m; ++i) {
-      std::vector<const double*> hs(h.begin(), h.begin() + i + 1);
-      std::vector<double> zz(i + 1);
-      multi_dot(b[i], hs.data(), i + 1, n, zz.data());
-      for (int k = 0; k <= i; ++k) {
-        bhb[(size_t)k * m + i] = zz[k];
-        bhb[(size_t)i * m + k] = zz[k];
+    for (int k = 0; k < m; ++k) {
+      k_zero<<<sb, 256, 0, stream()>>>(u, n);
+      hv(s, slot[k], u);
+      std::vector<const double*> ys(slot.begin() + k, slot.begin() + m);
+      std::vector<double> zz(m - k);
+      multi_dot(u, ys.data(), m - k, n, zz.data());
+      for (int i = k; i < m; ++i) {
+        bhb[(size_t)k * m + i] = zz[i - k];
+        bhb[(size_t)i * m + k] = zz[i - k];
       }
     }
-    for (int k = 0; k < K; ++k) val[k] = mu[k];
   }
   if (iter == max_iter) fail("Davidson method failed to converge!");
   *niter = iter;
-  for (int k = 0; k < K; ++k)
-    QED_HIP(hipMemcpyAsync(xout + (uint64_t)k * n, x[k], n * 8, hipMemcpyDeviceToDevice, stream()));
+  // the states are in the first K slots, which are the output vectors unless Gram-Schmidt
+  // removed one of them: then exchange contents (through t) until each is in place
+  for (int k = 0; k < K; ++k) {
+    double* want = xout + (uint64_t)k * n;
+    if (slot[k] == want) continue;
+    const int j = (int)(std::find(slot.begin(), slot.end(), want) - slot.begin());
+    QED_HIP(hipMemcpyAsync(t, want, n * 8, hipMemcpyDeviceToDevice, stream()));
+    QED_HIP(hipMemcpyAsync(want, slot[k], n * 8, hipMemcpyDeviceToDevice, stream()));
+    QED_HIP(hipMemcpyAsync(slot[k], t, n * 8, hipMemcpyDeviceToDevice, stream()));
+    std::swap(slot[k], slot[j]);
+  }
   QED_HIP(hipStreamSynchronize(stream()));
 }
 
@@ -1034,7 +1105,7 @@ int qed_davidson(qed_sector* s, int k, double accur, int max_iter, double* evals
   std::lock_guard<std::recursive_mutex> lk(lib_mutex());
   return guard([&] {
     std::vector<double> e;
-    davidson(*s, k, accur, max_iter, e, psi, niter);
+    davidson(*s, k, accur, max_iter, e, psi, niter, verbose_ed());
     std::copy(e.begin(), e.end(), evals);
   });
 }

@@ -218,8 +218,12 @@ int block_lanczos_blocktri(int p, int Me, int Mh, const std::vector<double>& Abl
 // floors of pa x pa blocks; W pa x q; host arrays, G q x q complex column-major per z
 void mcf_eval(int pa, int q, int M, const double* A, const double* B, const double* W, int nz, const double* z,
               bool transpose, double* G);
+// Davidson-Liu for the k lowest states; psi receives them (k x D); verb prints the
+// progress every 10 iterations, as the reference's verb_ED
 void davidson(qed_sector& s, int k, double accur, int max_iter, std::vector<double>& evals,
-              double* psi, int* niter);
+              double* psi, int* niter, bool verb = false);
+// the global parameter verb_ED
+bool verbose_ed();
 std::recursive_mutex& lib_mutex();
 
 }  // namespace qed

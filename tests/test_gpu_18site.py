@@ -5,8 +5,10 @@ so the H.v runs its global-memory form.
 * H.v of a product vector u (x) w against the closed form assembled from one-spin
   oracle products: H (u x w) = u (x) (H_up w) + (H_dn u) (x) w + U n_up.n_dn (u x w),
   where H_up, H_dn are the oracle's Hamiltonians of the 9-electron one-spin sectors;
-* the Lanczos ground state: residual |H psi - E0 psi| and the Rayleigh-quotient bound."""
+* the Lanczos ground state: residual |H psi - E0 psi| and the Rayleigh-quotient bound;
+* (with QED_SLOW_TESTS=1) Davidson for the 4 lowest states within the GPU's memory."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -102,3 +104,56 @@ def test_18site_lanczos_ground_state(gpu_lib, chain18):
     assert res2 < 1e-10 * e * e
     psi.free()
     y.free()
+
+
+@pytest.mark.skipif(os.environ.get("QED_SLOW_TESTS") != "1",
+                    reason="about 12 minutes on one MI355X (226 iterations of about 20 products of "
+                           "120 ms each); run with QED_SLOW_TESTS=1")
+def test_18site_davidson_four_lowest_states(gpu_lib, chain18):
+    """Davidson for the 4 lowest states at 18 sites within one GPU's memory: the K output
+    vectors plus 2K + 2 work vectors (14 x 18.9 GB = 265 GB). Each residual
+    |H v - e v| <= accur_Davidson (1e-5), orthonormal vectors, and the lowest value against
+    the Lanczos ground state of the same sector."""
+    from pyqcm_amd import qcm
+    from pyqcm_amd._lib import DeviceArray
+    model, vals, S = chain18
+    D = S.dim
+    K = 4
+    psi = DeviceArray(D)
+    gpu_lib.call("qed_start_vector", S.h, 0, psi.ptr)
+    E0 = np.zeros(1)
+    nit = ctypes.c_int()
+    gpu_lib.call("qed_lanczos_gs", S.h, 1e-12, 1000, psi.ptr, gpu_lib.dp(E0), ctypes.byref(nit), None, None)
+    psi.free()
+    print("18 sites: Lanczos E0 %.10f (%d iterations)" % (E0[0], nit.value), flush=True)
+    V = DeviceArray(K * D)
+    ev = np.zeros(K)
+    qcm.set_global_parameter("verb_ED", True)  # progress every 10 iterations
+    try:
+        gpu_lib.call("qed_davidson", S.h, K, 1e-5, 1000, gpu_lib.dp(ev), V.ptr, ctypes.byref(nit))
+    finally:
+        qcm.set_global_parameter("verb_ED", False)
+    print("18 sites: Davidson %d iterations, values %s" % (nit.value, ev), flush=True)
+    assert abs(ev[0] - E0[0]) < 1e-8
+    y = DeviceArray(D)
+    for k in range(K):
+        v = V.offset(k * D)
+        y.zeros()
+        gpu_lib.call("qed_hv", S.h, v, y.ptr)
+        d = np.zeros(3)
+        gpu_lib.call("qed_blas_dot", v, v, D, gpu_lib.dp(d[0:1]))
+        gpu_lib.call("qed_blas_dot", v, y.ptr, D, gpu_lib.dp(d[1:2]))
+        gpu_lib.call("qed_blas_dot", y.ptr, y.ptr, D, gpu_lib.dp(d[2:3]))
+        res2 = d[2] - 2 * ev[k] * d[1] + ev[k] ** 2 * d[0]
+        ov = []
+        for j in range(k):
+            o = np.zeros(1)
+            gpu_lib.call("qed_blas_dot", V.offset(j * D), v, D, gpu_lib.dp(o))
+            ov.append(abs(o[0]))
+        print("18 sites: state %d  <v|v> - 1 = %.1e  |r|^2 = %.2e  max overlap %.1e"
+              % (k, d[0] - 1.0, res2, max(ov) if ov else 0.0), flush=True)
+        assert abs(d[0] - 1.0) < 1e-12
+        assert res2 <= (1.1e-5) ** 2  # accur_Davidson, recomputed from the returned vector
+        assert all(o < 1e-12 for o in ov)
+    y.free()
+    V.free()
