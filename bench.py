@@ -23,8 +23,12 @@ Prints one JSON line (rank 0, N=1):
                engine from the fixed cold start and from the learned warm start (the MLP
                trained on the GPU on the committed training set, BF16 inference), at the
                same convergence tests and tolerances. A "solve" is one parameter point
-               converged by the CDMFT loop. --cdmft-jacobian adds a leg that also starts
-               Broyden from the inverse Jacobian of the nearest training run.
+               converged by the CDMFT loop. A problem that fails from the learned start is
+               run again from the cold start afterwards (the plain loop is the fallback),
+               reported as warm_with_fallback with its time added; warm solves that reach
+               another fixed point than the plain loop's are reported separately
+               (speedup_at_plain_fixed_point). --cdmft-jacobian adds a leg that also
+               starts Broyden from the inverse Jacobian of the nearest training run.
 """
 import argparse
 import json
@@ -170,7 +174,7 @@ def main():
     dx.free()
     dy.free()
     del S
-    cpt = cpt_leg() if args.cpt else None
+    cpt = cpt_leg(cpu=not args.no_cpu) if args.cpt else None
     cdmft = cdmft_leg(args.cdmft, args.cdmft_batch, not args.no_graph, args.cdmft_jacobian, args.mlp_ensemble) if args.cdmft > 0 else None
 
     out = {
@@ -267,11 +271,14 @@ PEAK_FP64_TFLOPS = 78.6
 CPT_FLOP_PER_POINT = 41.5e3  # two complex 12 x 12 products and one LU inverse per (k, w)
 
 
-def cpt_leg(side=128, nw=512, reps=3):
+def cpt_leg(side=128, nw=512, reps=3, cpu=True):
     """CPT spectral trace on the 3x4 Hubbard cluster (t=1, U=8, mu=4, sector N=12 S=0,
     D = 853,776): G_cpt = G_c (1 - V(k) G_c)^-1 over a side^2 k-grid (k_i = (i+0.5)/side
     in the superdual basis) and nw frequencies w = linspace(-10, 10) + 0.1i, one device
-    launch; device time from HIP events, algorithmic flops 41.5 kflop per (k, w)."""
+    launch; device time from HIP events, algorithmic flops 41.5 kflop per (k, w).
+    With cpu: the same cluster's Lanczos ground state from the seed-0 start vector on the
+    device and through the oracle's recurrence (one core), and the oracle's dense
+    G_c (1 - V G_c)^-1 on a sample of 16 k-points x nw frequencies."""
     from pyqcm_amd import _lib
     from pyqcm_amd import lattice as LT
     T = LT.square_tiling(3, 4)
@@ -298,11 +305,58 @@ def cpt_leg(side=128, nw=512, reps=3):
     dev = float(np.median(devs))
     flops = CPT_FLOP_PER_POINT * nw * len(k)
     tf = flops / (dev * 1e-3) / 1e12
-    inst.erase()
     log("CPT 3x4: ED + G_c at %d frequencies %.2f s; k-sum over %d x %d points: %.2f ms device, %.2f ms wall "
         "-> %.2f TFLOP/s" % (nw, t_ed, nw, len(k), dev, float(np.median(walls)) * 1e3, tf))
+    extra = {}
+    if cpu:
+        from oracle import oracle as O
+        from pyqcm_amd import _lib
+        from pyqcm_amd.sector import Sector
+        # Lanczos ground state of the cluster: device and oracle (one core), same start vector
+        ops = [(nm, "one-body", [(r - 1, c - 1, v) for (r, c, v) in T.cluster_operator_elements(nm)])
+               for nm in ("t", "mu")]
+        ops.append(("U", "interaction", [(i, i + 12, 1.0) for i in range(12)]))
+        from pyqcm_amd.sector import flatten_ops
+        S = Sector(12, 6, 6, flatten_ops(ops, vals))
+        x0 = S.start_vector(0).to_host()
+        t0 = time.perf_counter()
+        Eg, _, ng, _, _ = S.lanczos_gs(start=x0)
+        _lib.call("qed_sync")
+        tg = time.perf_counter() - t0
+        S.close()
+        nn, _ = O.square_bonds(3, 4)
+        model, ovals = O.hubbard_cluster(12, nn, t=1.0, U=8.0, mu=4.0)
+        So = O.Sector(model, ovals, 6, 6)
+        t0 = time.perf_counter()
+        Eo, _, no, _, _ = So.lanczos_gs(start=x0)
+        tl = time.perf_counter() - t0
+        log("Lanczos 3x4: device %.3f s (%d iterations, E0 %.12f); oracle 1 core %.2f s (%d, %.12f)"
+            % (tg, ng, Eg, tl, no, Eo))
+        extra["lanczos"] = {"E0_device": Eg, "E0_oracle": Eo, "iterations_device": ng, "iterations_oracle": no,
+                            "seconds_device": round(tg, 4), "seconds_oracle_1core": round(tl, 3),
+                            "kind": "port", "cores": 1}
+        # CPT: the oracle's dense restatement on 16 k-points x nw frequencies
+        Gc = inst.green(z)
+        tc = T.cluster_matrix(vals)
+        ks = k[:: len(k) // 16][:16]
+        tk = T.tk(vals, ks)
+        t0 = time.perf_counter()
+        tr = np.array([[np.trace(O.cpt_green(Gc[a], tk[b] - tc)) for b in range(len(ks))] for a in range(nw)])
+        tcpu = time.perf_counter() - t0
+        dev_tr = LT.cpt_batch(Geff, V0, nv, mats, ks, mode="trace")
+        agree = float(np.abs(dev_tr - tr).max() / np.abs(tr).max())
+        pts = nw * len(ks)
+        log("CPT 3x4 oracle: %d points in %.2f s (1 core) -> %.0f points/s; device %.3g points/s; "
+            "max rel. difference %.1e" % (pts, tcpu, pts / tcpu, nw * len(k) / (dev * 1e-3), agree))
+        extra["cpu_baseline"] = {"value": round(pts / tcpu, 1), "unit": "points/s", "cores": 1, "kind": "port",
+                                 "sample": "%d k-points x %d frequencies of the oracle's G_c (1 - V G_c)^-1 "
+                                           "trace" % (len(ks), nw),
+                                 "device_points_per_s": round(nw * len(k) / (dev * 1e-3), 1),
+                                 "max_rel_difference": agree}
+    inst.erase()
     return {"workload": "3x4 Hubbard cluster t=1 U=8 mu=4, sector N=12 S=0 (D=853,776), CPT trace of "
                         "G_c (1 - V(k) G_c)^-1 on a %dx%d k-grid x %d frequencies" % (side, side, nw),
+            **extra,
             "E0": round(E0, 10), "points": nw * len(k), "ksum_ms_device": round(dev, 3),
             "ksum_ms_wall": round(float(np.median(walls)) * 1e3, 3), "ed_and_gc_s": round(t_ed, 3),
             "roofline": {"bound": "fp64", "achieved": round(tf, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
@@ -356,16 +410,39 @@ def cdmft_leg(n, batch, graph=True, jacobian=False, ensemble=1):
         res[name + "_x"] = np.array(B.x)
         res[name + "_ok"] = good
         log("CDMFT %s: %s" % (name, res[name]))
+    # the plain loop is the fallback: the problems that failed from the learned start run
+    # again from the cold start, and the warm leg with that fallback is reported beside it
+    bad = np.where(~res["warm_ok"])[0]
+    fb = {"retried_from_cold_start": int(len(bad)), "converged": res["warm"]["converged"],
+          "seconds": res["warm"]["seconds"]}
+    if len(bad):
+        t0 = time.perf_counter()
+        B2 = C.BatchedCDMFT(P, [pts[i] for i in bad], [cold[i] for i in bad], solver="batched", capacity=batch)
+        ok2 = B2.converged & ~B2.failed
+        fb["converged"] += int(ok2.sum())
+        fb["seconds"] = round(fb["seconds"] + time.perf_counter() - t0, 2)
+    fb["solves_per_s"] = round(fb["converged"] / fb["seconds"], 4)
+    log("CDMFT warm, then cold for its failures: %s" % fb)
     both = res["plain_ok"] & res["warm_ok"]
     dx = np.abs(C.canonical_bath(res["plain_x"][both]) - C.canonical_bath(res["warm_x"][both]))
     same = dx.max(1) < 1e-2 if both.any() else np.zeros(0, bool)
+    # a warm solve counted only where it reaches the plain loop's fixed point (or where the
+    # plain loop did not converge)
+    agree_warm = int(res["warm_ok"].sum() - (both.sum() - same.sum()))
+    res["warm"]["converged_at_plain_fixed_point"] = agree_warm
+    res["warm"]["solves_per_s_at_plain_fixed_point"] = round(agree_warm / res["warm"]["seconds"], 4)
     return {
         "workload": "2x2 plaquette + 8 bath orbitals, %d seeded points (%d active at a time), sectors "
                     "N=8..16 S=0, Legendre (1,10,5,10,5) grid, 32x32 k-grid, accur 1e-4 on parameters, "
                     "Broyden maxiter 32" % (n, batch),
         "plain": res["plain"], "warm": res["warm"],
         "speedup_solves_per_s": round(res["warm"]["solves_per_s"] / max(res["plain"]["solves_per_s"], 1e-12), 3),
+        "speedup_at_plain_fixed_point": round(res["warm"]["solves_per_s_at_plain_fixed_point"] /
+                                              max(res["plain"]["solves_per_s"], 1e-12), 3),
+        "warm_with_fallback": fb,
+        "speedup_with_fallback": round(fb["solves_per_s"] / max(res["plain"]["solves_per_s"], 1e-12), 3),
         "same_fixed_point": "%d of %d points converged by both" % (int(same.sum()), int(both.sum())),
+        "different_fixed_point": int(both.sum() - same.sum()),
         **({"warm_jacobian": res["warm_jacobian"], "jacobian_runs": int(len(FJ)),
             "speedup_jacobian": round(res["warm_jacobian"]["solves_per_s"] /
                                       max(res["plain"]["solves_per_s"], 1e-12), 3)} if jacobian else {}),

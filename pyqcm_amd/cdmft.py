@@ -428,13 +428,22 @@ class BatchedCDMFT:
 
     def __init__(self, problem, values_list, x0_list, maxiter=32, depth=2, accur=1e-4,
                  accur_bath=1e-6, accur_dist=1e-8, initial_step=0.1, max_function_eval=1_000_000,
-                 solver=None, verbose=False, fit="device", capacity=None, alpha_list=None):
+                 solver=None, verbose=False, fit="device", capacity=None, alpha_list=None,
+                 fallback_list=None, fallback_after=None):
         """alpha_list: per problem, the Broyden inverse-Jacobian start (the reference's CDMFT
         `alpha`: a float gives (1 + alpha) times the unit matrix, a matrix is used as the
         inverse Jacobian, e.g. from a previous run; pyqcm/cdmft.py:171, 414-421). The
-        inverse Jacobians reached are kept in self.alpha."""
+        inverse Jacobians reached are kept in self.alpha.
+
+        fallback_list: per problem, a second starting point (e.g. the cold start when x0 is
+        a predicted warm start). A problem whose bath fit fails, whose Broyden step is
+        undefined, or that has not converged after `fallback_after` iterations (default:
+        maxiter) starts again from it once, with the unit inverse Jacobian and a fresh
+        convergence test; its solves and iterations add up. self.restarted marks them."""
         self.problem = problem
         self.alpha_in = alpha_list
+        self.fallback = None if fallback_list is None else [np.array(x, float) for x in fallback_list]
+        self.fallback_after = maxiter if fallback_after is None else int(fallback_after)
         self.fit_where = fit
         self.capacity = capacity or len(values_list)
         B = len(values_list)
@@ -448,6 +457,8 @@ class BatchedCDMFT:
         self.solver = solver or self._solve_sequential
         self.verbose = verbose
         self.conv = [parameters_convergence(accur, depth) for _ in range(B)]
+        self._accur, self._depth = accur, depth
+        self.restarted = np.zeros(B, bool)
         self.n_solves = np.zeros(B, int)
         self.nfev = np.zeros(B, int)
         self.dist = np.full(B, np.nan)
@@ -517,7 +528,23 @@ class BatchedCDMFT:
         f0 = [None] * B
         dxs = [None] * B
         it = np.zeros(B, int)
+        it_done = np.zeros(B, int)  # iterations of abandoned first attempts
         started = np.zeros(B, bool)
+
+        def restart(i):
+            """the problem's second attempt from its fallback start; False when none is left"""
+            if self.fallback is None or self.restarted[i]:
+                return False
+            self.restarted[i] = True
+            self.failed[i] = False
+            self.x[i] = np.copy(self.fallback[i])
+            Ij[i] = np.eye(n)
+            f0[i] = dxs[i] = None
+            started[i] = False
+            it_done[i] += it[i]
+            it[i] = 0
+            self.conv[i] = parameters_convergence(self._accur, self._depth)
+            return True
         queue = deque(range(B))
         active = []
         self.rounds = 0
@@ -530,6 +557,8 @@ class BatchedCDMFT:
             nxt = []
             for i, x, xn in zip(active, pts, new):
                 if self.failed[i]:
+                    if restart(i):
+                        nxt.append(i)
                     continue
                 fi = x - xn
                 if not started[i]:
@@ -540,13 +569,20 @@ class BatchedCDMFT:
                     T = np.outer(dx - Ij[i] @ df, dx)
                     Ij[i] = Ij[i] + T @ Ij[i] / (dx @ (Ij[i] @ df))
                     it[i] += 1
-                    if it[i] > self.maxiter:
-                        self.failed[i] = True
+                    if it[i] > self.maxiter or (it[i] > self.fallback_after and not self.restarted[i]
+                                                and self.fallback is not None):
+                        if restart(i):
+                            nxt.append(i)
+                        else:
+                            self.failed[i] = True
                         continue
                 f0[i] = fi.copy()
                 dx = -Ij[i] @ fi
                 if np.any(np.isnan(dx)):
-                    self.failed[i] = True
+                    if restart(i):
+                        nxt.append(i)
+                    else:
+                        self.failed[i] = True
                     continue
                 dxs[i] = dx
                 self.x[i] = x + dx
@@ -562,7 +598,7 @@ class BatchedCDMFT:
             if self.verbose:
                 print("batched CDMFT: %d active, %d queued, %d converged, %d failed"
                       % (len(active), len(queue), self.converged.sum(), self.failed.sum()), flush=True)
-        self.niter = it
+        self.niter = it + it_done
 
 
 # ------------------------------------------------------------- 2x2 + 8 bath model

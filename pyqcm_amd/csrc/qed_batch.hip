@@ -584,9 +584,12 @@ void lanczos_gs_batch(const std::vector<qed_sector*>& S, const std::vector<doubl
     QED_HIP(hipMemcpyAsync(hb.data(), beta, hb.size() * 8, hipMemcpyDeviceToHost, stream()));
     QED_HIP(hipStreamSynchronize(stream()));
     // stopping test of the scalar Lanczos for the new steps m = k+1 .. k1 of every active
-    // sector, in order: ritz(m) = |beta_m e_m^T y_min(T_m)|; the sector retires at its
-    // first converged step, as the sequential solver stops there (later steps of a small
-    // sector may follow an exhausted Krylov space and are never read)
+    // sector: ritz(m) = |beta_m e_m^T y_min(T_m)|; the sector retires at its first converged
+    // step, as the sequential solver stops there. A sector whose Krylov space can be
+    // exhausted within the chunk (D <= k1: the steps after it follow a zero vector) has
+    // every step tested in order; a larger one has its last step tested first and the
+    // earlier ones only when that has converged (one small eigenproblem per chunk instead
+    // of one per step)
     std::vector<int> newly(nj, -1);
     parallel_for(nj, [&](int j) {
       if (!active[j]) return;
@@ -598,6 +601,7 @@ void lanczos_gs_batch(const std::vector<qed_sector*>& S, const std::vector<doubl
         tridiag_lowest(a, b, &e0, ev);
         return std::fabs(ev.back() * b.back());
       };
+      if (S[j]->D > (uint64_t)k1 && ritz_at(k1) > accur) return;
       for (int m = k + 1; m <= k1; ++m)
         if (ritz_at(m) <= accur) {
           newly[j] = m;

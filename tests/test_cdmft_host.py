@@ -156,3 +156,44 @@ def test_broyden_matrix_start_is_the_inverse_jacobian():
     x3, it3, _ = C.broyden(F, np.zeros(3), 1.5 * np.eye(3), xtol=1e-12, maxiter=200)
     np.testing.assert_array_equal(x2, x3)
     assert it2 == it3
+
+
+class _MapProblem:
+    """A stand-in for a CDMFT problem whose 'impurity solve + bath fit' is the map g: the
+    host is the current point, the fit returns g(host); the fit fails where x[0] > 5."""
+
+    def __init__(self):
+        self.calls = 0
+
+    def solve_batch(self, values_list, x_list):
+        self.calls += len(x_list)
+        return [np.array(x, float) for x in x_list]
+
+    def fit(self, hosts, idx, X, *args):
+        hosts = np.asarray(hosts, float)
+        # contraction to the fixed point (1, 1); beyond x[0] = 10 a map whose pull
+        # vanishes, where Broyden's secant update breaks down
+        far = hosts[:, :1] > 10.0
+        Xn = np.where(far, hosts - 0.05 * np.tanh(hosts - 1.0), 1.0 + 0.5 * (hosts - 1.0))
+        ok = (hosts[:, 0] <= 5.0) | (hosts[:, 0] > 10.0)
+        return Xn, np.zeros(len(hosts)), np.ones(len(hosts), int), ok
+
+
+def test_batched_cdmft_falls_back_to_the_second_start():
+    """A problem whose first start fails (a failed fit; an undefined Broyden step or no
+    convergence within the allowed iterations) starts again from its fallback point and
+    converges there; without a fallback the same problems fail."""
+    from pyqcm_amd.cdmft import BatchedCDMFT
+    vals = [{}, {}, {}]
+    x0 = [[7.0, 1.0], [40.0, -30.0], [0.5, 0.8]]  # fit fails; far away; close
+    cold = [[1.2, 0.9]] * 3
+    P = _MapProblem()
+    plain = BatchedCDMFT(P, vals, x0, solver=P.solve_batch, fit="host")
+    assert plain.failed[0] and plain.failed[1] and not plain.failed[2] and not plain.restarted.any()
+    P = _MapProblem()
+    B = BatchedCDMFT(P, vals, x0, solver=P.solve_batch, fit="host", fallback_list=cold, fallback_after=8)
+    assert B.restarted[0] and B.restarted[1] and not B.restarted[2]
+    assert (B.converged & ~B.failed).all()
+    for i in range(3):
+        np.testing.assert_allclose(B.x[i], [1.0, 1.0], atol=1e-4)
+    assert B.niter[1] > 2  # both attempts counted
