@@ -61,6 +61,23 @@ int qed_sector_create(int L, int n_up, int n_dn, int n_ops, const int* types, co
                       const int* nels, const int* er, const int* ec, const double* ev,
                       qed_sector** out);
 int qed_sector_destroy(qed_sector* s);
+
+/* A sector of the symmetry-adapted (mixed) basis: the representatives of the orbits of an
+ * Abelian point group in irreducible representation `irrep`, for fixed N_up, N_down. The
+ * operators are given as for qed_sector_create; gens holds n_gen generators of L entries,
+ * 0-based: site images for the first L' entries and, with bath_irrep, the phase labels of
+ * the bath orbitals (L' = n_sites with bath_irrep, L otherwise). Generators must commute
+ * and have order <= 2 (real characters). The sector works with every solver entry point
+ * of a sector (qed_hv, qed_lanczos_gs, qed_band_lanczos, ...) except qed_davidson and
+ * qed_apply_c. Replaces ED_mixed_basis + symmetry_group + HS_one_body_operator
+ * (src_ed/ED_basis.cpp:185-361, src_ed/symmetry_group.cpp:18-454,
+ * src_ed/Operators/HS_one_body_operator.hpp:8-40) for the unmixed sectors. */
+int qed_mixed_sector_create(int L, int n_up, int n_dn, int n_ops, const int* types, const double* coefs,
+                            const int* nels, const int* rows, const int* cols, const double* vals, int n_sites,
+                            int n_gen, const int* gens, int bath_irrep, int irrep, qed_sector** out);
+/* The basis of a symmetry-adapted sector: *n states (up | down << 32), ascending, written
+ * to states unless it is NULL (the reference's ED_mixed_basis::binlist). */
+int qed_mixed_basis(const qed_sector* s, uint64_t* states, uint64_t* n);
 int qed_sector_dims(const qed_sector* s, uint64_t* dim, uint32_t* dim_up, uint32_t* dim_dn);
 
 /* y_dev += H x_dev. Replaces Hamiltonian_Factorized::mult_add

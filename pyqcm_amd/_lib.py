@@ -43,6 +43,9 @@ SIGNATURES = [
     ("qed_start_vector", c_int, [c_void_p, c_int, c_void_p]),
     ("qed_lanczos_gs", c_int, [c_void_p, c_double, c_int, c_void_p, dptr, iptr, dptr, dptr]),
     ("qed_lanczos_gs_noalloc", c_int, [c_void_p, c_double, c_int, c_void_p, dptr, iptr, dptr, dptr]),
+    ("qed_mixed_sector_create", c_int, [c_int, c_int, c_int, c_int, iptr, dptr, iptr, iptr, iptr, dptr, c_int,
+                                        c_int, iptr, c_int, c_int, P(c_void_p)]),
+    ("qed_mixed_basis", c_int, [c_void_p, P(c_u64), P(c_u64)]),
     ("qed_lanczos_green", c_int, [c_void_p, c_void_p, c_double, c_int, dptr, dptr, iptr]),
     ("qed_band_lanczos", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double, dptr,
                                  dptr, c_int, iptr]),

@@ -26,7 +26,7 @@ namespace qed {
 
 namespace {
 
-constexpr uint32_t kTargetBits = 20;
+constexpr uint32_t kTargetBits = kEllTargetBits;
 constexpr uint32_t kTargetMask = (1u << kTargetBits) - 1u;
 constexpr uint32_t kMaxVals = 1u << (32 - kTargetBits);
 
@@ -86,14 +86,7 @@ __global__ void k_hop_fill(const uint32_t* bin, uint32_t dim, const Bond* bonds,
 
 // density-density energy of the factorized basis state (su, sd)
 __device__ __forceinline__ double interaction(const HvView& v, uint32_t su, uint32_t sd) {
-  if (v.hubbard_uniform) return v.hub_U * (double)__popc(su & sd & v.hub_sites);
-  uint64_t w = (uint64_t)sd | ((uint64_t)su << 32);
-  double acc = 0.0;
-  for (int k = 0; k < v.n_int; ++k) {
-    uint64_t m = v.int_mask[k];
-    if ((w & m) == m) acc += v.int_val[k];
-  }
-  return acc;
+  return density_density(v, su, sd);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -780,6 +773,10 @@ void ensure_row_order(const qed_sector& s) {
 
 void hv(const qed_sector& s, const double* x, double* y, int variant) {
   if (s.D == 0) return;
+  if (s.mixed) {
+    hv_mixed(s, x, y);
+    return;
+  }
   set_lds_attributes();
   if (variant < 0) variant = s.hv_variant;
   if (variant < 0) variant = 12;
@@ -857,7 +854,7 @@ void hv(const qed_sector& s, const double* x, double* y, int variant) {
 
 
 bool hv_lanczos(const qed_sector& s, const double* x, double* y, const double* lz, double* lz_part) {
-  if (s.D == 0 || s.hv_variant >= 0) return false;
+  if (s.D == 0 || s.hv_variant >= 0 || s.mixed) return false;
   // the default row kernel (tiled row order); the LZ epilogue needs LDS for the row, the
   // amplitudes and eight doubles of reduction scratch
   const size_t lds2 = ((size_t)s.du + s.up_nvals + 1 + 16) * sizeof(double);

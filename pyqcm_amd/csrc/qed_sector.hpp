@@ -53,6 +53,9 @@ struct qed_sector {
   std::vector<double> pu_t, pd_t;
   std::vector<double> dsite_up, dsite_dn;
   mutable std::shared_ptr<void> split;  // plan and device tables, built on first use
+  // symmetry-adapted (mixed) basis of one irreducible representation: the group tables,
+  // the representatives and the factorized sector they live in (null: factorized sector)
+  std::shared_ptr<void> mixed;
 };
 
 namespace qed {
@@ -80,6 +83,21 @@ struct HvView {
 };
 
 HvView view(const qed_sector& s);
+
+// ELL hop entries: target rank | index of the signed amplitude << kEllTargetBits
+constexpr uint32_t kEllTargetBits = 20;
+
+// density-density energy of the factorized basis state (su, sd)
+__device__ __forceinline__ double density_density(const HvView& v, uint32_t su, uint32_t sd) {
+  if (v.hubbard_uniform) return v.hub_U * (double)__popc(su & sd & v.hub_sites);
+  const uint64_t w = (uint64_t)sd | ((uint64_t)su << 32);
+  double acc = 0.0;
+  for (int k = 0; k < v.n_int; ++k) {
+    const uint64_t m = v.int_mask[k];
+    if ((w & m) == m) acc += v.int_val[k];
+  }
+  return acc;
+}
 
 // one sector of a batched H.v: y[parity] += H x[parity]
 struct HvJob {
@@ -159,6 +177,8 @@ std::string split_status(const qed_sector& s);
 // the vector, the partial sums of <x, y_new> in lz_part (dd doubles); false when the
 // sector does not take the fused kernel (nothing launched)
 bool hv_lanczos(const qed_sector& s, const double* x, double* y, const double* lz, double* lz_part);
+// y += H x on a symmetry-adapted sector (s.mixed set)
+void hv_mixed(const qed_sector& s, const double* x, double* y);
 
 // BLAS-1 on the library stream. dot/nrm2 return host scalars (synchronizing). `work`, where
 // given, is device memory for the block partials (kMaxBlocks * 65 doubles); the library's

@@ -111,6 +111,34 @@ class Sector:
         return e[:m].copy(), Q[:m * p].reshape(m, p).T.copy()
 
 
+class MixedSector(Sector):
+    """H on one sector of the symmetry-adapted basis: orbit representatives of an Abelian
+    point group in one irreducible representation (qed_mixed_sector_create). generators:
+    lists of L 0-based entries (site images; with bath_irrep, bath phase labels)."""
+
+    def __init__(self, L, nup, ndn, flat, n_sites, generators, irrep, bath_irrep=False):
+        self.L, self.nup, self.ndn = L, nup, ndn
+        t, c, n, r, cc, v = flat
+        self._flat = flat
+        g = np.ascontiguousarray(np.asarray(generators, np.int32).reshape(-1, L))
+        h = ctypes.c_void_p()
+        call("qed_mixed_sector_create", L, nup, ndn, len(t), ip(t), dp(c), ip(n), ip(r), ip(cc), dp(v),
+             int(n_sites), len(g), ip(g.ravel()), int(bool(bath_irrep)), int(irrep), ctypes.byref(h))
+        self.h = h
+        dim, du, dd = ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_uint32()
+        call("qed_sector_dims", h, ctypes.byref(dim), ctypes.byref(du), ctypes.byref(dd))
+        self.dim, self.du, self.dd = dim.value, du.value, dd.value
+        self.irrep = irrep
+
+    def basis(self):
+        """the representatives as 64-bit states (up | down << 32), ascending"""
+        n = ctypes.c_uint64()
+        call("qed_mixed_basis", self.h, None, ctypes.byref(n))
+        out = np.zeros(n.value, np.uint64)
+        call("qed_mixed_basis", self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(n))
+        return out
+
+
 def apply_c(L, nup, ndn, a, pm, x: DeviceArray, y: DeviceArray, z=1.0):
     """y += z c_a x (pm=-1) or z c†_a x (pm=+1), device vectors."""
     call("qed_apply_c", int(L), int(nup), int(ndn), int(a), int(pm), float(z), x.ptr, y.ptr)

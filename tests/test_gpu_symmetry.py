@@ -1,0 +1,137 @@
+"""Symmetry-adapted (mixed) basis on the device (qed_mixed_sector_create) against the
+oracle's restatement of the reference's point-group reduction (oracle/symmetry.py, itself
+pinned by the union of the irreps' spectra being the sector's dense spectrum):
+
+* the basis (orbit representatives in each irrep) bit-exact;
+* H.v against the oracle's dense mixed-basis Hamiltonian;
+* the Lanczos ground state of each irrep against the oracle's lowest eigenvalue, and the
+  lowest over the irreps against the factorized sector's;
+* the reference's recorded plaquette + 8 bath ground state
+  (tests/test_diagonalization/S_format.ref: GS_energy -75.0353 in R0:N12:S0 with the two
+  mirrors as generators and bath_irrep) in the irrep 0 sector.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import symmetry as Y
+from tests import models as TM
+
+pytestmark = pytest.mark.gpu
+
+MIRRORS = [[1, 0, 3, 2], [2, 3, 0, 1]]
+
+
+def _elements(m, vals):
+    one, inter = [], []
+    for name, op in m.operators.items():
+        z = vals.get(name, 0.0)
+        if z == 0.0:
+            continue
+        for (r, c, v) in op.elements:
+            (one if op.kind == "one-body" else inter).append((r, c, z * v))
+    return one, inter
+
+
+def _plaquette4():
+    rng = np.random.default_rng(3)
+    p = TM.random_bath_params(rng, n_bath=4)
+    m, vals = TM.plaquette_bath(p, n_bath=4)
+    gens = [MIRRORS[0] + [0, 0, 0, 0], MIRRORS[1] + [0, 0, 2, 2]]
+    return m, vals, 4, gens, True
+
+
+def _chain4():
+    m, vals = TM.chain(4, tp=-0.3, U=3.0, mu=1.1)
+    return m, vals, 4, [[3, 2, 1, 0]], False
+
+
+CASES = {"plaquette4": _plaquette4, "chain4": _chain4}
+
+
+@pytest.mark.parametrize("case,nup,ndn", [("plaquette4", 4, 4), ("plaquette4", 3, 4), ("plaquette4", 2, 1),
+                                          ("chain4", 2, 2), ("chain4", 3, 1)])
+def test_mixed_basis_and_hv_match_oracle(gpu_lib, case, nup, ndn):
+    from pyqcm_amd.sector import MixedSector
+    m, vals, n_sites, gens, bath_irrep = CASES[case]()
+    G = Y.SymmetryGroup(m.L, n_sites, gens, bath_irrep=bath_irrep)
+    one, inter = _elements(m, vals)
+    rng = np.random.default_rng(7)
+    for r in range(G.g):
+        ref = Y.mixed_basis(G, nup, ndn, r)
+        S = MixedSector(m.L, nup, ndn, m.flat(vals), n_sites, gens, r, bath_irrep)
+        got = S.basis()
+        assert S.dim == len(ref)
+        np.testing.assert_array_equal(got, np.array(ref, np.uint64))
+        if not ref:
+            S.close()
+            continue
+        H = Y.mixed_hamiltonian(G, ref, r, one, inter)
+        x = rng.standard_normal(len(ref))
+        y0 = rng.standard_normal(len(ref))
+        hy = S.hv(x, y0)
+        err = np.abs(hy - (y0 + H @ x)).max() / np.abs(H @ x).max()
+        print(case, nup, ndn, "irrep", r, "dim", len(ref), "H.v rel err %.1e" % err)
+        assert err < 1e-13
+        S.close()
+
+
+@pytest.mark.parametrize("case,nup,ndn", [("plaquette4", 4, 4), ("chain4", 2, 2)])
+def test_mixed_lanczos_per_irrep(gpu_lib, case, nup, ndn):
+    from pyqcm_amd.sector import MixedSector, Sector
+    m, vals, n_sites, gens, bath_irrep = CASES[case]()
+    G = Y.SymmetryGroup(m.L, n_sites, gens, bath_irrep=bath_irrep)
+    one, inter = _elements(m, vals)
+    lows = []
+    for r in range(G.g):
+        ref = Y.mixed_basis(G, nup, ndn, r)
+        if len(ref) < 2:
+            continue
+        e_ref = np.linalg.eigvalsh(Y.mixed_hamiltonian(G, ref, r, one, inter))[0]
+        S = MixedSector(m.L, nup, ndn, m.flat(vals), n_sites, gens, r, bath_irrep)
+        E0 = S.lanczos_gs(accur=1e-12)[0]
+        print(case, "irrep", r, "E0 %.12f (oracle %.12f)" % (E0, e_ref))
+        assert abs(E0 - e_ref) <= 1e-10 * max(1.0, abs(e_ref))
+        lows.append(E0)
+        S.close()
+    F = Sector(m.L, nup, ndn, m.flat(vals))
+    E_full = F.lanczos_gs(accur=1e-12)[0]
+    assert abs(min(lows) - E_full) <= 1e-10 * abs(E_full)
+
+
+def test_trivial_group_is_the_factorized_sector(gpu_lib):
+    """No generators: one irrep, every factorized state its own representative; the
+    mixed-basis product is the factorized one."""
+    from pyqcm_amd.sector import MixedSector, Sector
+    m, vals = TM.square(3, 3, tp=-0.3)
+    S = MixedSector(m.L, 5, 4, m.flat(vals), m.L, np.zeros((0, m.L), np.int32), 0, False)
+    F = Sector(m.L, 5, 4, m.flat(vals))
+    assert S.dim == F.dim
+    b = S.basis()
+    up, dn = O.halfbasis(m.L, 5), O.halfbasis(m.L, 4)
+    np.testing.assert_array_equal(b, (np.tile(up.astype(np.uint64), len(dn)) |
+                                      (np.repeat(dn.astype(np.uint64), len(up)) << np.uint64(32))))
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(F.dim)
+    y = rng.standard_normal(F.dim)
+    np.testing.assert_allclose(S.hv(x, y), F.hv(x, y), rtol=0, atol=1e-12)
+
+
+def test_plaquette_8bath_ground_state_in_irrep_0(gpu_lib, golden):
+    """tests/test_diagonalization/S_format.ref: new_cluster_model('clus', 4, 8,
+    generators=[[2,1,4,3,0,0,0,0,2,2,2,2],[3,4,1,2,0,0,2,2,0,0,2,2]], bath_irrep=True);
+    the recorded GS_energy -75.0353 lies in R0:N12:S0 (about a quarter of the 853,776
+    factorized states)."""
+    from pyqcm_amd.sector import MixedSector, Sector
+    g = golden["plaquette_8bath"]
+    m, vals = TM.plaquette_bath(g["params"])
+    gens = [[1, 0, 3, 2, 0, 0, 0, 0, 2, 2, 2, 2], [2, 3, 0, 1, 0, 0, 2, 2, 0, 0, 2, 2]]
+    S = MixedSector(m.L, 6, 6, m.flat(vals), 4, gens, 0, True)
+    E0 = S.lanczos_gs(accur=1e-12)[0]
+    F = Sector(m.L, 6, 6, m.flat(vals))
+    E_full = F.lanczos_gs(accur=1e-12)[0]
+    print("irrep 0: dim %d, E0 %.10f; factorized: dim %d, E0 %.10f; recorded %.4f"
+          % (S.dim, E0, F.dim, E_full, g["E0"]))
+    assert abs(E0 - g["E0"]) < 6e-5
+    assert abs(E0 - E_full) <= 1e-10 * abs(E_full)
+    assert S.dim < F.dim / 3
