@@ -78,6 +78,18 @@ int qed_mixed_sector_create(int L, int n_up, int n_dn, int n_ops, const int* typ
 /* The basis of a symmetry-adapted sector: *n states (up | down << 32), ascending, written
  * to states unless it is NULL (the reference's ED_mixed_basis::binlist). */
 int qed_mixed_basis(const qed_sector* s, uint64_t* states, uint64_t* n);
+/* y += c_o x (pm = -1) or c^dagger_o x (pm = +1) between symmetry-adapted sectors, for the
+ * symmetric orbital o = sum_a coef[a] c_{a, spin} over the n_sites sites (spin 0 = up):
+ * src is x's sector, tgt the one of y (one particle less or more of that spin). Replaces
+ * destruction_operator (src_ed/Operators/destruction_operator.cpp:14-33, with Destroy,
+ * src_ed/ED_basis.cpp:60-77); the creation operator is its transpose, as the reference
+ * applies it (src_ed/model.cpp:160-215). */
+int qed_mixed_apply_c(const qed_sector* src, const qed_sector* tgt, int spin, int pm, int n_sites,
+                      const double* coef, const double* x_dev, double* y_dev);
+/* The group of a symmetry-adapted sector: its order g, its elements (g x L entries: site
+ * images, bath phase labels) and phi (g x g, in units of 2 pi / g), as symmetry_group
+ * builds them (src_ed/symmetry_group.cpp:183-352); elements and phi may be NULL. */
+int qed_mixed_group(const qed_sector* s, int* g, int* elements, int* phi);
 int qed_sector_dims(const qed_sector* s, uint64_t* dim, uint32_t* dim_up, uint32_t* dim_dn);
 
 /* y_dev += H x_dev. Replaces Hamiltonian_Factorized::mult_add

@@ -135,3 +135,19 @@ def test_plaquette_8bath_ground_state_in_irrep_0(gpu_lib, golden):
     assert abs(E0 - g["E0"]) < 6e-5
     assert abs(E0 - E_full) <= 1e-10 * abs(E_full)
     assert S.dim < F.dim / 3
+
+
+@pytest.mark.parametrize("case,nup,ndn", [("plaquette4", 4, 4), ("chain4", 2, 2)])
+def test_mixed_green_function_matches_dense(gpu_lib, case, nup, ndn):
+    """The Green function of the irrep-0 ground state through symmetric orbitals, the
+    creation and destruction operators between irreps and a band Lanczos per orbital
+    irrep, turned to the site basis, against the exact G of the dense Jordan-Wigner
+    Hamiltonian of the whole sector (the ground state is unique and in irrep 0 here)."""
+    from pyqcm_amd import symmetry as SY
+    m, vals, n_sites, gens, bath_irrep = CASES[case]()
+    zs = np.array([0.3j, 1.0j, 2.5j, 0.7 + 0.2j])
+    E0, G = SY.green_function(m.L, n_sites, m.flat(vals), gens, bath_irrep, nup, ndn, 0, zs)
+    E0d, Gd = O.dense_gf(m, vals, nup, ndn, list(range(n_sites)), zs)
+    print(case, "E0 %.12f dense %.12f" % (E0, E0d), "max |dG| %.2e" % np.abs(G - Gd).max())
+    assert abs(E0 - E0d) <= 1e-10 * abs(E0d)
+    np.testing.assert_allclose(G, Gd, atol=1e-9)
