@@ -247,8 +247,17 @@ int qed_cdmft_fit_batch(int n, int nb, int nw, const double* w, const double* we
 
 /* ---- the ED solver interface (namespace ED, src_ed/qcm_ED.hpp:23-340) -------------- */
 
-/* ED::new_model (src_ed/qcm_ED.cpp:39-58); trivial symmetry group only. */
+/* ED::new_model (src_ed/qcm_ED.cpp:39-58) without generators: the trivial group. */
 int qed_ed_new_model(const char* name, int n_sites, int n_bath);
+/* ED::new_model (src_ed/qcm_ED.cpp:39-58) with n_gen generators of an Abelian point group
+ * (gens: n_gen x (n_sites + n_bath), 0-based site images, then the bath entries: phase
+ * labels when bath_irrep, else 0-based orbital images, as the reference's binding passes
+ * them, src_python/qcm_ED_wrap.hpp:145-163). Generators of order > 2 (complex characters)
+ * are rejected. Instances of such a model solve each "R<r>:N<n>:S<s>" sector in the
+ * symmetry-adapted basis (qed_mixed_sector_create) and build the Green function from the
+ * symmetric orbitals (src_ed/model_instance.hpp:710-817), reported in the site basis. */
+int qed_ed_new_model_generators(const char* name, int n_sites, int n_bath, int n_gen, const int* gens,
+                                int bath_irrep);
 /* ED::new_operator (src_ed/qcm_ED.cpp:73-92): type "one-body" or "interaction";
  * 0-based indices (the binding converts 1-based labels). */
 int qed_ed_new_operator(const char* model, const char* name, const char* type, int nel,

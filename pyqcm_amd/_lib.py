@@ -69,6 +69,7 @@ SIGNATURES = [
                                     dptr, dptr, dptr, c_double, c_double, c_double, c_int, c_double,
                                     dptr, dptr, iptr, iptr]),
     ("qed_ed_new_model", c_int, [c_char_p, c_int, c_int]),
+    ("qed_ed_new_model_generators", c_int, [c_char_p, c_int, c_int, c_int, iptr, c_int]),
     ("qed_ed_new_operator", c_int, [c_char_p, c_char_p, c_char_p, c_int, iptr, iptr, dptr]),
     ("qed_ed_new_model_instance", c_int, [c_char_p, c_int, P(c_char_p), dptr, c_char_p, c_u64]),
     ("qed_ed_erase_model_instance", c_int, [c_u64]),

@@ -40,7 +40,7 @@ struct Globals {
                                   {"band_lanczos_minimum_gap", 1e-5}};
   std::map<std::string, bool> b{{"verb_ED", false}, {"no_degenerate_BL", false}, {"continued_fraction", false},
                               {"block_Lanczos_QR", false}, {"combine_mcf", false}};
-  std::map<std::string, char> c{{"GF_method", 'L'}, {"Ground_state_method", 'L'}, {"Hamiltonian_format", 'F'}};
+  std::map<std::string, char> c{{"GF_method", 'L'}, {"Ground_state_method", 'L'}, {"Hamiltonian_format", 'E'}};
 };
 Globals& G() {
   static Globals g;
@@ -79,6 +79,15 @@ struct Model {
   std::map<std::string, size_t> idx;
   int mixing = -1;
   bool closed = false;
+  // point group (src_ed/symmetry_group.hpp): generators as given to the library (0-based
+  // site images, then bath entries), the group's order, elements and characters, and the
+  // symmetric orbitals of the sites (rows of sorb, n_sites x n_sites, and their irreps)
+  std::vector<std::vector<int>> gens;
+  bool bath_irrep = false;
+  GroupTables group;
+  std::vector<double> sorb;
+  std::vector<int> sorb_irrep;
+  bool symmetric() const { return group.g > 1; }
 };
 
 struct QMat {
@@ -102,7 +111,7 @@ struct MCFrac {
 };
 
 struct State {
-  int nup = 0, ndn = 0;
+  int nup = 0, ndn = 0, irrep = 0;
   double energy = 0.0, weight = 1.0;
   DevBuf psi;
   QMat gf[2];
@@ -116,6 +125,7 @@ struct Instance {
   Model* model = nullptr;
   std::map<std::string, double> value;
   std::vector<std::pair<int, int>> sectors;  // (N, S)
+  std::vector<int> irreps;                   // R of each target sector
   int mixing = 0;
   bool correlated = false, gs_solved = false, gf_solved = false;
   bool cf = false;   // continued fractions ('F')
@@ -175,15 +185,35 @@ void flatten(const Instance& I, std::vector<int>& types, std::vector<double>& co
   }
 }
 
-SectorPtr make_sector(const Instance& I, int nup, int ndn, const std::string* only = nullptr) {
+// the sector (nup, ndn) of the instance's Hamiltonian (or of one operator): factorized for
+// a trivial group, else the symmetry-adapted sector of irreducible representation irrep
+SectorPtr make_sector(const Instance& I, int nup, int ndn, const std::string* only = nullptr, int irrep = 0) {
   std::vector<int> types, nels, er, ec;
   std::vector<double> coefs, ev;
   flatten(I, types, coefs, nels, er, ec, ev, only);
   qed_sector* s = nullptr;
-  if (qed_sector_create(I.model->L, nup, ndn, (int)types.size(), types.data(), coefs.data(), nels.data(),
-                        er.data(), ec.data(), ev.data(), &s) != 0)
-    fail(qed_last_error());
+  const Model& M = *I.model;
+  int rc;
+  if (M.symmetric()) {
+    std::vector<int> gflat;
+    for (const auto& gen : M.gens) gflat.insert(gflat.end(), gen.begin(), gen.end());
+    rc = qed_mixed_sector_create(M.L, nup, ndn, (int)types.size(), types.data(), coefs.data(), nels.data(),
+                                 er.data(), ec.data(), ev.data(), M.n_sites, (int)M.gens.size(), gflat.data(),
+                                 M.bath_irrep ? 1 : 0, irrep, &s);
+  } else {
+    if (irrep != 0) fail("irreducible representation " + std::to_string(irrep) + " of the trivial group");
+    rc = qed_sector_create(M.L, nup, ndn, (int)types.size(), types.data(), coefs.data(), nels.data(), er.data(),
+                           ec.data(), ev.data(), &s);
+  }
+  if (rc != 0) fail(qed_last_error());
   return SectorPtr(s);
+}
+
+// a function of the trivial-group solver only
+void require_trivial_group(const Instance& I, const char* what) {
+  if (I.model->symmetric())
+    fail(std::string(what) + " is not available for a model with a nontrivial point group (model " +
+         I.model->name + ")");
 }
 
 // ---- one-body matrices (model_instance::set_hopping_matrix, model_instance.hpp:424-475) --
@@ -361,7 +391,127 @@ bool sector_valid(const Instance& I, int nup, int ndn) {
   return nup >= 0 && ndn >= 0 && nup <= L && ndn <= L;
 }
 
+// the character of irrep r at element i, real groups (every generator of order <= 2)
+double character(const GroupTables& G, int r, int i) {
+  const int ph = G.phi[(size_t)r * G.g + i];
+  if (ph == 0) return 1.0;
+  if (2 * ph == G.g) return -1.0;
+  fail("complex characters are not supported");
+  return 0.0;
+}
+
+// the irrep whose characters are the products of those of a and b (symmetry_group::tensor)
+int irrep_tensor(const GroupTables& G, int a, int b) {
+  for (int r = 0; r < G.g; ++r) {
+    bool ok = true;
+    for (int i = 0; i < G.g && ok; ++i)
+      ok = G.phi[(size_t)r * G.g + i] == (G.phi[(size_t)a * G.g + i] + G.phi[(size_t)b * G.g + i]) % G.g;
+    if (ok) return r;
+  }
+  fail("no product of the irreducible representations " + std::to_string(a) + " and " + std::to_string(b));
+  return 0;
+}
+
+// symmetric orbitals of the sites (symmetry_group::build, src_ed/symmetry_group.cpp:293-327):
+// for each irrep, each orbit of sites gives sum_i chi(r, e_i) |e_i(site)>, kept (normalized)
+// unless it vanishes; n_sites of them in all
+void build_symmetric_orbitals(Model& M) {
+  const int n = M.n_sites, g = M.group.g;
+  M.sorb.clear();
+  M.sorb_irrep.clear();
+  for (int r = 0; r < g; ++r) {
+    std::vector<char> skip(n, 0);
+    for (int site = 0; site < n; ++site) {
+      if (skip[site]) continue;
+      skip[site] = 1;
+      std::vector<double> orb(n, 0.0);
+      for (int i = 0; i < g; ++i) {
+        const int img = M.group.elements[i][site];
+        if (img < 0 || img >= n) fail("the point group maps a cluster site outside the cluster");
+        skip[img] = 1;
+        orb[img] += character(M.group, r, i);
+      }
+      double nrm = 0.0;
+      for (double v : orb) nrm += v * v;
+      nrm = std::sqrt(nrm);
+      if (nrm < 1e-12) continue;
+      for (double v : orb) M.sorb.push_back(v / nrm);
+      M.sorb_irrep.push_back(r);
+    }
+  }
+  if ((int)M.sorb_irrep.size() != n)
+    fail("Problem with construction of symmetric orbitals: " + std::to_string(M.sorb_irrep.size()) +
+         " found instead of " + std::to_string(n));
+}
+
+// Lehmann form of one state of a symmetry-adapted sector (model_instance::build_qmatrix,
+// src_ed/model_instance.hpp:710-817): per irrep r_o of the symmetric orbitals, c_o|psi>
+// and c^dagger_o|psi> lie in the sectors of irrep r (x) r_o with one particle less or
+// more; a band Lanczos (or full diagonalization) there gives that irrep's block, which is
+// streamlined per part and checked for unitarity in the symmetric basis, as the reference
+// does, then turned to the site basis (Q_site = S_o^T Q_o) and concatenated with the others
+void build_qmatrix_sym(Instance& I, State& st, int spin) {
+  Model& M = *I.model;
+  const int n = M.n_sites;
+  SectorPtr B = make_sector(I, st.nup, st.ndn, nullptr, st.irrep);
+  QMat tot;
+  tot.L = n;
+  for (int ro = 0; ro < M.group.g; ++ro) {
+    std::vector<int> sel;
+    for (int o = 0; o < n; ++o)
+      if (M.sorb_irrep[o] == ro) sel.push_back(o);
+    const int p = (int)sel.size();
+    if (p == 0) continue;
+    const int rt = irrep_tensor(M.group, st.irrep, ro);
+    QMat parts[2];
+    for (int s = 0; s < 2; ++s) {
+      const int pm = 2 * s - 1;
+      const int tnup = st.nup + (spin == 0 ? pm : 0), tndn = st.ndn + (spin == 1 ? pm : 0);
+      if (!sector_valid(I, tnup, tndn)) continue;
+      SectorPtr T = make_sector(I, tnup, tndn, nullptr, rt);
+      if (T->D == 0) continue;
+      DevBuf phi((size_t)p * T->D * 8);
+      QED_HIP(hipMemsetAsync(phi.p, 0, (size_t)p * T->D * 8, stream()));
+      for (int k = 0; k < p; ++k)
+        if (qed_mixed_apply_c(B.get(), T.get(), spin, pm, n, &M.sorb[(size_t)sel[k] * n], st.psi.as<double>(),
+                              phi.as<double>() + (uint64_t)k * T->D) != 0)
+          fail(qed_last_error());
+      QMat q = sector_qmatrix(*T, phi.as<double>(), p);
+      for (double& e : q.e) {
+        e -= st.energy;
+        if (pm == -1) e = -e;
+      }
+      parts[s] = streamline(q);
+    }
+    QMat blk;
+    blk.L = p;
+    append(blk, parts[0]);
+    append(blk, parts[1]);
+    blk.L = p;
+    check_norm(blk, gd("accur_Q_matrix"));
+    QMat site;
+    site.L = n;
+    site.e = blk.e;
+    site.Q.assign((size_t)n * blk.M(), 0.0);
+    for (int m = 0; m < blk.M(); ++m)
+      for (int k = 0; k < p; ++k) {
+        const double w = blk.Q[k + (size_t)m * p];
+        const double* row = &M.sorb[(size_t)sel[k] * n];
+        for (int a = 0; a < n; ++a) site.Q[a + (size_t)m * n] += row[a] * w;
+      }
+    append(tot, site);
+  }
+  tot.L = n;
+  const double sw = std::sqrt(st.weight);
+  for (double& v : tot.Q) v *= sw;
+  st.gf[spin] = tot;
+}
+
 void build_qmatrix(Instance& I, State& st, int spin) {
+  if (I.model->symmetric()) {
+    build_qmatrix_sym(I, st, spin);
+    return;
+  }
   Model& M = *I.model;
   const int p = M.n_sites, L = M.L;
   QMat parts[2];
@@ -397,6 +547,7 @@ void build_qmatrix(Instance& I, State& st, int spin) {
 
 // continued fractions (model_instance<double>::build_cf, src_ed/model_instance.cpp:34-99)
 void build_cf(Instance& I, State& st, int spin) {
+  require_trivial_group(I, "the continued-fraction Green function");
   Model& M = *I.model;
   const int n = M.n_sites, L = M.L;
   st.cfe[spin].assign((size_t)n * n, CFrac());
@@ -436,6 +587,7 @@ void build_cf(Instance& I, State& st, int spin) {
 // start block c_i^(+-)|psi>, its modified Gram-Schmidt factor W (first pa columns that do
 // not deflate), block Lanczos on the orthonormal block, diagonal blocks shifted by E0
 void build_mcf(Instance& I, State& st, int spin) {
+  require_trivial_group(I, "the matrix continued-fraction Green function");
   Model& M = *I.model;
   const int p = M.n_sites, L = M.L;
   const double acc = gd("accur_deflation");
@@ -496,8 +648,8 @@ void build_mcf(Instance& I, State& st, int spin) {
   }
 }
 
-std::string sector_name(int N, int S) {
-  return "R0:N" + std::to_string(N) + ":S" + std::to_string(S);
+std::string sector_name(int N, int S, int R = 0) {
+  return "R" + std::to_string(R) + ":N" + std::to_string(N) + ":S" + std::to_string(S);
 }
 
 // ---- ground state (model_instance::low_energy_states, model_instance.hpp:245-417) ------
@@ -543,13 +695,14 @@ void finish_states(Instance& I, std::vector<std::unique_ptr<State>>& states);
 
 // keeps the states of one sector within max_gap of the lowest energy seen so far
 void add_states(Instance& I, int nup, int ndn, const std::vector<double>& evals, std::vector<DevBuf>& vecs,
-                double max_gap, std::vector<std::unique_ptr<State>>& states) {
+                double max_gap, std::vector<std::unique_ptr<State>>& states, int irrep = 0) {
   if (evals[0] < I.GS_energy) I.GS_energy = evals[0];
   for (size_t k = 0; k < evals.size(); ++k) {
     if (evals[k] - I.GS_energy > max_gap) continue;
     auto st = std::make_unique<State>();
     st->nup = nup;
     st->ndn = ndn;
+    st->irrep = irrep;
     st->energy = evals[k];
     st->psi = std::move(vecs[k]);
     states.push_back(std::move(st));
@@ -568,11 +721,11 @@ void low_energy_states(Instance& I) {
   if (max_gap < MIN_GAP) max_gap = MIN_GAP;
   I.GS_energy = 1.0e12;
   std::vector<std::unique_ptr<State>> states;
-  for (auto& sec : I.sectors) {
-    const int N = sec.first, S = sec.second;
+  for (size_t is = 0; is < I.sectors.size(); ++is) {
+    const int N = I.sectors[is].first, S = I.sectors[is].second, R = I.irreps[is];
     const int nup = (N + S) / 2, ndn = (N - S) / 2;
     if (!sector_valid(I, nup, ndn)) continue;
-    SectorPtr H = make_sector(I, nup, ndn);
+    SectorPtr H = make_sector(I, nup, ndn, nullptr, R);
     if (H->D == 0) continue;
     int n_states = std::max(1, gi("n_states"));
     if ((uint64_t)n_states > H->D) n_states = (int)H->D;
@@ -605,9 +758,9 @@ void low_energy_states(Instance& I) {
       }
     } catch (const std::exception& e) {
       fail(std::string(e.what()) + "\n  in cluster model '" + I.model->name + "', instance '" + I.model->name +
-           "_" + std::to_string(I.label) + "', sector '" + sector_name(N, S) + "'");
+           "_" + std::to_string(I.label) + "', sector '" + sector_name(N, S, R) + "'");
     }
-    add_states(I, nup, ndn, evals, vecs, max_gap, states);
+    add_states(I, nup, ndn, evals, vecs, max_gap, states, R);
   }
   finish_states(I, states);
 }
@@ -659,7 +812,7 @@ void finish_states(Instance& I, std::vector<std::unique_ptr<State>>& states) {
     I.E0 += s->weight * s->energy;
     I.GS_energy = std::min(I.GS_energy, s->energy);
     if (o.tellp() > 0) o << "/";
-    o << sector_name(s->nup + s->ndn, s->nup - s->ndn) << ":1";
+    o << sector_name(s->nup + s->ndn, s->nup - s->ndn, s->irrep) << ":1";
   }
   I.gs_string = o.str();
   set_hopping_matrix(I);
@@ -802,7 +955,9 @@ void batch_solve(std::vector<Instance*>& insts, bool graph, int chunk, BatchStat
   std::vector<Instance*> batched;
   for (Instance* I : insts) {
     if (I->gs_solved) continue;
-    const bool plain = I->correlated && gc("Ground_state_method") == 'L' && std::max(1, gi("n_states")) == 1;
+    // symmetry-adapted instances take the one-by-one path
+    const bool plain = I->correlated && gc("Ground_state_method") == 'L' && std::max(1, gi("n_states")) == 1 &&
+                       !I->model->symmetric();
     if (!plain) {
       low_energy_states(*I);
       continue;
@@ -878,7 +1033,7 @@ void batch_solve(std::vector<Instance*>& insts, bool graph, int chunk, BatchStat
   for (Instance* I : insts) {
     if (I->gf_solved || !I->correlated) continue;
     set_gf_format(*I);
-    if (I->cf || I->mcf || I->q_to_mcf) {
+    if (I->cf || I->mcf || I->q_to_mcf || I->model->symmetric()) {
       green_function_solve(*I);
       continue;
     }
@@ -1180,6 +1335,7 @@ struct SusceptibilityPart {
 };
 
 std::vector<SusceptibilityPart> susceptibility_parts(Instance& I, const std::string& op) {
+  require_trivial_group(I, "the susceptibility");
   if (!I.model->idx.count(op)) fail("Operator " + op + " is not defined in the model.");
   if (!I.value.count(op)) fail("operator " + op + " is not activated in instance " + std::to_string(I.label));
   low_energy_states(I);
@@ -1342,6 +1498,7 @@ void read_gf(const h5::Node& g, const Instance& I, State& s, int spin, const std
 }
 
 void write_instance_file(Instance& I, const std::string& path, const std::string& group) {
+  require_trivial_group(I, "the instance file");
   green_function_solve(I);
   const std::string fmt = gf_format_name(I);
   try {
@@ -1376,6 +1533,7 @@ void write_instance_file(Instance& I, const std::string& path, const std::string
 std::pair<int, int> sector_numbers(const std::string& s);
 
 void read_instance_file(Instance& I, const std::string& path, const std::string& group) {
+  require_trivial_group(I, "the instance file");
   std::vector<std::unique_ptr<State>> states;
   double gse = 0.0;
   std::string fmt, gs;
@@ -1457,7 +1615,12 @@ std::vector<std::string> split(const std::string& s, char c) {
 }
 
 // "R0:N12:S0" -> (N, S)
-std::pair<int, int> parse_sector(const std::string& s) {
+// "R<r>:N<n>:S<s>" (sector::sector, src_ed/sector.cpp): irrep r of a group of order g,
+// N particles, S = N_up - N_down
+struct SectorLabel {
+  int R, N, S;
+};
+SectorLabel parse_sector_label(const std::string& s, int g) {
   int R = -1, N = -1000, S = -1000;
   for (auto& tok : split(s, ':')) {
     if (tok.empty()) continue;
@@ -1471,9 +1634,14 @@ std::pair<int, int> parse_sector(const std::string& s) {
       fail("sector " + s + " is not valid");
     }
   }
-  if (R != 0 || N == -1000 || S == -1000) fail("sector " + s + " is not valid");
+  if (R < 0 || R >= g || N == -1000 || S == -1000) fail("sector " + s + " is not valid");
   if ((N + S) % 2 != 0 || S > N || S < -N || N < 0) fail("sector " + s + " is not valid");
-  return {N, S};
+  return {R, N, S};
+}
+
+std::pair<int, int> parse_sector(const std::string& s) {
+  const SectorLabel l = parse_sector_label(s, 1);
+  return {l.N, l.S};
 }
 }  // namespace
 
@@ -1482,15 +1650,25 @@ std::pair<int, int> qed::ed::sector_numbers(const std::string& s) { return parse
 extern "C" {
 
 int qed_ed_new_model(const char* name, int n_sites, int n_bath) {
+  return qed_ed_new_model_generators(name, n_sites, n_bath, 0, nullptr, 0);
+}
+
+int qed_ed_new_model_generators(const char* name, int n_sites, int n_bath, int n_gen, const int* gens,
+                                int bath_irrep) {
   std::lock_guard<std::recursive_mutex> lk(lib_mutex());
   return guard([&] {
     if (models().count(name)) fail(std::string("the name '") + name + "' has already been used for another cluster model");
     if (n_sites <= 0 || n_bath < 0 || n_sites + n_bath > 32) fail("invalid model size");
+    if (n_gen < 0 || (n_gen > 0 && !gens)) fail("invalid generators");
     auto m = std::make_unique<Model>();
     m->name = name;
     m->n_sites = n_sites;
     m->n_bath = n_bath;
     m->L = n_sites + n_bath;
+    for (int k = 0; k < n_gen; ++k) m->gens.emplace_back(gens + (size_t)k * m->L, gens + (size_t)(k + 1) * m->L);
+    m->bath_irrep = n_gen > 0 && bath_irrep != 0;
+    m->group = group_tables(m->L, n_sites, m->gens, m->bath_irrep);
+    if (m->symmetric()) build_symmetric_orbitals(*m);
     models()[name] = std::move(m);
   });
 }
@@ -1561,11 +1739,16 @@ int qed_ed_new_model_instance(const char* model, int nparam, const char* const* 
       }
       if (up != dn) mixing = 4;
     }
+    // a nontrivial group needs a format other than 'F' (model_instance_base.cpp:86-88);
+    // every other format is served by the device's symmetry-adapted product
+    if (M.symmetric() && gc("Hamiltonian_format") == 'F')
+      fail("the Hamiltonian format 'factorized' is not compatible with a nontrivial point group symmetry");
     for (auto& s : split(sectors, '/')) {
       if (s.empty()) continue;
-      auto ns = parse_sector(s);
-      if (ns.second != 0 && mixing == 0) mixing = 4;
-      I->sectors.push_back(ns);
+      const SectorLabel l = parse_sector_label(s, M.group.g);
+      if (l.S != 0 && mixing == 0) mixing = 4;
+      I->sectors.push_back({l.N, l.S});
+      I->irreps.push_back(l.R);
     }
     if (I->sectors.empty()) fail("no target sectors were specified for model " + M.name);
     if (M.mixing == -1) M.mixing = mixing;
@@ -1885,6 +2068,8 @@ int qed_ed_fidelity(uint64_t label1, uint64_t label2, double* fidelity) {
   return guard([&] {
     Instance& A = inst(label1);
     Instance& B = inst(label2);
+    require_trivial_group(A, "the fidelity");
+    require_trivial_group(B, "the fidelity");
     low_energy_states(A);
     low_energy_states(B);
     double z = 0.0;
@@ -1950,6 +2135,7 @@ int qed_ed_print_wavefunction(uint64_t label, char* buf, int buflen, int* needed
     // (ED_factorized_basis::print_state, src_ed/ED_basis.cpp:167-170; PrintBinaryDouble,
     // src_ed/binary_state.cpp:104-112)
     Instance& I = inst(label);
+    require_trivial_group(I, "print_wavefunction");
     low_energy_states(I);
     const int L = I.model->L;
     std::ostringstream o;

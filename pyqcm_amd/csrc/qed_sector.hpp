@@ -179,6 +179,16 @@ std::string split_status(const qed_sector& s);
 bool hv_lanczos(const qed_sector& s, const double* x, double* y, const double* lz, double* lz_part);
 // y += H x on a symmetry-adapted sector (s.mixed set)
 void hv_mixed(const qed_sector& s, const double* x, double* y);
+// The Abelian group of a list of generators (0-based site images, then bath entries:
+// phase labels with bath_irrep, orbital images otherwise), on the host: order g, elements
+// (g x N entries) and phi(a, b) in units of 2 pi / g (g x g); throws on invalid generators
+// (src_ed/symmetry_group.cpp:36-100, 183-352)
+struct GroupTables {
+  int g = 1;
+  std::vector<std::vector<int>> elements;
+  std::vector<int> phi;
+};
+GroupTables group_tables(int N, int n_sites, const std::vector<std::vector<int>>& gens, bool bath_irrep);
 
 // BLAS-1 on the library stream. dot/nrm2 return host scalars (synchronizing). `work`, where
 // given, is device memory for the block partials (kMaxBlocks * 65 doubles); the library's

@@ -31,11 +31,29 @@ def _b(s):
 # ---------------------------------------------------------------- models and operators
 
 def new_model(name, n_sites, n_bath, generators=None, bath_irrep=False):
-    """ED::new_model (src_ed/qcm_ED.cpp:39-58). Only the trivial point group is supported:
-    the factorized solver requires it (src_ed/model_instance_base.cpp:87-88)."""
-    if generators:
-        raise RuntimeError("the factorized format is not compatible with a nontrivial point group symmetry")
-    call("qed_ed_new_model", _b(name), int(n_sites), int(n_bath))
+    """ED::new_model (src_ed/qcm_ED.cpp:39-58). generators: a two-dimensional array of
+    n_sites + n_bath entries per generator, site images 1-based, then bath entries (phase
+    labels if bath_irrep, else 1-based orbital images), converted to 0-based as the
+    reference's binding does (src_python/qcm_ED_wrap.hpp:145-163). A model with a
+    nontrivial group needs a Hamiltonian_format other than 'F'
+    (src_ed/model_instance_base.cpp:86-88)."""
+    n_orb = int(n_sites) + int(n_bath)
+    if generators is None or len(generators) == 0:
+        call("qed_ed_new_model", _b(name), int(n_sites), int(n_bath))
+    else:
+        gen = np.asarray(generators)
+        if gen.ndim != 2:
+            raise RuntimeError("generators or positions should form a two-dimensional array!")
+        gen = gen.astype(np.int32)
+        for j in range(len(gen)):
+            if gen.shape[1] != n_orb:
+                raise RuntimeError("generator %d should have %d elements" % (j + 1, n_orb))
+        gen[:, :int(n_sites)] -= 1
+        if not bath_irrep:
+            gen[:, int(n_sites):] -= 1
+        gen = np.ascontiguousarray(gen)
+        call("qed_ed_new_model_generators", _b(name), int(n_sites), int(n_bath), len(gen), ip(gen.ravel()),
+             int(bool(bath_irrep)))
     _model_sizes[name] = (int(n_sites), int(n_bath))
 
 

@@ -1,0 +1,156 @@
+"""Models with generators through the instance layer (qcm.new_model(..., generators,
+bath_irrep)): sectors "R<r>:N<n>:S<s>" solved in the symmetry-adapted basis and the Green
+function built from the symmetric orbitals (src_ed/model_instance.hpp:710-817), against
+
+* the exact G of the dense Jordan-Wigner Hamiltonian of the whole (N_up, N_down) sector
+  (oracle.dense_gf), including a ground state outside irrep 0 and both spins of an
+  S = 1 sector;
+* the same model without generators (E0, G, averages);
+* the reference's recorded plaquette + 8 bath run with its two mirrors
+  (tests/test_diagonalization/S_format.ref: GS_energy -75.0353 in R0:N12:S0, the Lehmann
+  poles of each C2v irrep);
+* the reference's error behaviour (src_ed/model_instance_base.cpp:86-88,
+  src_ed/symmetry_group.cpp:36-100).
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import models as TM
+
+pytestmark = pytest.mark.gpu
+
+_counter = itertools.count(1)
+
+# 1-based site images; bath entries are phase labels (bath_irrep)
+PLAQ_MIRRORS_4B = [[2, 1, 4, 3, 0, 0, 0, 0], [3, 4, 1, 2, 0, 0, 2, 2]]
+PLAQ_MIRRORS_8B = [[2, 1, 4, 3, 0, 0, 0, 0, 2, 2, 2, 2], [3, 4, 1, 2, 0, 0, 2, 2, 0, 0, 2, 2]]
+
+
+def register(model, values, sectors, n_sites, generators=None, bath_irrep=False):
+    from pyqcm_amd import qcm
+    name = "sym%d" % next(_counter)
+    qcm.new_model(name, n_sites, model.L - n_sites, generators, bath_irrep)
+    for opname, op in model.operators.items():
+        qcm.new_operator(name, opname, op.kind, [(r + 1, c + 1, v) for (r, c, v) in op.elements])
+    label = 9000 + next(_counter)
+    qcm.new_model_instanceC(name, values, sectors, label)
+    return label
+
+
+def _plaquette4():
+    p = TM.random_bath_params(np.random.default_rng(3), n_bath=4)
+    m, vals = TM.plaquette_bath(p, n_bath=4)
+    return m, vals, 4, PLAQ_MIRRORS_4B, True
+
+
+def _chain4():
+    m, vals = TM.chain(4, tp=-0.3, U=3.0, mu=1.1)
+    return m, vals, 4, [[4, 3, 2, 1]], False
+
+
+CASES = {"plaquette4": _plaquette4, "chain4": _chain4}
+ZS = np.array([0.3j, 1.0j, 2.5j, 0.7 + 0.2j])
+
+
+def _lowest_irrep(m, vals, n_sites, gens, bath_irrep, N, S):
+    """(irrep, E0) of the lowest state over the irreps of sector (N, S)"""
+    from pyqcm_amd import qcm
+    best = None
+    for r in range(4):
+        try:
+            lab = register(m, vals, "R%d:N%d:S%d" % (r, N, S), n_sites, gens, bath_irrep)
+        except Exception:
+            break
+        E0, _ = qcm.ground_state_solve(lab)
+        if best is None or E0 < best[1] - 1e-9:
+            best = (r, E0)
+    return best
+
+
+@pytest.mark.parametrize("case,N,S", [("plaquette4", 8, 0), ("chain4", 4, 0), ("chain4", 3, 1),
+                                      ("plaquette4", 8, 2)])
+def test_instance_green_function_matches_dense(gpu_lib, case, N, S):
+    """Ground state in its own irrep (the lowest over the irreps; irrep 2 for the
+    plaquette's N=8, S=2), then G of both spins against the dense sector Hamiltonians (the
+    spin-down G of (N_up, N_down) is the spin-up G of (N_down, N_up) for these
+    spin-symmetric models). The cases have a nondegenerate sector ground state, so that
+    the dense G is that of the same state."""
+    from pyqcm_amd import qcm
+    m, vals, ns, gens, bi = CASES[case]()
+    nup, ndn = (N + S) // 2, (N - S) // 2
+    e = np.linalg.eigvalsh(O.dense_hamiltonian(m, vals, nup, ndn)[0])
+    assert e[1] - e[0] > 1e-6
+    r, _ = _lowest_irrep(m, vals, ns, gens, bi, N, S)
+    lab = register(m, vals, "R%d:N%d:S%d" % (r, N, S), ns, gens, bi)
+    E0, sec = qcm.ground_state_solve(lab)
+    assert sec == "R%d:N%d:S%d:1" % (r, N, S)
+    E0d, Gd = O.dense_gf(m, vals, nup, ndn, list(range(ns)), ZS)
+    print(case, N, S, "irrep", r, "E0 %.12f dense %.12f" % (E0, E0d))
+    assert abs(E0 - E0d) <= 1e-10 * abs(E0d)
+    G = qcm.Green_function(ZS, 0, lab)
+    np.testing.assert_allclose(G, Gd, atol=1e-9)
+    if S != 0:
+        _, Gdn = O.dense_gf(m, vals, ndn, nup, list(range(ns)), ZS)
+        np.testing.assert_allclose(qcm.Green_function(ZS, 1, lab), Gdn, atol=1e-9)
+
+
+def test_plaquette_8bath_with_mirrors(gpu_lib, golden):
+    """S_format.ref's run: E0 in R0:N12:S0 (recorded -75.0353), equal to the trivial-group
+    solve; each C2v irrep's G(iw) from the recorded poles; the averages of every operator
+    against the trivial-group instance."""
+    from pyqcm_amd import qcm
+    g = golden["plaquette_8bath"]
+    m, vals = TM.plaquette_bath(g["params"])
+    lab = register(m, vals, "R0:N12:S0", 4, PLAQ_MIRRORS_8B, True)
+    ref = register(m, vals, "R0:N12:S0", 4)
+    E0, sec = qcm.ground_state_solve(lab)
+    E0t, _ = qcm.ground_state_solve(ref)
+    print("E0 symmetric %.10f trivial %.10f recorded %.4f" % (E0, E0t, g["E0"]))
+    assert sec == "R0:N12:S0:1"
+    assert abs(E0 - g["E0"]) < 6e-5
+    assert abs(E0 - E0t) <= 1e-10 * abs(E0t)
+    ws = np.array([0.1, 0.5, 1.0, 2.0, 5.0])
+    G = qcm.Green_function(1j * ws, 0, lab)
+    S = 0.5 * np.array([[1, 1, 1, 1], [1, -1, 1, -1], [1, 1, -1, -1], [1, -1, -1, 1]], float)
+    Girr = np.array([[S[r] @ G[i] @ S[r] for r in range(4)] for i in range(len(ws))])
+    for blk in g["blocks"][:4]:
+        e = np.array(blk["e"])
+        q = np.array(blk["Q"])[:, 0, 0]
+        Gref = np.array([np.sum(q * q / (1j * w - e)) for w in ws])
+        err = min(np.abs(Girr[:, r] - Gref).max() for r in range(4))
+        assert err < 2e-5, err
+    # the irrep blocks do not mix: S G S^T is diagonal
+    for i in range(len(ws)):
+        off = S @ G[i] @ S.T
+        assert np.abs(off - np.diag(np.diag(off))).max() < 1e-12
+    a, b = qcm.cluster_averages(lab), qcm.cluster_averages(ref)
+    for op in a:
+        assert abs(a[op][0] - b[op][0]) < 1e-8, op
+        assert abs(a[op][1] - b[op][1]) < 1e-7, op
+
+
+def test_symmetric_model_errors(gpu_lib):
+    from pyqcm_amd import qcm
+    m, vals = TM.chain(4, U=3.0, mu=1.1)
+    with pytest.raises(RuntimeError, match="not valid"):
+        qcm.new_model("symbad%d" % next(_counter), 4, 0, [[2, 2, 4, 3]])
+    with pytest.raises(RuntimeError, match="complex characters"):
+        qcm.new_model("symrot%d" % next(_counter), 4, 0, [[2, 3, 4, 1]])
+    with pytest.raises(RuntimeError, match="do not commute"):
+        qcm.new_model("symnc%d" % next(_counter), 3, 0, [[2, 1, 3], [1, 3, 2]])
+    with pytest.raises(RuntimeError, match="should have 4 elements"):
+        qcm.new_model("symlen%d" % next(_counter), 4, 0, [[4, 3, 2]])
+    with pytest.raises(RuntimeError, match="not valid"):
+        register(m, vals, "R2:N4:S0", 4, [[4, 3, 2, 1]])
+    qcm.set_global_parameter("Hamiltonian_format", "F")
+    try:
+        with pytest.raises(RuntimeError, match="factorized"):
+            register(m, vals, "R0:N4:S0", 4, [[4, 3, 2, 1]])
+    finally:
+        qcm.set_global_parameter("Hamiltonian_format", "E")
+    lab = register(m, vals, "R0:N4:S0", 4, [[4, 3, 2, 1]])
+    with pytest.raises(RuntimeError, match="point group"):
+        qcm.susceptibility("U", np.array([0.5j]), lab)
