@@ -15,6 +15,8 @@ Prints one JSON line (rank 0, N=1):
   lanczos      ms and GB/s per ground-state Lanczos iteration on the same sector (48 D bytes:
                the product plus the vector update), fused and as separate passes, with
                3 iterations of the oracle's recurrence timed on 1 and up to 16 host cores
+  ground_state the workload's Lanczos ground state (E0, iterations, seconds) on the
+               factorized sector and on irrep 0 of the cluster's two-mirror basis
   cpt          the CPT k-sum of the 3x4 cluster (128^2 k-points x 512 frequencies), device
                time and FP64 rate on 41.5 kflop per (k, w)
   cdmft        the second half of the metric: converged CDMFT solves per second on the
@@ -174,6 +176,7 @@ def main():
     dx.free()
     dy.free()
     del S
+    gs = ground_state_leg(cluster_hamiltonian_4x4(vals)) if args.lanczos else None
     cpt = cpt_leg(cpu=not args.no_cpu) if args.cpt else None
     cdmft = cdmft_leg(args.cdmft, args.cdmft_batch, not args.no_graph, args.cdmft_jacobian, args.mlp_ensemble) if args.cdmft > 0 else None
 
@@ -204,6 +207,7 @@ def main():
         "cpu_baseline": cpu,
         "cpu_parallel": cpu_par,
         "lanczos": lanczos,
+        "ground_state": gs,
         "cpt": cpt,
         "cdmft": cdmft,
     }
@@ -242,6 +246,47 @@ def lanczos_leg(S, dx, steps=20):
             "ms_per_iteration_separate_passes": round(float(ms[1]), 4),
             "GB_s": round(per / ((fused if fused else ms[1]) * 1e-3) / 1e9, 1),
             "fused": fused is not None}
+
+
+def ground_state_leg(flat):
+    """The workload's ground state by Lanczos (seed 0, accur 1e-12) on the factorized sector
+    and on irrep 0 of the symmetry-adapted basis of the cluster's two mirrors (C2v; the
+    ground state is fully symmetric): E0, iterations and wall seconds of each, and the
+    mirror basis's product time (HIP events)."""
+    from pyqcm_amd import _lib
+    from pyqcm_amd._lib import DeviceArray
+    from pyqcm_amd.sector import MixedSector, Sector
+    out = {}
+    S = Sector(16, 8, 8, flat)
+    t0 = time.perf_counter()
+    E0, psi, n, *_ = S.lanczos_gs(seed=0, accur=1e-12, max_iter=600)
+    out["factorized"] = {"dim": S.dim, "E0": E0, "iterations": n, "seconds": round(time.perf_counter() - t0, 3)}
+    psi.free()
+    S.close()
+    mx = [(3 - i % 4) + 4 * (i // 4) for i in range(16)]
+    my = [i % 4 + 4 * (3 - i // 4) for i in range(16)]
+    t0 = time.perf_counter()
+    M = MixedSector(16, 8, 8, flat, 16, [mx, my], 0, False)
+    tb = time.perf_counter() - t0
+    rng = np.random.default_rng(3)
+    dx = DeviceArray.from_host(rng.standard_normal(M.dim))
+    dy = DeviceArray.from_host(rng.standard_normal(M.dim))
+    ms = np.zeros(1)
+    _lib.call("qed_hv_timed", M.h, -1, dx.ptr, dy.ptr, 2, 10, _lib.dp(ms))
+    dx.free()
+    dy.free()
+    t0 = time.perf_counter()
+    E1, psi, n1, *_ = M.lanczos_gs(seed=0, accur=1e-12, max_iter=600)
+    out["mirror_basis_irrep0"] = {"dim": M.dim, "E0": E1, "iterations": n1,
+                                  "seconds": round(time.perf_counter() - t0, 3), "build_seconds": round(tb, 3),
+                                  "hv_ms": round(float(ms[0]), 3),
+                                  "hv_GB_s": round(24.0 * M.dim / (ms[0] * 1e-3) / 1e9, 1)}
+    psi.free()
+    M.close()
+    out["E0_difference"] = abs(E1 - E0)
+    log("ground state: factorized E0 %.12f (%d it, %.2f s); mirror basis irrep 0 E0 %.12f (%d it, %.2f s, "
+        "H.v %.2f ms)" % (E0, n, out["factorized"]["seconds"], E1, n1, out["mirror_basis_irrep0"]["seconds"], ms[0]))
+    return out
 
 
 def copy_calibration():

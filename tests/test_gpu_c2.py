@@ -6,6 +6,7 @@ mu=4) on its half-filled N=16, Sz=0 sector (D = 165,636,900).
   residual check of the returned eigenvector.
 * Band-Lanczos Green function with 16 start vectors per excitation kind (N=15 and N=17
   sectors, D = 147,232,800 each): Q-matrix sum rule, symmetry, high-frequency tail.
+* The ground state in the symmetry-adapted basis of the two mirrors (irrep 0).
 * VCA k-integral on the Legendre grid (20 frequencies) and a 64^2 k-grid: the device
   integrand against the dense restatement on sampled k points, and Omega finite.
 """
@@ -67,6 +68,34 @@ def test_lanczos_16_sites_matches_oracle_fixture(gpu_lib):
     res2 = float(np.dot(r, r))
     print("residual |H psi - E0 psi|^2 = %.3e" % res2)
     assert res2 <= 1e-10 * E0 * E0
+
+
+def test_ground_state_16_sites_in_the_mirror_basis(gpu_lib):
+    """The same sector in the symmetry-adapted basis of the cluster's two mirrors
+    (x -> 3 - x, y -> 3 - y; C2v, 4 irreps): irrep 0 holds D = 41,412,900 of the
+    165,636,900 states, and its Lanczos ground state is the fixture's E0 (the ground state
+    is fully symmetric), with H psi = E0 psi in that basis."""
+    from pyqcm_amd._lib import DeviceArray
+    from pyqcm_amd.sector import MixedSector
+    with open(os.path.join(HERE, "golden", "c2_lanczos.json")) as f:
+        fx = json.load(f)
+    m, vals = TM.square(4, 4, tp=-0.3, U=8.0, mu=4.0)
+    mx = [(3 - i % 4) + 4 * (i // 4) for i in range(16)]
+    my = [i % 4 + 4 * (3 - i // 4) for i in range(16)]
+    S = MixedSector(m.L, 8, 8, m.flat(vals), 16, [mx, my], 0, False)
+    assert S.dim == 41412900
+    E0, psi, n, *_ = S.lanczos_gs(seed=0, accur=1e-12, max_iter=600)
+    print("irrep 0: E0 %.14f (fixture %.14f), %d iterations" % (E0, fx["E0"], n))
+    assert abs(E0 - fx["E0"]) <= 1e-10 * abs(fx["E0"])
+    hp = DeviceArray(S.dim).zeros()
+    S.hv_dev(psi, hp)
+    g, h = psi.to_host(), hp.to_host()
+    hp.free()
+    psi.free()
+    S.close()
+    r = h - E0 * g
+    assert abs(np.dot(g, g) - 1.0) < 1e-10
+    assert float(np.dot(r, r)) <= 1e-10 * E0 * E0
 
 
 def test_band_lanczos_green_function_16_sites(c2_instance):

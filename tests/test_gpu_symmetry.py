@@ -151,3 +151,25 @@ def test_mixed_green_function_matches_dense(gpu_lib, case, nup, ndn):
     print(case, "E0 %.12f dense %.12f" % (E0, E0d), "max |dG| %.2e" % np.abs(G - Gd).max())
     assert abs(E0 - E0d) <= 1e-10 * abs(E0d)
     np.testing.assert_allclose(G, Gd, atol=1e-9)
+
+
+def test_mixed_lanczos_4x4_quarter_filling(gpu_lib):
+    """The 4x4 cluster at N = 8 (N_up = N_down = 4, 3.3 M factorized states) in the
+    two-mirror basis: the irreps' dimensions add up to the factorized sector's, and the
+    lowest Lanczos E0 over the irreps is the factorized sector's."""
+    from pyqcm_amd.sector import MixedSector, Sector
+    m, vals = TM.square(4, 4, tp=-0.3, U=8.0, mu=2.0)
+    mx = [(3 - i % 4) + 4 * (i // 4) for i in range(16)]
+    my = [i % 4 + 4 * (3 - i // 4) for i in range(16)]
+    lows, dims = [], []
+    for r in range(4):
+        S = MixedSector(m.L, 4, 4, m.flat(vals), 16, [mx, my], r, False)
+        dims.append(S.dim)
+        lows.append(S.lanczos_gs(accur=1e-12)[0])
+        S.close()
+    F = Sector(m.L, 4, 4, m.flat(vals))
+    E_full = F.lanczos_gs(accur=1e-12)[0]
+    print("irrep dims", dims, "of", F.dim, "E0 per irrep", lows, "factorized", E_full)
+    # an orbit of length l survives in the l irreps that are trivial on its stabilizer
+    assert sum(dims) == F.dim
+    assert abs(min(lows) - E_full) <= 1e-10 * abs(E_full)
