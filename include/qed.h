@@ -90,6 +90,24 @@ int qed_mixed_apply_c(const qed_sector* src, const qed_sector* tgt, int spin, in
  * images, bath phase labels) and phi (g x g, in units of 2 pi / g), as symmetry_group
  * builds them (src_ed/symmetry_group.cpp:183-352); elements and phi may be NULL. */
 int qed_mixed_group(const qed_sector* s, int* g, int* elements, int* phi);
+/* A sector of the reference's mixed basis for the spin-flip and anomalous mixings
+ * (src_ed/ED_basis.cpp:185-361, trivial group): the union of the (N_up, N_down) blocks
+ * pairs[2k], pairs[2k+1] (k < npairs), as 64-bit states ascending. Operator types 0
+ * one-body (spin-flip elements allowed), 1 density-density interaction, 2 anomalous
+ * (elements r < c as ED::new_operator receives them, HS_anomalous_operator,
+ * src_ed/Operators/HS_anomalous_operator.hpp:8-40). qed_hv and the Krylov entry points
+ * (not qed_davidson) take such a sector; its dim_up and dim_dn are 0. */
+int qed_general_sector_create(int L, int npairs, const int* pairs, int n_ops, const int* types,
+                              const double* coefs, const int* nels, const int* rows, const int* cols,
+                              const double* vals, qed_sector** out);
+/* The basis of such a sector (64-bit states up | down << 32, ascending), as
+ * qed_mixed_basis. */
+int qed_general_basis(const qed_sector* s, uint64_t* states, uint64_t* n);
+/* y += z c_a x (pm = -1) or z c^dagger_a x (pm = +1) between two such sectors, spin
+ * orbital a (down orbitals at L + i), with the Jordan-Wigner sign of the 64-bit word
+ * (Destroy, src_ed/ED_basis.cpp:60-77; the creation operator is its transpose). */
+int qed_general_apply(const qed_sector* src, const qed_sector* tgt, int orbital, int pm, double z,
+                      const double* x_dev, double* y_dev);
 int qed_sector_dims(const qed_sector* s, uint64_t* dim, uint32_t* dim_up, uint32_t* dim_dn);
 
 /* y_dev += H x_dev. Replaces Hamiltonian_Factorized::mult_add

@@ -282,9 +282,9 @@ def sector_states(L, nup, ndn):
     return np.array([u | (d << L) for d in bd for u in bu], np.int64)
 
 
-def dense_hamiltonian(model: ClusterModel, values, nup, ndn):
-    """Dense H on one sector, built from Jordan-Wigner matrices (independent of the
-    restatement's half-operator tables)."""
+def dense_hamiltonian(model: ClusterModel, values, nup, ndn, states=None):
+    """Dense H on one sector (or on the Fock states listed in states), built from
+    Jordan-Wigner matrices (independent of the restatement's half-operator tables)."""
     L = model.L
     c = _fock_ops(L)
     cd = [m.T.tocsr() for m in c]
@@ -301,11 +301,69 @@ def dense_hamiltonian(model: ClusterModel, values, nup, ndn):
                     H = H + z * v * (cd[r] @ c[cc] + cd[cc] @ c[r])
                 elif r == cc:
                     H = H + z * v * (cd[r] @ c[r])
+            elif op.kind == "anomalous":
+                # (r, c, v) with r < c as the user passes it to new_operator: the binding
+                # halves v (src_python/qcm_ED_wrap.hpp:179), anomalous_operator stores
+                # (r, c, v/2) and (c, r, -v/2) (src_ed/Operators/anomalous_operator.hpp:54-58),
+                # and HS_anomalous_operator (HS_anomalous_operator.hpp:20-36) puts
+                # X = 2 (-v/2) pauli at <J|.|I> for J = I with bits r and c removed by
+                # pair_annihilate(mask(c), mask(r)) (src_ed/binary_state.cpp:36-43, pauli =
+                # (-1)^(bits strictly between)), and X at <I|.|J>. With Jordan-Wigner
+                # operators c_c c_r |I> = (-1)^(bits strictly between) |J>, so the operator
+                # is v (c_r c_c + h.c.).
+                if r < cc:
+                    P = c[r] @ c[cc]
+                    H = H + z * v * (P + P.T)
             else:
                 a, b = min(r, cc), max(r, cc)
                 H = H + z * v * ((cd[a] @ c[a]) @ (cd[b] @ c[b]))
-    idx = sector_states(L, nup, ndn)
+    idx = sector_states(L, nup, ndn) if states is None else np.asarray(states, np.int64)
     return H[idx][:, idx].toarray(), (c, cd)
+
+
+def block_states(L, pairs):
+    """Fock-space labels (up | down << L) of the union of the (N_up, N_down) blocks in
+    pairs, ascending: the order of the reference's mixed basis for sectors that conserve
+    only N or only S (src_ed/ED_basis.cpp:185-361, trivial group), whose binary states
+    (up | down << 32) sort the same way."""
+    out = []
+    for (nu, nd) in pairs:
+        out.extend(int(x) for x in sector_states(L, nu, nd))
+    return np.array(sorted(out), np.int64)
+
+
+def dense_spinor_gf(model, values, states0, spinor, states_minus, states_plus, zs):
+    """Brute-force G_ab(z) = <A_a (z-H+E0)^-1 A_b^+> + <A_b^+ (z+H-E0)^-1 A_a> for the
+    spinor components A_a = c_a (nambu False) or c_a^+ (nambu True), spinor =
+    [(spin orbital a, nambu)], from the ground state of the basis states0 (labels);
+    A_a |0> lies in states_minus and A_a^+ |0> in states_plus. T = 0, nondegenerate
+    ground state."""
+    L = model.L
+    H0, (c, cd) = dense_hamiltonian(model, values, 0, 0, states=states0)
+    e, U = np.linalg.eigh(H0)
+    E0, psi = e[0], U[:, 0]
+    dimF = 1 << (2 * L)
+    full = np.zeros(dimF)
+    full[np.asarray(states0)] = psi
+    A = [(cd[a] if nb else c[a]) for (a, nb) in spinor]
+    out = np.zeros((len(zs), len(spinor), len(spinor)), complex)
+    for part, st in ((-1, states_minus), (1, states_plus)):
+        if len(st) == 0:
+            continue
+        Hs, _ = dense_hamiltonian(model, values, 0, 0, states=st)
+        es, Us = np.linalg.eigh(Hs)
+        st = np.asarray(st)
+        if part < 0:
+            phi = np.array([(Aa @ full)[st] for Aa in A])
+            Q = phi @ Us
+            for i, z in enumerate(zs):
+                out[i] += (Q / (z + (es - E0))) @ Q.T
+        else:
+            phi = np.array([(Aa.T @ full)[st] for Aa in A])
+            Q = phi @ Us
+            for i, z in enumerate(zs):
+                out[i] += (Q / (z - (es - E0))) @ Q.T
+    return E0, out
 
 
 def dense_gf(model, values, nup, ndn, orbitals, zs):

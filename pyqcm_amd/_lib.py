@@ -46,6 +46,10 @@ SIGNATURES = [
     ("qed_mixed_sector_create", c_int, [c_int, c_int, c_int, c_int, iptr, dptr, iptr, iptr, iptr, dptr, c_int,
                                         c_int, iptr, c_int, c_int, P(c_void_p)]),
     ("qed_mixed_basis", c_int, [c_void_p, P(c_u64), P(c_u64)]),
+    ("qed_general_sector_create", c_int, [c_int, c_int, iptr, c_int, iptr, dptr, iptr, iptr, iptr, dptr,
+                                          P(c_void_p)]),
+    ("qed_general_basis", c_int, [c_void_p, P(c_u64), P(c_u64)]),
+    ("qed_general_apply", c_int, [c_void_p, c_void_p, c_int, c_int, c_double, c_void_p, c_void_p]),
     ("qed_mixed_apply_c", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, dptr, c_void_p, c_void_p]),
     ("qed_mixed_group", c_int, [c_void_p, iptr, iptr, iptr]),
     ("qed_lanczos_green", c_int, [c_void_p, c_void_p, c_double, c_int, dptr, dptr, iptr]),

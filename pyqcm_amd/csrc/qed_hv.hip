@@ -777,6 +777,10 @@ void hv(const qed_sector& s, const double* x, double* y, int variant) {
     hv_mixed(s, x, y);
     return;
   }
+  if (s.general) {
+    hv_general(s, x, y);
+    return;
+  }
   set_lds_attributes();
   if (variant < 0) variant = s.hv_variant;
   if (variant < 0) variant = 12;
@@ -854,7 +858,7 @@ void hv(const qed_sector& s, const double* x, double* y, int variant) {
 
 
 bool hv_lanczos(const qed_sector& s, const double* x, double* y, const double* lz, double* lz_part) {
-  if (s.D == 0 || s.hv_variant >= 0 || s.mixed) return false;
+  if (s.D == 0 || s.hv_variant >= 0 || !s.factorized()) return false;
   // the default row kernel (tiled row order); the LZ epilogue needs LDS for the row, the
   // amplitudes and eight doubles of reduction scratch
   const size_t lds2 = ((size_t)s.du + s.up_nvals + 1 + 16) * sizeof(double);

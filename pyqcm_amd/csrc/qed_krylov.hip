@@ -801,7 +801,7 @@ void davidson(qed_sector& s, int K, double accur, int max_iter, std::vector<doub
               int* niter, bool verb) {
   const uint64_t n = s.D;
   const double tiny = 1e-14;
-  if (s.mixed) fail("Davidson: not available on a symmetry-adapted sector");
+  if (!s.factorized()) fail("Davidson: available on factorized sectors only");
   if ((uint64_t)K > n) K = (int)n;
   if (K < 1) fail("Davidson: no state requested");
   const int J = (int)std::min<uint64_t>(3 * (uint64_t)K, n);

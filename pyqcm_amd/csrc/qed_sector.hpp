@@ -56,6 +56,10 @@ struct qed_sector {
   // symmetry-adapted (mixed) basis of one irreducible representation: the group tables,
   // the representatives and the factorized sector they live in (null: factorized sector)
   std::shared_ptr<void> mixed;
+  // basis of a sector that conserves N or S but not both (spin-flip and anomalous
+  // mixings): the sorted 64-bit states and the Hamiltonian's terms (null otherwise)
+  std::shared_ptr<void> general;
+  bool factorized() const { return !mixed && !general; }
 };
 
 namespace qed {
@@ -179,6 +183,8 @@ std::string split_status(const qed_sector& s);
 bool hv_lanczos(const qed_sector& s, const double* x, double* y, const double* lz, double* lz_part);
 // y += H x on a symmetry-adapted sector (s.mixed set)
 void hv_mixed(const qed_sector& s, const double* x, double* y);
+// y += H x on a sector of the spin-flip / anomalous mixings (s.general set)
+void hv_general(const qed_sector& s, const double* x, double* y);
 // The Abelian group of a list of generators (0-based site images, then bath entries:
 // phase labels with bath_irrep, orbital images otherwise), on the host: order g, elements
 // (g x N entries) and phi(a, b) in units of 2 pi / g (g x g); throws on invalid generators

@@ -17,9 +17,9 @@ def flatten_ops(ops, values):
     for name, kind, elements in ops:
         if name not in values or values[name] == 0.0:
             continue
-        if kind not in ("one-body", "interaction"):
-            raise ValueError("operator type %r is not supported by the factorized solver" % kind)
-        types.append(0 if kind == "one-body" else 1)
+        if kind not in ("one-body", "interaction", "anomalous"):
+            raise ValueError("operator type %r is not supported" % kind)
+        types.append({"one-body": 0, "interaction": 1, "anomalous": 2}[kind])
         coefs.append(float(values[name]))
         nels.append(len(elements))
         for (r, c, v) in elements:
@@ -137,6 +137,41 @@ class MixedSector(Sector):
         out = np.zeros(n.value, np.uint64)
         call("qed_mixed_basis", self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(n))
         return out
+
+
+class GeneralSector(Sector):
+    """H on a sector that conserves N or S but not both (spin-flip and anomalous mixings,
+    trivial group): the union of the (N_up, N_down) blocks in pairs, 64-bit states
+    ascending (qed_general_sector_create). flat: flatten_ops output; operator type 2 is
+    anomalous."""
+
+    def __init__(self, L, pairs, flat):
+        self.L = L
+        self.nup = self.ndn = -1
+        t, c, n, r, cc, v = flat
+        self._flat = flat
+        pr = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
+        h = ctypes.c_void_p()
+        call("qed_general_sector_create", L, len(pr), ip(pr.ravel()), len(t), ip(t), dp(c), ip(n), ip(r), ip(cc),
+             dp(v), ctypes.byref(h))
+        self.h = h
+        dim = ctypes.c_uint64()
+        call("qed_sector_dims", h, ctypes.byref(dim), None, None)
+        self.dim, self.du, self.dd = dim.value, 0, 0
+
+    def basis(self):
+        """the basis as 64-bit states (up | down << 32), ascending"""
+        n = ctypes.c_uint64()
+        call("qed_general_basis", self.h, None, ctypes.byref(n))
+        out = np.zeros(n.value, np.uint64)
+        call("qed_general_basis", self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(n))
+        return out
+
+    def apply(self, target, orbital, pm, x, y, z=1.0):
+        """y += z c_a x (pm = -1) or z c+_a x (pm = +1), x in this sector, y in target's
+        (DeviceArray or device address)"""
+        call("qed_general_apply", self.h, target.h, int(orbital), int(pm), float(z), getattr(x, "ptr", x),
+             getattr(y, "ptr", y))
 
 
 def apply_c(L, nup, ndn, a, pm, x: DeviceArray, y: DeviceArray, z=1.0):

@@ -434,3 +434,42 @@ def test_mcf_with_deflated_start_block(qr):
                 assert np.abs(O.mcf_evaluate(A, B, W, z) - O.qmatrix_gf(e - E0, Q, z)).max() < 1e-11
         else:
             assert len(q) == 5 and len(A) == 1
+
+
+def test_anomalous_dense_term_matches_bdg_closed_form():
+    """The oracle's anomalous term, v (c_r c_c + h.c.) for an element (r, c, v) as the user
+    gives it (the binding's 1/2 and the HS operator's 2 restated), on a noninteracting
+    chain with singlet pairing: the ground-state energy of the S = 0 blocks equals the
+    Bogoliubov-de Gennes closed form tr(h) + sum of the negative eigenvalues of
+    [[h, D], [D^T, -h]] with D_ij = -v_ij for the element (i up, j down); a mistake of the
+    factor 2 or of the operator's form changes it."""
+    n = 4
+    m, vals = hubbard_like_chain(n)
+    L = m.L
+    d, s = 0.45, 0.3
+    m.add("D", "anomalous", [(i, i + 1 + L, 1.0) for i in range(n - 1)] + [(i + 1, i + L, 1.0) for i in range(n - 1)])
+    m.add("S", "anomalous", [(i, i + L, 1.0) for i in range(n)])
+    vals.update({"D": d, "S": s})
+    st = O.block_states(L, [(k, k) for k in range(L + 1)])
+    H, _ = O.dense_hamiltonian(m, vals, 0, 0, states=st)
+    E0 = np.linalg.eigvalsh(H)[0]
+    h = np.zeros((n, n))
+    for (r, c, v) in m.operators["t"].elements:
+        if r < L and c < L:
+            h[r, c] += vals["t"] * v
+            h[c, r] += vals["t"] * v if r != c else 0.0
+    for (r, c, v) in m.operators["mu"].elements:
+        if r < L:
+            h[r, r] += vals["mu"] * v
+    D = np.zeros((n, n))
+    for name, z in (("D", d), ("S", s)):
+        for (r, c, v) in m.operators[name].elements:
+            D[r, c - L] += -z * v
+    M = np.block([[h, D], [D.T, -h]])
+    e = np.linalg.eigvalsh(M)
+    closed = np.trace(h) + e[e < 0].sum()
+    assert abs(E0 - closed) < 1e-12, (E0, closed)
+
+
+def hubbard_like_chain(n):
+    return O.hubbard_cluster(n, [(i, i + 1) for i in range(n - 1)], t=1.0, U=0.0, mu=0.7)
