@@ -276,8 +276,11 @@ int qed_ed_new_model(const char* name, int n_sites, int n_bath);
  * symmetric orbitals (src_ed/model_instance.hpp:710-817), reported in the site basis. */
 int qed_ed_new_model_generators(const char* name, int n_sites, int n_bath, int n_gen, const int* gens,
                                 int bath_irrep);
-/* ED::new_operator (src_ed/qcm_ED.cpp:73-92): type "one-body" or "interaction";
- * 0-based indices (the binding converts 1-based labels). */
+/* ED::new_operator (src_ed/qcm_ED.cpp:73-92): type "one-body", "interaction" or
+ * "anomalous" (elements r < c, halved by the binding as the reference's); one-body elements
+ * between the spins and anomalous operators set the instance's spin-flip / anomalous
+ * mixing; an operator has one target (cluster, bath or cluster-bath); 0-based indices (the
+ * binding converts 1-based labels). */
 int qed_ed_new_operator(const char* model, const char* name, const char* type, int nel,
                         const int* r, const int* c, const double* v);
 /* ED::new_model_instance (src_ed/qcm_ED.cpp:127-158); sectors "R0:N4:S0/R0:N6:S0". */

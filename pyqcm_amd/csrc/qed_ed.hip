@@ -70,7 +70,18 @@ struct Op {
   std::vector<double> v;
   int target = 1;  // 1 cluster, 2 bath, 3 cluster-bath (src_ed/Operators/Hermitian_operator.hpp)
   bool interaction = false;
+  bool anomalous = false;
+  // the mixing the operator needs: 1 anomalous, 2 spin flip (one-body elements between the
+  // spins, or same-spin pairing), as one_body_operator::check_spin_flip and
+  // anomalous_operator::check_spin_flip set it (one_body_operator.hpp:198-206,
+  // anomalous_operator.hpp:113-121)
+  int mixing = 0;
 };
+
+// the reference's codes for a particle number or spin that is not conserved
+// (src_ed/sector.hpp:24-25)
+constexpr int kSecOdd = 9999, kSecEven = 10000;
+bool sec_code(int x) { return x >= kSecOdd; }
 
 struct Model {
   std::string name;
@@ -112,6 +123,7 @@ struct MCFrac {
 
 struct State {
   int nup = 0, ndn = 0, irrep = 0;
+  int secN = 0, secS = 0;  // the sector (N, S; codes where not conserved)
   double energy = 0.0, weight = 1.0;
   DevBuf psi;
   QMat gf[2];
@@ -124,7 +136,7 @@ struct Instance {
   uint64_t label = 0;
   Model* model = nullptr;
   std::map<std::string, double> value;
-  std::vector<std::pair<int, int>> sectors;  // (N, S)
+  std::vector<std::pair<int, int>> sectors;  // (N, S); kSecOdd / kSecEven where not conserved
   std::vector<int> irreps;                   // R of each target sector
   int mixing = 0;
   bool correlated = false, gs_solved = false, gf_solved = false;
@@ -158,6 +170,38 @@ Instance& inst(uint64_t label) {
   return *it->second;
 }
 
+// spinor components per site: 2 with anomalous terms (Nambu), 2 with spin flip, 4 with both
+int n_mixed(int mixing) { return ((mixing & 1) ? 2 : 1) * ((mixing & 2) ? 2 : 1); }
+bool spinor_mixing(int mixing) { return (mixing & 3) != 0; }
+// dimension of the Green function (model_instance_base.cpp:75-80)
+int dim_gf(const Instance& I) { return I.model->n_sites * n_mixed(I.mixing); }
+
+// a function that the spin-flip and anomalous mixings do not reach
+void require_no_spinor(const Instance& I, const char* what) {
+  if (spinor_mixing(I.mixing))
+    fail(std::string(what) + " is not available with Hilbert space mixing " + std::to_string(I.mixing));
+}
+
+// the (N_up, N_down) blocks of a sector (ED_mixed_basis, src_ed/ED_basis.cpp:185-361)
+std::vector<std::pair<int, int>> sector_blocks(int L, int N, int S) {
+  std::vector<std::pair<int, int>> out;
+  if (!sec_code(N) && !sec_code(S)) {
+    out.push_back({(N + S) / 2, (N - S) / 2});
+  } else if (sec_code(N) && !sec_code(S)) {
+    for (int nu = 0; nu <= L; ++nu)
+      if (nu - S >= 0 && nu - S <= L) out.push_back({nu, nu - S});
+  } else if (!sec_code(N) && sec_code(S)) {
+    for (int nu = 0; nu <= L; ++nu)
+      if (N - nu >= 0 && N - nu <= L) out.push_back({nu, N - nu});
+  } else {
+    const int par = N == kSecOdd ? 1 : 0;
+    for (int nu = 0; nu <= L; ++nu)
+      for (int nd = 0; nd <= L; ++nd)
+        if ((nu + nd) % 2 == par) out.push_back({nu, nd});
+  }
+  return out;
+}
+
 struct SectorDeleter {
   void operator()(qed_sector* s) const { qed_sector_destroy(s); }
 };
@@ -176,7 +220,7 @@ void flatten(const Instance& I, std::vector<int>& types, std::vector<double>& co
       if (it == I.value.end() || it->second == 0.0) continue;
       z = it->second;
     }
-    types.push_back(op.interaction ? 1 : 0);
+    types.push_back(op.interaction ? 1 : (op.anomalous ? 2 : 0));
     coefs.push_back(z);
     nels.push_back((int)op.v.size());
     er.insert(er.end(), op.r.begin(), op.r.end());
@@ -209,6 +253,30 @@ SectorPtr make_sector(const Instance& I, int nup, int ndn, const std::string* on
   return SectorPtr(s);
 }
 
+// a sector (N, S) of the spin-flip / anomalous mixings (codes where not conserved)
+SectorPtr make_general_sector(const Instance& I, int N, int S, const std::string* only = nullptr) {
+  std::vector<int> types, nels, er, ec;
+  std::vector<double> coefs, ev;
+  flatten(I, types, coefs, nels, er, ec, ev, only);
+  const auto blocks = sector_blocks(I.model->L, N, S);
+  std::vector<int> pairs;
+  for (auto& b : blocks) {
+    pairs.push_back(b.first);
+    pairs.push_back(b.second);
+  }
+  qed_sector* s = nullptr;
+  if (qed_general_sector_create(I.model->L, (int)blocks.size(), pairs.data(), (int)types.size(), types.data(),
+                                coefs.data(), nels.data(), er.data(), ec.data(), ev.data(), &s) != 0)
+    fail(qed_last_error());
+  return SectorPtr(s);
+}
+
+// the sector of a state (factorized, symmetry-adapted or mixed-number)
+SectorPtr state_sector(const Instance& I, const State& st, const std::string* only = nullptr) {
+  if (spinor_mixing(I.mixing)) return make_general_sector(I, st.secN, st.secS, only);
+  return make_sector(I, st.nup, st.ndn, only, st.irrep);
+}
+
 // a function of the trivial-group solver only
 void require_trivial_group(const Instance& I, const char* what) {
   if (I.model->symmetric())
@@ -216,8 +284,147 @@ void require_trivial_group(const Instance& I, const char* what) {
          I.model->name + ")");
 }
 
+// diagonalized bath of spin index sp: eb, and tcb rotated into the bath's eigenbasis
+// (nc cluster rows, nbg bath orbitals)
+void diagonalize_bath(Instance& I, int sp, int nc, int nbg) {
+  I.eb[sp].assign(nbg, 0.0);
+  I.tcb_d[sp] = I.tcb[sp];
+  if (nbg == 0) return;
+  std::vector<double> a(I.tb[sp]), w(nbg);
+  bool diag = true;
+  for (int i = 0; i < nbg && diag; ++i)
+    for (int j = 0; j < nbg; ++j)
+      if (i != j && a[i + (size_t)j * nbg] != 0.0) {
+        diag = false;
+        break;
+      }
+  if (diag) {
+    for (int i = 0; i < nbg; ++i) I.eb[sp][i] = a[i + (size_t)i * nbg];
+    return;
+  }
+  sym_eigen(nbg, a.data(), w.data(), true);
+  I.eb[sp] = w;
+  std::vector<double> t((size_t)nc * nbg, 0.0);
+  for (int i = 0; i < nc; ++i)
+    for (int m = 0; m < nbg; ++m) {
+      double acc = 0.0;
+      for (int k = 0; k < nbg; ++k) acc += I.tcb[sp][i + (size_t)k * nc] * a[k + (size_t)m * nbg];
+      t[i + (size_t)m * nc] = acc;
+    }
+  I.tcb_d[sp] = t;
+}
+
+// One-body matrices of the spinor for the spin-flip, anomalous and full mixings
+// (one_body_operator::set_hopping_matrix_templ, src_ed/Operators/one_body_operator.hpp:
+// 326-411; anomalous_operator::set_hopping_matrix_templ, anomalous_operator.hpp:241-284):
+// with M the operator's matrix over the 2L spin orbitals (one-body: (r, c, v), (c, r, v);
+// anomalous: (r, c, v), (c, r, -v)) and e = [[M_uu, M_ud], [M_du, M_dd]] restricted to
+// the operator's rows and columns (sites, bath or sites x bath):
+//   spin flip: one-body e;
+//   anomalous: one-body diag(M_uu, -M_dd^T) (hybrid: -M_dd), pairing 2 M_du below and
+//     2 M_du^T above the diagonal blocks (hybrid: -2 M_ud above);
+//   full: one-body diag(e, -e^T) (hybrid: -e), pairing 2 e below, 2 e^T above (hybrid: -2 e).
+void set_hopping_matrix_spinor(Instance& I) {
+  Model& M = *I.model;
+  const int ns = M.n_sites, nb = M.n_bath, L = M.L, m = I.mixing, nm = n_mixed(m);
+  const int dg = nm * ns, dbg = nm * nb;
+  I.SEF_bath = 0.0;
+  for (int sp = 0; sp < 2; ++sp) {
+    I.tc[sp].assign((size_t)dg * dg, 0.0);
+    I.tb[sp].assign((size_t)dbg * dbg, 0.0);
+    I.tcb[sp].assign((size_t)dg * dbg, 0.0);
+  }
+  for (const Op& op : M.ops) {
+    if (op.interaction) continue;
+    auto it = I.value.find(op.name);
+    if (it == I.value.end() || it->second == 0.0) continue;
+    const double z = it->second;
+    std::vector<double> F((size_t)4 * L * L, 0.0);  // M over spin orbitals, 2L x 2L column-major
+    auto at = [&](int a, int b) -> double& { return F[a + (size_t)b * 2 * L]; };
+    for (size_t q = 0; q < op.v.size(); ++q) {
+      const int r = op.r[q], c = op.c[q];
+      const double v = op.v[q];
+      if (op.anomalous) {
+        if (r < c) {
+          at(r, c) += v;
+          at(c, r) -= v;
+        }
+      } else if (r < c) {
+        at(r, c) += v;
+        at(c, r) += v;
+      } else if (r == c) {
+        at(r, r) += v;
+      }
+    }
+    // the operator's rows and columns (orbitals within a spin)
+    int r0, nR, c0, nC;
+    std::vector<double>* dst;
+    int ld;  // leading dimension of the destination (its row count)
+    if (op.target == 1) {
+      r0 = 0, nR = ns, c0 = 0, nC = ns, dst = &I.tc[0], ld = dg;
+    } else if (op.target == 2) {
+      r0 = ns, nR = nb, c0 = ns, nC = nb, dst = &I.tb[0], ld = dbg;
+    } else {
+      r0 = 0, nR = ns, c0 = ns, nC = nb, dst = &I.tcb[0], ld = dg;
+    }
+    const bool hyb = op.target == 3;
+    // e: the (2 nR) x (2 nC) spin block matrix of the operator
+    auto e = [&](int a, int b) {
+      const int ra = a < nR ? r0 + a : L + r0 + (a - nR);
+      const int cb = b < nC ? c0 + b : L + c0 + (b - nC);
+      return at(ra, cb);
+    };
+    auto put = [&](int a, int b, double val) { (*dst)[a + (size_t)b * ld] += z * val; };
+    if (!op.anomalous) {
+      if (m == 2) {
+        for (int a = 0; a < 2 * nR; ++a)
+          for (int b = 0; b < 2 * nC; ++b) put(a, b, e(a, b));
+      } else if (m == 1) {
+        for (int a = 0; a < nR; ++a)
+          for (int b = 0; b < nC; ++b) {
+            put(a, b, e(a, b));
+            // lower-right: -M_dd^T (square blocks), -M_dd for the hybridization
+            if (hyb) put(nR + a, nC + b, -e(nR + a, nC + b));
+          }
+        if (!hyb)
+          for (int a = 0; a < nR; ++a)
+            for (int b = 0; b < nC; ++b) put(nR + a, nC + b, -e(nR + b, nC + a));
+      } else {
+        for (int a = 0; a < 2 * nR; ++a)
+          for (int b = 0; b < 2 * nC; ++b) {
+            put(a, b, e(a, b));
+            put(2 * nR + a, 2 * nC + b, hyb ? -e(a, b) : -e(b, a));
+          }
+      }
+    } else {
+      if (m == 1) {
+        for (int a = 0; a < nR; ++a)
+          for (int b = 0; b < nC; ++b) {
+            put(nR + a, b, 2.0 * e(nR + a, b));                            // 2 M_du
+            put(a, nC + b, hyb ? -2.0 * e(a, nC + b) : 2.0 * e(nR + b, a));  // 2 M_du^T or -2 M_ud
+          }
+      } else if (m == 3) {
+        for (int a = 0; a < 2 * nR; ++a)
+          for (int b = 0; b < 2 * nC; ++b) {
+            put(2 * nR + a, b, 2.0 * e(a, b));
+            put(a, 2 * nC + b, hyb ? -2.0 * e(a, b) : 2.0 * e(b, a));
+          }
+      } else {
+        fail("anomalous operator " + op.name + " in an instance without anomalous mixing");
+      }
+    }
+  }
+  diagonalize_bath(I, 0, dg, dbg);
+  I.eb[1] = I.eb[0];
+  I.tcb_d[1] = I.tcb_d[0];
+}
+
 // ---- one-body matrices (model_instance::set_hopping_matrix, model_instance.hpp:424-475) --
 void set_hopping_matrix(Instance& I) {
+  if (spinor_mixing(I.mixing)) {
+    set_hopping_matrix_spinor(I);
+    return;
+  }
   Model& M = *I.model;
   const int ns = M.n_sites, nb = M.n_bath, L = M.L;
   I.SEF_bath = 0.0;
@@ -231,11 +438,15 @@ void set_hopping_matrix(Instance& I) {
     if (op.interaction) continue;
     auto it = I.value.find(op.name);
     if (it == I.value.end() || it->second == 0.0) continue;
+    // spin-flip and anomalous terms belong to the spinor matrices (mixing 1, 2, 3)
+    if (op.anomalous || (op.mixing & 2))
+      fail("operator " + op.name + " needs the spin-flip or anomalous mixing");
     const double z = it->second;
     for (size_t q = 0; q < op.v.size(); ++q) {
       int r = op.r[q], c = op.c[q];
       double v = op.v[q] * z;
       if (r > c) continue;
+      if ((r >= L) != (c >= L)) fail("operator " + op.name + " couples the two spins");
       int sp = r >= L ? 1 : 0;
       int rr = r - sp * L, cc = c - sp * L;
       auto add = [&](int a, int b, double val) {
@@ -507,7 +718,61 @@ void build_qmatrix_sym(Instance& I, State& st, int spin) {
   st.gf[spin] = tot;
 }
 
+// Lehmann form of one state for the spin-flip, anomalous and full mixings
+// (model_instance::build_qmatrix, src_ed/model_instance.hpp:710-817, trivial group): the
+// spinor of build_symmetric_orbitals (src_ed/symmetry_group.cpp:616-645: per Nambu index,
+// per spin, per site; the Nambu components are creation operators, model::create_or_destroy,
+// src_ed/model.cpp:155-157), one band Lanczos per shifted sector (shift_sector,
+// src_ed/symmetry_group.cpp:540-558: every component lands in the same one), one Q-matrix
+// of dimension n_mixed x n_sites
+void build_qmatrix_spinor(Instance& I, State& st) {
+  Model& M = *I.model;
+  const int ns = M.n_sites, L = M.L, m = I.mixing;
+  const int nambu_max = (m & 1) ? 2 : 1, spin_max = (m & 2) ? 2 : 1;
+  std::vector<std::pair<int, bool>> spinor;
+  for (int inb = 0; inb < nambu_max; ++inb)
+    for (int isp = 0; isp < spin_max; ++isp)
+      for (int i = 0; i < ns; ++i) {
+        const int spin = inb ? (spin_max == 2 ? isp : (isp + 1) % 2) : isp;
+        spinor.push_back({i + spin * L, inb != 0});
+      }
+  const int p = (int)spinor.size();
+  SectorPtr B = state_sector(I, st);
+  auto shift = [](int x, int d) { return x == kSecOdd ? kSecEven : (x == kSecEven ? kSecOdd : x + d); };
+  QMat parts[2];
+  for (int s = 0; s < 2; ++s) {
+    const int pm = 2 * s - 1;
+    SectorPtr T = make_general_sector(I, shift(st.secN, pm), shift(st.secS, pm));
+    if (T->D == 0) continue;
+    DevBuf phi((size_t)p * T->D * 8);
+    QED_HIP(hipMemsetAsync(phi.p, 0, (size_t)p * T->D * 8, stream()));
+    for (int k = 0; k < p; ++k)
+      if (qed_general_apply(B.get(), T.get(), spinor[k].first, spinor[k].second ? -pm : pm, 1.0,
+                            st.psi.as<double>(), phi.as<double>() + (uint64_t)k * T->D) != 0)
+        fail(qed_last_error());
+    QMat q = sector_qmatrix(*T, phi.as<double>(), p);
+    for (double& e : q.e) {
+      e -= st.energy;
+      if (pm == -1) e = -e;
+    }
+    parts[s] = streamline(q);
+  }
+  QMat tot;
+  tot.L = p;
+  append(tot, parts[0]);
+  append(tot, parts[1]);
+  tot.L = p;
+  check_norm(tot, gd("accur_Q_matrix"));
+  const double sw = std::sqrt(st.weight);
+  for (double& v : tot.Q) v *= sw;
+  st.gf[0] = tot;
+}
+
 void build_qmatrix(Instance& I, State& st, int spin) {
+  if (spinor_mixing(I.mixing)) {
+    build_qmatrix_spinor(I, st);
+    return;
+  }
   if (I.model->symmetric()) {
     build_qmatrix_sym(I, st, spin);
     return;
@@ -548,6 +813,7 @@ void build_qmatrix(Instance& I, State& st, int spin) {
 // continued fractions (model_instance<double>::build_cf, src_ed/model_instance.cpp:34-99)
 void build_cf(Instance& I, State& st, int spin) {
   require_trivial_group(I, "the continued-fraction Green function");
+  require_no_spinor(I, "the continued-fraction Green function");
   Model& M = *I.model;
   const int n = M.n_sites, L = M.L;
   st.cfe[spin].assign((size_t)n * n, CFrac());
@@ -588,6 +854,7 @@ void build_cf(Instance& I, State& st, int spin) {
 // not deflate), block Lanczos on the orthonormal block, diagonal blocks shifted by E0
 void build_mcf(Instance& I, State& st, int spin) {
   require_trivial_group(I, "the matrix continued-fraction Green function");
+  require_no_spinor(I, "the matrix continued-fraction Green function");
   Model& M = *I.model;
   const int p = M.n_sites, L = M.L;
   const double acc = gd("accur_deflation");
@@ -648,12 +915,19 @@ void build_mcf(Instance& I, State& st, int spin) {
   }
 }
 
+// the reference's sector string (operator<< of sector, src_ed/sector.hpp:100-106)
 std::string sector_name(int N, int S, int R = 0) {
-  return "R" + std::to_string(R) + ":N" + std::to_string(N) + ":S" + std::to_string(S);
+  std::string o;
+  if (S == kSecOdd && N == kSecOdd) o += "O";
+  o += "R" + std::to_string(R);
+  if (!sec_code(N)) o += ":N" + std::to_string(N);
+  if (!sec_code(S)) o += ":S" + std::to_string(S);
+  return o;
 }
 
 // ---- ground state (model_instance::low_energy_states, model_instance.hpp:245-417) ------
 void one_body_solve(Instance& I) {
+  require_no_spinor(I, "the uncorrelated (one-body) solution");
   Model& M = *I.model;
   const int ns = M.n_sites, nb = M.n_bath, d = ns + nb;
   I.GS_energy = 0.0;
@@ -695,7 +969,8 @@ void finish_states(Instance& I, std::vector<std::unique_ptr<State>>& states);
 
 // keeps the states of one sector within max_gap of the lowest energy seen so far
 void add_states(Instance& I, int nup, int ndn, const std::vector<double>& evals, std::vector<DevBuf>& vecs,
-                double max_gap, std::vector<std::unique_ptr<State>>& states, int irrep = 0) {
+                double max_gap, std::vector<std::unique_ptr<State>>& states, int irrep = 0, int secN = 0,
+                int secS = 0) {
   if (evals[0] < I.GS_energy) I.GS_energy = evals[0];
   for (size_t k = 0; k < evals.size(); ++k) {
     if (evals[k] - I.GS_energy > max_gap) continue;
@@ -703,6 +978,8 @@ void add_states(Instance& I, int nup, int ndn, const std::vector<double>& evals,
     st->nup = nup;
     st->ndn = ndn;
     st->irrep = irrep;
+    st->secN = secN;
+    st->secS = secS;
     st->energy = evals[k];
     st->psi = std::move(vecs[k]);
     states.push_back(std::move(st));
@@ -723,9 +1000,10 @@ void low_energy_states(Instance& I) {
   std::vector<std::unique_ptr<State>> states;
   for (size_t is = 0; is < I.sectors.size(); ++is) {
     const int N = I.sectors[is].first, S = I.sectors[is].second, R = I.irreps[is];
-    const int nup = (N + S) / 2, ndn = (N - S) / 2;
-    if (!sector_valid(I, nup, ndn)) continue;
-    SectorPtr H = make_sector(I, nup, ndn, nullptr, R);
+    const bool spinor = spinor_mixing(I.mixing);
+    const int nup = spinor ? -1 : (N + S) / 2, ndn = spinor ? -1 : (N - S) / 2;
+    if (!spinor && !sector_valid(I, nup, ndn)) continue;
+    SectorPtr H = spinor ? make_general_sector(I, N, S) : make_sector(I, nup, ndn, nullptr, R);
     if (H->D == 0) continue;
     int n_states = std::max(1, gi("n_states"));
     if ((uint64_t)n_states > H->D) n_states = (int)H->D;
@@ -760,7 +1038,7 @@ void low_energy_states(Instance& I) {
       fail(std::string(e.what()) + "\n  in cluster model '" + I.model->name + "', instance '" + I.model->name +
            "_" + std::to_string(I.label) + "', sector '" + sector_name(N, S, R) + "'");
     }
-    add_states(I, nup, ndn, evals, vecs, max_gap, states, R);
+    add_states(I, nup, ndn, evals, vecs, max_gap, states, R, N, S);
   }
   finish_states(I, states);
 }
@@ -812,7 +1090,8 @@ void finish_states(Instance& I, std::vector<std::unique_ptr<State>>& states) {
     I.E0 += s->weight * s->energy;
     I.GS_energy = std::min(I.GS_energy, s->energy);
     if (o.tellp() > 0) o << "/";
-    o << sector_name(s->nup + s->ndn, s->nup - s->ndn, s->irrep) << ":1";
+    if (spinor_mixing(I.mixing)) o << sector_name(s->secN, s->secS, 0) << ":1";
+    else o << sector_name(s->nup + s->ndn, s->nup - s->ndn, s->irrep) << ":1";
   }
   I.gs_string = o.str();
   set_hopping_matrix(I);
@@ -957,7 +1236,7 @@ void batch_solve(std::vector<Instance*>& insts, bool graph, int chunk, BatchStat
     if (I->gs_solved) continue;
     // symmetry-adapted instances take the one-by-one path
     const bool plain = I->correlated && gc("Ground_state_method") == 'L' && std::max(1, gi("n_states")) == 1 &&
-                       !I->model->symmetric();
+                       !I->model->symmetric() && !spinor_mixing(I->mixing);
     if (!plain) {
       low_energy_states(*I);
       continue;
@@ -1033,7 +1312,7 @@ void batch_solve(std::vector<Instance*>& insts, bool graph, int chunk, BatchStat
   for (Instance* I : insts) {
     if (I->gf_solved || !I->correlated) continue;
     set_gf_format(*I);
-    if (I->cf || I->mcf || I->q_to_mcf || I->model->symmetric()) {
+    if (I->cf || I->mcf || I->q_to_mcf || I->model->symmetric() || spinor_mixing(I->mixing)) {
       green_function_solve(*I);
       continue;
     }
@@ -1160,7 +1439,7 @@ void green_function_eval(Instance& I, int nz, const double* z, int sp, double* o
 void green_function(Instance& I, int nz, const double* z, int spin_down, double* out) {
   green_function_solve(I);
   const int sp = spin_index(I, spin_down);
-  const int d = I.model->n_sites;
+  const int d = dim_gf(I);
   const size_t dd2 = (size_t)2 * d * d;
   auto& cache = I.gf_cache[sp];
   const size_t depth = (size_t)std::max(0, gi("GF_lookup_depth"));
@@ -1190,7 +1469,7 @@ void green_function(Instance& I, int nz, const double* z, int spin_down, double*
 }
 
 void green_function_eval(Instance& I, int nz, const double* z, int sp, double* out) {
-  const int d = I.model->n_sites;
+  const int d = dim_gf(I);
   std::fill(out, out + (size_t)2 * nz * d * d, 0.0);
   for (auto& s : I.states) {
     if ((I.mcf || I.q_to_mcf) && I.correlated && s->mcf_comb[sp].M() > 0) {
@@ -1255,7 +1534,7 @@ void green_function_eval(Instance& I, int nz, const double* z, int sp, double* o
 void hybridization(Instance& I, int nz, const double* z, int spin_down, double* out) {
   low_energy_states(I);
   const int sp = (spin_down && (I.mixing & 4)) ? 1 : 0;
-  const int d = I.model->n_sites, nb = I.model->n_bath;
+  const int d = dim_gf(I), nb = I.model->n_bath * n_mixed(I.mixing);
   for (int iz = 0; iz < nz; ++iz) {
     cd w(z[2 * iz], z[2 * iz + 1]);
     double* g = out + (size_t)2 * iz * d * d;
@@ -1277,6 +1556,7 @@ void hybridization(Instance& I, int nz, const double* z, int spin_down, double* 
 // b_0 of the hole fractions; mcf_set::integrated_Green_function, mcf_set.hpp:146-158:
 // (W_h^T W_h)^T). M(b, a) accumulates over the states, d x d column-major.
 std::vector<cd> gf_average(Instance& I, int sp) {
+  require_no_spinor(I, "the Green-function average");
   const int d = I.model->n_sites;
   std::vector<cd> M((size_t)d * d, 0.0);
   auto at = [&](int r, int c) -> cd& { return M[r + (size_t)c * d]; };
@@ -1336,6 +1616,7 @@ struct SusceptibilityPart {
 
 std::vector<SusceptibilityPart> susceptibility_parts(Instance& I, const std::string& op) {
   require_trivial_group(I, "the susceptibility");
+  require_no_spinor(I, "the susceptibility");
   if (!I.model->idx.count(op)) fail("Operator " + op + " is not defined in the model.");
   if (!I.value.count(op)) fail("operator " + op + " is not activated in instance " + std::to_string(I.label));
   low_energy_states(I);
@@ -1499,6 +1780,7 @@ void read_gf(const h5::Node& g, const Instance& I, State& s, int spin, const std
 
 void write_instance_file(Instance& I, const std::string& path, const std::string& group) {
   require_trivial_group(I, "the instance file");
+  require_no_spinor(I, "the instance file");
   green_function_solve(I);
   const std::string fmt = gf_format_name(I);
   try {
@@ -1534,6 +1816,7 @@ std::pair<int, int> sector_numbers(const std::string& s);
 
 void read_instance_file(Instance& I, const std::string& path, const std::string& group) {
   require_trivial_group(I, "the instance file");
+  require_no_spinor(I, "the instance file");
   std::vector<std::unique_ptr<State>> states;
   double gse = 0.0;
   std::string fmt, gs;
@@ -1621,6 +1904,32 @@ struct SectorLabel {
   int R, N, S;
 };
 SectorLabel parse_sector_label(const std::string& s, int g) {
+  if (s.find('N') == std::string::npos || s.find('S') == std::string::npos || s.find('O') != std::string::npos) {
+    // N or S not conserved (src_ed/sector.hpp:52-95): a missing label is "even", 'O'
+    // makes both odd, and an odd value of the other makes it "odd"
+    int R = 0, N = kSecEven, S = kSecEven;
+    for (auto& tok : split(s, ':')) {
+      if (tok.empty()) continue;
+      std::string t = tok;
+      if (t[0] == 'O') t = t.substr(1);
+      if (t.empty()) continue;
+      const char k = t[0];
+      try {
+        if (k == 'R') R = std::stoi(t.substr(1));
+        else if (k == 'N') N = std::stoi(t.substr(1));
+        else if (k == 'S') S = std::stoi(t.substr(1));
+        else fail("sector string " + s + " does not conform to standard!");
+      } catch (const std::invalid_argument&) {
+        fail("sector string " + s + " does not conform to standard!");
+      }
+    }
+    if (s.find('O') != std::string::npos) N = S = kSecOdd;
+    if (S == kSecEven && N != kSecEven && (N % 2)) S = kSecOdd;
+    if (N == kSecEven && S != kSecEven && (S % 2)) N = kSecOdd;
+    if (R < 0 || R >= g) fail("sector " + s + " is not valid");
+    if (!sec_code(N) && N < 0) fail("sector " + s + " is not valid");
+    return {R, N, S};
+  }
   int R = -1, N = -1000, S = -1000;
   for (auto& tok : split(s, ':')) {
     if (tok.empty()) continue;
@@ -1687,7 +1996,8 @@ int qed_ed_new_operator(const char* model, const char* name, const char* type, i
     op.name = name;
     op.type = t;
     if (t == "interaction") op.interaction = true;
-    else if (t != "one-body") fail("type of operator " + op.name + " is not supported by the factorized solver");
+    else if (t == "anomalous") op.anomalous = true;
+    else if (t != "one-body") fail("type of operator " + op.name + " is not supported by this solver");
     bool touches_bath = false, touches_site = false, hybrid = false;
     for (int i = 0; i < nel; ++i) {
       int a = r[i], b = c[i];
@@ -1702,6 +2012,28 @@ int qed_ed_new_operator(const char* model, const char* name, const char* type, i
       hybrid |= ab != bb;
     }
     op.target = hybrid ? 3 : (touches_bath && !touches_site ? 2 : 1);
+    // one target per operator (set_target, src_ed/Operators/one_body_operator.hpp and
+    // anomalous_operator.hpp:131-162): cluster only, bath only, or cluster-bath only
+    if (!op.interaction) {
+      bool mixed_target = false;
+      for (int i = 0; i < nel; ++i) {
+        const bool ab = (op.r[i] % M.L) >= M.n_sites, bb = (op.c[i] % M.L) >= M.n_sites;
+        const int t = ab != bb ? 3 : (ab ? 2 : 1);
+        mixed_target |= t != op.target;
+      }
+      if (mixed_target) fail("operator " + op.name + " has no unique target");
+    }
+    for (size_t q = 0; q < op.v.size(); ++q) {
+      const bool cross = (op.r[q] < M.L) != (op.c[q] < M.L);
+      if (op.anomalous && !cross) op.mixing |= 2;  // triplet pairing needs both spins
+      if (!op.anomalous && !op.interaction && cross) op.mixing |= 2;
+    }
+    if (op.anomalous) {
+      op.mixing |= 1;
+      bool any = false;
+      for (size_t q = 0; q < op.v.size(); ++q) any |= op.r[q] < op.c[q];
+      if (!any) fail("building empty anomalous operator.");
+    }
     if (M.idx.count(op.name)) {
       M.ops[M.idx[op.name]] = op;
     } else {
@@ -1728,25 +2060,54 @@ int qed_ed_new_model_instance(const char* model, int nparam, const char* const* 
     if (I->value.empty()) fail("ED model instance (label = " + std::to_string(label) + ") for model " + M.name + " has no parameters!");
     // mixing: normal unless spin-asymmetric one-body terms or S != 0 sectors
     int mixing = 0;
+    bool nonzero_anomalous = false;
     for (auto& kv : I->value) {
       const Op& op = M.ops[M.idx[kv.first]];
       if (op.interaction && kv.second != 0.0) I->correlated = true;
-      if (op.interaction || kv.second == 0.0) continue;
+      // spin flip and anomalous terms set the mixing whatever their value
+      // (model_instance_base.cpp:56-66)
+      mixing |= op.mixing;
+      if (op.anomalous) nonzero_anomalous = true;
+      // spin-asymmetric one-body terms (check_spin_symmetry); spin-flip terms are
+      // counted above
+      if (op.interaction || op.anomalous || (op.mixing & 2) || kv.second == 0.0) continue;
       std::map<std::pair<int, int>, double> up, dn;
       for (size_t q = 0; q < op.v.size(); ++q) {
         if (op.r[q] < M.L) up[{op.r[q], op.c[q]}] += op.v[q];
         else dn[{op.r[q] - M.L, op.c[q] - M.L}] += op.v[q];
       }
-      if (up != dn) mixing = 4;
+      if (up != dn) mixing |= 4;
     }
+    if (mixing > 5) mixing &= 3;
+    if (mixing == 5) mixing = 1;
+    if (n_mixed(mixing) == 4 && !nonzero_anomalous)
+      fail("cluster anomalous operators cannot all be zero in the full Nambu case.");
     // a nontrivial group needs a format other than 'F' (model_instance_base.cpp:86-88);
     // every other format is served by the device's symmetry-adapted product
     if (M.symmetric() && gc("Hamiltonian_format") == 'F')
       fail("the Hamiltonian format 'factorized' is not compatible with a nontrivial point group symmetry");
+    if (spinor_mixing(mixing) && gc("Hamiltonian_format") == 'F')
+      fail("the Hamiltonian format 'factorized' is not compatible with Hilbert space mixing " +
+           std::to_string(mixing));
+    if (spinor_mixing(mixing) && M.symmetric())
+      fail("spin-flip and anomalous terms together with a nontrivial point group are not supported by this solver");
     for (auto& s : split(sectors, '/')) {
       if (s.empty()) continue;
       const SectorLabel l = parse_sector_label(s, M.group.g);
-      if (l.S != 0 && mixing == 0) mixing = 4;
+      if (!sec_code(l.S) && l.S != 0 && mixing == 0) mixing = 4;
+      // src_ed/model_instance_base.cpp:112-125
+      if (sec_code(l.S) && !(mixing & 2))
+        fail("sector string " + s + " defines a non conserved spin, but spin is conserved in the model (instance " +
+             std::to_string(label) + ")");
+      if (sec_code(l.N) && !(mixing & 1))
+        fail("sector string " + s + " defines a non conserved particle number, but particle number is conserved "
+             "in the model (instance " + std::to_string(label) + ")");
+      if (!sec_code(l.S) && (mixing & 2))
+        fail("sector string " + s + " defines a conserved spin, but spin is not conserved in the model (instance " +
+             std::to_string(label) + ")");
+      if (!sec_code(l.N) && (mixing & 1))
+        fail("sector string " + s + " defines a conserved particle number, but particle number is not conserved "
+             "in the model (instance " + std::to_string(label) + ")");
       I->sectors.push_back({l.N, l.S});
       I->irreps.push_back(l.R);
     }
@@ -1869,7 +2230,7 @@ int qed_ed_self_energy(uint64_t label, int nz, const double* z, int spin_down, d
   std::lock_guard<std::recursive_mutex> lk(lib_mutex());
   return guard([&] {
     Instance& I = inst(label);
-    const int d = I.model->n_sites;
+    const int d = dim_gf(I);
     std::vector<double> G((size_t)2 * nz * d * d), H((size_t)2 * nz * d * d);
     green_function(I, nz, z, spin_down, G.data());
     hybridization(I, nz, z, spin_down, H.data());
@@ -1896,7 +2257,7 @@ int qed_ed_hopping_matrix(uint64_t label, int spin_down, double* tc) {
   return guard([&] {
     Instance& I = inst(label);
     const int sp = (spin_down && (I.mixing & 4)) ? 1 : 0;
-    const int d = I.model->n_sites;
+    const int d = dim_gf(I);
     for (size_t q = 0; q < (size_t)d * d; ++q) {
       tc[2 * q] = I.tc[sp][q];
       tc[2 * q + 1] = 0.0;
@@ -1922,7 +2283,7 @@ int qed_ed_qmatrix(uint64_t label, int spin_down, int* M, double* e, double* Q) 
     if ((I.cf || I.mcf || I.q_to_mcf) && I.correlated)
       fail("the Q-matrix is not available with the continued-fraction Green function");
     const int sp = spin_index(I, spin_down);
-    const int d = I.model->n_sites;
+    const int d = dim_gf(I);
     int tot = 0;
     for (auto& s : I.states) tot += s->gf[sp].M();
     *M = tot;
@@ -1941,6 +2302,7 @@ int qed_ed_potthoff_functional(uint64_t label, double* omega) {
   std::lock_guard<std::recursive_mutex> lk(lib_mutex());
   return guard([&] {
     Instance& I = inst(label);
+    require_no_spinor(I, "the Potthoff functional");
     green_function_solve(I);
     *omega = I.SEF_bath + I.E0;
   });
@@ -1963,7 +2325,7 @@ int qed_ed_mixing(uint64_t label, int* mixing) {
 
 int qed_ed_dim_gf(uint64_t label, int* d) {
   std::lock_guard<std::recursive_mutex> lk(lib_mutex());
-  return guard([&] { *d = inst(label).model->n_sites; });
+  return guard([&] { *d = dim_gf(inst(label)); });
 }
 
 int qed_ed_cluster_averages(uint64_t label, int nop, const char* const* names, double* ave, double* var) {
@@ -2011,7 +2373,7 @@ int qed_ed_cluster_averages(uint64_t label, int nop, const char* const* names, d
       const Op& op = I.model->ops[I.model->idx.at(nm)];
       double a = 0.0, a2 = 0.0;
       for (auto& s : I.states) {
-        SectorPtr H = make_sector(I, s->nup, s->ndn, &nm);
+        SectorPtr H = state_sector(I, *s, &nm);
         DevBuf y(H->D * 8);
         QED_HIP(hipMemsetAsync(y.p, 0, H->D * 8, stream()));
         hv(*H, s->psi.as<double>(), y.as<double>());
@@ -2070,6 +2432,8 @@ int qed_ed_fidelity(uint64_t label1, uint64_t label2, double* fidelity) {
     Instance& B = inst(label2);
     require_trivial_group(A, "the fidelity");
     require_trivial_group(B, "the fidelity");
+    require_no_spinor(A, "the fidelity");
+    require_no_spinor(B, "the fidelity");
     low_energy_states(A);
     low_energy_states(B);
     double z = 0.0;
@@ -2095,6 +2459,7 @@ int qed_ed_tr_sigma_inf(uint64_t label, double* value) {
   std::lock_guard<std::recursive_mutex> lk(lib_mutex());
   return guard([&] {
     Instance& I = inst(label);
+    require_no_spinor(I, "tr_sigma_inf");
     const int d = I.model->n_sites;
     std::vector<double> eps(3), S(3, 0.0), z(6), sig((size_t)2 * 3 * d * d);
     for (int i = 0; i < 3; ++i) {
@@ -2117,6 +2482,7 @@ int qed_ed_hybridization_lehmann(uint64_t label, int spin_down, double* E, doubl
   std::lock_guard<std::recursive_mutex> lk(lib_mutex());
   return guard([&] {
     Instance& I = inst(label);
+    require_no_spinor(I, "the Lehmann form of the hybridization");
     const int nb = I.model->n_bath, ns = I.model->n_sites;
     if (nb == 0) fail("System has no bath. Hybridization not defined.");
     const int sp = (spin_down && (I.mixing & 4)) ? 1 : 0;
@@ -2136,6 +2502,7 @@ int qed_ed_print_wavefunction(uint64_t label, char* buf, int buflen, int* needed
     // src_ed/binary_state.cpp:104-112)
     Instance& I = inst(label);
     require_trivial_group(I, "print_wavefunction");
+    require_no_spinor(I, "print_wavefunction");
     low_energy_states(I);
     const int L = I.model->L;
     std::ostringstream o;

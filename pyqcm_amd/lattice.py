@@ -437,6 +437,9 @@ class LatticeInstance:
 
     def green(self, z):
         self.ground_state()
+        if qcm.mixingC(self.label) not in (0, 4):
+            raise RuntimeError("the lattice functions take the normal or up/down mixing only "
+                               "(instance mixing %d)" % qcm.mixingC(self.label))
         return qcm.Green_function(np.atleast_1d(z), 0, self.label)
 
     def _geff(self, z):

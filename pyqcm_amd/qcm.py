@@ -2,9 +2,8 @@
 
 Names, argument meaning, defaults and errors follow the reference binding
 (src_python/qcm_ED_wrap.hpp:27-381, src_python/qcm_wrap.hpp:292-910): operator element
-labels are 1-based with r <= c, anomalous elements are scaled by 1/2 as there (anomalous
-operators themselves are not supported by the factorized solver and are rejected),
-failures raise RuntimeError with the library's message.
+labels are 1-based with r <= c, anomalous elements are scaled by 1/2 as there, failures
+raise RuntimeError with the library's message.
 
 Beyond the reference's one-frequency calls, every Green-function entry point also takes
 an array of frequencies and evaluates them in one device batch.
@@ -239,9 +238,12 @@ def hopping_matrix(spin_down=0, diag=0, label=0, full=0):
 
 
 def bath(label=0, spin_down=0):
-    """Diagonalized bath: energies (nb) and cluster-bath hybridization (d x nb)."""
+    """Diagonalized bath: energies (nb) and cluster-bath hybridization (d x nb); with the
+    spin-flip or anomalous mixings both count the spinor components (d = n_mixed x n_sites,
+    nb = n_mixed x n_bath)."""
     d = Green_function_dimensionC(label)
-    nb = model_sizeC(_instance_model[int(label)])[1]
+    ns, nb, _ = model_sizeC(_instance_model[int(label)])
+    nb *= d // ns
     eb = np.zeros(max(nb, 1))
     t = np.zeros(max(d * nb, 1))
     call("qed_ed_bath", int(label), int(spin_down), dp(eb), dp(t))

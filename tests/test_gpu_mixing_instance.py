@@ -1,0 +1,198 @@
+"""The spin-flip, anomalous (Nambu) and full mixings through the instance layer:
+anomalous operators and spin-flip one-body terms, sector strings without N or S
+("R0:S0", "R0:N4", "R0"), the spinor Green function of dimension n_mixed x n_sites
+(src_ed/model_instance.hpp:710-817, src_ed/symmetry_group.cpp:616-645), and the spinor
+one-body matrices (one_body_operator.hpp:326-411, anomalous_operator.hpp:241-284), against
+
+* dense Jordan-Wigner Hamiltonians of the same states (E0, G of the spinor);
+* the noninteracting limit: with U -> 0 the self-energy G0^-1 - G^-1 vanishes only if the
+  spinor hopping matrices (cluster, bath and hybridization blocks, one-body and pairing
+  terms) are those of the Hamiltonian the solver diagonalized;
+* the reference's errors for sector strings that contradict the mixing.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import models as TM
+
+pytestmark = pytest.mark.gpu
+
+_counter = itertools.count(1)
+
+
+def register(model, values, sectors, n_sites):
+    from pyqcm_amd import qcm
+    name = "mix%d" % next(_counter)
+    qcm.new_model(name, n_sites, model.L - n_sites)
+    for opname, op in model.operators.items():
+        qcm.new_operator(name, opname, op.kind, [(r + 1, c + 1, v) for (r, c, v) in op.elements])
+    label = 9500 + next(_counter)
+    qcm.new_model_instanceC(name, values, sectors, label)
+    return label
+
+
+def chain(hx=0.0, d=0.0, s=0.0, U=3.0):
+    n = 4
+    m, vals = TM.chain(n, tp=-0.3, U=U, mu=1.1)
+    L = m.L
+    if hx:
+        m.add("hx", "one-body", [(i, i + L, 1.0) for i in range(n)])
+        vals["hx"] = hx
+    if d:
+        m.add("D", "anomalous", [(i, i + 1 + L, 1.0) for i in range(n - 1)] + [(i + 1, i + L, 1.0) for i in range(n - 1)])
+        vals["D"] = d
+    if s:
+        m.add("S", "anomalous", [(i, i + L, 1.0) for i in range(n)])
+        vals["S"] = s
+    return m, vals, n
+
+
+def blocks(L, N=None, S=None, parity=None):
+    if N is not None:
+        return [(nu, N - nu) for nu in range(L + 1) if 0 <= N - nu <= L]
+    if S is not None:
+        return [(nu, nu - S) for nu in range(L + 1) if 0 <= nu - S <= L]
+    return [(nu, nd) for nu in range(L + 1) for nd in range(L + 1) if (nu + nd) % 2 == parity]
+
+
+def spinor(L, n, nambu, flip):
+    out = [(i, False) for i in range(n)]
+    if flip:
+        out += [(i + L, False) for i in range(n)]
+    if nambu:
+        out += [(i, True) for i in range(n)] + [(i + L, True) for i in range(n)] if flip else [(i + L, True) for i in range(n)]
+    return out
+
+
+ZS = np.array([0.3j, 1.0j, 2.5j, 0.6 + 0.2j])
+
+
+@pytest.mark.parametrize("case", ["nambu", "spin_flip", "full"])
+def test_mixing_instance_ground_state_and_green_function(gpu_lib, case):
+    from pyqcm_amd import qcm
+    if case == "nambu":
+        m, vals, n = chain(d=0.5, s=0.2)
+        sec, b0, bm, bp = "R0:S0", blocks(m.L, S=0), blocks(m.L, S=-1), blocks(m.L, S=1)
+        sp, mixing = spinor(m.L, n, True, False), 1
+    elif case == "spin_flip":
+        m, vals, n = chain(hx=0.4)
+        sec, b0, bm, bp = "R0:N4", blocks(m.L, N=4), blocks(m.L, N=3), blocks(m.L, N=5)
+        sp, mixing = spinor(m.L, n, False, True), 2
+    else:
+        m, vals, n = chain(hx=0.3, d=0.4)
+        sec, b0, bm, bp = "R0", blocks(m.L, parity=0), blocks(m.L, parity=1), blocks(m.L, parity=1)
+        sp, mixing = spinor(m.L, n, True, True), 3
+    lab = register(m, vals, sec, n)
+    E0, gs = qcm.ground_state_solve(lab)
+    assert qcm.mixingC(lab) == mixing
+    assert gs == sec + ":1"
+    assert qcm.Green_function_dimensionC(lab) == len(sp)
+    E0d, Gd = O.dense_spinor_gf(m, vals, O.block_states(m.L, b0), sp, O.block_states(m.L, bm),
+                                O.block_states(m.L, bp), ZS)
+    G = qcm.Green_function(ZS, 0, lab)
+    print(case, "E0 %.12f dense %.12f" % (E0, E0d), "max |dG| %.2e" % np.abs(G - Gd).max())
+    assert abs(E0 - E0d) <= 1e-10 * abs(E0d)
+    np.testing.assert_allclose(G, Gd, atol=1e-9)
+    # G(z) = G(z*)^* and the particle-hole structure leaves G real-symmetric
+    np.testing.assert_allclose(G, np.transpose(G, (0, 2, 1)), atol=1e-12)
+
+
+def nambu_bath_model(U):
+    """2 sites + 2 bath orbitals with every kind of term: site hopping and mu, bath
+    energies, site-bath hybridization, site pairing (cluster), bath pairing (bath) and
+    site-bath pairing (hybridization)."""
+    ns, nb = 2, 2
+    L = ns + nb
+    m = O.ClusterModel(L)
+    m.add("t", "one-body", [(0, 1, -1.0), (L, L + 1, -1.0)])
+    m.add("U", "interaction", [(i, i + L, 1.0) for i in range(ns)])
+    m.add("mu", "one-body", [(i, i, -1.0) for i in range(ns)] + [(i + L, i + L, -1.0) for i in range(ns)])
+    m.add("eb", "one-body", [(2, 2, 1.0), (2 + L, 2 + L, 1.0), (3, 3, -1.0), (3 + L, 3 + L, -1.0)])
+    m.add("tb", "one-body", [(0, 2, 1.0), (0 + L, 2 + L, 1.0), (1, 3, 1.0), (1 + L, 3 + L, 1.0)])
+    m.add("D", "anomalous", [(0, 1 + L, 1.0), (1, 0 + L, 1.0)])
+    m.add("Db", "anomalous", [(2, 2 + L, 1.0), (3, 3 + L, 1.0)])
+    m.add("Dh", "anomalous", [(0, 2 + L, 1.0), (1, 3 + L, 0.5)])
+    vals = {"t": 1.0, "U": U, "mu": 0.4, "eb": 0.8, "tb": 0.6, "D": 0.3, "Db": 0.25, "Dh": 0.2}
+    return m, vals, ns
+
+
+def test_nambu_self_energy_vanishes_without_interaction(gpu_lib):
+    """U = 1e-9: Sigma = z - t_c - Gamma(z) - G^-1 vanishes to O(U) only if the spinor
+    matrices t_c, t_b, t_cb (and the pairing blocks of each) match the Hamiltonian the
+    solver diagonalized; the ED G also equals the dense one."""
+    from pyqcm_amd import qcm
+    m, vals, ns = nambu_bath_model(1e-9)
+    lab = register(m, vals, "R0:S0", ns)
+    E0, gs = qcm.ground_state_solve(lab)
+    assert qcm.mixingC(lab) == 1
+    sp = [(i, False) for i in range(ns)] + [(i + m.L, True) for i in range(ns)]
+    _, Gd = O.dense_spinor_gf(m, vals, O.block_states(m.L, blocks(m.L, S=0)), sp,
+                              O.block_states(m.L, blocks(m.L, S=-1)), O.block_states(m.L, blocks(m.L, S=1)), ZS)
+    G = qcm.Green_function(ZS, 0, lab)
+    np.testing.assert_allclose(G, Gd, atol=1e-9)
+    Sig = qcm.self_energyC(ZS, 0, lab)
+    print("max |Sigma| at U = 1e-9: %.2e" % np.abs(Sig).max())
+    assert np.abs(Sig).max() < 1e-7
+    # with the interaction on, Sigma is no longer small (the check above has teeth)
+    m2, vals2, _ = nambu_bath_model(2.0)
+    lab2 = register(m2, vals2, "R0:S0", ns)
+    assert np.abs(qcm.self_energyC(ZS, 0, lab2)).max() > 1e-3
+
+
+def test_full_mixing_self_energy_vanishes_without_interaction(gpu_lib):
+    """The same model with a transverse field on sites and bath (spin flip and pairing:
+    mixing 3, spinor of 4 n_sites): Sigma vanishes at U = 1e-9 and G is the dense one."""
+    from pyqcm_amd import qcm
+    m, vals, ns = nambu_bath_model(1e-9)
+    L = m.L
+    m.add("hx", "one-body", [(i, i + L, 1.0) for i in range(ns)])
+    m.add("hxb", "one-body", [(i, i + L, 1.0) for i in range(ns, L)])
+    vals["hx"] = 0.3
+    vals["hxb"] = 0.15
+    lab = register(m, vals, "R0", ns)
+    qcm.ground_state_solve(lab)
+    assert qcm.mixingC(lab) == 3
+    sp = ([(i, False) for i in range(ns)] + [(i + L, False) for i in range(ns)] + [(i, True) for i in range(ns)]
+          + [(i + L, True) for i in range(ns)])
+    _, Gd = O.dense_spinor_gf(m, vals, O.block_states(L, blocks(L, parity=0)), sp,
+                              O.block_states(L, blocks(L, parity=1)), O.block_states(L, blocks(L, parity=1)), ZS)
+    np.testing.assert_allclose(qcm.Green_function(ZS, 0, lab), Gd, atol=1e-9)
+    Sig = qcm.self_energyC(ZS, 0, lab)
+    print("full: max |Sigma| at U = 1e-9: %.2e" % np.abs(Sig).max())
+    assert np.abs(Sig).max() < 1e-7
+
+
+def test_spin_flip_self_energy_vanishes_without_interaction(gpu_lib):
+    from pyqcm_amd import qcm
+    m, vals, n = chain(hx=0.35, U=1e-9)
+    lab = register(m, vals, "R0:N4", n)
+    qcm.ground_state_solve(lab)
+    Sig = qcm.self_energyC(ZS, 0, lab)
+    print("spin flip: max |Sigma| at U = 1e-9: %.2e" % np.abs(Sig).max())
+    assert np.abs(Sig).max() < 1e-7
+
+
+def test_mixing_sector_errors(gpu_lib):
+    from pyqcm_amd import qcm
+    m, vals, ns = nambu_bath_model(1.0)
+    m.add("bad", "one-body", [(0, 0, 1.0), (2, 2, 1.0)])  # a site and a bath orbital
+    with pytest.raises(RuntimeError, match="no unique target"):
+        register(m, vals, "R0:S0", ns)
+    m, vals, n = chain(d=0.5)
+    with pytest.raises(RuntimeError, match="non conserved particle number|conserved particle number"):
+        register(m, vals, "R0:N4:S0", n)
+    m, vals, n = chain()
+    with pytest.raises(RuntimeError, match="non conserved particle number"):
+        register(m, vals, "R0:S0", n)
+    m, vals, n = chain(hx=0.3)
+    with pytest.raises(RuntimeError, match="conserved spin"):
+        register(m, vals, "R0:N4:S0", n)
+    m, vals, n = chain(d=0.5)
+    lab = register(m, vals, "R0:S0", n)
+    with pytest.raises(RuntimeError, match="mixing"):
+        qcm.Green_function(ZS, 1, lab)
+    with pytest.raises(RuntimeError, match="mixing"):
+        qcm.susceptibility("U", np.array([0.5j]), lab)
