@@ -37,11 +37,30 @@ struct GTerm {
   int kind;
 };
 
+// Index of a state without a search, when every down word's allowed up words are those of
+// one particle number or of one parity (the sectors of the spin-flip, anomalous and full
+// mixings): states sort by down word, then up word, so
+//   index(up, dn) = prefix[dn] + rank[up],
+// with prefix[dn] the number of basis states of smaller down words and rank[up] the rank
+// of up among the up words of its particle number (or of its parity); allowed[ndn] holds
+// the allowed up particle numbers of down particle number ndn as bits.
+struct LocateTables {
+  const uint32_t* prefix;
+  const uint32_t* rank;
+  const uint64_t* allowed;
+  int tabled;
+};
+
 struct GeneralTables {
   DevBuf states;  // uint64, ascending
   DevBuf terms;   // GTerm
   int nterms = 0;
   std::vector<uint64_t> host_states;
+  DevBuf prefix, rank, allowed;
+  bool tabled = false;
+  LocateTables loc() const {
+    return {prefix.as<uint32_t>(), rank.as<uint32_t>(), allowed.as<uint64_t>(), tabled ? 1 : 0};
+  }
 };
 
 __host__ __device__ __forceinline__ uint64_t orbital_mask(int a, int L) {
@@ -60,6 +79,15 @@ __device__ __forceinline__ uint32_t find_state(const uint64_t* __restrict__ st, 
   return (lo < n && st[lo] == s) ? lo : n;
 }
 
+// index of state s in the basis, or n when absent
+__device__ __forceinline__ uint32_t locate(const LocateTables& T, const uint64_t* __restrict__ st, uint32_t n,
+                                           uint64_t s) {
+  if (!T.tabled) return find_state(st, n, s);
+  const uint32_t up = (uint32_t)s, dn = (uint32_t)(s >> 32);
+  if (!((T.allowed[__popc(dn)] >> __popc(up)) & 1ull)) return n;
+  return T.prefix[dn] + T.rank[up];
+}
+
 // parity of the bits of s strictly between the single-bit masks a and b
 __device__ __forceinline__ int between_parity(uint64_t s, uint64_t a, uint64_t b) {
   const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
@@ -69,7 +97,7 @@ __device__ __forceinline__ int between_parity(uint64_t s, uint64_t a, uint64_t b
 
 // y[I] += sum_J H[I, J] x[J]: every term applied to basis state I gives <J|T|I>, which is
 // H[I, J] since the term list is Hermitian and real
-__global__ __launch_bounds__(256) void k_hv_general(const uint64_t* __restrict__ st, uint32_t n,
+__global__ __launch_bounds__(256) void k_hv_general(const uint64_t* __restrict__ st, uint32_t n, LocateTables loc,
                                                     const GTerm* __restrict__ terms, int nterms,
                                                     const double* __restrict__ x, double* __restrict__ y) {
   for (uint32_t I = blockIdx.x * blockDim.x + threadIdx.x; I < n; I += gridDim.x * blockDim.x) {
@@ -84,7 +112,7 @@ __global__ __launch_bounds__(256) void k_hv_general(const uint64_t* __restrict__
           if (s & t.mr) diag += t.w;
         } else if ((s & t.mc) && !(s & t.mr)) {
           const uint64_t s2 = s - t.mc + t.mr;
-          const uint32_t J = find_state(st, n, s2);
+          const uint32_t J = locate(loc, st, n, s2);
           if (J == n) continue;
           const double v = t.w * x[J];
           acc += between_parity(s, t.mr, t.mc) ? -v : v;
@@ -95,7 +123,7 @@ __global__ __launch_bounds__(256) void k_hv_general(const uint64_t* __restrict__
         const bool both = (s & t.mr) && (s & t.mc), none = !(s & t.mr) && !(s & t.mc);
         if (!both && !none) continue;
         const uint64_t s2 = both ? s - t.mr - t.mc : s + t.mr + t.mc;
-        const uint32_t J = find_state(st, n, s2);
+        const uint32_t J = locate(loc, st, n, s2);
         if (J == n) continue;
         const double v = t.w * x[J];
         acc += between_parity(s, t.mr, t.mc) ? v : -v;
@@ -110,7 +138,8 @@ __global__ __launch_bounds__(256) void k_hv_general(const uint64_t* __restrict__
 // c^+_a |s> = (-1)^(bits of s below a) |s + a> (the 64-bit word's order, as Destroy,
 // src_ed/ED_basis.cpp:60-77); the bits below a are the same in source and target
 __global__ __launch_bounds__(256) void k_general_apply(const uint64_t* __restrict__ src, uint32_t ns,
-                                                       const uint64_t* __restrict__ tgt, uint32_t nt, uint64_t m,
+                                                       LocateTables sloc, const uint64_t* __restrict__ tgt,
+                                                       uint32_t nt, uint64_t m,
                                                        int pm, double z, const double* __restrict__ x,
                                                        double* __restrict__ y) {
   for (uint32_t J = blockIdx.x * blockDim.x + threadIdx.x; J < nt; J += gridDim.x * blockDim.x) {
@@ -118,7 +147,7 @@ __global__ __launch_bounds__(256) void k_general_apply(const uint64_t* __restric
     const bool has = (t & m) != 0;
     if (pm < 0 ? has : !has) continue;
     const uint64_t s = pm < 0 ? t + m : t - m;
-    const uint32_t I = find_state(src, ns, s);
+    const uint32_t I = locate(sloc, src, ns, s);
     if (I == ns) continue;
     const double v = z * x[I];
     y[J] += (__popcll(t & (m - 1ull)) & 1) ? -v : v;
@@ -129,11 +158,14 @@ uint32_t grid_of(uint64_t n) { return (uint32_t)std::min<uint64_t>(std::max<uint
 
 }  // namespace
 
-void hv_general(const qed_sector& s, const double* x, double* y) {
+void hv_general(const qed_sector& s, const double* x, double* y, int variant) {
   auto T = std::static_pointer_cast<GeneralTables>(s.general);
   const uint32_t n = (uint32_t)s.D;
   if (n == 0) return;
-  k_hv_general<<<grid_of(n), 256, 0, stream()>>>(T->states.as<uint64_t>(), n, T->terms.as<GTerm>(), T->nterms, x, y);
+  LocateTables loc = T->loc();
+  if (variant == 0) loc.tabled = 0;  // the binary search (reference form of the lookup)
+  k_hv_general<<<grid_of(n), 256, 0, stream()>>>(T->states.as<uint64_t>(), n, loc, T->terms.as<GTerm>(), T->nterms,
+                                                 x, y);
   QED_HIP(hipGetLastError());
 }
 
@@ -170,6 +202,47 @@ int qed_general_sector_create(int L, int npairs, const int* pairs, int n_ops, co
     std::sort(T->host_states.begin(), T->host_states.end());
     T->host_states.erase(std::unique(T->host_states.begin(), T->host_states.end()), T->host_states.end());
     D = T->host_states.size();
+    // lookup tables: every down particle number allows one up particle number, or every
+    // up particle number of one parity
+    std::vector<uint64_t> allowed(kMaxOrbGeneral + 1, 0);
+    for (auto& bl : blocks) allowed[bl.second] |= 1ull << bl.first;
+    bool single = true, parity = true;
+    for (int nd = 0; nd <= L; ++nd) {
+      const uint64_t a = allowed[nd];
+      if (!a) continue;
+      if (__builtin_popcountll(a) != 1) single = false;
+      for (int nu = 0; nu <= L; ++nu) {
+        const bool want = ((nu + nd) % 2) == ((__builtin_ctzll(a) + nd) % 2);
+        if (((a >> nu) & 1ull) != (uint64_t)want) parity = false;
+      }
+    }
+    if ((single || parity) && L <= 20) {
+      const uint32_t nw = 1u << L;
+      std::vector<uint32_t> rank(nw), prefix(nw);
+      std::vector<uint32_t> cnt(L + 2, 0);
+      for (uint32_t u = 0; u < nw; ++u) {
+        const int pc = __builtin_popcount(u);
+        rank[u] = single ? cnt[pc]++ : cnt[pc & 1]++;
+      }
+      std::vector<uint64_t> per(L + 1, 0);  // basis states per down word of each particle number
+      for (int nd = 0; nd <= L; ++nd)
+        for (int nu = 0; nu <= L; ++nu)
+          if ((allowed[nd] >> nu) & 1ull) per[nd] += half[nu].size();
+      uint64_t run = 0;
+      for (uint32_t d = 0; d < nw; ++d) {
+        prefix[d] = (uint32_t)run;
+        run += per[__builtin_popcount(d)];
+      }
+      if (run != D) fail("mixed basis: inconsistent lookup tables");
+      T->prefix.alloc((size_t)nw * 4);
+      T->rank.alloc((size_t)nw * 4);
+      T->allowed.alloc(allowed.size() * 8);
+      QED_HIP(hipMemcpyAsync(T->prefix.p, prefix.data(), (size_t)nw * 4, hipMemcpyHostToDevice, stream()));
+      QED_HIP(hipMemcpyAsync(T->rank.p, rank.data(), (size_t)nw * 4, hipMemcpyHostToDevice, stream()));
+      QED_HIP(hipMemcpyAsync(T->allowed.p, allowed.data(), allowed.size() * 8, hipMemcpyHostToDevice, stream()));
+      QED_HIP(hipStreamSynchronize(stream()));
+      T->tabled = true;
+    }
     // the terms
     std::vector<GTerm> terms;
     size_t off = 0;
@@ -235,7 +308,7 @@ int qed_general_apply(const qed_sector* src, const qed_sector* tgt, int orbital,
     auto S = std::static_pointer_cast<GeneralTables>(src->general);
     auto T = std::static_pointer_cast<GeneralTables>(tgt->general);
     if (tgt->D && src->D)
-      k_general_apply<<<grid_of(tgt->D), 256, 0, stream()>>>(S->states.as<uint64_t>(), (uint32_t)src->D,
+      k_general_apply<<<grid_of(tgt->D), 256, 0, stream()>>>(S->states.as<uint64_t>(), (uint32_t)src->D, S->loc(),
                                                              T->states.as<uint64_t>(), (uint32_t)tgt->D,
                                                              orbital_mask(orbital, src->L), pm, z, x, y);
     QED_HIP(hipGetLastError());

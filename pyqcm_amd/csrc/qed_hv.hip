@@ -778,7 +778,7 @@ void hv(const qed_sector& s, const double* x, double* y, int variant) {
     return;
   }
   if (s.general) {
-    hv_general(s, x, y);
+    hv_general(s, x, y, variant);
     return;
   }
   set_lds_attributes();

@@ -183,8 +183,9 @@ std::string split_status(const qed_sector& s);
 bool hv_lanczos(const qed_sector& s, const double* x, double* y, const double* lz, double* lz_part);
 // y += H x on a symmetry-adapted sector (s.mixed set)
 void hv_mixed(const qed_sector& s, const double* x, double* y);
-// y += H x on a sector of the spin-flip / anomalous mixings (s.general set)
-void hv_general(const qed_sector& s, const double* x, double* y);
+// y += H x on a sector of the spin-flip / anomalous mixings (s.general set); variant 0
+// locates the targets by binary search, otherwise by lookup tables where they exist
+void hv_general(const qed_sector& s, const double* x, double* y, int variant = -1);
 // The Abelian group of a list of generators (0-based site images, then bath entries:
 // phase labels with bath_irrep, orbital images otherwise), on the host: order g, elements
 // (g x N entries) and phi(a, b) in units of 2 pi / g (g x g); throws on invalid generators

@@ -145,3 +145,38 @@ def test_general_sector_spinor_green_function(gpu_lib, case):
     np.testing.assert_allclose(G, Gd, atol=1e-9)
     psi.free()
     B.close()
+
+
+@pytest.mark.parametrize("case", ["nambu_bath", "spin_flip_bath", "full_bath"])
+def test_general_lookup_tables_equal_binary_search(gpu_lib, case):
+    """2x2 plaquette + 4 bath orbitals (L = 8) with pairing or spin flip: the product that
+    locates targets through the rank tables is bit-identical to the one that binary
+    searches the sorted basis, and the Lanczos E0 of the Nambu sector is the lowest of the
+    blocks' union (the pairing-free limit: the factorized sectors' lowest)."""
+    from pyqcm_amd import _lib
+    from pyqcm_amd._lib import DeviceArray
+    from pyqcm_amd.sector import GeneralSector
+    p = TM.random_bath_params(np.random.default_rng(4), n_bath=4)
+    m, vals = TM.plaquette_bath(p, n_bath=4)
+    L = m.L
+    if case in ("nambu_bath", "full_bath"):
+        m.add("D", "anomalous", [(0, 1 + L, 1.0), (1, 0 + L, 1.0), (2, 3 + L, 1.0), (3, 2 + L, 1.0),
+                                 (0, 2 + L, -1.0), (2, 0 + L, -1.0), (1, 3 + L, -1.0), (3, 1 + L, -1.0)])
+        vals["D"] = 0.3
+    if case in ("spin_flip_bath", "full_bath"):
+        m.add("hx", "one-body", [(i, i + L, 1.0) for i in range(4)])
+        vals["hx"] = 0.25
+    kind, value = {"nambu_bath": ("S", 0), "spin_flip_bath": ("N", 8), "full_bath": ("P", 0)}[case]
+    S = GeneralSector(L, _pairs(L, kind, value), _device_flat(m, vals))
+    rng = np.random.default_rng(9)
+    x = DeviceArray.from_host(rng.standard_normal(S.dim))
+    y0 = rng.standard_normal(S.dim)
+    ya, yb = DeviceArray.from_host(y0), DeviceArray.from_host(y0)
+    S.hv_dev(x, ya)
+    _lib.call("qed_hv_variant", S.h, 0, x.ptr, yb.ptr)
+    a, b = ya.to_host(), yb.to_host()
+    print(case, "dim", S.dim, "max |table - search| %.1e" % np.abs(a - b).max())
+    np.testing.assert_array_equal(a, b)
+    for v in (x, ya, yb):
+        v.free()
+    S.close()
