@@ -390,6 +390,50 @@ void ensure(DevBuf& b, size_t bytes) {
   if (b.bytes < bytes) b.alloc(bytes);
 }
 
+// One host-to-device copy for a group of descriptor tables: they are written into a pinned
+// host buffer at 256-byte aligned offsets whose device addresses are known at once, and
+// sent together by flush(). The host buffer is rewritten only after the stream has passed
+// a synchronization that follows the previous flush (the callers' step structure).
+struct Stager {
+  unsigned char* host = nullptr;
+  size_t cap = 0, off = 0;
+  DevBuf dev;
+  Stager() = default;
+  Stager(const Stager&) = delete;
+  Stager& operator=(const Stager&) = delete;
+  ~Stager() {
+    if (host) (void)hipHostFree(host);
+  }
+  void init(size_t bytes) {
+    cap = bytes;
+    QED_HIP(hipHostMalloc((void**)&host, cap, hipHostMallocDefault));
+    dev.alloc(cap);
+  }
+  // space for n bytes: (host address, device address)
+  std::pair<unsigned char*, unsigned char*> reserve(size_t n) {
+    off = (off + 255) & ~(size_t)255;
+    if (off + n > cap) fail("batched band Lanczos: descriptor staging overflow");
+    std::pair<unsigned char*, unsigned char*> r{host + off, dev.as<unsigned char>() + off};
+    off += n;
+    return r;
+  }
+  template <class D>
+  D* put(const Table<D>& t) {  // descriptors followed by their first blocks (find_job)
+    const size_t n = t.h.size();
+    auto r = reserve(std::max<size_t>(1, n) * (sizeof(D) + sizeof(uint32_t)));
+    if (n) {
+      std::memcpy(r.first, t.h.data(), n * sizeof(D));
+      uint32_t* b0 = reinterpret_cast<uint32_t*>(r.first + n * sizeof(D));
+      for (size_t j = 0; j < n; ++j) b0[j] = t.h[j].blk0;
+    }
+    return reinterpret_cast<D*>(r.second);
+  }
+  void flush() {
+    if (off) QED_HIP(hipMemcpyAsync(dev.p, host, off, hipMemcpyHostToDevice, stream()));
+    off = 0;
+  }
+};
+
 // Persistent worker pool for the per-step host work of the batched passes (one call per
 // Krylov step: spawning threads per call costs more than the work at small batches).
 class HostPool {
@@ -697,7 +741,6 @@ struct BandJob {
   // then the deflated-vector overlaps and T(j, j)
   double* scd = nullptr;  // this job's kSc scalars in the batch's device array
   double* sch = nullptr;  // and their host copy
-  double* hcd = nullptr;  // kBM host-supplied coefficients of phase 4 (device)
   // pending host reads from the previous step
   int prev_j = -1, prev_nk = 0, prev_pc = 0, prev_nI = 0, nk = 0;
   bool check_pending = false;
@@ -760,19 +803,28 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
     hv_of[i] = (int)cs.size();
     cs.push_back(S[i]);
   }
-  // the step scalars of all jobs in one device array (one download per step) and the
-  // host-supplied phase-4 coefficients in another (one upload per step)
+  // the step scalars of all jobs in one device array (one download per step, into pinned
+  // host memory); the descriptor tables of a step go up in two staged copies: phase 1's,
+  // and those of phases 2-6 together with the host-supplied phase-4 coefficients
   DevBuf sc_all((size_t)std::max(1, nj) * kSc * sizeof(double));
-  DevBuf hc_all((size_t)std::max(1, nj) * kBM * sizeof(double));
-  std::vector<double> sch_all((size_t)std::max(1, nj) * kSc, 0.0), hch_all((size_t)std::max(1, nj) * kBM, 0.0);
+  double* sch_all = nullptr;
+  QED_HIP(hipHostMalloc((void**)&sch_all, (size_t)std::max(1, nj) * kSc * sizeof(double), hipHostMallocDefault));
+  std::unique_ptr<double, void (*)(double*)> sch_keep(sch_all, [](double* p) { (void)hipHostFree(p); });
+  std::fill(sch_all, sch_all + (size_t)std::max(1, nj) * kSc, 0.0);
   for (int i = 0; i < nj; ++i) {
     J[i].scd = sc_all.as<double>() + (size_t)i * kSc;
-    J[i].sch = sch_all.data() + (size_t)i * kSc;
-    J[i].hcd = hc_all.as<double>() + (size_t)i * kBM;
+    J[i].sch = sch_all + (size_t)i * kSc;
   }
+  // staging bounds: per step at most 8 + 2 kBM tables of nj descriptors, the H.v job table,
+  // the activity flags and the coefficients
+  const size_t per_table = (size_t)std::max(1, nj) * (sizeof(MAxpy) + sizeof(uint32_t)) + 256;
+  Stager stage1, stage2;
+  stage1.init(per_table + 256);
+  stage2.init((8 + 2 * (size_t)kBM) * per_table + (size_t)std::max(1, nj) * (sizeof(HvJob) + sizeof(int)) +
+              (size_t)std::max(1, nj) * kBM * sizeof(double) + 4096);
+  std::vector<std::function<void()>> launches;  // the staged group's kernels, in order
   HvBatch HB;
   if (!cs.empty()) hv_batch_prepare(HB, cs);
-  DevBuf dact(std::max<size_t>(1, cs.size()) * sizeof(int));
   DevBuf part;
   Table<MDot> TD;
   Table<MAxpy> TA;
@@ -782,22 +834,37 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
   bool tuned = false;
   int steps = 0;
   std::vector<int> act(cs.size(), 0);
+  // the run_* helpers stage their table into stage2 and queue the launch; flush_launches
+  // sends the group in one copy and then launches the queued kernels in order
   auto run_mdot = [&]() {
     if (TD.h.empty()) return;
-    MDot* d = TD.upload();
-    ensure(part, (size_t)TD.total * kBM * sizeof(double));
-    k_mdot_part<<<TD.total, kBT, 0, stream()>>>(d, (int)TD.h.size(), part.as<double>());
-    k_mdot_final<<<((int)TD.h.size() * kBM + 3) / 4, 256, 0, stream()>>>(d, (int)TD.h.size(), part.as<double>());
+    MDot* d = stage2.put(TD);
+    const uint32_t total = TD.total;
+    const int n = (int)TD.h.size();
+    ensure(part, (size_t)total * kBM * sizeof(double));
+    launches.push_back([&part, d, total, n] {
+      k_mdot_part<<<total, kBT, 0, stream()>>>(d, n, part.as<double>());
+      k_mdot_final<<<(n * kBM + 3) / 4, 256, 0, stream()>>>(d, n, part.as<double>());
+    });
   };
   auto run_maxpy = [&]() {
     if (TA.h.empty()) return;
-    MAxpy* d = TA.upload();
-    k_maxpy<<<TA.total, kBT, 0, stream()>>>(d, (int)TA.h.size());
+    MAxpy* d = stage2.put(TA);
+    const uint32_t total = TA.total;
+    const int n = (int)TA.h.size();
+    launches.push_back([d, total, n] { k_maxpy<<<total, kBT, 0, stream()>>>(d, n); });
   };
   auto run_mrev = [&]() {
     if (TR.h.empty()) return;
-    MRev* d = TR.upload();
-    k_mrev<<<TR.total, kBT, 0, stream()>>>(d, (int)TR.h.size());
+    MRev* d = stage2.put(TR);
+    const uint32_t total = TR.total;
+    const int n = (int)TR.h.size();
+    launches.push_back([d, total, n] { k_mrev<<<total, kBT, 0, stream()>>>(d, n); });
+  };
+  auto flush_launches = [&]() {
+    stage2.flush();
+    for (auto& f : launches) f();
+    launches.clear();
   };
   while (true) {
     // ---- phase 1: norm of v[j] and overlaps with the start vectors (all live jobs) ----
@@ -818,9 +885,14 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
       TD.add(d, b.n);
     }
     if (!live) break;
-    run_mdot();
-    QED_HIP(hipMemcpyAsync(sch_all.data(), sc_all.p, (size_t)nj * kSc * sizeof(double), hipMemcpyDeviceToHost,
-                           stream()));
+    {
+      MDot* d = stage1.put(TD);
+      ensure(part, (size_t)TD.total * kBM * sizeof(double));
+      stage1.flush();
+      k_mdot_part<<<TD.total, kBT, 0, stream()>>>(d, (int)TD.h.size(), part.as<double>());
+      k_mdot_final<<<((int)TD.h.size() * kBM + 3) / 4, 256, 0, stream()>>>(d, (int)TD.h.size(), part.as<double>());
+    }
+    QED_HIP(hipMemcpyAsync(sch_all, sc_all.p, (size_t)nj * kSc * sizeof(double), hipMemcpyDeviceToHost, stream()));
     QED_HIP(hipStreamSynchronize(stream()));
     ++steps;
     // ---- host: complete the previous step's band entries and its eigenvalue test, then
@@ -982,8 +1054,10 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
       }
     }
     if (!TF.h.empty()) {
-      VFill* d = TF.upload();
-      k_fill0<<<TF.total, kBT, 0, stream()>>>(d, (int)TF.h.size());
+      VFill* d = stage2.put(TF);
+      const uint32_t total = TF.total;
+      const int n = (int)TF.h.size();
+      launches.push_back([d, total, n] { k_fill0<<<total, kBT, 0, stream()>>>(d, n); });
     }
     for (size_t q = 0; q < cs.size(); ++q) act[q] = 0;
     for (int i = 0; i < nj; ++i) {
@@ -994,29 +1068,38 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
       HB.host[h].y[0] = w[i];
     }
     if (!cs.empty()) {
-      hv_batch_upload(HB);
-      QED_HIP(hipMemcpyAsync(dact.p, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, stream()));
-      if (!tuned) {
-        // tune on scratch outputs (the trial products must not land in w)
-        std::vector<std::unique_ptr<DevBuf>> sy(cs.size());
-        HvBatch tb;
-        hv_batch_prepare(tb, cs);
-        for (size_t q = 0; q < cs.size(); ++q) {
-          sy[q].reset(new DevBuf(cs[q]->D * 8));
-          tb.host[q].x[0] = tb.host[q].x[1] = HB.host[q].x[0] ? HB.host[q].x[0] : sy[q]->as<double>();
-          tb.host[q].y[0] = tb.host[q].y[1] = sy[q]->as<double>();
+      auto jr = stage2.reserve(HB.host.size() * sizeof(HvJob));
+      std::memcpy(jr.first, HB.host.data(), HB.host.size() * sizeof(HvJob));
+      auto ar = stage2.reserve(act.size() * sizeof(int));
+      std::memcpy(ar.first, act.data(), act.size() * sizeof(int));
+      HvJob* jobs_dev = reinterpret_cast<HvJob*>(jr.second);
+      const int* act_dev = reinterpret_cast<const int*>(ar.second);
+      launches.push_back([&, jobs_dev, act_dev] {
+        if (!tuned) {
+          // tune on scratch outputs (the trial products must not land in w)
+          std::vector<std::unique_ptr<DevBuf>> sy(cs.size());
+          HvBatch tb;
+          hv_batch_prepare(tb, cs);
+          for (size_t q = 0; q < cs.size(); ++q) {
+            sy[q].reset(new DevBuf(cs[q]->D * 8));
+            tb.host[q].x[0] = tb.host[q].x[1] = HB.host[q].x[0] ? HB.host[q].x[0] : sy[q]->as<double>();
+            tb.host[q].y[0] = tb.host[q].y[1] = sy[q]->as<double>();
+          }
+          hv_batch_upload(tb);
+          cfg = hv_batch_tune(tb, act_dev);
+          QED_HIP(hipStreamSynchronize(stream()));
+          tuned = true;
+          if (stats) stats->hv_config = cfg;
         }
-        hv_batch_upload(tb);
-        cfg = hv_batch_tune(tb, dact.as<int>());
-        QED_HIP(hipStreamSynchronize(stream()));
-        tuned = true;
-        if (stats) stats->hv_config = cfg;
-      }
-      hv_batch(HB, cfg, 0, dact.as<int>());
+        HB.jobs = jobs_dev;
+        hv_batch(HB, cfg, 0, act_dev);
+      });
     }
     // ---- phase 4: w -= sum_k T(k, j) v[k] over the band; release the oldest vector ----
     TA.clear();
-    bool any_hc = false;
+    auto hcr = stage2.reserve((size_t)std::max(1, nj) * kBM * sizeof(double));
+    double* hch = reinterpret_cast<double*>(hcr.first);
+    const double* hcd = reinterpret_cast<const double*>(hcr.second);
     for (int i = 0; i < nj; ++i) {
       if (!proceed[i]) continue;
       BandJob& b = J[i];
@@ -1055,15 +1138,11 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
       }
       a.m = m;
       if (!b.hcoef.empty()) {
-        std::copy(b.hcoef.begin(), b.hcoef.end(), hch_all.begin() + (size_t)i * kBM);
-        any_hc = true;
-        for (size_t u = 0; u < hslot.size(); ++u) a.c[hslot[u]] = b.hcd + u;
+        std::copy(b.hcoef.begin(), b.hcoef.end(), hch + (size_t)i * kBM);
+        for (size_t u = 0; u < hslot.size(); ++u) a.c[hslot[u]] = hcd + (size_t)i * kBM + u;
       }
       if (m > 0) TA.add(a, b.n);
     }
-    if (any_hc)
-      QED_HIP(hipMemcpyAsync(hc_all.p, hch_all.data(), (size_t)nj * kBM * sizeof(double), hipMemcpyHostToDevice,
-                             stream()));
     run_maxpy();
     // ---- phase 5: deflated vectors, one at a time (w -= <v_k, w> v_k) ----
     int maxI = 0;
@@ -1121,6 +1200,7 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
     }
     run_mdot();
     run_maxpy();
+    flush_launches();
     for (int i = 0; i < nj; ++i) {
       if (!proceed[i]) continue;
       BandJob& b = J[i];
