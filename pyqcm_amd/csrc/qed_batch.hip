@@ -885,7 +885,7 @@ void band_lanczos_batch(const std::vector<qed_sector*>& S, const std::vector<con
       TD.add(d, b.n);
     }
     if (!live) break;
-    {
+    if (!TD.h.empty()) {  // (every live job may have reached M0: it retires below)
       MDot* d = stage1.put(TD);
       ensure(part, (size_t)TD.total * kBM * sizeof(double));
       stage1.flush();

@@ -74,6 +74,35 @@ def test_batch_graph_replay_equals_direct_launches(gpu_lib):
         np.testing.assert_array_equal(q1, q2)
 
 
+def test_batch_band_lanczos_at_iteration_limit_matches_sequential(gpu_lib):
+    """max_iter_BL = 24: every Green-function sector stops at its iteration limit instead
+    of the eigenvalue test, so a step can find all remaining sectors at the limit (they
+    retire without a phase-1 launch); the batched Lehmann forms equal the sequential ones."""
+    from pyqcm_amd import _lib, qcm
+    C, M, cases = _setup(3, 13)
+    zs = 1j * np.array([0.3, 1.0, 3.0])
+    qcm.set_global_parameter("max_iter_BL", 24)
+    try:
+        seq = [M.instance(_values(C, v, x), C.plaquette_sectors(v)) for v, x in cases]
+        bat = [M.instance(_values(C, v, x), C.plaquette_sectors(v)) for v, x in cases]
+        _lib.call("qed_reset_start_vector_state")
+        G1 = []
+        for I in seq:
+            I.ground_state()
+            G1.append(qcm.Green_function(zs, 0, I.label))
+        _lib.call("qed_reset_start_vector_state")
+        qcm.batch_solve([I.label for I in bat], graph=True)
+        for I, J, G in zip(seq, bat, G1):
+            G2 = qcm.Green_function(zs, 0, J.label)
+            err = np.abs(G2 - G).max()
+            print("max |G_batch - G_seq| %.2e, poles %d" % (err, len(qcm.qmatrix(J.label)[0])))
+            assert err <= 1e-7 * max(1.0, np.abs(G).max())
+        for I in seq + bat:
+            I.erase()
+    finally:
+        qcm.set_global_parameter("max_iter_BL", 600)
+
+
 def test_qed_batch_solve_abi_matches_instances(gpu_lib):
     """The packed C entry point: E0 and the Lehmann form of each instance."""
     from pyqcm_amd import _lib, qcm
