@@ -473,3 +473,24 @@ def test_anomalous_dense_term_matches_bdg_closed_form():
 
 def hubbard_like_chain(n):
     return O.hubbard_cluster(n, [(i, i + 1) for i in range(n - 1)], t=1.0, U=0.0, mu=0.7)
+
+
+def test_dense_spinor_gf_reduces_to_dense_gf():
+    """The spinor Green function with normal components on one (N_up, N_down) block is
+    the plain dense G, and a Nambu component c^+_(i down) gives G^N_ii(z) = -G_down,ii(-z)
+    (exchanging c and c^+ in both Lehmann terms); the down-spin G of this spin-symmetric
+    chain is the up-spin one."""
+    m, vals = O.hubbard_cluster(4, [(0, 1), (1, 2), (2, 3)], t=1.0, U=3.0, mu=1.5)
+    L = m.L
+    zs = np.array([0.4j, 1.3j, 0.5 + 0.3j])
+    E0, G = O.dense_gf(m, vals, 2, 2, list(range(4)), zs)
+    st0 = O.block_states(L, [(2, 2)])
+    E1, Gs = O.dense_spinor_gf(m, vals, st0, [(i, False) for i in range(4)], O.block_states(L, [(1, 2)]),
+                               O.block_states(L, [(3, 2)]), zs)
+    assert abs(E0 - E1) < 1e-12
+    np.testing.assert_allclose(Gs, G, atol=1e-12)
+    _, Gn = O.dense_spinor_gf(m, vals, st0, [(i + L, True) for i in range(4)], O.block_states(L, [(2, 3)]),
+                              O.block_states(L, [(2, 1)]), zs)
+    _, Gm = O.dense_gf(m, vals, 2, 2, list(range(4)), -zs)
+    for i in range(len(zs)):
+        np.testing.assert_allclose(np.diag(Gn[i]), -np.diag(Gm[i]), atol=1e-12)
