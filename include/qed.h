@@ -68,8 +68,8 @@ int qed_sector_destroy(qed_sector* s);
  * 0-based: site images for the first L' entries and, with bath_irrep, the phase labels of
  * the bath orbitals (L' = n_sites with bath_irrep, L otherwise). Generators must commute
  * and have order <= 2 (real characters). The sector works with every solver entry point
- * of a sector (qed_hv, qed_lanczos_gs, qed_band_lanczos, ...) except qed_davidson and
- * qed_apply_c. Replaces ED_mixed_basis + symmetry_group + HS_one_body_operator
+ * of a sector (qed_hv, qed_lanczos_gs, qed_band_lanczos, qed_davidson, qed_diag, ...)
+ * except qed_apply_c. Replaces ED_mixed_basis + symmetry_group + HS_one_body_operator
  * (src_ed/ED_basis.cpp:185-361, src_ed/symmetry_group.cpp:18-454,
  * src_ed/Operators/HS_one_body_operator.hpp:8-40) for the unmixed sectors. */
 int qed_mixed_sector_create(int L, int n_up, int n_dn, int n_ops, const int* types, const double* coefs,
@@ -95,8 +95,8 @@ int qed_mixed_group(const qed_sector* s, int* g, int* elements, int* phi);
  * pairs[2k], pairs[2k+1] (k < npairs), as 64-bit states ascending. Operator types 0
  * one-body (spin-flip elements allowed), 1 density-density interaction, 2 anomalous
  * (elements r < c as ED::new_operator receives them, HS_anomalous_operator,
- * src_ed/Operators/HS_anomalous_operator.hpp:8-40). qed_hv and the Krylov entry points
- * (not qed_davidson) take such a sector; its dim_up and dim_dn are 0. */
+ * src_ed/Operators/HS_anomalous_operator.hpp:8-40). qed_hv, qed_diag and the Krylov
+ * entry points (qed_davidson included) take such a sector; its dim_up and dim_dn are 0. */
 int qed_general_sector_create(int L, int npairs, const int* pairs, int n_ops, const int* types,
                               const double* coefs, const int* nels, const int* rows, const int* cols,
                               const double* vals, qed_sector** out);
@@ -117,7 +117,11 @@ int qed_hv(qed_sector* s, const double* x_dev, double* y_dev);
 int qed_hv_host(qed_sector* s, const double* x, double* y);
 /* d_dev += diagonal of H as the reference's factorized format reports it: interaction
  * terms only (HS_factorized_operator::diag is empty, src_ed/Operators/HS_factorized_operator.hpp:103).
- * Replaces Hamiltonian_Factorized::diag (src_ed/Hamiltonian/Hamiltonian_Factorized.hpp:65-68). */
+ * Replaces Hamiltonian_Factorized::diag (src_ed/Hamiltonian/Hamiltonian_Factorized.hpp:65-68).
+ * On a symmetry-adapted or mixed-number sector, the whole diagonal <I|H|I>, one-body terms
+ * included, as the operator formats report it (Hamiltonian_CSR::diag,
+ * src_ed/Hamiltonian/Hamiltonian_CSR.hpp:103-105; HS_nondiagonal_operator::diag,
+ * src_ed/Operators/HS_nondiagonal_operator.hpp:176). */
 int qed_diag(qed_sector* s, double* d_dev);
 
 /* The reference's start vector: x = N(0,1) deviates of std::default_random_engine(seed)

@@ -154,6 +154,26 @@ __global__ __launch_bounds__(256) void k_general_apply(const uint64_t* __restric
   }
 }
 
+// d[I] += H[I, I]: the one-body diagonal and density-density terms (no hop or pair term
+// maps a state to itself)
+__global__ __launch_bounds__(256) void k_diag_general(const uint64_t* __restrict__ st, uint32_t n,
+                                                      const GTerm* __restrict__ terms, int nterms,
+                                                      double* __restrict__ d) {
+  for (uint32_t I = blockIdx.x * blockDim.x + threadIdx.x; I < n; I += gridDim.x * blockDim.x) {
+    const uint64_t s = st[I];
+    double acc = 0.0;
+    for (int k = 0; k < nterms; ++k) {
+      const GTerm t = terms[k];
+      if (t.kind == 1) {
+        if ((s & t.mr) && (s & t.mc)) acc += t.w;
+      } else if (t.kind == 0 && t.mr == t.mc) {
+        if (s & t.mr) acc += t.w;
+      }
+    }
+    d[I] += acc;
+  }
+}
+
 uint32_t grid_of(uint64_t n) { return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, (n + 255) / 256), 8192); }
 
 }  // namespace
@@ -166,6 +186,14 @@ void hv_general(const qed_sector& s, const double* x, double* y, int variant) {
   if (variant == 0) loc.tabled = 0;  // the binary search (reference form of the lookup)
   k_hv_general<<<grid_of(n), 256, 0, stream()>>>(T->states.as<uint64_t>(), n, loc, T->terms.as<GTerm>(), T->nterms,
                                                  x, y);
+  QED_HIP(hipGetLastError());
+}
+
+void diag_general(const qed_sector& s, double* d) {
+  auto T = std::static_pointer_cast<GeneralTables>(s.general);
+  const uint32_t n = (uint32_t)s.D;
+  if (n == 0) return;
+  k_diag_general<<<grid_of(n), 256, 0, stream()>>>(T->states.as<uint64_t>(), n, T->terms.as<GTerm>(), T->nterms, d);
   QED_HIP(hipGetLastError());
 }
 

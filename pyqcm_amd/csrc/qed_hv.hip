@@ -1448,6 +1448,10 @@ int qed_hv_host(qed_sector* s, const double* x, double* y) {
 int qed_diag(qed_sector* s, double* d) {
   return guard([&] {
     if (s->D == 0) return;
+    if (!s->factorized()) {
+      sector_diagonal(*s, d);
+      return;
+    }
     k_diag_interaction<<<(uint32_t)((s->D + 255) / 256), 256, 0, stream()>>>(view(*s), d, s->D);
     QED_HIP(hipGetLastError());
   });

@@ -694,12 +694,21 @@ __device__ __forceinline__ double diag_interaction(const HvView& v, uint64_t I) 
   return e;
 }
 
+// the diagonal Davidson reads: the factorized format's (interaction part, on the fly) or a
+// stored one (the symmetry-adapted and mixed-number sectors: the true diagonal of H, as the
+// reference's other formats report it)
+struct DavDiag {
+  HvView v;
+  const double* d;  // stored diagonal, or null
+  __device__ __forceinline__ double operator()(uint64_t I) const { return d ? d[I] : diag_interaction(v, I); }
+};
+
 // correction vector of one unconverged root: c = (H x - mu x) / (mu - d + tiny)
-__global__ void k_dav_correction(HvView v, const double* __restrict__ hx, const double* __restrict__ x, double mu,
+__global__ void k_dav_correction(DavDiag dg, const double* __restrict__ hx, const double* __restrict__ x, double mu,
                                  double tiny, double* __restrict__ c, uint64_t n) {
   for (uint64_t I = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n;
        I += (uint64_t)gridDim.x * blockDim.x)
-    c[I] = (hx[I] - mu * x[I]) / (-diag_interaction(v, I) + mu + tiny);
+    c[I] = (hx[I] - mu * x[I]) / (-dg(I) + mu + tiny);
 }
 
 // the order of find_lowest (src_util/vector_num.hpp:242-264): ascending values, and among
@@ -710,7 +719,7 @@ __device__ __forceinline__ bool dav_before(double va, uint64_t ia, double vb, ui
 
 // block partials of the first diagonal entry after (pv, pi) in that order (all entries
 // when have_prev is 0); a block with no candidate reports +inf
-__global__ __launch_bounds__(256) void k_dav_next_lowest(HvView v, uint64_t n, int have_prev, double pv, uint64_t pi,
+__global__ __launch_bounds__(256) void k_dav_next_lowest(DavDiag dg, uint64_t n, int have_prev, double pv, uint64_t pi,
                                                          double* __restrict__ part_v, uint64_t* __restrict__ part_i) {
   __shared__ double sv[256];
   __shared__ uint64_t si[256];
@@ -718,7 +727,7 @@ __global__ __launch_bounds__(256) void k_dav_next_lowest(HvView v, uint64_t n, i
   uint64_t bi = 0;
   for (uint64_t I = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n;
        I += (uint64_t)gridDim.x * blockDim.x) {
-    const double d = diag_interaction(v, I);
+    const double d = dg(I);
     if (have_prev && !dav_before(pv, pi, d, I)) continue;
     if (dav_before(d, I, bv, bi)) {
       bv = d;
@@ -742,7 +751,7 @@ __global__ __launch_bounds__(256) void k_dav_next_lowest(HvView v, uint64_t n, i
 }
 
 // the K lowest diagonal entries in find_lowest's order, one device pass per entry
-std::vector<uint64_t> lowest_diagonal(const qed_sector& s, int K) {
+std::vector<uint64_t> lowest_diagonal(const qed_sector& s, const DavDiag& dg, int K) {
   const uint64_t n = s.D;
   const uint32_t nb = (uint32_t)std::min<uint64_t>(kMaxBlocks, std::max<uint64_t>(1, (n + 255) / 256));
   DevBuf pv(nb * sizeof(double)), pi(nb * sizeof(uint64_t));
@@ -752,7 +761,7 @@ std::vector<uint64_t> lowest_diagonal(const qed_sector& s, int K) {
   double lv = 0.0;
   uint64_t li = 0;
   for (int k = 0; k < K; ++k) {
-    k_dav_next_lowest<<<nb, 256, 0, stream()>>>(view(s), n, k > 0, lv, li, pv.as<double>(), pi.as<uint64_t>());
+    k_dav_next_lowest<<<nb, 256, 0, stream()>>>(dg, n, k > 0, lv, li, pv.as<double>(), pi.as<uint64_t>());
     QED_HIP(hipMemcpyAsync(hv_.data(), pv.p, nb * sizeof(double), hipMemcpyDeviceToHost, stream()));
     QED_HIP(hipMemcpyAsync(hi_.data(), pi.p, nb * sizeof(uint64_t), hipMemcpyDeviceToHost, stream()));
     QED_HIP(hipStreamSynchronize(stream()));
@@ -796,12 +805,13 @@ void dav_transform(const std::vector<double*>& in, const std::vector<double>& C,
 // and dropped), the residuals H x - mu x are formed from a fresh product of the Ritz vector,
 // and the basis is transformed in place. Device memory: the K output vectors (the first K
 // basis slots) plus 2K further slots and two scratch vectors, 3K + 2 vectors in all (14 for
-// K = 4) instead of the reference's ~9K; the diagonal is never stored.
+// K = 4) instead of the reference's ~9K. The diagonal (preconditioner and start vectors) is
+// recomputed per entry on factorized sectors; symmetry-adapted and mixed-number sectors
+// store it (one more vector), since their diagonal gathers over the orbit's hops.
 void davidson(qed_sector& s, int K, double accur, int max_iter, std::vector<double>& val, double* xout,
               int* niter, bool verb) {
   const uint64_t n = s.D;
   const double tiny = 1e-14;
-  if (!s.factorized()) fail("Davidson: available on factorized sectors only");
   if ((uint64_t)K > n) K = (int)n;
   if (K < 1) fail("Davidson: no state requested");
   const int J = (int)std::min<uint64_t>(3 * (uint64_t)K, n);
@@ -810,7 +820,7 @@ void davidson(qed_sector& s, int K, double accur, int max_iter, std::vector<doub
   {
     size_t fr = 0, tot = 0;
     QED_HIP(hipMemGetInfo(&fr, &tot));
-    const double need = (double)(J - K + 2) * n * 8;
+    const double need = (double)(J - K + 2 + (s.factorized() ? 0 : 1)) * n * 8;
     if (need > (double)fr)
       fail("Davidson: " + std::to_string(J - K + 2) + " work vectors of the sector need " +
            std::to_string(need / 1e9) + " GB of device memory, " + std::to_string(fr / 1e9) + " GB free");
@@ -830,7 +840,15 @@ void davidson(qed_sector& s, int K, double accur, int max_iter, std::vector<doub
   if (verb)
     printf("\nDavidson method for %d vectors in sector (%d, %d), dim = %llu\n", K, s.nup, s.ndn,
            (unsigned long long)n);
-  const std::vector<uint64_t> low = lowest_diagonal(s, K);
+  DevBuf dstore;
+  DavDiag dg{s.factorized() ? view(s) : HvView{}, nullptr};
+  if (!s.factorized()) {
+    dstore.alloc(n * 8);
+    k_zero<<<sb, 256, 0, stream()>>>(dstore.as<double>(), n);
+    sector_diagonal(s, dstore.as<double>());
+    dg.d = dstore.as<double>();
+  }
+  const std::vector<uint64_t> low = lowest_diagonal(s, dg, K);
   const double one = 1.0;
   for (int k = 0; k < K; ++k) {
     k_zero<<<sb, 256, 0, stream()>>>(slot[k], n);
@@ -882,7 +900,7 @@ void davidson(qed_sector& s, int K, double accur, int max_iter, std::vector<doub
       const int k = unconv[q];
       k_zero<<<sb, 256, 0, stream()>>>(u, n);
       hv(s, slot[k], u);
-      k_dav_correction<<<sb, 256, 0, stream()>>>(view(s), u, slot[k], mu[k], tiny, slot[K + Q + q], n);
+      k_dav_correction<<<sb, 256, 0, stream()>>>(dg, u, slot[k], mu[k], tiny, slot[K + Q + q], n);
     }
     // Gram-Schmidt in the reference's order (vectors below tiny are removed)
     std::vector<double*> b(slot.begin(), slot.begin() + K + 2 * Q);

@@ -183,6 +183,9 @@ std::string split_status(const qed_sector& s);
 bool hv_lanczos(const qed_sector& s, const double* x, double* y, const double* lz, double* lz_part);
 // y += H x on a symmetry-adapted sector (s.mixed set)
 void hv_mixed(const qed_sector& s, const double* x, double* y);
+// d += the diagonal of H on a symmetry-adapted or mixed-number sector (D doubles)
+void sector_diagonal(const qed_sector& s, double* d);
+void diag_general(const qed_sector& s, double* d);
 // y += H x on a sector of the spin-flip / anomalous mixings (s.general set); variant 0
 // locates the targets by binary search, otherwise by lookup tables where they exist
 void hv_general(const qed_sector& s, const double* x, double* y, int variant = -1);

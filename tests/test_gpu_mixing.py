@@ -180,3 +180,34 @@ def test_general_lookup_tables_equal_binary_search(gpu_lib, case):
     for v in (x, ya, yb):
         v.free()
     S.close()
+
+
+@pytest.mark.parametrize("case", ["spin_flip", "nambu_onsite", "full"])
+def test_general_sector_diagonal_and_davidson(gpu_lib, case):
+    """The diagonal of H on a mixed-number sector (one-body diagonal and density terms)
+    against the dense Jordan-Wigner matrix, and Davidson-Liu's 4 lowest states against its
+    spectrum: values, residuals <= accur_Davidson, orthonormal vectors."""
+    import ctypes
+    from pyqcm_amd._lib import DeviceArray
+    from pyqcm_amd.sector import GeneralSector
+    kw, kind, value = CASES[case]
+    m, vals = _chain_model(**kw)
+    L = m.L
+    pairs = _pairs(L, kind, value)
+    S = GeneralSector(L, pairs, _device_flat(m, vals))
+    H, _ = O.dense_hamiltonian(m, vals, 0, 0, states=O.block_states(L, pairs))
+    np.testing.assert_allclose(S.diag(), np.diag(H), rtol=0, atol=1e-12 * np.abs(H).max())
+    K = 4
+    psi = DeviceArray(K * S.dim).zeros()
+    ev = np.zeros(K)
+    nit = ctypes.c_int()
+    gpu_lib.call("qed_davidson", S.h, K, 1e-5, 1000, gpu_lib.dp(ev), psi.ptr, ctypes.byref(nit))
+    V = psi.to_host().reshape(K, S.dim)
+    e = np.linalg.eigvalsh(H)[:K]
+    print(case, "dim", S.dim, "Davidson %d iterations" % nit.value, ev, "dense", e)
+    np.testing.assert_allclose(ev, e, rtol=0, atol=1e-8)
+    np.testing.assert_allclose(V @ V.T, np.eye(K), atol=1e-10)
+    for k in range(K):
+        assert np.linalg.norm(H @ V[k] - ev[k] * V[k]) <= 1.1e-5
+    psi.free()
+    S.close()

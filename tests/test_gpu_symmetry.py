@@ -173,3 +173,45 @@ def test_mixed_lanczos_4x4_quarter_filling(gpu_lib):
     # an orbit of length l survives in the l irreps that are trivial on its stabilizer
     assert sum(dims) == F.dim
     assert abs(min(lows) - E_full) <= 1e-10 * abs(E_full)
+
+
+def _davidson(gpu_lib, S, K, accur=1e-5):
+    import ctypes
+    from pyqcm_amd._lib import DeviceArray
+    psi = DeviceArray(K * S.dim).zeros()
+    ev = np.zeros(K)
+    nit = ctypes.c_int()
+    gpu_lib.call("qed_davidson", S.h, K, accur, 1000, gpu_lib.dp(ev), psi.ptr, ctypes.byref(nit))
+    V = psi.to_host().reshape(K, S.dim)
+    psi.free()
+    return ev, V, nit.value
+
+
+@pytest.mark.parametrize("case,nup,ndn", [("plaquette4", 4, 4), ("chain4", 2, 2)])
+def test_mixed_diagonal_and_davidson(gpu_lib, case, nup, ndn):
+    """The diagonal of H in the symmetry-adapted basis (the factorized state's diagonal plus
+    the hops that stay inside its own orbit, which the plaquette's mirrors produce) against
+    the dense oracle matrix, and Davidson-Liu's 4 lowest states of each irrep (the true
+    diagonal as its preconditioner) against the dense spectrum: values, residuals
+    <= accur_Davidson, orthonormal vectors."""
+    from pyqcm_amd.sector import MixedSector
+    m, vals, n_sites, gens, bath_irrep = CASES[case]()
+    G = Y.SymmetryGroup(m.L, n_sites, gens, bath_irrep=bath_irrep)
+    one, inter = _elements(m, vals)
+    K = 4
+    for r in range(G.g):
+        ref = Y.mixed_basis(G, nup, ndn, r)
+        if len(ref) < 2 * K:
+            continue
+        H = Y.mixed_hamiltonian(G, ref, r, one, inter)
+        S = MixedSector(m.L, nup, ndn, m.flat(vals), n_sites, gens, r, bath_irrep)
+        d = S.diag()
+        np.testing.assert_allclose(d, np.diag(H), rtol=0, atol=1e-12 * np.abs(H).max())
+        ev, V, nit = _davidson(gpu_lib, S, K)
+        e = np.linalg.eigvalsh(H)[:K]
+        print(case, "irrep", r, "dim", len(ref), "Davidson %d iterations" % nit, ev, "dense", e)
+        np.testing.assert_allclose(ev, e, rtol=0, atol=1e-8)
+        np.testing.assert_allclose(V @ V.T, np.eye(K), atol=1e-10)
+        for k in range(K):
+            assert np.linalg.norm(H @ V[k] - ev[k] * V[k]) <= 1.1e-5
+        S.close()

@@ -154,3 +154,34 @@ def test_symmetric_model_errors(gpu_lib):
     lab = register(m, vals, "R0:N4:S0", 4, [[4, 3, 2, 1]])
     with pytest.raises(RuntimeError, match="point group"):
         qcm.susceptibility("U", np.array([0.5j]), lab)
+
+
+def test_davidson_method_on_symmetric_and_nambu_instances(gpu_lib):
+    """Ground_state_method 'D' (Davidson-Liu with the true diagonal of each sector) on an
+    instance over every irrep of the plaquette's N=8, S=0 and on a Nambu chain (sector
+    R0:S0): the same ground state and Green function as the dense oracle."""
+    from pyqcm_amd import qcm
+    from tests import test_gpu_mixing_instance as MI
+    qcm.set_global_parameter("Ground_state_method", "D")
+    # residual 1e-7: the vector's error (~ residual / gap) bounds G's to ~1e-6, E0's is ~1e-14
+    qcm.set_global_parameter("accur_Davidson", 1e-7)
+    try:
+        m, vals, ns, gens, bi = _plaquette4()
+        lab = register(m, vals, "/".join("R%d:N8:S0" % r for r in range(4)), ns, gens, bi)
+        E0, sec = qcm.ground_state_solve(lab)
+        E0d, Gd = O.dense_gf(m, vals, 4, 4, list(range(ns)), ZS)
+        print("plaquette Davidson E0 %.12f dense %.12f" % (E0, E0d), sec)
+        assert abs(E0 - E0d) <= 1e-9 * abs(E0d)
+        np.testing.assert_allclose(qcm.Green_function(ZS, 0, lab), Gd, atol=1e-5)
+        m, vals, n = MI.chain(d=0.5, s=0.2)
+        lab = MI.register(m, vals, "R0:S0", n)
+        E0, _ = qcm.ground_state_solve(lab)
+        E0d, Gd = O.dense_spinor_gf(m, vals, O.block_states(m.L, MI.blocks(m.L, S=0)), MI.spinor(m.L, n, True, False),
+                                    O.block_states(m.L, MI.blocks(m.L, S=-1)), O.block_states(m.L, MI.blocks(m.L, S=1)),
+                                    MI.ZS)
+        print("Nambu Davidson E0 %.12f dense %.12f" % (E0, E0d))
+        assert abs(E0 - E0d) <= 1e-9 * abs(E0d)
+        np.testing.assert_allclose(qcm.Green_function(MI.ZS, 0, lab), Gd, atol=1e-5)
+    finally:
+        qcm.set_global_parameter("Ground_state_method", "L")
+        qcm.set_global_parameter("accur_Davidson", 1e-5)
