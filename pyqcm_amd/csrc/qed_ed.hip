@@ -417,6 +417,16 @@ void set_hopping_matrix_spinor(Instance& I) {
   diagonalize_bath(I, 0, dg, dbg);
   I.eb[1] = I.eb[0];
   I.tcb_d[1] = I.tcb_d[0];
+  // the bath's part of the Potthoff functional: minus the negative ones among the n_bath
+  // lowest energies of the spinor bath, no spin factor (model_instance::set_hopping_matrix,
+  // src_ed/model_instance.hpp:469-473, which reads the first n_bath diagonal entries: the
+  // lowest when the bath was diagonalized; a bath matrix that is already diagonal is
+  // sorted here too, since its stored order lists the particle block first and the
+  // Nambu functional then misses the hole block's negative energies)
+  std::vector<double> e = I.eb[0];
+  std::sort(e.begin(), e.end());
+  for (int i = 0; i < nb; ++i)
+    if (e[i] < 0) I.SEF_bath -= e[i];
 }
 
 // ---- one-body matrices (model_instance::set_hopping_matrix, model_instance.hpp:424-475) --
@@ -2302,7 +2312,6 @@ int qed_ed_potthoff_functional(uint64_t label, double* omega) {
   std::lock_guard<std::recursive_mutex> lk(lib_mutex());
   return guard([&] {
     Instance& I = inst(label);
-    require_no_spinor(I, "the Potthoff functional");
     green_function_solve(I);
     *omega = I.SEF_bath + I.E0;
   });

@@ -48,7 +48,14 @@ class Tiling:
                                    (pyqcm hopping_operator);
       onsite(name, amplitude)      amplitude on every site (mu uses -1);
       density_wave(name, Q)        cos(pi Q.r_i) on site i, both spins
-                                   (src_qcm/lattice_model.cpp:1149-1215, type 'N').
+                                   (src_qcm/lattice_model.cpp:1149-1215, type 'N');
+    and as real pairing matrices P^n, P^n_ij (c_{i up, 0} c_{j down, n} + h.c.):
+      pairing(name, link, amplitude)  singlet pairing, amplitude on
+                                   c_i up c_j down - c_i down c_j up + h.c., j = i + link,
+                                   and amplitude on c_i up c_i down + h.c. for link 0
+                                   (pyqcm anomalous_operator, type 'singlet',
+                                   src_qcm/lattice_model.cpp:308-341,1101-1133).
+    Operators with pairing terms need the Nambu spinor (c_up(k), c+_down(-k)).
     """
 
     def __init__(self, sites, superlattice):
@@ -71,6 +78,7 @@ class Tiling:
         self.S = full
         self.Sinv = np.linalg.inv(full.T)
         self.terms = {}  # name -> {n (tuple of dim ints): n x n real matrix}
+        self.pairs = {}  # name -> {n: n x n real pairing matrix P^n}
         self.n_orb = self.n
 
     def _zero(self):
@@ -102,6 +110,25 @@ class Tiling:
             self._mat(name, neg)[j, i] += amplitude
         return self
 
+    def pairing(self, name, link, amplitude=1.0):
+        link = np.array(list(link) + [0] * (3 - len(link)), np.int64)
+        d = self.pairs.setdefault(name, {})
+
+        def mat(nvec):
+            if nvec not in d:
+                d[nvec] = np.zeros((self.n, self.n))
+            return d[nvec]
+        for i, r in enumerate(self.sites):
+            loc = self._locate(r + link)
+            if loc is None:
+                raise ValueError("site %d + link %s is not a lattice site" % (i, link))
+            j, nvec = loc
+            mat(nvec)[i, j] += amplitude
+            if np.any(link):
+                # c_i up c_j down - c_i down c_j up = c_i up c_j down + c_j up c_i down
+                mat(tuple(-x for x in nvec))[j, i] += amplitude
+        return self
+
     def onsite(self, name, amplitude=1.0):
         M = self._mat(name, self._zero())
         M[np.diag_indices(self.n)] += amplitude
@@ -116,8 +143,11 @@ class Tiling:
 
     # ---- decompositions
 
-    def cluster_matrix(self, values):
-        """Intra-cluster one-body matrix T_0 = sum_op value_op T_0^op."""
+    def cluster_matrix(self, values, nambu=False):
+        """Intra-cluster one-body matrix T_0 = sum_op value_op T_0^op (with nambu, the
+        2n x 2n Nambu matrix of the translation n = 0)."""
+        if nambu:
+            return self.nambu_terms(values).get(self._zero(), np.zeros((2 * self.n, 2 * self.n))).real.copy()
         T0 = np.zeros((self.n, self.n))
         for name, d in self.terms.items():
             z = values.get(name, 0.0)
@@ -125,10 +155,49 @@ class Tiling:
                 T0 += z * d[self._zero()]
         return T0
 
+    def nambu_terms(self, values):
+        """{n: 2n x 2n matrix} of the lattice Hamiltonian in the Nambu spinor
+        Psi_k = (c_up(k), c+_down(-k)), h(k) = sum_n H^n exp(2 pi i k.n) with
+          H^n = [[T^n, -P^n], [-(P^-n)^T, -(T^-n)^T]]
+        (the hole block from c+_down c_down = 1 - c_down c+_down; the pairing blocks from
+        v c_up c_down = -v c_down c_up), the convention of the cluster's spinor matrices
+        (src_ed/Operators/anomalous_operator.hpp:241-284)."""
+        n = self.n
+        T, P = {}, {}
+        for src, dst in ((self.terms, T), (self.pairs, P)):
+            for name, d in src.items():
+                z = values.get(name, 0.0)
+                if not z:
+                    continue
+                for nvec, M in d.items():
+                    dst[nvec] = dst.get(nvec, 0.0) + z * M
+        out = {}
+        for nvec in set(T) | set(P) | {tuple(-x for x in v) for v in list(T) + list(P)}:
+            neg = tuple(-x for x in nvec)
+            H = np.zeros((2 * n, 2 * n))
+            if nvec in T:
+                H[:n, :n] = T[nvec]
+            if neg in T:
+                H[n:, n:] = -np.asarray(T[neg]).T
+            if nvec in P:
+                H[:n, n:] = -P[nvec]
+            if neg in P:
+                H[n:, :n] = -np.asarray(P[neg]).T
+            if np.any(H):
+                out[nvec] = H
+        return out
+
     def cluster_operator_elements(self, name):
         """Intra-cluster part of operator `name` as 1-based (r, c, v), r <= c, over both
         spins (site i -> label i+1 for spin up, i+1+n_orb for spin down), the form
-        qcm.new_operator takes."""
+        qcm.new_operator takes; for a pairing operator, the anomalous elements
+        (i up, j down, P^0_ij), each standing for P^0_ij (c_i up c_j down + h.c.)."""
+        if name in self.pairs:
+            P0 = self.pairs[name].get(self._zero())
+            if P0 is None:
+                return []
+            return [(i + 1, j + 1 + self.n_orb, float(P0[i, j])) for i in range(self.n) for j in range(self.n)
+                    if P0[i, j] != 0.0]
         T0 = self.terms.get(name, {}).get(self._zero())
         out = []
         if T0 is None:
@@ -140,10 +209,17 @@ class Tiling:
                         out.append((i + 1 + s, j + 1 + s, float(T0[i, j])))
         return out
 
-    def fourier_terms(self, values):
-        """Inter-cluster terms: (nvecs (m x 3) float, mats (m x n x n) complex), n != 0."""
+    def fourier_terms(self, values, nambu=False):
+        """Inter-cluster terms: (nvecs (m x 3) float, mats (m x n x n) complex), n != 0
+        (with nambu, the 2n x 2n Nambu matrices)."""
         acc = {}
         zero = self._zero()
+        if nambu:
+            acc = {k: v for k, v in self.nambu_terms(values).items() if k != zero}
+            keys = list(acc)
+            nv = np.array([list(k) + [0] * (3 - self.dim) for k in keys], float).reshape(-1, 3)
+            mats = np.array([acc[k] for k in keys], complex).reshape(-1, 2 * self.n, 2 * self.n)
+            return nv, mats
         for name, d in self.terms.items():
             z = values.get(name, 0.0)
             if not z:
@@ -157,11 +233,11 @@ class Tiling:
         mats = np.array([acc[k] for k in keys], complex).reshape(-1, self.n, self.n)
         return nv, mats
 
-    def tk(self, values, k):
-        """t(k) for wavevectors k (nk x 3), dense host evaluation."""
+    def tk(self, values, k, nambu=False):
+        """t(k) for wavevectors k (nk x 3), dense host evaluation (with nambu, h(k))."""
         k = np.atleast_2d(np.asarray(k, float))
-        nv, mats = self.fourier_terms(values)
-        T0 = self.cluster_matrix(values).astype(complex)
+        nv, mats = self.fourier_terms(values, nambu)
+        T0 = self.cluster_matrix(values, nambu).astype(complex)
         if not len(nv):
             return np.repeat(T0[None], len(k), 0)
         ph = np.exp(2j * np.pi * (k @ nv.T))
@@ -349,7 +425,7 @@ class LatticeModel:
         for op in lattice_ops:
             el = tiling.cluster_operator_elements(op)
             if el:
-                qcm.new_operator(name, op, "one-body", el)
+                qcm.new_operator(name, op, "anomalous" if op in tiling.pairs else "one-body", el)
                 self.cluster_ops.append(op)
         L = self.L
         qcm.new_operator(name, interaction, "interaction",
@@ -362,10 +438,14 @@ class LatticeModel:
     def L(self):
         return self.n + self.n_bath
 
-    def add_cluster_operator(self, name, elements):
-        """One-body cluster operator with 1-based (r, c, v) elements over 2L spin orbitals;
-        its bath-bath and cluster-bath spin-up blocks are recorded for the bath fit."""
-        qcm.new_operator(self.name, name, "one-body", elements)
+    def add_cluster_operator(self, name, elements, kind="one-body"):
+        """Cluster operator with 1-based (r, c, v) elements over 2L spin orbitals (kind
+        "one-body" or "anomalous"); a one-body operator's bath-bath and cluster-bath spin-up
+        blocks are recorded for the bath fit."""
+        qcm.new_operator(self.name, name, kind, elements)
+        if kind != "one-body":
+            self.cluster_ops.append(name)
+            return
         B = np.zeros((self.n_bath, self.n_bath))
         C = np.zeros((self.n, self.n_bath))
         for r, c, v in elements:
@@ -435,37 +515,55 @@ class LatticeInstance:
             self._gs = qcm.ground_state_solve(self.label)
         return self._gs
 
-    def green(self, z):
+    @property
+    def mixing(self):
+        """The instance's mixing: 0 normal, 1 Nambu (pairing terms), 4 up/down (spin
+        asymmetric cluster terms), the lattice model's mixing being the instance's
+        (src_qcm/lattice_model.cpp:458-468)."""
         self.ground_state()
-        if qcm.mixingC(self.label) not in (0, 4):
-            raise RuntimeError("the lattice functions take the normal or up/down mixing only "
-                               "(instance mixing %d)" % qcm.mixingC(self.label))
-        return qcm.Green_function(np.atleast_1d(z), 0, self.label)
+        m = qcm.mixingC(self.label)
+        if m not in (0, 1, 4):
+            raise RuntimeError("the lattice functions take the normal, Nambu or up/down mixing only "
+                               "(instance mixing %d)" % m)
+        return m
 
-    def _geff(self, z):
-        G = self.green(z)
+    @property
+    def nambu(self):
+        return self.mixing == 1
+
+    def spins(self):
+        """The spin blocks the lattice functions sum over: both for the up/down mixing."""
+        return (0, 1) if self.mixing == 4 else (0,)
+
+    def green(self, z, spin_down=0):
+        self.mixing
+        return qcm.Green_function(np.atleast_1d(z), spin_down, self.label)
+
+    def _geff(self, z, spin_down=0):
+        G = self.green(z, spin_down)
         if self.model.n_bath == 0:
             return G, G
-        Gam = qcm.hybridization_functionC(np.atleast_1d(z), 0, self.label)
+        Gam = qcm.hybridization_functionC(np.atleast_1d(z), spin_down, self.label)
         return G, np.linalg.inv(np.linalg.inv(G) + Gam)
 
-    def v0(self):
-        """V0 = T_0(lattice) - t_c, the k-independent part of V."""
-        tc = qcm.hopping_matrix(0, 0, self.label).real
-        return self.model.tiling.cluster_matrix(self.values) - tc
+    def v0(self, spin_down=0):
+        """V0 = T_0(lattice) - t_c, the k-independent part of V (Nambu: 2n x 2n)."""
+        tc = qcm.hopping_matrix(spin_down, 0, self.label).real
+        return self.model.tiling.cluster_matrix(self.values, self.nambu) - tc
 
-    def cpt_green(self, k, z, mode="full"):
+    def cpt_green(self, k, z, mode="full", spin_down=0):
         """CPT Green function at frequencies z and wavevectors k (QCM::CPT_Green_function,
-        src_qcm/QCM.cpp:328-344)."""
-        _, Geff = self._geff(z)
-        nv, mats = self.model.tiling.fourier_terms(self.values)
-        return cpt_batch(Geff, self.v0(), nv, mats, k, mode)
+        src_qcm/QCM.cpp:328-344); with the Nambu mixing, of the spinor
+        (c_up(k), c+_down(-k)), 2n x 2n."""
+        _, Geff = self._geff(z, spin_down)
+        nv, mats = self.model.tiling.fourier_terms(self.values, self.nambu)
+        return cpt_batch(Geff, self.v0(spin_down), nv, mats, k, mode)
 
-    def sef_integrand(self, z, k):
-        """(1/nk) sum_k log|det(V(k, z) G_c(z) - 1)| per frequency (one spin)."""
-        G, Geff = self._geff(z)
-        nv, mats = self.model.tiling.fourier_terms(self.values)
-        I = sef_batch(Geff, self.v0(), nv, mats, k)
+    def sef_integrand(self, z, k, spin_down=0):
+        """(1/nk) sum_k log|det(V(k, z) G_c(z) - 1)| per frequency (one spin block)."""
+        G, Geff = self._geff(z, spin_down)
+        nv, mats = self.model.tiling.fourier_terms(self.values, self.nambu)
+        I = sef_batch(Geff, self.v0(spin_down), nv, mats, k)
         if self.model.n_bath:
             I = I + np.log(np.abs(np.linalg.det(G))) - np.log(np.abs(np.linalg.det(Geff)))
         return I
@@ -473,15 +571,25 @@ class LatticeInstance:
     def potthoff_functional(self, grid=(1, 10, 10, 10, 10), kside=64):
         """Self-energy functional on a fixed (w, k) grid, the grid path of
         lattice_model_instance::Potthoff_functional (src_qcm/lattice_model_instance.cpp:503-584):
-        Omega = [Omega_c - sum_w weight_w 2 (1/nk) sum_k log|det(V G - 1)|
-                 + sum_i (H_ii - Hc_ii)] / n_sites   (spin-symmetric case),
-        with the k-grid k = i/side (src_util/integrate.cpp:177-239)."""
+          Omega = [Omega_c - sum_w weight_w f (1/nk) sum_k log|det(V G - 1)| - Omega_diag] / n_sites,
+          Omega_diag = (1/2) f sum_i s_i (H_ii - Hc_ii),
+        with f = 2 for the normal mixing (two equal spins) and 1 otherwise (the up/down
+        mixing sums both spin blocks), s_i = +1 on the Nambu (hole) components and -1 on
+        the others, and the k-grid k = i/side (src_util/integrate.cpp:177-239)."""
         ws, wt = legendre_frequency_grid(*grid)
         k = self.model.tiling.kgrid(kside, shift=0.0)
-        Iw = 2.0 * self.sef_integrand(1j * ws, k)
+        f = 2.0 if self.mixing == 0 else 1.0
+        Iw = f * sum(self.sef_integrand(1j * ws, k, sp) for sp in self.spins())
         omega_trace = -np.sum(Iw * wt)
-        omega_diag = -np.trace(self.v0())
-        self.ground_state()
+        omega_diag = 0.0
+        for sp in self.spins():
+            d = np.diag(self.v0(sp))
+            if self.nambu:
+                n = self.model.n
+                omega_diag += d[n:].sum() - d[:n].sum()
+            else:
+                omega_diag -= d.sum()
+        omega_diag *= 0.5 * f
         omega_clus = qcm.Potthoff_functional(self.label)
         return (omega_trace - omega_diag + omega_clus) / self.model.n
 
