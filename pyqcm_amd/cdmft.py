@@ -244,8 +244,11 @@ class CDMFTProblem:
     cluster suffix '_1', as in pyqcm); `sectors` is the target-sector string, or a function
     of the parameter values returning it."""
 
-    def __init__(self, model, varia, sectors, grid=None, kside=32, graph=True):
+    def __init__(self, model, varia, sectors, grid=None, kside=32, graph=True, nambu=None):
         self.model = model
+        # the spinor of the fit: None until the first solve tells the instance's mixing
+        # (1: pairing terms, the Nambu spinor; 0 or 4: one spin block)
+        self.nambu = nambu
         self.graph = graph  # replay the batched Lanczos steps as hipGraphs
         self.varia = [v[:-2] if v.endswith("_1") else v for v in varia]
         self.sectors = sectors
@@ -257,7 +260,17 @@ class CDMFTProblem:
         vals.update(zip(self.varia, np.asarray(x, float)))
         inst = self.model.instance(vals, self.sectors_of(values), label)
         inst.ground_state()
+        self._check_mixing(inst)
         return inst
+
+    def _check_mixing(self, inst):
+        m = inst.mixing
+        if m not in (0, 1):
+            raise RuntimeError("CDMFT takes the normal and Nambu mixings (instance mixing %d)" % m)
+        if self.nambu is None:
+            self.nambu = m == 1
+        elif self.nambu != (m == 1):
+            raise RuntimeError("the instance's mixing changed during CDMFT (%d)" % m)
 
     def host(self, inst):
         return inst.cdmft_host(self.grid.wr, self.kside)
@@ -276,9 +289,10 @@ class CDMFTProblem:
         for I in insts:
             E0 = I.ground_state()[0]
             if np.isfinite(E0):
+                self._check_mixing(I)
                 hosts.append(self.host(I))
             else:  # the impurity solve failed for this instance
-                n = self.model.n
+                n = self.model.n * (2 if self.nambu else 1)
                 hosts.append(np.full((len(self.grid.wr), n, n), np.nan, complex))
             I.erase()
         return hosts
@@ -287,17 +301,18 @@ class CDMFTProblem:
         return self.sectors(values) if callable(self.sectors) else self.sectors
 
     def distance(self, Ghost, X):
-        eb, tcb = self.model.diagonal_bath(self.varia, X)
-        return LT.cdmft_distance_batch(self.grid.wr, self.grid.cdmft_weight, Ghost, eb, tcb)
+        eb, tcb = self.model.diagonal_bath(self.varia, X, bool(self.nambu))
+        return LT.cdmft_distance_batch(self.grid.wr, self.grid.cdmft_weight, Ghost, eb, tcb,
+                                       1 if self.nambu else 0)
 
     def distance_multi(self, hosts, requests):
         """requests: list of (host index, points (m x nvar)); all points are evaluated in
         one device call (qed_cdmft_distance_multi), values returned per request."""
         pts = np.concatenate([np.atleast_2d(p) for _, p in requests])
         host_of = np.concatenate([np.full(len(np.atleast_2d(p)), h, np.int32) for h, p in requests])
-        eb, tcb = self.model.diagonal_bath(self.varia, pts)
+        eb, tcb = self.model.diagonal_bath(self.varia, pts, bool(self.nambu))
         d = LT.cdmft_distance_multi(self.grid.wr, self.grid.cdmft_weight, np.asarray(hosts),
-                                    host_of, eb, tcb)
+                                    host_of, eb, tcb, 1 if self.nambu else 0)
         out, o = [], 0
         for _, p in requests:
             m = len(np.atleast_2d(p))
@@ -307,7 +322,9 @@ class CDMFTProblem:
 
     def linear_bath_map(self):
         """(E (nvar x nb), T (nvar x n x nb)) when every variational parameter acts on a
-        diagonal bath (eb and tcb linear in x); None otherwise."""
+        diagonal bath (eb and tcb linear in x) of one spin block; None otherwise."""
+        if self.nambu:
+            return None
         nb, n = self.model.n_bath, self.model.n
         E = np.zeros((len(self.varia), nb))
         T = np.zeros((len(self.varia), n, nb))

@@ -21,6 +21,8 @@ this module only assembles the small per-frequency matrices.
 """
 from __future__ import annotations
 
+import itertools
+
 import numpy as np
 
 from . import _lib, qcm
@@ -408,6 +410,10 @@ def legendre_frequency_grid(w1, w2, n1, n2, n3):
 
 # ---------------------------------------------------------------- cluster side
 
+# instance labels shared by every lattice model, so that two models never reuse one
+_instance_labels = itertools.count(1)
+
+
 class LatticeModel:
     """A lattice model: a tiling plus the ED cluster model (cluster sites first, then
     bath orbitals) registered through qcm. `lattice_ops` are the tiling operators whose
@@ -432,7 +438,7 @@ class LatticeModel:
                          [(i + 1, i + 1 + L, 1.0) for i in range(self.n)])
         self.cluster_ops.append(interaction)
         self.bath_ops = {}  # name -> (B (nb x nb), C (n x nb)) spin-up linear blocks
-        self._next_label = 0
+        self.bath_elements = {}  # name -> (1-based elements, kind) of add_cluster_operator
 
     @property
     def L(self):
@@ -443,7 +449,9 @@ class LatticeModel:
         "one-body" or "anomalous"); a one-body operator's bath-bath and cluster-bath spin-up
         blocks are recorded for the bath fit."""
         qcm.new_operator(self.name, name, kind, elements)
+        self.bath_elements[name] = (list(elements), kind)
         if kind != "one-body":
+            self.bath_ops[name] = (np.zeros((self.n_bath, self.n_bath)), np.zeros((self.n, self.n_bath)))
             self.cluster_ops.append(name)
             return
         B = np.zeros((self.n_bath, self.n_bath))
@@ -461,11 +469,58 @@ class LatticeModel:
         self.bath_ops[name] = (B, C)
         self.cluster_ops.append(name)
 
-    def diagonal_bath(self, names, X):
+    def nambu_bath_blocks(self, name):
+        """(B (2nb x 2nb), C (2n x 2nb)): the bath-bath and cluster-bath blocks of the Nambu
+        spinor matrices of a cluster operator at unit value, by the rules of the cluster's
+        spinor one-body matrices (one_body_operator.hpp:326-411, anomalous_operator.hpp:
+        241-284): with M the operator over the 2L spin orbitals (one-body: v on (r, c) and
+        (c, r); anomalous: v/2 on (r, c) and -v/2 on (c, r), the binding's halving),
+          one-body   bath  diag(M_uu, -M_dd^T)    cluster-bath  diag(M_uu, -M_dd)
+          anomalous  bath  2 M_du below, 2 M_du^T above
+                     cluster-bath  2 M_du below, -2 M_ud above."""
+        el, kind = self.bath_elements[name]
+        n, nb, L = self.n, self.n_bath, self.L
+        F = np.zeros((2 * L, 2 * L))
+        for r, c, v in el:
+            r, c = r - 1, c - 1
+            if kind == "anomalous":
+                if r < c:
+                    F[r, c] += 0.5 * v
+                    F[c, r] -= 0.5 * v
+            elif r < c:
+                F[r, c] += v
+                F[c, r] += v
+            elif r == c:
+                F[r, r] += v
+        S, Bt = slice(0, n), slice(n, L)
+        uu, dd = F[:L, :L], F[L:, L:]
+        du, ud = F[L:, :L], F[:L, L:]
+        Bn = np.zeros((2 * nb, 2 * nb))
+        Cn = np.zeros((2 * n, 2 * nb))
+        if kind == "anomalous":
+            Bn[nb:, :nb] = 2.0 * du[Bt, Bt]
+            Bn[:nb, nb:] = 2.0 * du[Bt, Bt].T
+            Cn[n:, :nb] = 2.0 * du[S, Bt]
+            Cn[:n, nb:] = -2.0 * ud[S, Bt]
+        else:
+            Bn[:nb, :nb] = uu[Bt, Bt]
+            Bn[nb:, nb:] = -dd[Bt, Bt].T
+            Cn[:n, :nb] = uu[S, Bt]
+            Cn[n:, nb:] = -dd[S, Bt]
+        return Bn, Cn
+
+    def diagonal_bath(self, names, X, nambu=False):
         """(eb (B x nb), tcb (B x n x nb)) for rows X (B x len(names)) of bath parameters;
         a non-diagonal bath matrix is diagonalized as in the reference's hybridization
-        function (src_ed/model_instance.hpp:1004-1018)."""
+        function (src_ed/model_instance.hpp:1004-1018). With nambu, the spinor forms:
+        eb (B x 2nb), tcb (B x 2n x 2nb)."""
         X = np.atleast_2d(np.asarray(X, float))
+        if nambu:
+            blocks = [self.nambu_bath_blocks(nm) for nm in names]
+            E = np.einsum("bi,ijk->bjk", X, np.array([b[0] for b in blocks]))
+            T = np.einsum("bi,ijk->bjk", X, np.array([b[1] for b in blocks]))
+            eb, U = np.linalg.eigh(E)
+            return eb, T @ U
         E = np.zeros((len(X), self.n_bath, self.n_bath))
         T = np.zeros((len(X), self.n, self.n_bath))
         for i, nm in enumerate(names):
@@ -484,8 +539,7 @@ class LatticeModel:
         other names are lattice parameters, inherited by the cluster operator of the same
         name unless that has its own '_1' value."""
         if label is None:
-            self._next_label += 1
-            label = 7000 + self._next_label
+            label = 7000 + next(_instance_labels)
         lattice_vals = {k: v for k, v in values.items() if not k.endswith("_1")}
         cluster_vals = {k: v for k, v in lattice_vals.items() if k in self.cluster_ops}
         for k, v in values.items():
