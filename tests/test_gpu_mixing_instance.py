@@ -196,3 +196,40 @@ def test_mixing_sector_errors(gpu_lib):
         qcm.Green_function(ZS, 1, lab)
     with pytest.raises(RuntimeError, match="mixing"):
         qcm.susceptibility("U", np.array([0.5j]), lab)
+
+
+@pytest.mark.parametrize("case", ["nambu", "spin_flip", "full"])
+def test_mixing_cluster_averages_match_dense(gpu_lib, case):
+    """Ground-state averages and variances of every operator, the pairing and spin-flip
+    ones included (model_instance::cluster_averages, src_ed/model_instance.hpp:152-163),
+    against <psi|O|psi> and <psi|O^2|psi> - <O>^2 of the dense ground state; site
+    operators are divided by the number of sites, mu with the opposite sign."""
+    from pyqcm_amd import qcm
+    if case == "nambu":
+        m, vals, n = chain(d=0.5, s=0.2)
+        sec, b0 = "R0:S0", blocks(m.L, S=0)
+    elif case == "spin_flip":
+        m, vals, n = chain(hx=0.4)
+        sec, b0 = "R0:N4", blocks(m.L, N=4)
+    else:
+        m, vals, n = chain(hx=0.3, d=0.4)
+        sec, b0 = "R0", blocks(m.L, parity=0)
+    lab = register(m, vals, sec, n)
+    qcm.ground_state_solve(lab)
+    ave = qcm.cluster_averages(lab)
+    states = O.block_states(m.L, b0)
+    e, U = np.linalg.eigh(O.dense_hamiltonian(m, vals, 0, 0, states=states)[0])
+    assert e[1] - e[0] > 1e-6
+    psi = U[:, 0]
+    for name, op in m.operators.items():
+        if vals.get(name, 0.0) == 0.0:
+            continue
+        one = O.ClusterModel(m.L)
+        one.add(name, op.kind, op.elements)
+        Om = O.dense_hamiltonian(one, {name: 1.0}, 0, 0, states=states)[0]
+        a = psi @ Om @ psi
+        v = psi @ Om @ (Om @ psi) - a * a
+        sgn = -1.0 if name == "mu" else 1.0
+        print(case, name, "average %.12f dense %.12f" % (ave[name][0], sgn * a / n))
+        assert abs(ave[name][0] - sgn * a / n) < 1e-9, name
+        assert abs(ave[name][1] - v / n ** 2) < 1e-9, name
