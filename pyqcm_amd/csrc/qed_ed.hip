@@ -121,6 +121,14 @@ struct MCFrac {
   int M() const { return pa ? (int)(A.size() / ((size_t)pa * pa)) : 0; }
 };
 
+// continued fractions of one block of p orbitals (an irrep of the symmetric orbitals, or
+// the whole spinor): orbital k = sum_a W[k d + a] (component a of the Green function)
+struct CFBlock {
+  int p = 0;
+  std::vector<double> W;     // p x d, row-major
+  std::vector<CFrac> e, h;   // p x p (o1 >= o2 at o1 + o2 p), electron / hole parts
+};
+
 struct State {
   int nup = 0, ndn = 0, irrep = 0;
   int secN = 0, secS = 0;  // the sector (N, S; codes where not conserved)
@@ -128,6 +136,7 @@ struct State {
   DevBuf psi;
   QMat gf[2];
   std::vector<CFrac> cfe[2], cfh[2];  // n x n (o1 >= o2), electron / hole parts
+  std::vector<CFBlock> cfb[2];        // symmetric orbitals or spinor: fractions per block
   MCFrac mcf[2][2];                   // [spin][0: holes, 1: particles]
   MCFrac mcf_comb[2];                 // combined fraction (combine_mcf), evaluated alone
 };
@@ -735,8 +744,8 @@ void build_qmatrix_sym(Instance& I, State& st, int spin) {
 // src_ed/model.cpp:155-157), one band Lanczos per shifted sector (shift_sector,
 // src_ed/symmetry_group.cpp:540-558: every component lands in the same one), one Q-matrix
 // of dimension n_mixed x n_sites
-void build_qmatrix_spinor(Instance& I, State& st) {
-  Model& M = *I.model;
+std::vector<std::pair<int, bool>> spinor_components(const Instance& I) {
+  const Model& M = *I.model;
   const int ns = M.n_sites, L = M.L, m = I.mixing;
   const int nambu_max = (m & 1) ? 2 : 1, spin_max = (m & 2) ? 2 : 1;
   std::vector<std::pair<int, bool>> spinor;
@@ -746,13 +755,21 @@ void build_qmatrix_spinor(Instance& I, State& st) {
         const int spin = inb ? (spin_max == 2 ? isp : (isp + 1) % 2) : isp;
         spinor.push_back({i + spin * L, inb != 0});
       }
+  return spinor;
+}
+
+// the sector code or number after adding d particles (shift_sector,
+// src_ed/symmetry_group.cpp:540-558): parity codes swap, numbers move by d
+int shift_code(int x, int d) { return x == kSecOdd ? kSecEven : (x == kSecEven ? kSecOdd : x + d); }
+
+void build_qmatrix_spinor(Instance& I, State& st) {
+  const std::vector<std::pair<int, bool>> spinor = spinor_components(I);
   const int p = (int)spinor.size();
   SectorPtr B = state_sector(I, st);
-  auto shift = [](int x, int d) { return x == kSecOdd ? kSecEven : (x == kSecEven ? kSecOdd : x + d); };
   QMat parts[2];
   for (int s = 0; s < 2; ++s) {
     const int pm = 2 * s - 1;
-    SectorPtr T = make_general_sector(I, shift(st.secN, pm), shift(st.secS, pm));
+    SectorPtr T = make_general_sector(I, shift_code(st.secN, pm), shift_code(st.secS, pm));
     if (T->D == 0) continue;
     DevBuf phi((size_t)p * T->D * 8);
     QED_HIP(hipMemsetAsync(phi.p, 0, (size_t)p * T->D * 8, stream()));
@@ -820,10 +837,98 @@ void build_qmatrix(Instance& I, State& st, int spin) {
   st.gf[spin] = tot;
 }
 
+// The continued fractions of one block of p orbitals (model_instance<double>::build_cf,
+// src_ed/model_instance.cpp:34-99): for pm = -1, +1 the target sector target(pm) (null if
+// none), and for o1 >= o2 the start vector A_o1 |psi> (+ A_o2 |psi>) accumulated by
+// apply(T, k, pm, y), normalized, tridiagonalized by Lanczos; b[0] = norm x weight.
+template <class Target, class Apply>
+void cf_block(State& st, int p, Target target, Apply apply, CFBlock& blk) {
+  blk.p = p;
+  blk.e.assign((size_t)p * p, CFrac());
+  blk.h.assign((size_t)p * p, CFrac());
+  for (int pm = -1; pm < 2; pm += 2) {
+    SectorPtr T = target(pm);
+    if (!T || T->D == 0) continue;
+    DevBuf psi(T->D * 8);
+    for (int o1 = 0; o1 < p; ++o1)
+      for (int o2 = 0; o2 <= o1; ++o2) {
+        QED_HIP(hipMemsetAsync(psi.p, 0, T->D * 8, stream()));
+        apply(*T, o1, pm, psi.as<double>());
+        if (o1 != o2) apply(*T, o2, pm, psi.as<double>());
+        const double nrm = dot(psi.as<double>(), psi.as<double>(), T->D);
+        if (nrm <= 1e-14) continue;
+        scal(1.0 / std::sqrt(nrm), psi.as<double>(), T->D);
+        std::vector<double> a, b;
+        lanczos_green(*T, psi.as<double>(), gd("accur_continued_fraction"), gi("max_iter_CF"), a, b);
+        CFrac f;
+        for (size_t i = 0; i < b.size(); ++i) b[i] *= b[i];
+        b[0] = nrm * st.weight;
+        for (double& x : a) x = (pm == 1) ? x - st.energy : -x + st.energy;
+        f.a = a;
+        f.b = b;
+        f.set = true;
+        (pm == 1 ? blk.e : blk.h)[o1 + (size_t)o2 * p] = f;
+      }
+  }
+}
+
+// continued fractions of the symmetric orbitals, one block per irrep (the orbitals'
+// coefficients are the rows of W), or of the spinor of the spin-flip, anomalous and full
+// mixings (one block, W the identity), with the start vectors of the Lehmann forms
+void build_cf_blocks(Instance& I, State& st, int spin) {
+  Model& M = *I.model;
+  st.cfb[spin].clear();
+  if (spinor_mixing(I.mixing)) {
+    const std::vector<std::pair<int, bool>> spinor = spinor_components(I);
+    const int p = (int)spinor.size();
+    SectorPtr B = state_sector(I, st);
+    CFBlock blk;
+    blk.W.assign((size_t)p * p, 0.0);
+    for (int k = 0; k < p; ++k) blk.W[(size_t)k * p + k] = 1.0;
+    cf_block(
+        st, p, [&](int pm) { return make_general_sector(I, shift_code(st.secN, pm), shift_code(st.secS, pm)); },
+        [&](qed_sector& T, int k, int pm, double* y) {
+          if (qed_general_apply(B.get(), &T, spinor[k].first, spinor[k].second ? -pm : pm, 1.0,
+                                st.psi.as<double>(), y) != 0)
+            fail(qed_last_error());
+        },
+        blk);
+    st.cfb[0].push_back(std::move(blk));
+    return;
+  }
+  const int n = M.n_sites;
+  SectorPtr B = make_sector(I, st.nup, st.ndn, nullptr, st.irrep);
+  for (int ro = 0; ro < M.group.g; ++ro) {
+    std::vector<int> sel;
+    for (int o = 0; o < n; ++o)
+      if (M.sorb_irrep[o] == ro) sel.push_back(o);
+    const int p = (int)sel.size();
+    if (p == 0) continue;
+    const int rt = irrep_tensor(M.group, st.irrep, ro);
+    CFBlock blk;
+    for (int k = 0; k < p; ++k) blk.W.insert(blk.W.end(), &M.sorb[(size_t)sel[k] * n], &M.sorb[(size_t)sel[k] * n] + n);
+    cf_block(
+        st, p,
+        [&](int pm) -> SectorPtr {
+          const int tnup = st.nup + (spin == 0 ? pm : 0), tndn = st.ndn + (spin == 1 ? pm : 0);
+          if (!sector_valid(I, tnup, tndn)) return nullptr;
+          return make_sector(I, tnup, tndn, nullptr, rt);
+        },
+        [&](qed_sector& T, int k, int pm, double* y) {
+          if (qed_mixed_apply_c(B.get(), &T, spin, pm, n, &M.sorb[(size_t)sel[k] * n], st.psi.as<double>(), y) != 0)
+            fail(qed_last_error());
+        },
+        blk);
+    st.cfb[spin].push_back(std::move(blk));
+  }
+}
+
 // continued fractions (model_instance<double>::build_cf, src_ed/model_instance.cpp:34-99)
 void build_cf(Instance& I, State& st, int spin) {
-  require_trivial_group(I, "the continued-fraction Green function");
-  require_no_spinor(I, "the continued-fraction Green function");
+  if (spinor_mixing(I.mixing) || I.model->symmetric()) {
+    build_cf_blocks(I, st, spin);
+    return;
+  }
   Model& M = *I.model;
   const int n = M.n_sites, L = M.L;
   st.cfe[spin].assign((size_t)n * n, CFrac());
@@ -1478,6 +1583,62 @@ void green_function(Instance& I, int nz, const double* z, int spin_down, double*
   }
 }
 
+// The p x p matrix of one block from per-pair values v(o1 >= o2): the diagonal entries and
+// 0.5 (v(o1, o2) - v(o1, o1) - v(o2, o2)) off it (continued_fraction_set::Green_function),
+// then added to g (d x d, column-major) as W^T G_block W.
+void add_block(const CFBlock& blk, int d, const std::vector<cd>& v, cd* g) {
+  const int p = blk.p;
+  std::vector<cd> G((size_t)p * p, 0.0);
+  for (int o = 0; o < p; ++o) G[o + (size_t)o * p] = v[o + (size_t)o * p];
+  for (int o1 = 0; o1 < p; ++o1)
+    for (int o2 = 0; o2 < o1; ++o2) {
+      const cd t = 0.5 * (v[o1 + (size_t)o2 * p] - v[o1 + (size_t)o1 * p] - v[o2 + (size_t)o2 * p]);
+      G[o1 + (size_t)o2 * p] = G[o2 + (size_t)o1 * p] = t;
+    }
+  for (int k = 0; k < p; ++k)
+    for (int l = 0; l < p; ++l) {
+      const cd gkl = G[k + (size_t)l * p];
+      if (gkl == 0.0) continue;
+      for (int b = 0; b < d; ++b) {
+        const double wl = blk.W[(size_t)l * d + b];
+        if (wl == 0.0) continue;
+        for (int a = 0; a < d; ++a) g[a + (size_t)b * d] += blk.W[(size_t)k * d + a] * gkl * wl;
+      }
+    }
+}
+
+// G(z) of the block continued fractions of one state: all set fractions of all blocks in
+// one device evaluation
+void cf_blocks_eval(const std::vector<CFBlock>& blocks, int d, int nz, const double* z, double* out) {
+  std::vector<int> off, dep, blk_of, pair_of;
+  std::vector<double> aa, bb;
+  for (size_t ib = 0; ib < blocks.size(); ++ib)
+    for (int h = 0; h < 2; ++h) {
+      const auto& cf = h ? blocks[ib].h : blocks[ib].e;
+      for (size_t q = 0; q < cf.size(); ++q) {
+        if (!cf[q].set) continue;
+        off.push_back((int)aa.size());
+        dep.push_back((int)cf[q].a.size());
+        aa.insert(aa.end(), cf[q].a.begin(), cf[q].a.end());
+        bb.insert(bb.end(), cf[q].b.begin(), cf[q].b.begin() + cf[q].a.size());
+        blk_of.push_back((int)ib);
+        pair_of.push_back((int)q);
+      }
+    }
+  const int nf = (int)off.size();
+  if (nf == 0) return;
+  std::vector<double> vals((size_t)2 * nf * nz);
+  if (qed_cf_eval(nf, off.data(), dep.data(), aa.data(), bb.data(), nz, z, vals.data()) != 0) fail(qed_last_error());
+  for (int iz = 0; iz < nz; ++iz) {
+    std::vector<std::vector<cd>> v(blocks.size());
+    for (size_t ib = 0; ib < blocks.size(); ++ib) v[ib].assign((size_t)blocks[ib].p * blocks[ib].p, 0.0);
+    for (int f = 0; f < nf; ++f)
+      v[blk_of[f]][pair_of[f]] += cd(vals[2 * ((size_t)iz * nf + f)], vals[2 * ((size_t)iz * nf + f) + 1]);
+    cd* g = reinterpret_cast<cd*>(out + (size_t)2 * iz * d * d);
+    for (size_t ib = 0; ib < blocks.size(); ++ib) add_block(blocks[ib], d, v[ib], g);
+  }
+}
+
 void green_function_eval(Instance& I, int nz, const double* z, int sp, double* out) {
   const int d = dim_gf(I);
   std::fill(out, out + (size_t)2 * nz * d * d, 0.0);
@@ -1497,6 +1658,8 @@ void green_function_eval(Instance& I, int nz, const double* z, int sp, double* o
       const QMat& q = s->gf[sp];
       if (q.M() == 0) continue;
       if (qed_qmatrix_gf(d, q.M(), q.e.data(), q.Q.data(), nz, z, out) != 0) fail(qed_last_error());
+    } else if (!s->cfb[sp].empty()) {
+      cf_blocks_eval(s->cfb[sp], d, nz, z, out);
     } else {
       // continued fractions: diagonal entries and the (o1 + o2) combinations
       std::vector<int> off, dep;
@@ -1580,6 +1743,17 @@ std::vector<cd> gf_average(Instance& I, int sp) {
           for (int k = 0; k < h.pa; ++k) w += h.W[k + (size_t)h.pa * a] * h.W[k + (size_t)h.pa * b];
           at(b, a) += w;  // (W^T W)^T
         }
+    } else if (I.cf && I.correlated && !s->cfb[sp].empty()) {
+      // the hole fractions' weights b[0] per block, the same combination
+      std::vector<cd> g((size_t)d * d, 0.0);
+      for (const CFBlock& blk : s->cfb[sp]) {
+        std::vector<cd> v((size_t)blk.p * blk.p, 0.0);
+        for (size_t q = 0; q < v.size(); ++q)
+          if (blk.h[q].set) v[q] = blk.h[q].b[0];
+        add_block(blk, d, v, g.data());
+      }
+      for (int a = 0; a < d; ++a)
+        for (int b = 0; b < d; ++b) at(a, b) += g[a + (size_t)b * d];
     } else if (I.cf && I.correlated) {
       const auto& h = s->cfh[sp];
       if (h.size() != (size_t)d * d) continue;

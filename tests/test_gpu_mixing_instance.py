@@ -233,3 +233,40 @@ def test_mixing_cluster_averages_match_dense(gpu_lib, case):
         print(case, name, "average %.12f dense %.12f" % (ave[name][0], sgn * a / n))
         assert abs(ave[name][0] - sgn * a / n) < 1e-9, name
         assert abs(ave[name][1] - v / n ** 2) < 1e-9, name
+
+
+@pytest.mark.parametrize("case", ["nambu", "spin_flip", "full"])
+def test_mixing_continued_fraction_green_function(gpu_lib, case):
+    """GF_method 'F' (one Lanczos continued fraction per pair of spinor components,
+    model_instance<double>::build_cf, src_ed/model_instance.cpp:34-99) against the dense
+    spinor G and against the instance's own Lehmann form; the stop criterion is tightened
+    so that the fractions are exhaustive on these sectors. The integrated G (the hole
+    fractions' weights) equals the Lehmann form's."""
+    from pyqcm_amd import qcm
+    if case == "nambu":
+        m, vals, n = chain(d=0.5, s=0.2)
+        sec, b0, bm, bp = "R0:S0", blocks(m.L, S=0), blocks(m.L, S=-1), blocks(m.L, S=1)
+        sp = spinor(m.L, n, True, False)
+    elif case == "spin_flip":
+        m, vals, n = chain(hx=0.4)
+        sec, b0, bm, bp = "R0:N4", blocks(m.L, N=4), blocks(m.L, N=3), blocks(m.L, N=5)
+        sp = spinor(m.L, n, False, True)
+    else:
+        m, vals, n = chain(hx=0.3, d=0.4)
+        sec, b0, bm, bp = "R0", blocks(m.L, parity=0), blocks(m.L, parity=1), blocks(m.L, parity=1)
+        sp = spinor(m.L, n, True, True)
+    lab_q = register(m, vals, sec, n)
+    Gq = qcm.Green_function(ZS, 0, lab_q)
+    qcm.set_global_parameter("GF_method", "F")
+    qcm.set_global_parameter("accur_continued_fraction", 1e-12)
+    try:
+        lab = register(m, vals, sec, n)
+        G = qcm.Green_function(ZS, 0, lab)
+    finally:
+        qcm.set_global_parameter("GF_method", "L")
+        qcm.set_global_parameter("accur_continued_fraction", 0.01)
+    _, Gd = O.dense_spinor_gf(m, vals, O.block_states(m.L, b0), sp, O.block_states(m.L, bm),
+                              O.block_states(m.L, bp), ZS)
+    print(case, "max |G_cf - G_dense| %.2e, |G_cf - G_lehmann| %.2e" % (np.abs(G - Gd).max(), np.abs(G - Gq).max()))
+    np.testing.assert_allclose(G, Gd, atol=1e-9)
+    np.testing.assert_allclose(G, Gq, atol=1e-9)

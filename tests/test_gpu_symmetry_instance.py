@@ -185,3 +185,40 @@ def test_davidson_method_on_symmetric_and_nambu_instances(gpu_lib):
     finally:
         qcm.set_global_parameter("Ground_state_method", "L")
         qcm.set_global_parameter("accur_Davidson", 1e-5)
+
+
+@pytest.mark.parametrize("case,N,S", [("plaquette4", 8, 0), ("chain4", 3, 1)])
+def test_symmetric_continued_fraction_green_function(gpu_lib, case, N, S):
+    """GF_method 'F' on a model with generators: one continued fraction per pair of
+    symmetric orbitals of each irrep, the start vectors in the target irrep
+    (model_instance<double>::build_cf, src_ed/model_instance.cpp:34-99), G brought to the
+    site basis; against the dense G, and the integrated G (the hole fractions' weights)
+    against the dense ground state's <c+_j c_i> (the Lehmann form's integral sums every
+    pole below zero, particle poles included when the sector's lowest state is not the
+    lowest over N, as the reference's does)."""
+    from pyqcm_amd import qcm
+    m, vals, ns, gens, bi = CASES[case]()
+    nup, ndn = (N + S) // 2, (N - S) // 2
+    r, _ = _lowest_irrep(m, vals, ns, gens, bi, N, S)
+    sec = "R%d:N%d:S%d" % (r, N, S)
+    qcm.set_global_parameter("GF_method", "F")
+    qcm.set_global_parameter("accur_continued_fraction", 1e-12)
+    try:
+        lab = register(m, vals, sec, ns, gens, bi)
+        qcm.ground_state_solve(lab)
+        G = qcm.Green_function(ZS, 0, lab)
+        Gdn = qcm.Green_function(ZS, 1, lab)
+        ave = qcm.Green_function_average(lab)
+    finally:
+        qcm.set_global_parameter("GF_method", "L")
+        qcm.set_global_parameter("accur_continued_fraction", 0.01)
+    _, Gd = O.dense_gf(m, vals, nup, ndn, list(range(ns)), ZS)
+    _, Gdd = O.dense_gf(m, vals, ndn, nup, list(range(ns)), ZS)
+    print(case, "irrep", r, "max |G_cf - G_dense| %.2e" % np.abs(G - Gd).max())
+    np.testing.assert_allclose(G, Gd, atol=1e-9)
+    np.testing.assert_allclose(Gdn, Gdd, atol=1e-9)
+    H, (c, cd) = O.dense_hamiltonian(m, vals, nup, ndn)
+    psi = np.zeros(1 << (2 * m.L))
+    psi[O.sector_states(m.L, nup, ndn)] = np.linalg.eigh(H)[1][:, 0]
+    D = np.array([[psi @ (cd[j] @ (c[i] @ psi)) for j in range(ns)] for i in range(ns)])
+    np.testing.assert_allclose(ave, D, atol=1e-9)
