@@ -1729,8 +1729,7 @@ void hybridization(Instance& I, int nz, const double* z, int spin_down, double* 
 // b_0 of the hole fractions; mcf_set::integrated_Green_function, mcf_set.hpp:146-158:
 // (W_h^T W_h)^T). M(b, a) accumulates over the states, d x d column-major.
 std::vector<cd> gf_average(Instance& I, int sp) {
-  require_no_spinor(I, "the Green-function average");
-  const int d = I.model->n_sites;
+  const int d = dim_gf(I);
   std::vector<cd> M((size_t)d * d, 0.0);
   auto at = [&](int r, int c) -> cd& { return M[r + (size_t)c * d]; };
   for (auto& s : I.states) {
@@ -2521,6 +2520,7 @@ int qed_ed_cluster_averages(uint64_t label, int nop, const char* const* names, d
       // src_ed/model_instance.hpp:152-163; one_body_operator::average_from_GF,
       // src_ed/Operators/one_body_operator.hpp:420-441): one-body operators on the cluster
       // only, no variance
+      require_no_spinor(I, "the averages from the Green function");
       green_function_solve(I);
       const int d = I.model->n_sites, L = I.model->L;
       const int nsp = (I.mixing & 4) ? 2 : 1;
@@ -2594,10 +2594,16 @@ int qed_ed_green_function_density(uint64_t label, double* density) {
   return guard([&] {
     Instance& I = inst(label);
     green_function_solve(I);
-    const int d = I.model->n_sites;
+    // model_instance::Green_function_density (src_ed/model_instance.hpp:662-675): the
+    // spin-down particles of the spin-flip spinor, 1 - <c c+> of the Nambu components
+    const int d = I.model->n_sites, dg = dim_gf(I);
     double n = 0.0;
     const auto m = gf_average(I, 0);
-    for (int i = 0; i < d; ++i) n += m[i + (size_t)i * d].real();
+    for (int i = 0; i < d; ++i) n += m[i + (size_t)i * dg].real();
+    if (I.mixing & 2)
+      for (int i = d; i < 2 * d; ++i) n += m[i + (size_t)i * dg].real();
+    else if (I.mixing & 1)
+      for (int i = d; i < 2 * d; ++i) n += 1.0 - m[i + (size_t)i * dg].real();
     if (I.mixing & 4) {
       const auto md = gf_average(I, 1);
       for (int i = 0; i < d; ++i) n += md[i + (size_t)i * d].real();

@@ -270,3 +270,37 @@ def test_mixing_continued_fraction_green_function(gpu_lib, case):
     print(case, "max |G_cf - G_dense| %.2e, |G_cf - G_lehmann| %.2e" % (np.abs(G - Gd).max(), np.abs(G - Gq).max()))
     np.testing.assert_allclose(G, Gd, atol=1e-9)
     np.testing.assert_allclose(G, Gq, atol=1e-9)
+
+
+@pytest.mark.parametrize("case", ["nambu", "spin_flip", "full"])
+def test_mixing_green_function_average_and_density(gpu_lib, case):
+    """The integrated spinor G (model_instance::Green_function_average,
+    src_ed/model_instance.hpp:643-654: the Lehmann poles below zero) equals
+    <A_b psi, A_a psi> of the dense ground state for the spinor components A (c or c+),
+    and the density (src_ed/model_instance.hpp:662-675: 1 - <c c+> on the Nambu
+    components) equals <n> per site."""
+    from pyqcm_amd import qcm
+    if case == "nambu":
+        m, vals, n = chain(d=0.5, s=0.2)
+        sec, b0 = "R0:S0", blocks(m.L, S=0)
+        sp = spinor(m.L, n, True, False)
+    elif case == "spin_flip":
+        m, vals, n = chain(hx=0.4)
+        sec, b0 = "R0:N4", blocks(m.L, N=4)
+        sp = spinor(m.L, n, False, True)
+    else:
+        m, vals, n = chain(hx=0.3, d=0.4)
+        sec, b0 = "R0", blocks(m.L, parity=0)
+        sp = spinor(m.L, n, True, True)
+    lab = register(m, vals, sec, n)
+    qcm.ground_state_solve(lab)
+    M = qcm.Green_function_average(lab)
+    H, (c, cd) = O.dense_hamiltonian(m, vals, 0, 0, states=O.block_states(m.L, b0))
+    psi = np.zeros(1 << (2 * m.L))
+    psi[O.block_states(m.L, b0)] = np.linalg.eigh(H)[1][:, 0]
+    A = [(cd[a] if nambu else c[a]) @ psi for a, nambu in sp]
+    ref = np.array([[A[b] @ A[a] for b in range(len(sp))] for a in range(len(sp))])
+    print(case, "max |M - dense| %.1e" % np.abs(M - ref).max())
+    np.testing.assert_allclose(M, ref, atol=1e-10)
+    dens = sum(psi @ (cd[a] @ (c[a] @ psi)) for a in list(range(n)) + list(range(m.L, m.L + n))) / n
+    assert abs(qcm.Green_function_density(lab) - dens) < 1e-10
