@@ -1797,9 +1797,10 @@ struct SusceptibilityPart {
   QMat q;
 };
 
+// O|psi> with the operator restricted to the state's own sector, as HS_operator.at(sec)
+// (the symmetry-adapted and mixed-number sectors included: the part of O|psi> in the
+// state's irrep and number sector)
 std::vector<SusceptibilityPart> susceptibility_parts(Instance& I, const std::string& op) {
-  require_trivial_group(I, "the susceptibility");
-  require_no_spinor(I, "the susceptibility");
   if (!I.model->idx.count(op)) fail("Operator " + op + " is not defined in the model.");
   if (!I.value.count(op)) fail("operator " + op + " is not activated in instance " + std::to_string(I.label));
   low_energy_states(I);
@@ -1807,9 +1808,9 @@ std::vector<SusceptibilityPart> susceptibility_parts(Instance& I, const std::str
   std::vector<SusceptibilityPart> out;
   for (auto& st : I.states) {
     if (!st->psi.p) fail("the susceptibility needs the ground-state vectors of the instance");
-    SectorPtr H = make_sector(I, st->nup, st->ndn);
+    SectorPtr H = state_sector(I, *st);
     if (H->D == 0) continue;
-    SectorPtr O = make_sector(I, st->nup, st->ndn, &op);
+    SectorPtr O = state_sector(I, *st, &op);
     DevBuf b(H->D * 8);
     QED_HIP(hipMemsetAsync(b.p, 0, H->D * 8, stream()));
     hv(*O, st->psi.as<double>(), b.as<double>());

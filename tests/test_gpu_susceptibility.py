@@ -98,3 +98,36 @@ def test_susceptibility_errors(gpu_lib):
         qcm.susceptibility("nope", WS, lab)
     with pytest.raises(Exception, match="operator mu is not activated in instance"):
         qcm.susceptibility_poles("mu", lab)
+
+
+def test_susceptibility_of_symmetric_and_nambu_instances(gpu_lib):
+    """The operator applied within the state's own sector (HS_operator.at(sec)): on the
+    chain's mirror-symmetric basis (irrep of the ground state; t and U are symmetric, so
+    O|psi> stays in it) the result equals the trivial group's brute-force sum; on the
+    Nambu sector S = 0 of a paired chain, for the pairing operator and U, it equals the
+    sum over the dense Jordan-Wigner Hamiltonian of the same states. Both sectors are
+    below max_dim_full (exact Lehmann forms)."""
+    from pyqcm_amd import qcm
+    from tests import test_gpu_mixing_instance as MI
+    from tests import test_gpu_symmetry_instance as SI
+    m, vals, ns, gens, bi = SI._chain4()
+    r, _ = SI._lowest_irrep(m, vals, ns, gens, bi, 4, 0)
+    lab = SI.register(m, vals, "R%d:N4:S0" % r, ns, gens, bi)
+    for op in ("t", "U"):
+        chi = qcm.susceptibility(op, WS, lab)
+        de, Q = _lehmann(m, vals, op, 2, 2)
+        ref = np.array([np.sum(Q ** 2 / (w - de)) for w in WS])
+        print("mirror basis", op, "max relative error %.2e" % (np.abs(chi - ref).max() / np.abs(ref).max()))
+        assert np.abs(chi - ref).max() <= 1e-12 * np.abs(ref).max()
+    m, vals, n = MI.chain(d=0.5, s=0.2)
+    lab = MI.register(m, vals, "R0:S0", n)
+    states = O.block_states(m.L, MI.blocks(m.L, S=0))
+    e, U = np.linalg.eigh(O.dense_hamiltonian(m, vals, 0, 0, states=states)[0])
+    for op in ("D", "U"):
+        chi = qcm.susceptibility(op, WS, lab)
+        one = O.ClusterModel(m.L)
+        one.add(op, m.operators[op].kind, m.operators[op].elements)
+        Q = U.T @ (O.dense_hamiltonian(one, {op: 1.0}, 0, 0, states=states)[0] @ U[:, 0])
+        ref = np.array([np.sum(Q ** 2 / (w - (e - e[0]))) for w in WS])
+        print("Nambu", op, "max relative error %.2e" % (np.abs(chi - ref).max() / np.abs(ref).max()))
+        assert np.abs(chi - ref).max() <= 1e-12 * np.abs(ref).max()
