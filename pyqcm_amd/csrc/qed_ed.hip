@@ -2620,16 +2620,21 @@ int qed_ed_fidelity(uint64_t label1, uint64_t label2, double* fidelity) {
   return guard([&] {
     Instance& A = inst(label1);
     Instance& B = inst(label2);
-    require_trivial_group(A, "the fidelity");
-    require_trivial_group(B, "the fidelity");
-    require_no_spinor(A, "the fidelity");
-    require_no_spinor(B, "the fidelity");
     low_energy_states(A);
     low_energy_states(B);
+    // states of the same sector share a basis: the same (N_up, N_down) for the factorized
+    // form; the same irrep or number sector of one model for the symmetry-adapted and
+    // mixed-number bases
+    const bool special = A.model->symmetric() || spinor_mixing(A.mixing) || B.model->symmetric() ||
+                         spinor_mixing(B.mixing);
+    if (special && (A.model != B.model || A.mixing != B.mixing))
+      fail("the fidelity of symmetry-adapted or mixed-number states needs two instances of one model");
     double z = 0.0;
     for (auto& s1 : A.states)
       for (auto& s2 : B.states) {
-        if (s1->nup != s2->nup || s1->ndn != s2->ndn) continue;
+        if (s1->nup != s2->nup || s1->ndn != s2->ndn || s1->irrep != s2->irrep || s1->secN != s2->secN ||
+            s1->secS != s2->secS)
+          continue;
         if (!s1->psi.p || !s2->psi.p) fail("the fidelity needs the ground-state vectors of both instances");
         if (A.model->L != B.model->L) fail("the two instances have different Hilbert spaces");
         auto binom = [](int n, int k) {
@@ -2637,7 +2642,8 @@ int qed_ed_fidelity(uint64_t label1, uint64_t label2, double* fidelity) {
           for (int i = 1; i <= k; ++i) c = c * (uint64_t)(n - k + i) / (uint64_t)i;
           return c;
         };
-        const uint64_t D = binom(A.model->L, s1->nup) * binom(A.model->L, s1->ndn);
+        const uint64_t D = special ? state_sector(A, *s1)->D
+                                   : binom(A.model->L, s1->nup) * binom(A.model->L, s1->ndn);
         const double zz = dot(s1->psi.as<double>(), s2->psi.as<double>(), D);
         z += zz * zz * s1->weight * s2->weight;
       }
@@ -2648,9 +2654,11 @@ int qed_ed_fidelity(uint64_t label1, uint64_t label2, double* fidelity) {
 int qed_ed_tr_sigma_inf(uint64_t label, double* value) {
   std::lock_guard<std::recursive_mutex> lk(lib_mutex());
   return guard([&] {
+    // model_instance::tr_sigma_inf (src_ed/model_instance.hpp:1279-1311): with the Nambu
+    // spinor, the hole components enter with the opposite sign
     Instance& I = inst(label);
-    require_no_spinor(I, "tr_sigma_inf");
-    const int d = I.model->n_sites;
+    low_energy_states(I);
+    const int d = dim_gf(I), d2 = (I.mixing & 1) ? d / 2 : d;
     std::vector<double> eps(3), S(3, 0.0), z(6), sig((size_t)2 * 3 * d * d);
     for (int i = 0; i < 3; ++i) {
       eps[i] = (i + 1) * 1.0e-5;
@@ -2660,7 +2668,7 @@ int qed_ed_tr_sigma_inf(uint64_t label, double* value) {
     for (int spin = 0; spin < ((I.mixing & 4) ? 2 : 1); ++spin) {
       if (qed_ed_self_energy(label, 3, z.data(), spin, sig.data()) != 0) fail(qed_last_error());
       for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < d; ++j) S[i] += sig[2 * ((size_t)i * d * d + j + (size_t)j * d)];
+        for (int j = 0; j < d; ++j) S[i] += (j < d2 ? 1.0 : -1.0) * sig[2 * ((size_t)i * d * d + j + (size_t)j * d)];
     }
     if (I.mixing == 0)
       for (double& v : S) v *= 2.0;
@@ -2671,13 +2679,17 @@ int qed_ed_tr_sigma_inf(uint64_t label, double* value) {
 int qed_ed_hybridization_lehmann(uint64_t label, int spin_down, double* E, double* T) {
   std::lock_guard<std::recursive_mutex> lk(lib_mutex());
   return guard([&] {
+    // model_instance::hybridization (src_ed/model_instance.hpp:1455-1466): the bath as
+    // set_hopping_matrix leaves it, diagonalized in place (energies, and the cluster-bath
+    // matrix in the bath's eigenbasis), of the spinor's dimensions for the spinor mixings
     Instance& I = inst(label);
-    require_no_spinor(I, "the Lehmann form of the hybridization");
-    const int nb = I.model->n_bath, ns = I.model->n_sites;
+    const int nb = I.model->n_bath;
     if (nb == 0) fail("System has no bath. Hybridization not defined.");
+    low_energy_states(I);
     const int sp = (spin_down && (I.mixing & 4)) ? 1 : 0;
-    for (int i = 0; i < nb; ++i) E[i] = I.tb[sp][i + (size_t)i * nb];
-    std::copy(I.tcb[sp].begin(), I.tcb[sp].begin() + (size_t)ns * nb, T);
+    const int dg = dim_gf(I), dbg = dg / I.model->n_sites * nb;
+    for (int i = 0; i < dbg; ++i) E[i] = I.eb[sp][i];
+    std::copy(I.tcb_d[sp].begin(), I.tcb_d[sp].begin() + (size_t)dg * dbg, T);
   });
 }
 

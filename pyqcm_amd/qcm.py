@@ -316,10 +316,12 @@ def tr_sigma_inf(label=0):
 
 def hybridization(label=0, spin_down=0):
     """Lehmann form of the hybridization function as the reference binding returns it:
-    (bath diagonal, the column-major n_sites x n_bath cluster-bath matrix read row-major
-    into shape (n_sites, n_bath)) (src_python/qcm_ED_wrap.hpp:351-362)."""
+    (bath energies, the column-major n_sites x n_bath cluster-bath matrix in the bath's
+    eigenbasis read row-major into shape (n_sites, n_bath)), of the spinor's dimensions
+    for the spin-flip and anomalous mixings (src_python/qcm_ED_wrap.hpp:351-362)."""
     d = Green_function_dimensionC(label)
-    nb = model_sizeC(_instance_model[int(label)])[1]
+    ns, nb = model_sizeC(_instance_model[int(label)])[:2]
+    nb *= d // ns  # the spinor's bath orbitals for the spin-flip and anomalous mixings
     E = np.zeros(max(nb, 1))
     T = np.zeros(max(d * nb, 1))
     call("qed_ed_hybridization_lehmann", int(label), int(spin_down), dp(E), dp(T))

@@ -194,8 +194,6 @@ def test_mixing_sector_errors(gpu_lib):
     lab = register(m, vals, "R0:S0", n)
     with pytest.raises(RuntimeError, match="mixing"):
         qcm.Green_function(ZS, 1, lab)
-    with pytest.raises(RuntimeError, match="mixing"):
-        qcm.susceptibility("U", np.array([0.5j]), lab)
 
 
 @pytest.mark.parametrize("case", ["nambu", "spin_flip", "full"])
@@ -304,3 +302,65 @@ def test_mixing_green_function_average_and_density(gpu_lib, case):
     np.testing.assert_allclose(M, ref, atol=1e-10)
     dens = sum(psi @ (cd[a] @ (c[a] @ psi)) for a in list(range(n)) + list(range(m.L, m.L + n))) / n
     assert abs(qcm.Green_function_density(lab) - dens) < 1e-10
+
+
+def test_nambu_tr_sigma_inf_and_hybridization_lehmann(gpu_lib):
+    """tr_sigma_inf with the Nambu spinor (src_ed/model_instance.hpp:1279-1311: hole
+    components with the opposite sign) is the Hartree term U <N> of the dense ground
+    state; the Lehmann form of the hybridization (the bath diagonalized in place,
+    model_instance::hybridization) rebuilds the solver's Gamma(z), the paired bath
+    included."""
+    from pyqcm_amd import qcm
+    m, vals, ns = nambu_bath_model(2.0)
+    lab = register(m, vals, "R0:S0", ns)
+    qcm.ground_state_solve(lab)
+    L = m.L
+    states = O.block_states(L, blocks(L, S=0))
+    H, (c, cd) = O.dense_hamiltonian(m, vals, 0, 0, states=states)
+    psi = np.zeros(1 << (2 * L))
+    psi[states] = np.linalg.eigh(H)[1][:, 0]
+    N = sum(psi @ (cd[a] @ (c[a] @ psi)) for a in list(range(ns)) + list(range(L, L + ns)))
+    t = qcm.tr_sigma_inf(lab)
+    print("Nambu tr_sigma_inf %.8f, U <N> %.8f" % (t, vals["U"] * N))
+    assert abs(t - vals["U"] * N) < 1e-5
+    E, T = qcm.hybridization(lab)
+    assert E.shape == (2 * (L - ns),) and T.shape == (2 * ns, 2 * (L - ns))
+    tcb = T.reshape(-1).reshape(2 * (L - ns), 2 * ns).T.real  # column-major 2ns x 2nb
+    z = np.array([0.4j, 1.7j, 0.3 + 0.2j])
+    Gam = np.array([(tcb / (zz - E)) @ tcb.T for zz in z])
+    np.testing.assert_allclose(Gam, qcm.hybridization_functionC(z, 0, lab), atol=1e-13)
+
+
+def test_fidelity_of_nambu_and_symmetric_instances(gpu_lib):
+    """|<psi_A|psi_B>|^2 of two instances of one model in one sector: the Nambu chain at
+    two pairing amplitudes, and the plaquette's mirror basis at two values of U, against
+    the dense ground states."""
+    from pyqcm_amd import qcm
+    from tests import test_gpu_symmetry_instance as SI
+    m, va, n = chain(d=0.5, s=0.2)
+    vb = dict(va, D=0.35)
+    name = "fid%d" % next(_counter)
+    qcm.new_model(name, n, m.L - n)
+    for opname, op in m.operators.items():
+        qcm.new_operator(name, opname, op.kind, [(r + 1, c + 1, v) for (r, c, v) in op.elements])
+    qcm.new_model_instanceC(name, va, "R0:S0", 9801)
+    qcm.new_model_instanceC(name, vb, "R0:S0", 9802)
+    states = O.block_states(m.L, blocks(m.L, S=0))
+    ga = np.linalg.eigh(O.dense_hamiltonian(m, va, 0, 0, states=states)[0])[1][:, 0]
+    gb = np.linalg.eigh(O.dense_hamiltonian(m, vb, 0, 0, states=states)[0])[1][:, 0]
+    f = qcm.fidelity(9801, 9802)
+    print("Nambu fidelity %.12f dense %.12f" % (f, (ga @ gb) ** 2))
+    assert abs(f - (ga @ gb) ** 2) < 1e-9
+    m, vals, ns, gens, bi = SI._plaquette4()
+    name = "fids%d" % next(_counter)
+    qcm.new_model(name, ns, m.L - ns, gens, bi)
+    for opname, op in m.operators.items():
+        qcm.new_operator(name, opname, op.kind, [(r + 1, c + 1, v) for (r, c, v) in op.elements])
+    v2 = dict(vals, U=vals["U"] + 0.5)
+    qcm.new_model_instanceC(name, vals, "R0:N8:S0", 9803)
+    qcm.new_model_instanceC(name, v2, "R0:N8:S0", 9804)
+    g1 = np.linalg.eigh(O.dense_hamiltonian(m, vals, 4, 4)[0])[1][:, 0]
+    g2 = np.linalg.eigh(O.dense_hamiltonian(m, v2, 4, 4)[0])[1][:, 0]
+    f = qcm.fidelity(9803, 9804)
+    print("mirror-basis fidelity %.12f dense %.12f" % (f, (g1 @ g2) ** 2))
+    assert abs(f - (g1 @ g2) ** 2) < 1e-9
