@@ -1041,8 +1041,60 @@ std::string sector_name(int N, int S, int R = 0) {
 }
 
 // ---- ground state (model_instance::low_energy_states, model_instance.hpp:245-417) ------
+// model_instance::one_body_solve for the spin-flip, anomalous and full mixings
+// (src_ed/model_instance.hpp:1164-1256): the spinor one-body matrix of cluster and bath
+// diagonalized, the Lehmann form of its cluster rows, the energy of the negative levels
+// plus the Nambu correction (the diagonal one-body elements of the spin-up orbitals for
+// the anomalous mixing, of all orbitals for the full one, operators not of the
+// cluster-bath kind; one_body_operator.hpp:95-103), halved for the full mixing
+void one_body_solve_spinor(Instance& I) {
+  Model& M = *I.model;
+  const int ns = M.n_sites, nb = M.n_bath, dg = dim_gf(I), dbg = dg / ns * nb, d = dg + dbg;
+  std::vector<double> H((size_t)d * d, 0.0), e(d);
+  for (int a = 0; a < dg; ++a)
+    for (int b = 0; b < dg; ++b) H[a + (size_t)b * d] = I.tc[0][a + (size_t)b * dg];
+  for (int a = 0; a < dbg; ++a)
+    for (int b = 0; b < dbg; ++b) H[(dg + a) + (size_t)(dg + b) * d] = I.tb[0][a + (size_t)b * dbg];
+  for (int a = 0; a < dg; ++a)
+    for (int b = 0; b < dbg; ++b) {
+      H[a + (size_t)(dg + b) * d] = I.tcb[0][a + (size_t)b * dg];
+      H[(dg + b) + (size_t)a * d] = I.tcb[0][a + (size_t)b * dg];
+    }
+  sym_eigen(d, H.data(), e.data(), true);
+  auto st = std::make_unique<State>();
+  st->weight = 1.0;
+  QMat q;
+  q.L = dg;
+  q.e = e;
+  q.Q.assign((size_t)dg * d, 0.0);
+  for (int m = 0; m < d; ++m)
+    for (int a = 0; a < dg; ++a) q.Q[a + (size_t)m * dg] = H[a + (size_t)m * d];
+  st->gf[0] = q;
+  double E = 0.0;
+  for (double x : e)
+    if (x < 0) E += x;
+  double corr = 0.0;
+  if (I.mixing & 1)
+    for (const Op& op : M.ops) {
+      auto it = I.value.find(op.name);
+      if (it == I.value.end() || op.interaction || op.anomalous || op.target == 3) continue;
+      for (size_t k = 0; k < op.v.size(); ++k)
+        if (op.r[k] == op.c[k] && (I.mixing == 3 || op.r[k] < M.L)) corr += op.v[k] * it->second;
+    }
+  E += corr;
+  if (I.mixing == 3) E *= 0.5;
+  I.GS_energy = st->energy = I.E0 = E;
+  I.states.clear();
+  I.states.push_back(std::move(st));
+  I.gs_solved = I.gf_solved = true;
+  I.gs_string = "uncorrelated";
+}
+
 void one_body_solve(Instance& I) {
-  require_no_spinor(I, "the uncorrelated (one-body) solution");
+  if (spinor_mixing(I.mixing)) {
+    one_body_solve_spinor(I);
+    return;
+  }
   Model& M = *I.model;
   const int ns = M.n_sites, nb = M.n_bath, d = ns + nb;
   I.GS_energy = 0.0;

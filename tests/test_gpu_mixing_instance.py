@@ -364,3 +364,32 @@ def test_fidelity_of_nambu_and_symmetric_instances(gpu_lib):
     f = qcm.fidelity(9803, 9804)
     print("mirror-basis fidelity %.12f dense %.12f" % (f, (g1 @ g2) ** 2))
     assert abs(f - (g1 @ g2) ** 2) < 1e-9
+
+
+@pytest.mark.parametrize("case", ["nambu", "spin_flip", "full"])
+def test_mixing_uncorrelated_solution(gpu_lib, case):
+    """No interaction: the one-body solution of the spinor (model_instance::one_body_solve,
+    src_ed/model_instance.hpp:1164-1256; Nambu correction of the energy) against the
+    lowest state of the whole Fock space and its spinor G from dense ED."""
+    from pyqcm_amd import qcm
+    if case == "nambu":
+        m, vals, n = chain(d=0.5, s=0.2)
+        sec, sp = "R0:S0", spinor(m.L, n, True, False)
+    elif case == "spin_flip":
+        m, vals, n = chain(hx=0.4)
+        sec, sp = "R0:N4", spinor(m.L, n, False, True)
+    else:
+        m, vals, n = chain(hx=0.3, d=0.4)
+        sec, sp = "R0", spinor(m.L, n, True, True)
+    del vals["U"]
+    lab = register(m, vals, sec, n)
+    E0, gs = qcm.ground_state_solve(lab)
+    assert gs == "uncorrelated"
+    allst = O.block_states(m.L, [(nu, nd) for nu in range(m.L + 1) for nd in range(m.L + 1)])
+    e = np.linalg.eigvalsh(O.dense_hamiltonian(m, vals, 0, 0, states=allst)[0])
+    assert e[1] - e[0] > 1e-6
+    Ed, Gd = O.dense_spinor_gf(m, vals, allst, sp, allst, allst, ZS)
+    G = qcm.Green_function(ZS, 0, lab)
+    print(case, "E0 %.12f dense %.12f, max |dG| %.1e" % (E0, Ed, np.abs(G - Gd).max()))
+    assert abs(E0 - Ed) < 1e-10
+    np.testing.assert_allclose(G, Gd, atol=1e-10)
