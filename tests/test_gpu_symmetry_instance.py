@@ -151,9 +151,13 @@ def test_symmetric_model_errors(gpu_lib):
             register(m, vals, "R0:N4:S0", 4, [[4, 3, 2, 1]])
     finally:
         qcm.set_global_parameter("Hamiltonian_format", "E")
-    lab = register(m, vals, "R0:N4:S0", 4, [[4, 3, 2, 1]])
-    with pytest.raises(RuntimeError, match="point group"):
-        qcm.susceptibility("U", np.array([0.5j]), lab)
+    qcm.set_global_parameter("GF_method", "M")
+    try:
+        lab = register(m, vals, "R0:N4:S0", 4, [[4, 3, 2, 1]])
+        with pytest.raises(RuntimeError, match="point group"):
+            qcm.Green_function(np.array([0.5j]), 0, lab)
+    finally:
+        qcm.set_global_parameter("GF_method", "L")
 
 
 def test_davidson_method_on_symmetric_and_nambu_instances(gpu_lib):
