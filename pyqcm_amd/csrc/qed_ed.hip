@@ -967,27 +967,40 @@ void build_cf(Instance& I, State& st, int spin) {
 // matrix continued fractions (model_instance::build_mcf, src_ed/model_instance.hpp:846-946):
 // start block c_i^(+-)|psi>, its modified Gram-Schmidt factor W (first pa columns that do
 // not deflate), block Lanczos on the orthonormal block, diagonal blocks shifted by E0
+// For the spin-flip, anomalous and full mixings the start block is the spinor's
+// components (the Lehmann form's start vectors), q = n_mixed x n_sites.
 void build_mcf(Instance& I, State& st, int spin) {
   require_trivial_group(I, "the matrix continued-fraction Green function");
-  require_no_spinor(I, "the matrix continued-fraction Green function");
   Model& M = *I.model;
-  const int p = M.n_sites, L = M.L;
+  const bool spinor = spinor_mixing(I.mixing);
+  const std::vector<std::pair<int, bool>> comps = spinor ? spinor_components(I) : std::vector<std::pair<int, bool>>();
+  const int p = spinor ? (int)comps.size() : M.n_sites, L = M.L;
   const double acc = gd("accur_deflation");
+  SectorPtr B = spinor ? state_sector(I, st) : nullptr;
   for (int part = 0; part < 2; ++part) {
     const int pm = 2 * part - 1;
     MCFrac& f = st.mcf[spin][part];
     f = MCFrac();
-    const int tnup = st.nup + (spin == 0 ? pm : 0), tndn = st.ndn + (spin == 1 ? pm : 0);
-    if (!sector_valid(I, tnup, tndn)) continue;
-    SectorPtr T = make_sector(I, tnup, tndn);
+    SectorPtr T;
+    if (spinor) {
+      T = make_general_sector(I, shift_code(st.secN, pm), shift_code(st.secS, pm));
+    } else {
+      const int tnup = st.nup + (spin == 0 ? pm : 0), tndn = st.ndn + (spin == 1 ? pm : 0);
+      if (!sector_valid(I, tnup, tndn)) continue;
+      T = make_sector(I, tnup, tndn);
+    }
     if (T->D == 0) continue;
     const uint64_t D = T->D;
     DevBuf phi((size_t)p * D * 8);
     double* ph = phi.as<double>();
     QED_HIP(hipMemsetAsync(ph, 0, (size_t)p * D * 8, stream()));
-    for (int i = 0; i < p; ++i)
-      if (qed_apply_c(L, st.nup, st.ndn, i + spin * L, pm, 1.0, st.psi.as<double>(), ph + (uint64_t)i * D) != 0)
-        fail(qed_last_error());
+    for (int i = 0; i < p; ++i) {
+      const int rc = spinor ? qed_general_apply(B.get(), T.get(), comps[i].first, comps[i].second ? -pm : pm, 1.0,
+                                                st.psi.as<double>(), ph + (uint64_t)i * D)
+                            : qed_apply_c(L, st.nup, st.ndn, i + spin * L, pm, 1.0, st.psi.as<double>(),
+                                          ph + (uint64_t)i * D);
+      if (rc != 0) fail(qed_last_error());
+    }
     std::vector<double> W((size_t)p * p, 0.0);
     int pa = p;
     for (int l = 0; l < p; ++l) {

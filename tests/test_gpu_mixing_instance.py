@@ -393,3 +393,61 @@ def test_mixing_uncorrelated_solution(gpu_lib, case):
     print(case, "E0 %.12f dense %.12f, max |dG| %.1e" % (E0, Ed, np.abs(G - Gd).max()))
     assert abs(E0 - Ed) < 1e-10
     np.testing.assert_allclose(G, Gd, atol=1e-10)
+
+
+@pytest.mark.parametrize("case", ["nambu", "spin_flip", "full"])
+def test_mixing_matrix_continued_fraction(gpu_lib, case):
+    """GF_method 'M' with the spinor's components as the start block (block Lanczos per
+    shifted sector, model_instance::build_mcf, src_ed/model_instance.hpp:846-946),
+    against the oracle's block Lanczos and matrix continued fraction on the same start
+    blocks of the dense ground state. The spin-flip blocks deflate at the second floor
+    (the Krylov space grows by 4, not 8, per step: S_x is conserved) and the fraction stops
+    there, as the reference's does, so only the Nambu and full cases are also the exact G;
+    with combine_mcf the combined fraction gives the same G."""
+    from pyqcm_amd import qcm
+    if case == "nambu":
+        m, vals, n = chain(d=0.5, s=0.2)
+        sec, b0, bm, bp = "R0:S0", blocks(m.L, S=0), blocks(m.L, S=-1), blocks(m.L, S=1)
+        sp = spinor(m.L, n, True, False)
+    elif case == "spin_flip":
+        m, vals, n = chain(hx=0.4)
+        sec, b0, bm, bp = "R0:N4", blocks(m.L, N=4), blocks(m.L, N=3), blocks(m.L, N=5)
+        sp = spinor(m.L, n, False, True)
+    else:
+        m, vals, n = chain(hx=0.3, d=0.4)
+        sec, b0, bm, bp = "R0", blocks(m.L, parity=0), blocks(m.L, parity=1), blocks(m.L, parity=1)
+        sp = spinor(m.L, n, True, True)
+    _, Gd = O.dense_spinor_gf(m, vals, O.block_states(m.L, b0), sp, O.block_states(m.L, bm),
+                              O.block_states(m.L, bp), ZS)
+    qcm.set_global_parameter("GF_method", "M")
+    try:
+        G = qcm.Green_function(ZS, 0, register(m, vals, sec, n))
+        qcm.set_global_parameter("combine_mcf", True)
+        Gc = qcm.Green_function(ZS, 0, register(m, vals, sec, n))
+    finally:
+        qcm.set_global_parameter("GF_method", "L")
+        qcm.set_global_parameter("combine_mcf", False)
+    st0 = O.block_states(m.L, b0)
+    H, (c, cd) = O.dense_hamiltonian(m, vals, 0, 0, states=st0)
+    e, U = np.linalg.eigh(H)
+    psi = np.zeros(1 << (2 * m.L))
+    psi[st0] = U[:, 0]
+    Go = np.zeros_like(Gd)
+    for pm, bt in ((-1, bm), (1, bp)):
+        stt = O.block_states(m.L, bt)
+        Ht = O.dense_hamiltonian(m, vals, 0, 0, states=stt)[0]
+        # A_a = c_a (c+_a for a Nambu component), A+_a its conjugate
+        phi = [(((cd[a] if nambu else c[a]) if pm < 0 else (c[a] if nambu else cd[a])) @ psi)[stt]
+               for a, nambu in sp]
+        q, W = O.mgs_weight(phi)
+        A, B = O.block_lanczos(lambda x: Ht @ x, q)
+        A = O.mcf_shift(A, e[0], pm == 1)
+        for iz, z in enumerate(ZS):
+            g = O.mcf_evaluate(A, B, W, z)
+            Go[iz] += g if pm == 1 else g.T
+    print(case, "max |G_mcf - G_oracle| %.1e, |G_mcf - G_dense| %.1e, combined - oracle %.1e"
+          % (np.abs(G - Go).max(), np.abs(G - Gd).max(), np.abs(Gc - Go).max()))
+    np.testing.assert_allclose(G, Go, atol=1e-9)
+    np.testing.assert_allclose(Gc, Go, atol=1e-9)
+    if case != "spin_flip":
+        np.testing.assert_allclose(G, Gd, atol=1e-9)
