@@ -1998,8 +1998,8 @@ void write_gf(const h5::Node& g, const Instance& I, const State& s, int spin, co
   }
 }
 
-void read_gf(const h5::Node& g, const Instance& I, State& s, int spin, const std::string& fmt) {
-  const int d = I.model->n_sites;
+void read_gf(const h5::Node& g, const Instance& I, State& s, int spin, const std::string& fmt, int mixing) {
+  const int d = I.model->n_sites * n_mixed(mixing);
   if (fmt == "cf") {
     read_cf(g, s.cfe[spin], s.cfh[spin], d);
     return;
@@ -2029,9 +2029,11 @@ void read_gf(const h5::Node& g, const Instance& I, State& s, int spin, const std
 
 void write_instance_file(Instance& I, const std::string& path, const std::string& group) {
   require_trivial_group(I, "the instance file");
-  require_no_spinor(I, "the instance file");
   green_function_solve(I);
   const std::string fmt = gf_format_name(I);
+  if (spinor_mixing(I.mixing) && fmt == "cf")
+    fail("the instance file of a spin-flip or anomalous instance takes its Lehmann form or matrix continued "
+         "fraction, not continued fractions");
   try {
     h5::Node f = h5::open_or_create(path);
     if (h5::exists(f, group)) h5::unlink(f, group);
@@ -2049,7 +2051,8 @@ void write_instance_file(Instance& I, const std::string& path, const std::string
     for (size_t k = 0; k < I.states.size(); ++k) {
       const State& s = *I.states[k];
       h5::Node sg = h5::create_group(g, "state_" + std::to_string(k));
-      h5::write_attr(sg, "sec", sector_name(s.nup + s.ndn, s.nup - s.ndn));
+      h5::write_attr(sg, "sec", spinor_mixing(I.mixing) ? sector_name(s.secN, s.secS)
+                                                         : sector_name(s.nup + s.ndn, s.nup - s.ndn));
       h5::write_attr(sg, "energy", s.energy);
       h5::write_attr(sg, "weight", s.weight);
       h5::write_attr(sg, "gf_format", fmt);
@@ -2065,7 +2068,6 @@ std::pair<int, int> sector_numbers(const std::string& s);
 
 void read_instance_file(Instance& I, const std::string& path, const std::string& group) {
   require_trivial_group(I, "the instance file");
-  require_no_spinor(I, "the instance file");
   std::vector<std::unique_ptr<State>> states;
   double gse = 0.0;
   std::string fmt, gs;
@@ -2093,14 +2095,22 @@ void read_instance_file(Instance& I, const std::string& path, const std::string&
       h5::Node sg = h5::open_group(g, "state_" + std::to_string(k));
       auto st = std::make_unique<State>();
       const auto ns = sector_numbers(h5::read_attr_string(sg, "sec"));
-      st->nup = (ns.first + ns.second) / 2;
-      st->ndn = (ns.first - ns.second) / 2;
+      if (spinor_mixing(mixing)) {
+        st->nup = st->ndn = -1;
+        st->secN = ns.first;
+        st->secS = ns.second;
+        if (fmt != "bl" && fmt != "mcf")
+          fail("the instance file of a spin-flip or anomalous instance holds continued fractions");
+      } else {
+        st->nup = (ns.first + ns.second) / 2;
+        st->ndn = (ns.first - ns.second) / 2;
+      }
       st->energy = h5::read_attr_double(sg, "energy");
       st->weight = h5::read_attr_double(sg, "weight");
       std::string sf = h5::read_attr_string(sg, "gf_format");
       if (sf != "bl" && sf != "mcf") sf = "cf";
-      if (h5::exists(sg, "gf")) read_gf(h5::open_group(sg, "gf"), I, *st, 0, sf);
-      if (h5::exists(sg, "gf_down")) read_gf(h5::open_group(sg, "gf_down"), I, *st, 1, sf);
+      if (h5::exists(sg, "gf")) read_gf(h5::open_group(sg, "gf"), I, *st, 0, sf, mixing);
+      if (h5::exists(sg, "gf_down")) read_gf(h5::open_group(sg, "gf_down"), I, *st, 1, sf, mixing);
       states.push_back(std::move(st));
     }
   } catch (const std::exception& e) {

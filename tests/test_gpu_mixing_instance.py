@@ -451,3 +451,30 @@ def test_mixing_matrix_continued_fraction(gpu_lib, case):
     np.testing.assert_allclose(Gc, Go, atol=1e-9)
     if case != "spin_flip":
         np.testing.assert_allclose(G, Gd, atol=1e-9)
+
+
+@pytest.mark.parametrize("method", ["L", "M"])
+def test_nambu_instance_file_round_trip(gpu_lib, tmp_path, method):
+    """An instance of the Nambu mixing with a paired bath written to an instance file and
+    read into a fresh instance of the model: the sector string with its code for N
+    ("R0:S0"), G(z), Sigma(z) and the Potthoff functional bit for bit, for the Lehmann
+    form and the matrix continued fraction."""
+    from pyqcm_amd import qcm
+    m, vals, ns = nambu_bath_model(2.0)
+    qcm.set_global_parameter("GF_method", method)
+    try:
+        a = register(m, vals, "R0:S0", ns)
+        E0, sec = qcm.ground_state_solve(a)
+        G, S = qcm.Green_function(ZS, 0, a), qcm.self_energyC(ZS, 0, a)
+        path = str(tmp_path / "nambu.h5")
+        qcm.write_instance_to_hdf5(path, "cluster_0", a)
+        b = register(m, vals, "R0:S0", ns)
+        qcm.read_instance_from_hdf5(path, "cluster_0", b)
+        E0b, secb = qcm.ground_state_solve(b)
+        np.testing.assert_array_equal(qcm.Green_function(ZS, 0, b), G)
+        np.testing.assert_array_equal(qcm.self_energyC(ZS, 0, b), S)
+        assert qcm.Potthoff_functional(b) == qcm.Potthoff_functional(a)
+    finally:
+        qcm.set_global_parameter("GF_method", "L")
+    assert sec == secb == "R0:S0:1"
+    assert abs(E0b - E0) <= 1e-12 * abs(E0)
