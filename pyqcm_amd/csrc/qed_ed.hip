@@ -138,6 +138,7 @@ struct State {
   std::vector<CFrac> cfe[2], cfh[2];  // n x n (o1 >= o2), electron / hole parts
   std::vector<CFBlock> cfb[2];        // symmetric orbitals or spinor: fractions per block
   MCFrac mcf[2][2];                   // [spin][0: holes, 1: particles]
+  std::vector<MCFrac> mcfb[2][2];     // per irrep of the symmetric orbitals, W in the site basis
   MCFrac mcf_comb[2];                 // combined fraction (combine_mcf), evaluated alone
 };
 
@@ -964,18 +965,108 @@ void build_cf(Instance& I, State& st, int spin) {
   }
 }
 
+// The matrix continued fraction of one start block of p vectors ph (D amplitudes each,
+// overwritten): its modified Gram-Schmidt factor W (the first pa columns that do not
+// deflate), block Lanczos on the orthonormal block, diagonal blocks shifted by the
+// state's energy; W (pa x p) carries sqrt(weight). f is left empty when nothing remains.
+void mcf_of_block(qed_sector& T, double* ph, int p, const State& st, int pm, MCFrac& f) {
+  f = MCFrac();
+  const uint64_t D = T.D;
+  const double acc = gd("accur_deflation");
+  std::vector<double> W((size_t)p * p, 0.0);
+  int pa = p;
+  for (int l = 0; l < p; ++l) {
+    double* xl = ph + (uint64_t)l * D;
+    for (int k = 0; k < l; ++k) {
+      const double* xk = ph + (uint64_t)k * D;
+      const double z = dot(xk, xl, D);
+      W[k + (size_t)p * l] = z;
+      axpy(-z, xk, xl, D);
+    }
+    const double nrm = nrm2(xl, D);
+    if (nrm < acc) {
+      pa = l;
+      break;
+    }
+    W[l + (size_t)p * l] = nrm;
+    scal(1.0 / nrm, xl, D);
+  }
+  if (pa == 0) return;
+  const double sw = std::sqrt(st.weight);
+  std::vector<std::vector<double>> A, B;
+  const int M0 = (int)(14 * pa * std::log(1.0 * D));
+  block_lanczos(T, ph, pa, M0, gi("max_iter_BL"), acc, gd("accur_band_lanczos"), gb("block_Lanczos_QR"), A, B);
+  if (A.empty()) return;
+  f.pa = pa;
+  f.q = p;
+  f.W.assign((size_t)pa * p, 0.0);
+  for (int l = 0; l < p; ++l)
+    for (int k = 0; k < pa; ++k) f.W[k + (size_t)pa * l] = sw * W[k + (size_t)p * l];
+  const size_t pp = (size_t)pa * pa;
+  f.A.resize(A.size() * pp);
+  f.B.resize(A.size() * pp);
+  for (size_t j = 0; j < A.size(); ++j)
+    for (size_t t = 0; t < pp; ++t) {
+      const bool diag = (t % pa) == (t / pa);
+      const double a = A[j][t];
+      f.A[j * pp + t] = (pm == 1) ? a - (diag ? st.energy : 0.0) : (diag ? st.energy : 0.0) - a;
+      f.B[j * pp + t] = B[j][t];
+    }
+}
+
 // matrix continued fractions (model_instance::build_mcf, src_ed/model_instance.hpp:846-946):
-// start block c_i^(+-)|psi>, its modified Gram-Schmidt factor W (first pa columns that do
-// not deflate), block Lanczos on the orthonormal block, diagonal blocks shifted by E0
-// For the spin-flip, anomalous and full mixings the start block is the spinor's
-// components (the Lehmann form's start vectors), q = n_mixed x n_sites.
+// start block c_i^(+-)|psi> over the sites; for the spin-flip, anomalous and full mixings
+// the spinor's components (the Lehmann form's start vectors), q = n_mixed x n_sites; for a
+// model with generators one fraction per irrep of the symmetric orbitals, its weight
+// W (pa x p) times the orbitals' coefficients (p x n_sites), so that every block's
+// W^T F W lands in the site basis.
 void build_mcf(Instance& I, State& st, int spin) {
-  require_trivial_group(I, "the matrix continued-fraction Green function");
   Model& M = *I.model;
+  const int L = M.L;
+  if (M.symmetric()) {
+    if (gb("combine_mcf")) fail("combine_mcf is not available for a model with a nontrivial point group (model " + M.name + ")");
+    const int n = M.n_sites;
+    SectorPtr B = make_sector(I, st.nup, st.ndn, nullptr, st.irrep);
+    for (int part = 0; part < 2; ++part) st.mcfb[spin][part].clear();
+    for (int ro = 0; ro < M.group.g; ++ro) {
+      std::vector<int> sel;
+      for (int o = 0; o < n; ++o)
+        if (M.sorb_irrep[o] == ro) sel.push_back(o);
+      const int p = (int)sel.size();
+      if (p == 0) continue;
+      const int rt = irrep_tensor(M.group, st.irrep, ro);
+      for (int part = 0; part < 2; ++part) {
+        const int pm = 2 * part - 1;
+        const int tnup = st.nup + (spin == 0 ? pm : 0), tndn = st.ndn + (spin == 1 ? pm : 0);
+        if (!sector_valid(I, tnup, tndn)) continue;
+        SectorPtr T = make_sector(I, tnup, tndn, nullptr, rt);
+        if (T->D == 0) continue;
+        DevBuf phi((size_t)p * T->D * 8);
+        double* ph = phi.as<double>();
+        QED_HIP(hipMemsetAsync(ph, 0, (size_t)p * T->D * 8, stream()));
+        for (int k = 0; k < p; ++k)
+          if (qed_mixed_apply_c(B.get(), T.get(), spin, pm, n, &M.sorb[(size_t)sel[k] * n], st.psi.as<double>(),
+                                ph + (uint64_t)k * T->D) != 0)
+            fail(qed_last_error());
+        MCFrac f;
+        mcf_of_block(*T, ph, p, st, pm, f);
+        if (f.M() == 0) continue;
+        std::vector<double> Wn((size_t)f.pa * n, 0.0);
+        for (int k = 0; k < f.pa; ++k)
+          for (int l = 0; l < p; ++l) {
+            const double w = f.W[k + (size_t)f.pa * l];
+            for (int a = 0; a < n; ++a) Wn[k + (size_t)f.pa * a] += w * M.sorb[(size_t)sel[l] * n + a];
+          }
+        f.W = Wn;
+        f.q = n;
+        st.mcfb[spin][part].push_back(std::move(f));
+      }
+    }
+    return;
+  }
   const bool spinor = spinor_mixing(I.mixing);
   const std::vector<std::pair<int, bool>> comps = spinor ? spinor_components(I) : std::vector<std::pair<int, bool>>();
-  const int p = spinor ? (int)comps.size() : M.n_sites, L = M.L;
-  const double acc = gd("accur_deflation");
+  const int p = spinor ? (int)comps.size() : M.n_sites;
   SectorPtr B = spinor ? state_sector(I, st) : nullptr;
   for (int part = 0; part < 2; ++part) {
     const int pm = 2 * part - 1;
@@ -1001,45 +1092,7 @@ void build_mcf(Instance& I, State& st, int spin) {
                                           ph + (uint64_t)i * D);
       if (rc != 0) fail(qed_last_error());
     }
-    std::vector<double> W((size_t)p * p, 0.0);
-    int pa = p;
-    for (int l = 0; l < p; ++l) {
-      double* xl = ph + (uint64_t)l * D;
-      for (int k = 0; k < l; ++k) {
-        const double* xk = ph + (uint64_t)k * D;
-        const double z = dot(xk, xl, D);
-        W[k + (size_t)p * l] = z;
-        axpy(-z, xk, xl, D);
-      }
-      const double nrm = nrm2(xl, D);
-      if (nrm < acc) {
-        pa = l;
-        break;
-      }
-      W[l + (size_t)p * l] = nrm;
-      scal(1.0 / nrm, xl, D);
-    }
-    if (pa == 0) continue;
-    const double sw = std::sqrt(st.weight);
-    std::vector<std::vector<double>> A, B;
-    const int M0 = (int)(14 * pa * std::log(1.0 * D));
-    block_lanczos(*T, ph, pa, M0, gi("max_iter_BL"), acc, gd("accur_band_lanczos"), gb("block_Lanczos_QR"), A, B);
-    if (A.empty()) continue;
-    f.pa = pa;
-    f.q = p;
-    f.W.assign((size_t)pa * p, 0.0);
-    for (int l = 0; l < p; ++l)
-      for (int k = 0; k < pa; ++k) f.W[k + (size_t)pa * l] = sw * W[k + (size_t)p * l];
-    const size_t pp = (size_t)pa * pa;
-    f.A.resize(A.size() * pp);
-    f.B.resize(A.size() * pp);
-    for (size_t j = 0; j < A.size(); ++j)
-      for (size_t t = 0; t < pp; ++t) {
-        const bool diag = (t % pa) == (t / pa);
-        const double a = A[j][t];
-        f.A[j * pp + t] = (pm == 1) ? a - (diag ? st.energy : 0.0) : (diag ? st.energy : 0.0) - a;
-        f.B[j * pp + t] = B[j][t];
-      }
+    mcf_of_block(*T, ph, p, st, pm, f);
   }
 }
 
@@ -1716,8 +1769,9 @@ void green_function_eval(Instance& I, int nz, const double* z, int sp, double* o
       // particles added as they are, holes transposed (src_ed/mcf_set.hpp:111-131)
       for (int part = 1; part >= 0; --part) {
         const MCFrac& f = s->mcf[sp][part];
-        if (f.M() == 0) continue;
-        mcf_eval(f.pa, f.q, f.M(), f.A.data(), f.B.data(), f.W.data(), nz, z, part == 0, out);
+        if (f.M() > 0) mcf_eval(f.pa, f.q, f.M(), f.A.data(), f.B.data(), f.W.data(), nz, z, part == 0, out);
+        for (const MCFrac& fb : s->mcfb[sp][part])
+          mcf_eval(fb.pa, fb.q, fb.M(), fb.A.data(), fb.B.data(), fb.W.data(), nz, z, part == 0, out);
       }
     } else if (!I.cf || !I.correlated) {
       const QMat& q = s->gf[sp];
@@ -1799,14 +1853,18 @@ std::vector<cd> gf_average(Instance& I, int sp) {
   auto at = [&](int r, int c) -> cd& { return M[r + (size_t)c * d]; };
   for (auto& s : I.states) {
     if (I.mcf && I.correlated) {
-      const MCFrac& h = s->mcf[sp][0];
-      if (h.M() == 0 || h.pa == 0) continue;
-      for (int a = 0; a < h.q; ++a)
-        for (int b = 0; b < h.q; ++b) {
-          double w = 0.0;
-          for (int k = 0; k < h.pa; ++k) w += h.W[k + (size_t)h.pa * a] * h.W[k + (size_t)h.pa * b];
-          at(b, a) += w;  // (W^T W)^T
-        }
+      std::vector<const MCFrac*> hs{&s->mcf[sp][0]};
+      for (const MCFrac& fb : s->mcfb[sp][0]) hs.push_back(&fb);
+      for (const MCFrac* hp : hs) {
+        const MCFrac& h = *hp;
+        if (h.M() == 0 || h.pa == 0) continue;
+        for (int a = 0; a < h.q; ++a)
+          for (int b = 0; b < h.q; ++b) {
+            double w = 0.0;
+            for (int k = 0; k < h.pa; ++k) w += h.W[k + (size_t)h.pa * a] * h.W[k + (size_t)h.pa * b];
+            at(b, a) += w;  // (W^T W)^T
+          }
+      }
     } else if (I.cf && I.correlated && !s->cfb[sp].empty()) {
       // the hole fractions' weights b[0] per block, the same combination
       std::vector<cd> g((size_t)d * d, 0.0);

@@ -152,12 +152,14 @@ def test_symmetric_model_errors(gpu_lib):
     finally:
         qcm.set_global_parameter("Hamiltonian_format", "E")
     qcm.set_global_parameter("GF_method", "M")
+    qcm.set_global_parameter("combine_mcf", True)
     try:
         lab = register(m, vals, "R0:N4:S0", 4, [[4, 3, 2, 1]])
         with pytest.raises(RuntimeError, match="point group"):
             qcm.Green_function(np.array([0.5j]), 0, lab)
     finally:
         qcm.set_global_parameter("GF_method", "L")
+        qcm.set_global_parameter("combine_mcf", False)
 
 
 def test_davidson_method_on_symmetric_and_nambu_instances(gpu_lib):
@@ -226,3 +228,80 @@ def test_symmetric_continued_fraction_green_function(gpu_lib, case, N, S):
     psi[O.sector_states(m.L, nup, ndn)] = np.linalg.eigh(H)[1][:, 0]
     D = np.array([[psi @ (cd[j] @ (c[i] @ psi)) for j in range(ns)] for i in range(ns)])
     np.testing.assert_allclose(ave, D, atol=1e-9)
+
+
+@pytest.mark.parametrize("case,N,S", [("chain4", 4, 0), ("chain4", 3, 1), ("plaquette4", 8, 0)])
+def test_symmetric_matrix_continued_fraction(gpu_lib, case, N, S):
+    """GF_method 'M' on a model with generators: one matrix continued fraction per irrep of
+    the symmetric orbitals (block Lanczos from the irrep's start block in the target irrep,
+    model_instance::build_mcf, src_ed/model_instance.hpp:846-946), its weight carried to
+    the site basis; against the oracle's block Lanczos and fraction on the same start blocks
+    (each irrep's orbitals applied to the dense ground state, the irrep's span being all the
+    truncation depends on), both spins, and the integrated G against the dense
+    <c+_j c_i>. On the chain at N = 3, S = 1 the particle block of irrep 1 deflates at its
+    fourth floor (Krylov ranks 2, 4, 6, 7) and the fraction stops there, as the
+    reference's does; elsewhere the result is also the dense G."""
+    from pyqcm_amd import qcm
+    m, vals, ns, gens, bi = CASES[case]()
+    nup, ndn = (N + S) // 2, (N - S) // 2
+    r, _ = _lowest_irrep(m, vals, ns, gens, bi, N, S)
+    qcm.set_global_parameter("GF_method", "M")
+    try:
+        lab = register(m, vals, "R%d:N%d:S%d" % (r, N, S), ns, gens, bi)
+        G = qcm.Green_function(ZS, 0, lab)
+        Gdn = qcm.Green_function(ZS, 1, lab)
+        ave = qcm.Green_function_average(lab)
+    finally:
+        qcm.set_global_parameter("GF_method", "L")
+    _, Gd = O.dense_gf(m, vals, nup, ndn, list(range(ns)), ZS)
+    _, Gdd = O.dense_gf(m, vals, ndn, nup, list(range(ns)), ZS)
+    H, (c, cd) = O.dense_hamiltonian(m, vals, nup, ndn)
+    ev, U = np.linalg.eigh(H)
+    psi = np.zeros(1 << (2 * m.L))
+    psi[O.sector_states(m.L, nup, ndn)] = U[:, 0]
+    # one orthonormal basis of each irrep's span of site orbitals: the images under the
+    # group, symmetrized with the characters
+    spans = _irrep_spans(case, ns)
+
+    def oracle_mcf(spin):
+        out = np.zeros((len(ZS), ns, ns), complex)
+        for pm in (-1, 1):
+            tu, td = (nup + pm, ndn) if spin == 0 else (nup, ndn + pm)
+            if not (0 <= tu <= m.L and 0 <= td <= m.L):
+                continue
+            stt = O.sector_states(m.L, tu, td)
+            Ht = O.dense_hamiltonian(m, vals, tu, td)[0]
+            for Sb in spans:
+                ops = [c[a + spin * m.L] if pm < 0 else cd[a + spin * m.L] for a in range(ns)]
+                phi = [sum(w[a] * (ops[a] @ psi) for a in range(ns))[stt] for w in Sb]
+                q, W = O.mgs_weight(phi)
+                if not len(q):
+                    continue
+                A, B = O.block_lanczos(lambda x: Ht @ x, q)
+                A = O.mcf_shift(A, ev[0], pm == 1)
+                for iz, z in enumerate(ZS):
+                    g = O.mcf_evaluate(A, B, W, z)
+                    out[iz] += Sb.T @ (g if pm == 1 else g.T) @ Sb
+        return out
+
+    Go, God = oracle_mcf(0), oracle_mcf(1)
+    print(case, N, S, "irrep", r, "max |G_mcf - G_oracle| %.2e, |G_mcf - G_dense| %.2e"
+          % (np.abs(G - Go).max(), np.abs(G - Gd).max()))
+    np.testing.assert_allclose(G, Go, atol=1e-9)
+    np.testing.assert_allclose(Gdn, God, atol=1e-9)
+    tol = 1e-9 if case == "chain4" else 1e-6
+    if (case, N, S) != ("chain4", 3, 1):
+        np.testing.assert_allclose(G, Gd, atol=tol)
+        np.testing.assert_allclose(Gdn, Gdd, atol=tol)
+    D = np.array([[psi @ (cd[j] @ (c[i] @ psi)) for j in range(ns)] for i in range(ns)])
+    np.testing.assert_allclose(ave, D, atol=tol)
+
+
+def _irrep_spans(case, ns):
+    """Orthonormal bases (rows) of the site-orbital spans of each irrep: the chain's
+    mirror x -> 3 - x, the plaquette's two mirrors (C2v, one orbital per irrep)."""
+    if case == "chain4":
+        h = np.sqrt(0.5)
+        return [np.array([[h, 0, 0, h], [0, h, h, 0]]), np.array([[h, 0, 0, -h], [0, h, -h, 0]])]
+    S = 0.5 * np.array([[1, 1, 1, 1], [1, -1, 1, -1], [1, 1, -1, -1], [1, -1, -1, 1]], float)
+    return [S[r:r + 1] for r in range(4)]
