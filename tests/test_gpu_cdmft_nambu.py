@@ -116,3 +116,27 @@ def test_nambu_fit_recovers_a_planted_paired_bath(gpu_lib):
     print("planted", xs, "fitted", X[0], "distance %.2e" % F[0], "evaluations", NF[0])
     assert OK[0]
     np.testing.assert_allclose(X[0], xs, atol=1e-5)
+
+
+def test_batched_nambu_cdmft_equals_plain_loop(gpu_lib):
+    """A paired problem (lattice on-site pairing field S, variational bath pairing db):
+    two instances advanced in lockstep by the batched engine reach the plain loop's fixed
+    points (the spinor instances are solved one by one inside the batch), and those fixed
+    points carry bath pairing."""
+    from pyqcm_amd import cdmft as C
+    grid = C.frequency_grid("matsubara", (2, 50))
+    varia = ["eb1", "eb2", "tb1", "tb2", "db"]
+    P = C.CDMFTProblem(model(), varia, "R0:S0", grid=grid)
+    cases = [({"t": 1.0, "mu": 2.0, "U": 4.0, "S": 0.1}, [1.0, -1.0, 0.5, 0.5, 0.1]),
+             ({"t": 1.0, "mu": 1.5, "U": 3.0, "S": 0.1}, [1.2, -0.8, 0.6, 0.4, 0.1])]
+    plain = [C.CDMFT(P, v, x) for v, x in cases]
+    assert P.nambu
+    batch = C.BatchedCDMFT(P, [v for v, _ in cases], [x for _, x in cases])
+    for i, p in enumerate(plain):
+        print("plain", p.x, p.niter, "batched", batch.x[i], batch.niter[i])
+        assert batch.converged[i]
+        # the same fixed point within the loop's tolerance (accur = 1e-4 on the parameters):
+        # the batch draws its ED start vectors in another order, and a round-off-level
+        # difference can end the loop one iteration earlier
+        np.testing.assert_allclose(batch.x[i], p.x, atol=2e-4)
+        assert abs(p.x[4]) > 1e-3
