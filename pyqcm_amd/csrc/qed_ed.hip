@@ -135,6 +135,7 @@ struct State {
   double energy = 0.0, weight = 1.0;
   DevBuf psi;
   QMat gf[2];
+  std::vector<QMat> gfb[2];           // per irrep of the symmetric orbitals (their basis)
   std::vector<CFrac> cfe[2], cfh[2];  // n x n (o1 >= o2), electron / hole parts
   std::vector<CFBlock> cfb[2];        // symmetric orbitals or spinor: fractions per block
   MCFrac mcf[2][2];                   // [spin][0: holes, 1: particles]
@@ -681,17 +682,49 @@ void build_symmetric_orbitals(Model& M) {
 // more; a band Lanczos (or full diagonalization) there gives that irrep's block, which is
 // streamlined per part and checked for unitarity in the symmetric basis, as the reference
 // does, then turned to the site basis (Q_site = S_o^T Q_o) and concatenated with the others
+// The site-basis Lehmann form of per-irrep blocks in the symmetric-orbital basis: each
+// block's rows carried to the sites by the orbitals' coefficients, the blocks appended
+QMat sym_blocks_to_site(const Model& M, const std::vector<QMat>& blocks) {
+  const int n = M.n_sites;
+  QMat tot;
+  tot.L = n;
+  for (int ro = 0; ro < (int)blocks.size(); ++ro) {
+    std::vector<int> sel;
+    for (int o = 0; o < n; ++o)
+      if (M.sorb_irrep[o] == ro) sel.push_back(o);
+    const QMat& blk = blocks[ro];
+    const int p = (int)sel.size();
+    if (p == 0 || blk.M() == 0) continue;
+    if (blk.L != p) fail("Q matrix of irrep " + std::to_string(ro) + ": L = " + std::to_string(blk.L) + " for " +
+                         std::to_string(p) + " symmetric orbitals");
+    QMat site;
+    site.L = n;
+    site.e = blk.e;
+    site.Q.assign((size_t)n * blk.M(), 0.0);
+    for (int m = 0; m < blk.M(); ++m)
+      for (int k = 0; k < p; ++k) {
+        const double w = blk.Q[k + (size_t)m * p];
+        const double* row = &M.sorb[(size_t)sel[k] * n];
+        for (int a = 0; a < n; ++a) site.Q[a + (size_t)m * n] += row[a] * w;
+      }
+    append(tot, site);
+  }
+  tot.L = n;
+  return tot;
+}
+
 void build_qmatrix_sym(Instance& I, State& st, int spin) {
   Model& M = *I.model;
   const int n = M.n_sites;
   SectorPtr B = make_sector(I, st.nup, st.ndn, nullptr, st.irrep);
-  QMat tot;
-  tot.L = n;
+  st.gfb[spin].assign(M.group.g, QMat());
+  const double sw = std::sqrt(st.weight);
   for (int ro = 0; ro < M.group.g; ++ro) {
     std::vector<int> sel;
     for (int o = 0; o < n; ++o)
       if (M.sorb_irrep[o] == ro) sel.push_back(o);
     const int p = (int)sel.size();
+    st.gfb[spin][ro].L = p;
     if (p == 0) continue;
     const int rt = irrep_tensor(M.group, st.irrep, ro);
     QMat parts[2];
@@ -720,22 +753,10 @@ void build_qmatrix_sym(Instance& I, State& st, int spin) {
     append(blk, parts[1]);
     blk.L = p;
     check_norm(blk, gd("accur_Q_matrix"));
-    QMat site;
-    site.L = n;
-    site.e = blk.e;
-    site.Q.assign((size_t)n * blk.M(), 0.0);
-    for (int m = 0; m < blk.M(); ++m)
-      for (int k = 0; k < p; ++k) {
-        const double w = blk.Q[k + (size_t)m * p];
-        const double* row = &M.sorb[(size_t)sel[k] * n];
-        for (int a = 0; a < n; ++a) site.Q[a + (size_t)m * n] += row[a] * w;
-      }
-    append(tot, site);
+    for (double& v : blk.Q) v *= sw;
+    st.gfb[spin][ro] = blk;
   }
-  tot.L = n;
-  const double sw = std::sqrt(st.weight);
-  for (double& v : tot.Q) v *= sw;
-  st.gf[spin] = tot;
+  st.gf[spin] = sym_blocks_to_site(M, st.gfb[spin]);
 }
 
 // Lehmann form of one state for the spin-flip, anomalous and full mixings
@@ -2036,6 +2057,21 @@ void read_cf(const h5::Node& g, std::vector<CFrac>& e, std::vector<CFrac>& h, in
 
 void write_gf(const h5::Node& g, const Instance& I, const State& s, int spin, const std::string& fmt) {
   const int d = I.model->n_sites;
+  if (I.model->symmetric()) {
+    // one block per irrep, in the symmetric-orbital basis (Q_matrix_set::write_hdf5,
+    // src_ed/Q_matrix_set.hpp:183-195)
+    if (fmt != "bl") fail("the instance file of a model with generators holds its Lehmann form only");
+    h5::write_attr(g, "nblocks", (int)s.gfb[spin].size());
+    for (size_t r = 0; r < s.gfb[spin].size(); ++r) {
+      const QMat& q = s.gfb[spin][r];
+      h5::Node b = h5::create_group(g, "block_" + std::to_string(r));
+      h5::write_attr(b, "L", q.L);
+      h5::write_attr(b, "M", q.M());
+      h5::write_vec(b, "e", q.e);
+      h5::write_mat(b, "v", q.L, q.M(), q.Q.data());
+    }
+    return;
+  }
   if (fmt == "bl") {
     const QMat& q = s.gf[spin];
     h5::write_attr(g, "nblocks", 1);
@@ -2058,6 +2094,25 @@ void write_gf(const h5::Node& g, const Instance& I, const State& s, int spin, co
 
 void read_gf(const h5::Node& g, const Instance& I, State& s, int spin, const std::string& fmt, int mixing) {
   const int d = I.model->n_sites * n_mixed(mixing);
+  if (I.model->symmetric()) {
+    const int nbl = h5::read_attr_int(g, "nblocks");
+    if (nbl != I.model->group.g)
+      fail("the Green function has " + std::to_string(nbl) + " symmetry blocks; the model's group has " +
+           std::to_string(I.model->group.g) + " irreps");
+    s.gfb[spin].assign(nbl, QMat());
+    for (int r = 0; r < nbl; ++r) {
+      h5::Node b = h5::open_group(g, "block_" + std::to_string(r));
+      QMat& q = s.gfb[spin][r];
+      q.L = h5::read_attr_int(b, "L");
+      const int M = h5::read_attr_int(b, "M");
+      q.e = h5::read_vec(b, "e");
+      int rr = 0, cc = 0;
+      if (M > 0) q.Q = h5::read_mat_real(b, "v", rr, cc);
+      if ((int)q.e.size() != M || (M > 0 && (rr != q.L || cc != M))) fail("Q matrix: e and v do not match L and M");
+    }
+    s.gf[spin] = sym_blocks_to_site(*I.model, s.gfb[spin]);
+    return;
+  }
   if (fmt == "cf") {
     read_cf(g, s.cfe[spin], s.cfh[spin], d);
     return;
@@ -2086,7 +2141,6 @@ void read_gf(const h5::Node& g, const Instance& I, State& s, int spin, const std
 }
 
 void write_instance_file(Instance& I, const std::string& path, const std::string& group) {
-  require_trivial_group(I, "the instance file");
   green_function_solve(I);
   const std::string fmt = gf_format_name(I);
   if (spinor_mixing(I.mixing) && fmt == "cf")
@@ -2110,7 +2164,7 @@ void write_instance_file(Instance& I, const std::string& path, const std::string
       const State& s = *I.states[k];
       h5::Node sg = h5::create_group(g, "state_" + std::to_string(k));
       h5::write_attr(sg, "sec", spinor_mixing(I.mixing) ? sector_name(s.secN, s.secS)
-                                                         : sector_name(s.nup + s.ndn, s.nup - s.ndn));
+                                                         : sector_name(s.nup + s.ndn, s.nup - s.ndn, s.irrep));
       h5::write_attr(sg, "energy", s.energy);
       h5::write_attr(sg, "weight", s.weight);
       h5::write_attr(sg, "gf_format", fmt);
@@ -2122,10 +2176,10 @@ void write_instance_file(Instance& I, const std::string& path, const std::string
   }
 }
 
-std::pair<int, int> sector_numbers(const std::string& s);
+std::pair<int, int> sector_numbers(const std::string& s, int g = 1);
+int sector_irrep(const std::string& s, int g);
 
 void read_instance_file(Instance& I, const std::string& path, const std::string& group) {
-  require_trivial_group(I, "the instance file");
   std::vector<std::unique_ptr<State>> states;
   double gse = 0.0;
   std::string fmt, gs;
@@ -2152,7 +2206,7 @@ void read_instance_file(Instance& I, const std::string& path, const std::string&
     for (int k = 0; k < n; ++k) {
       h5::Node sg = h5::open_group(g, "state_" + std::to_string(k));
       auto st = std::make_unique<State>();
-      const auto ns = sector_numbers(h5::read_attr_string(sg, "sec"));
+      const auto ns = sector_numbers(h5::read_attr_string(sg, "sec"), I.model->group.g);
       if (spinor_mixing(mixing)) {
         st->nup = st->ndn = -1;
         st->secN = ns.first;
@@ -2162,6 +2216,9 @@ void read_instance_file(Instance& I, const std::string& path, const std::string&
       } else {
         st->nup = (ns.first + ns.second) / 2;
         st->ndn = (ns.first - ns.second) / 2;
+        st->irrep = sector_irrep(h5::read_attr_string(sg, "sec"), I.model->group.g);
+        if (I.model->symmetric() && fmt != "bl")
+          fail("the instance file of a model with generators holds continued fractions");
       }
       st->energy = h5::read_attr_double(sg, "energy");
       st->weight = h5::read_attr_double(sg, "weight");
@@ -2265,13 +2322,13 @@ SectorLabel parse_sector_label(const std::string& s, int g) {
   return {R, N, S};
 }
 
-std::pair<int, int> parse_sector(const std::string& s) {
-  const SectorLabel l = parse_sector_label(s, 1);
-  return {l.N, l.S};
-}
 }  // namespace
 
-std::pair<int, int> qed::ed::sector_numbers(const std::string& s) { return parse_sector(s); }
+std::pair<int, int> qed::ed::sector_numbers(const std::string& s, int g) {
+  const SectorLabel l = parse_sector_label(s, g);
+  return {l.N, l.S};
+}
+int qed::ed::sector_irrep(const std::string& s, int g) { return parse_sector_label(s, g).R; }
 
 extern "C" {
 

@@ -305,3 +305,31 @@ def _irrep_spans(case, ns):
         return [np.array([[h, 0, 0, h], [0, h, h, 0]]), np.array([[h, 0, 0, -h], [0, h, -h, 0]])]
     S = 0.5 * np.array([[1, 1, 1, 1], [1, -1, 1, -1], [1, 1, -1, -1], [1, -1, -1, 1]], float)
     return [S[r:r + 1] for r in range(4)]
+
+
+@pytest.mark.parametrize("case,N,S", [("plaquette4", 8, 0), ("chain4", 3, 1)])
+def test_symmetric_instance_file_round_trip(gpu_lib, tmp_path, case, N, S):
+    """An instance of a model with generators written to an instance file (one Lehmann
+    block per irrep in the symmetric-orbital basis, Q_matrix_set::write_hdf5,
+    src_ed/Q_matrix_set.hpp:183-195; the state's sector with its irrep) and read into a
+    fresh instance: sector string, G and Sigma of both spins and the Potthoff functional
+    bit for bit."""
+    from pyqcm_amd import qcm
+    m, vals, ns, gens, bi = CASES[case]()
+    r, _ = _lowest_irrep(m, vals, ns, gens, bi, N, S)
+    sec = "R%d:N%d:S%d" % (r, N, S)
+    a = register(m, vals, sec, ns, gens, bi)
+    E0, gs = qcm.ground_state_solve(a)
+    G = [qcm.Green_function(ZS, s, a) for s in (0, 1)]
+    Sg = [qcm.self_energyC(ZS, s, a) for s in (0, 1)]
+    path = str(tmp_path / "sym.h5")
+    qcm.write_instance_to_hdf5(path, "cluster_0", a)
+    b = register(m, vals, sec, ns, gens, bi)
+    qcm.read_instance_from_hdf5(path, "cluster_0", b)
+    E0b, gsb = qcm.ground_state_solve(b)
+    assert gsb == gs == sec + ":1"
+    assert abs(E0b - E0) <= 1e-12 * abs(E0)
+    for s in (0, 1):
+        np.testing.assert_array_equal(qcm.Green_function(ZS, s, b), G[s])
+        np.testing.assert_array_equal(qcm.self_energyC(ZS, s, b), Sg[s])
+    assert qcm.Potthoff_functional(b) == qcm.Potthoff_functional(a)
