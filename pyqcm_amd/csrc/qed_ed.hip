@@ -335,21 +335,13 @@ void diagonalize_bath(Instance& I, int sp, int nc, int nbg) {
 //   anomalous: one-body diag(M_uu, -M_dd^T) (hybrid: -M_dd), pairing 2 M_du below and
 //     2 M_du^T above the diagonal blocks (hybrid: -2 M_ud above);
 //   full: one-body diag(e, -e^T) (hybrid: -e), pairing 2 e below, 2 e^T above (hybrid: -2 e).
-void set_hopping_matrix_spinor(Instance& I) {
-  Model& M = *I.model;
-  const int ns = M.n_sites, nb = M.n_bath, L = M.L, m = I.mixing, nm = n_mixed(m);
+// Adds z times the spinor matrices of one operator to tc, tb and tcb (column-major,
+// dimensions nm n_sites and nm n_bath) by the rules above.
+void add_spinor_op(const Model& M, const Op& op, double z, int m, std::vector<double>& tc, std::vector<double>& tb,
+                   std::vector<double>& tcb) {
+  const int ns = M.n_sites, nb = M.n_bath, L = M.L, nm = n_mixed(m);
   const int dg = nm * ns, dbg = nm * nb;
-  I.SEF_bath = 0.0;
-  for (int sp = 0; sp < 2; ++sp) {
-    I.tc[sp].assign((size_t)dg * dg, 0.0);
-    I.tb[sp].assign((size_t)dbg * dbg, 0.0);
-    I.tcb[sp].assign((size_t)dg * dbg, 0.0);
-  }
-  for (const Op& op : M.ops) {
-    if (op.interaction) continue;
-    auto it = I.value.find(op.name);
-    if (it == I.value.end() || it->second == 0.0) continue;
-    const double z = it->second;
+  {
     std::vector<double> F((size_t)4 * L * L, 0.0);  // M over spin orbitals, 2L x 2L column-major
     auto at = [&](int a, int b) -> double& { return F[a + (size_t)b * 2 * L]; };
     for (size_t q = 0; q < op.v.size(); ++q) {
@@ -372,11 +364,11 @@ void set_hopping_matrix_spinor(Instance& I) {
     std::vector<double>* dst;
     int ld;  // leading dimension of the destination (its row count)
     if (op.target == 1) {
-      r0 = 0, nR = ns, c0 = 0, nC = ns, dst = &I.tc[0], ld = dg;
+      r0 = 0, nR = ns, c0 = 0, nC = ns, dst = &tc, ld = dg;
     } else if (op.target == 2) {
-      r0 = ns, nR = nb, c0 = ns, nC = nb, dst = &I.tb[0], ld = dbg;
+      r0 = ns, nR = nb, c0 = ns, nC = nb, dst = &tb, ld = dbg;
     } else {
-      r0 = 0, nR = ns, c0 = ns, nC = nb, dst = &I.tcb[0], ld = dg;
+      r0 = 0, nR = ns, c0 = ns, nC = nb, dst = &tcb, ld = dg;
     }
     const bool hyb = op.target == 3;
     // e: the (2 nR) x (2 nC) spin block matrix of the operator
@@ -424,6 +416,24 @@ void set_hopping_matrix_spinor(Instance& I) {
         fail("anomalous operator " + op.name + " in an instance without anomalous mixing");
       }
     }
+  }
+}
+
+void set_hopping_matrix_spinor(Instance& I) {
+  Model& M = *I.model;
+  const int ns = M.n_sites, nb = M.n_bath, m = I.mixing, nm = n_mixed(m);
+  const int dg = nm * ns, dbg = nm * nb;
+  I.SEF_bath = 0.0;
+  for (int sp = 0; sp < 2; ++sp) {
+    I.tc[sp].assign((size_t)dg * dg, 0.0);
+    I.tb[sp].assign((size_t)dbg * dbg, 0.0);
+    I.tcb[sp].assign((size_t)dg * dbg, 0.0);
+  }
+  for (const Op& op : M.ops) {
+    if (op.interaction) continue;
+    auto it = I.value.find(op.name);
+    if (it == I.value.end() || it->second == 0.0) continue;
+    add_spinor_op(M, op, it->second, m, I.tc[0], I.tb[0], I.tcb[0]);
   }
   diagonalize_bath(I, 0, dg, dbg);
   I.eb[1] = I.eb[0];
@@ -2711,8 +2721,37 @@ int qed_ed_cluster_averages(uint64_t label, int nop, const char* const* names, d
       // src_ed/model_instance.hpp:152-163; one_body_operator::average_from_GF,
       // src_ed/Operators/one_body_operator.hpp:420-441): one-body operators on the cluster
       // only, no variance
-      require_no_spinor(I, "the averages from the Green function");
       green_function_solve(I);
+      if (spinor_mixing(I.mixing)) {
+        // the trace of the integrated spinor G with the operator's spinor matrix E
+        // (one_body_operator::average_from_GF, anomalous_operator::average_from_GF), plus
+        // the Nambu correction of the diagonal one-body elements, halved for the full
+        // mixing (model_instance::cluster_averages, src_ed/model_instance.hpp:144-163)
+        const Model& Md = *I.model;
+        const int dg = dim_gf(I), dbg = dg / Md.n_sites * Md.n_bath;
+        const auto Mg = gf_average(I, 0);
+        for (int k = 0; k < nop; ++k) {
+          std::string nm(names[k]);
+          if (!I.value.count(nm)) fail("operator " + nm + " is not activated in instance " + std::to_string(label));
+          const Op& op = Md.ops[Md.idx.at(nm)];
+          ave[k] = var[k] = std::nan("");
+          if (op.interaction || op.target != 1) continue;
+          std::vector<double> E((size_t)dg * dg, 0.0), Eb((size_t)dbg * dbg, 0.0), Ecb((size_t)dg * dbg, 0.0);
+          add_spinor_op(Md, op, 1.0, I.mixing, E, Eb, Ecb);
+          double a = 0.0;
+          for (size_t q = 0; q < E.size(); ++q) a += Mg[q].real() * E[q];
+          double corr = 0.0;
+          if (!op.anomalous)
+            for (size_t e = 0; e < op.v.size(); ++e)
+              if (op.r[e] == op.c[e] && (I.mixing == 3 || op.r[e] < Md.L)) corr += op.v[e];
+          if (I.mixing == 1) a += corr;
+          if (I.mixing == 3) a = 0.5 * (a + corr);
+          double norm = 1.0 / Md.n_sites;
+          if (nm == "mu") norm *= -1.0;
+          ave[k] = a * norm;
+        }
+        return;
+      }
       const int d = I.model->n_sites, L = I.model->L;
       const int nsp = (I.mixing & 4) ? 2 : 1;
       std::vector<std::vector<cd>> M;

@@ -478,3 +478,35 @@ def test_nambu_instance_file_round_trip(gpu_lib, tmp_path, method):
         qcm.set_global_parameter("GF_method", "L")
     assert sec == secb == "R0:S0:1"
     assert abs(E0b - E0) <= 1e-12 * abs(E0)
+
+
+@pytest.mark.parametrize("case", ["nambu", "spin_flip", "full"])
+def test_mixing_uncorrelated_averages(gpu_lib, case):
+    """Operator averages of an uncorrelated spinor instance from its integrated Green
+    function and the operator's spinor matrix, with the Nambu correction
+    (src_ed/model_instance.hpp:144-163), against the lowest state of the whole Fock space."""
+    from pyqcm_amd import qcm
+    if case == "nambu":
+        m, vals, n = chain(d=0.5, s=0.2)
+        sec = "R0:S0"
+    elif case == "spin_flip":
+        m, vals, n = chain(hx=0.4)
+        sec = "R0:N4"
+    else:
+        m, vals, n = chain(hx=0.3, d=0.4)
+        sec = "R0"
+    del vals["U"]
+    lab = register(m, vals, sec, n)
+    qcm.ground_state_solve(lab)
+    ave = qcm.cluster_averages(lab)
+    allst = O.block_states(m.L, [(nu, nd) for nu in range(m.L + 1) for nd in range(m.L + 1)])
+    psi = np.linalg.eigh(O.dense_hamiltonian(m, vals, 0, 0, states=allst)[0])[1][:, 0]
+    for name, op in m.operators.items():
+        if vals.get(name, 0.0) == 0.0:
+            continue
+        one = O.ClusterModel(m.L)
+        one.add(name, op.kind, op.elements)
+        a = psi @ O.dense_hamiltonian(one, {name: 1.0}, 0, 0, states=allst)[0] @ psi
+        sgn = -1.0 if name == "mu" else 1.0
+        print(case, name, "average %.12f dense %.12f" % (ave[name][0], sgn * a / n))
+        assert abs(ave[name][0] - sgn * a / n) < 1e-10, name
