@@ -57,7 +57,11 @@ class Tiling:
                                    and amplitude on c_i up c_i down + h.c. for link 0
                                    (pyqcm anomalous_operator, type 'singlet',
                                    src_qcm/lattice_model.cpp:308-341,1101-1133).
-    Operators with pairing terms need the Nambu spinor (c_up(k), c+_down(-k)).
+    and as real spin-flip matrices F^n, F^n_ij (c+_{i up, 0} c_{j down, n} + h.c.):
+      spin_flip(name, amplitude)   amplitude on c+_i up c_i down + h.c. on every site (pyqcm
+                                   hopping_operator with tau = 0, sigma = 1: a field along x).
+    Operators with pairing terms need the Nambu spinor (c_up(k), c+_down(-k)), spin-flip
+    operators the spinor (c_up(k), c_down(k)) (mixing 2).
     """
 
     def __init__(self, sites, superlattice):
@@ -81,6 +85,7 @@ class Tiling:
         self.Sinv = np.linalg.inv(full.T)
         self.terms = {}  # name -> {n (tuple of dim ints): n x n real matrix}
         self.pairs = {}  # name -> {n: n x n real pairing matrix P^n}
+        self.flips = {}  # name -> {n: n x n real spin-flip matrix F^n}
         self.n_orb = self.n
 
     def _zero(self):
@@ -131,6 +136,14 @@ class Tiling:
                 mat(tuple(-x for x in nvec))[j, i] += amplitude
         return self
 
+    def spin_flip(self, name, amplitude=1.0):
+        d = self.flips.setdefault(name, {})
+        z = self._zero()
+        if z not in d:
+            d[z] = np.zeros((self.n, self.n))
+        d[z][np.diag_indices(self.n)] += amplitude
+        return self
+
     def onsite(self, name, amplitude=1.0):
         M = self._mat(name, self._zero())
         M[np.diag_indices(self.n)] += amplitude
@@ -145,11 +158,17 @@ class Tiling:
 
     # ---- decompositions
 
-    def cluster_matrix(self, values, nambu=False):
-        """Intra-cluster one-body matrix T_0 = sum_op value_op T_0^op (with nambu, the
-        2n x 2n Nambu matrix of the translation n = 0)."""
-        if nambu:
-            return self.nambu_terms(values).get(self._zero(), np.zeros((2 * self.n, 2 * self.n))).real.copy()
+    def spinor_terms(self, values, mixing):
+        """The lattice terms in the spinor of a mixing: 1 Nambu (nambu_terms), 2 spin flip
+        (spin_flip_terms)."""
+        return self.nambu_terms(values) if mixing == 1 else self.spin_flip_terms(values)
+
+    def cluster_matrix(self, values, nambu=False, mixing=None):
+        """Intra-cluster one-body matrix T_0 = sum_op value_op T_0^op (with nambu, or a
+        spinor mixing 1 or 2, the 2n x 2n spinor matrix of the translation n = 0)."""
+        mixing = (1 if nambu else 0) if mixing is None else mixing
+        if mixing:
+            return self.spinor_terms(values, mixing).get(self._zero(), np.zeros((2 * self.n, 2 * self.n))).real.copy()
         T0 = np.zeros((self.n, self.n))
         for name, d in self.terms.items():
             z = values.get(name, 0.0)
@@ -194,6 +213,12 @@ class Tiling:
         spins (site i -> label i+1 for spin up, i+1+n_orb for spin down), the form
         qcm.new_operator takes; for a pairing operator, the anomalous elements
         (i up, j down, P^0_ij), each standing for P^0_ij (c_i up c_j down + h.c.)."""
+        if name in self.flips:
+            F0 = self.flips[name].get(self._zero())
+            if F0 is None:
+                return []
+            return [(i + 1, j + 1 + self.n_orb, float(F0[i, j])) for i in range(self.n) for j in range(self.n)
+                    if F0[i, j] != 0.0]
         if name in self.pairs:
             P0 = self.pairs[name].get(self._zero())
             if P0 is None:
@@ -211,13 +236,43 @@ class Tiling:
                         out.append((i + 1 + s, j + 1 + s, float(T0[i, j])))
         return out
 
-    def fourier_terms(self, values, nambu=False):
+    def spin_flip_terms(self, values):
+        """{n: 2n x 2n matrix} of the lattice Hamiltonian in the spinor
+        Psi_k = (c_up(k), c_down(k)), h(k) = sum_n H^n exp(2 pi i k.n) with
+          H^n = [[T^n, F^n], [(F^-n)^T, T^n]]
+        (the one-body matrices of the spin-flip mixing, one_body_operator.hpp:326-411)."""
+        n = self.n
+        T, F = {}, {}
+        for src, dst in ((self.terms, T), (self.flips, F)):
+            for name, d in src.items():
+                z = values.get(name, 0.0)
+                if not z:
+                    continue
+                for nvec, M in d.items():
+                    dst[nvec] = dst.get(nvec, 0.0) + z * M
+        out = {}
+        for nvec in set(T) | set(F) | {tuple(-x for x in v) for v in F}:
+            neg = tuple(-x for x in nvec)
+            H = np.zeros((2 * n, 2 * n))
+            if nvec in T:
+                H[:n, :n] = T[nvec]
+                H[n:, n:] = T[nvec]
+            if nvec in F:
+                H[:n, n:] = F[nvec]
+            if neg in F:
+                H[n:, :n] = np.asarray(F[neg]).T
+            if np.any(H):
+                out[nvec] = H
+        return out
+
+    def fourier_terms(self, values, nambu=False, mixing=None):
         """Inter-cluster terms: (nvecs (m x 3) float, mats (m x n x n) complex), n != 0
-        (with nambu, the 2n x 2n Nambu matrices)."""
+        (with nambu, or a spinor mixing 1 or 2, the 2n x 2n spinor matrices)."""
         acc = {}
         zero = self._zero()
-        if nambu:
-            acc = {k: v for k, v in self.nambu_terms(values).items() if k != zero}
+        mixing = (1 if nambu else 0) if mixing is None else mixing
+        if mixing:
+            acc = {k: v for k, v in self.spinor_terms(values, mixing).items() if k != zero}
             keys = list(acc)
             nv = np.array([list(k) + [0] * (3 - self.dim) for k in keys], float).reshape(-1, 3)
             mats = np.array([acc[k] for k in keys], complex).reshape(-1, 2 * self.n, 2 * self.n)
@@ -235,11 +290,12 @@ class Tiling:
         mats = np.array([acc[k] for k in keys], complex).reshape(-1, self.n, self.n)
         return nv, mats
 
-    def tk(self, values, k, nambu=False):
-        """t(k) for wavevectors k (nk x 3), dense host evaluation (with nambu, h(k))."""
+    def tk(self, values, k, nambu=False, mixing=None):
+        """t(k) for wavevectors k (nk x 3), dense host evaluation (with nambu or a spinor
+        mixing, the spinor h(k))."""
         k = np.atleast_2d(np.asarray(k, float))
-        nv, mats = self.fourier_terms(values, nambu)
-        T0 = self.cluster_matrix(values, nambu).astype(complex)
+        nv, mats = self.fourier_terms(values, nambu, mixing)
+        T0 = self.cluster_matrix(values, nambu, mixing).astype(complex)
         if not len(nv):
             return np.repeat(T0[None], len(k), 0)
         ph = np.exp(2j * np.pi * (k @ nv.T))
@@ -576,10 +632,15 @@ class LatticeInstance:
         (src_qcm/lattice_model.cpp:458-468)."""
         self.ground_state()
         m = qcm.mixingC(self.label)
-        if m not in (0, 1, 4):
-            raise RuntimeError("the lattice functions take the normal, Nambu or up/down mixing only "
-                               "(instance mixing %d)" % m)
+        if m not in (0, 1, 2, 4):
+            raise RuntimeError("the lattice functions take the normal, Nambu, spin-flip or up/down mixing "
+                               "only (instance mixing %d)" % m)
         return m
+
+    @property
+    def spinor(self):
+        """The spinor mixing of the lattice functions: 1 Nambu, 2 spin flip, 0 none."""
+        return self.mixing if self.mixing in (1, 2) else 0
 
     @property
     def nambu(self):
@@ -603,20 +664,20 @@ class LatticeInstance:
     def v0(self, spin_down=0):
         """V0 = T_0(lattice) - t_c, the k-independent part of V (Nambu: 2n x 2n)."""
         tc = qcm.hopping_matrix(spin_down, 0, self.label).real
-        return self.model.tiling.cluster_matrix(self.values, self.nambu) - tc
+        return self.model.tiling.cluster_matrix(self.values, mixing=self.spinor) - tc
 
     def cpt_green(self, k, z, mode="full", spin_down=0):
         """CPT Green function at frequencies z and wavevectors k (QCM::CPT_Green_function,
         src_qcm/QCM.cpp:328-344); with the Nambu mixing, of the spinor
         (c_up(k), c+_down(-k)), 2n x 2n."""
         _, Geff = self._geff(z, spin_down)
-        nv, mats = self.model.tiling.fourier_terms(self.values, self.nambu)
+        nv, mats = self.model.tiling.fourier_terms(self.values, mixing=self.spinor)
         return cpt_batch(Geff, self.v0(spin_down), nv, mats, k, mode)
 
     def sef_integrand(self, z, k, spin_down=0):
         """(1/nk) sum_k log|det(V(k, z) G_c(z) - 1)| per frequency (one spin block)."""
         G, Geff = self._geff(z, spin_down)
-        nv, mats = self.model.tiling.fourier_terms(self.values, self.nambu)
+        nv, mats = self.model.tiling.fourier_terms(self.values, mixing=self.spinor)
         I = sef_batch(Geff, self.v0(spin_down), nv, mats, k)
         if self.model.n_bath:
             I = I + np.log(np.abs(np.linalg.det(G))) - np.log(np.abs(np.linalg.det(Geff)))

@@ -180,3 +180,31 @@ def test_nambu_cpt_interacting_matches_dense_restatement(gpu_lib):
         for b in range(len(k)):
             ref = np.linalg.inv(np.linalg.inv(Gc[a]) - (hk[b] - tc - Gam[a]))
             np.testing.assert_allclose(G[a, b], ref, atol=1e-10)
+
+
+def test_spin_flip_lattice_cpt_and_potthoff_at_U0(gpu_lib):
+    """A field along x on every site (pyqcm hopping_operator with tau = 0, sigma = 1): the
+    instance has the spin-flip mixing, and CPT and the functional work in the spinor
+    (c_up(k), c_down(k)). The lattice's intra-cluster spinor matrix is the cluster's t_c;
+    at U -> 0 CPT is (z - h(k))^-1 and the functional is the lattice ground-state energy
+    per site, (1/n) <sum_{E(k) < 0} E(k)> on the same k-grid."""
+    from pyqcm_amd import lattice as LT
+    T = LT.chain_tiling(4).spin_flip("hx")
+    M = LT.LatticeModel("sfchain_%d" % next(_names), T, lattice_ops=("t", "mu", "hx"))
+    vals = {"t": 1.0, "mu": 0.0, "hx": 0.3, "U": 1e-9}
+    # the cluster's spinor levels e +- 0.3 have four below zero: N = 4
+    inst = M.instance(vals, "R0:N4")
+    assert inst.mixing == 2
+    np.testing.assert_allclose(inst.v0(), 0.0, atol=1e-14)
+    k = np.array([[0.1, 0, 0], [0.37, 0, 0], [0.5, 0, 0]])
+    G = inst.cpt_green(k, ZS, "full")
+    hk = M.tiling.tk(vals, k, mixing=2)
+    for a, z in enumerate(ZS):
+        for b in range(len(k)):
+            np.testing.assert_allclose(G[a, b], np.linalg.inv(z * np.eye(8) - hk[b]), atol=1e-8)
+    om = inst.potthoff_functional(grid=(1, 10, 30, 30, 30), kside=128)
+    kg = M.tiling.kgrid(128, shift=0.0)
+    e = np.linalg.eigvalsh(M.tiling.tk(vals, kg, mixing=2))
+    ref = np.where(e < 0, e, 0.0).sum(1).mean() / 4
+    print("spin flip: Omega %.10f, lattice energy per site %.10f" % (om, ref))
+    assert abs(om - ref) < 2e-6
