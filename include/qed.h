@@ -305,6 +305,12 @@ int qed_ed_hybridization_function(uint64_t label, int nz, const double* z, int s
                                   double* Gam);
 /* ED::hopping_matrix (cluster part tc, complex d x d). */
 int qed_ed_hopping_matrix(uint64_t label, int spin_down, double* tc);
+/* The one-body matrix of cluster and bath, (d + d_b) x (d + d_b) complex column-major
+ * (d = dim_GF, d_b the bath's spinor dimension): [[t_c, t_cb], [t_cb^T, t_b]], the bath
+ * diagonalized (diag != 0) or as the operators give it. Replaces ED::hopping_matrix_full
+ * (src_ed/qcm_ED.cpp:278, src_ed/model_instance.hpp:1491-1525; binding hopping_matrix(...,
+ * full=1), src_python/qcm_ED_wrap.hpp:94-106). */
+int qed_ed_hopping_matrix_full(uint64_t label, int spin_down, int diag, double* H);
 /* Bath matrices after diagonalization: eb (nb), tcb (d x nb column-major). */
 int qed_ed_bath(uint64_t label, int spin_down, double* eb, double* tcb);
 /* ED::qmatrix (model_instance.hpp:103): M poles, e (M) and Q (d x M column-major);

@@ -83,6 +83,7 @@ SIGNATURES = [
     ("qed_ed_self_energy", c_int, [c_u64, c_int, dptr, c_int, dptr]),
     ("qed_ed_hybridization_function", c_int, [c_u64, c_int, dptr, c_int, dptr]),
     ("qed_ed_hopping_matrix", c_int, [c_u64, c_int, dptr]),
+    ("qed_ed_hopping_matrix_full", c_int, [c_u64, c_int, c_int, dptr]),
     ("qed_ed_bath", c_int, [c_u64, c_int, dptr, dptr]),
     ("qed_ed_qmatrix", c_int, [c_u64, c_int, iptr, dptr, dptr]),
     ("qed_ed_potthoff_functional", c_int, [c_u64, dptr]),

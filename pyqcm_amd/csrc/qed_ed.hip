@@ -2649,6 +2649,34 @@ int qed_ed_hopping_matrix(uint64_t label, int spin_down, double* tc) {
   });
 }
 
+int qed_ed_hopping_matrix_full(uint64_t label, int spin_down, int diag, double* H) {
+  std::lock_guard<std::recursive_mutex> lk(lib_mutex());
+  return guard([&] {
+    // model_instance::hopping_matrix_full (src_ed/model_instance.hpp:1491-1525): the
+    // one-body matrix of cluster and bath, [[t_c, t_cb], [t_cb^T, t_b]], with the bath in
+    // its eigenbasis (diag) or as the operators give it
+    Instance& I = inst(label);
+    const int sp = (spin_down && (I.mixing & 4)) ? 1 : 0;
+    const int dg = dim_gf(I), dbg = dg / I.model->n_sites * I.model->n_bath, d = dg + dbg;
+    std::vector<double> F((size_t)d * d, 0.0);
+    for (int a = 0; a < dg; ++a)
+      for (int b = 0; b < dg; ++b) F[a + (size_t)b * d] = I.tc[sp][a + (size_t)b * dg];
+    const std::vector<double>& tcb = diag ? I.tcb_d[sp] : I.tcb[sp];
+    for (int a = 0; a < dg; ++a)
+      for (int b = 0; b < dbg; ++b) {
+        F[a + (size_t)(dg + b) * d] = tcb[a + (size_t)b * dg];
+        F[(dg + b) + (size_t)a * d] = tcb[a + (size_t)b * dg];
+      }
+    for (int a = 0; a < dbg; ++a)
+      for (int b = 0; b < dbg; ++b)
+        F[(dg + a) + (size_t)(dg + b) * d] = diag ? (a == b ? I.eb[sp][a] : 0.0) : I.tb[sp][a + (size_t)b * dbg];
+    for (size_t q = 0; q < F.size(); ++q) {
+      H[2 * q] = F[q];
+      H[2 * q + 1] = 0.0;
+    }
+  });
+}
+
 int qed_ed_bath(uint64_t label, int spin_down, double* eb, double* tcb) {
   std::lock_guard<std::recursive_mutex> lk(lib_mutex());
   return guard([&] {

@@ -231,7 +231,15 @@ def hybridization_functionC(z, spin_down=0, label=0):
 
 
 def hopping_matrix(spin_down=0, diag=0, label=0, full=0):
+    """t_c (dim_GF x dim_GF); with full, the matrix of cluster and bath with the bath
+    diagonalized (diag) or as given (src_python/qcm_ED_wrap.hpp:94-106)."""
     d = Green_function_dimensionC(label)
+    if full:
+        ns, nb = model_sizeC(_instance_model[int(label)])[:2]
+        d += nb * (d // ns)
+        out = np.zeros(2 * d * d)
+        call("qed_ed_hopping_matrix_full", int(label), int(spin_down), int(diag), dp(out))
+        return out.view(np.complex128).reshape(d, d).T.copy()
     out = np.zeros(2 * d * d)
     call("qed_ed_hopping_matrix", int(label), int(spin_down), dp(out))
     return out.view(np.complex128).reshape(d, d).T.copy()

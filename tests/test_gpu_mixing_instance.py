@@ -510,3 +510,24 @@ def test_mixing_uncorrelated_averages(gpu_lib, case):
         sgn = -1.0 if name == "mu" else 1.0
         print(case, name, "average %.12f dense %.12f" % (ave[name][0], sgn * a / n))
         assert abs(ave[name][0] - sgn * a / n) < 1e-10, name
+
+
+def test_hopping_matrix_full_with_a_paired_bath(gpu_lib):
+    """hopping_matrix(full=1) of a Nambu instance with a paired bath: the cluster block is
+    t_c, the bath block as the operators give it (pairing off the diagonal) or, with diag,
+    the bath energies of the Lehmann form of the hybridization; both forms have the same
+    spectrum (the bath is rotated by an orthogonal matrix)."""
+    from pyqcm_amd import qcm
+    m, vals, ns = nambu_bath_model(2.0)
+    lab = register(m, vals, "R0:S0", ns)
+    qcm.ground_state_solve(lab)
+    H0 = qcm.hopping_matrix(0, 0, lab, 1)
+    Hd = qcm.hopping_matrix(0, 1, lab, 1)
+    d = 2 * ns
+    tc = qcm.hopping_matrix(0, 0, lab)
+    np.testing.assert_array_equal(H0[:d, :d], tc)
+    np.testing.assert_array_equal(Hd[:d, :d], tc)
+    E, _ = qcm.hybridization(lab)
+    np.testing.assert_allclose(np.diag(Hd[d:, d:]).real, E, atol=1e-14)
+    assert np.abs(H0[d:, d:] - np.diag(np.diag(H0[d:, d:]))).max() > 0.1
+    np.testing.assert_allclose(np.linalg.eigvalsh(H0), np.linalg.eigvalsh(Hd), atol=1e-12)
