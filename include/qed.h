@@ -361,10 +361,13 @@ int qed_ed_print_wavefunction(uint64_t label, char* buf, int buflen, int* needed
 int qed_ed_get_combined_mcf(uint64_t label, int spin_down, int* p, int* M, int* q, double* A, double* B,
                             double* W);
 /* ED::density_matrix (src_ed/qcm_ED.cpp:205-216): for the factorized Hamiltonian format
- * the reference returns an empty matrix and basis (model_instance::
- * density_matrix_factorized, src_ed/model_instance.hpp:1629-1633); so does this solver:
- * *dim = 0. */
-int qed_ed_density_matrix(uint64_t label, int nsites, const int* sites, int* dim);
+ * (Hamiltonian_format 'F') the reference returns an empty matrix and basis
+ * (density_matrix_factorized, src_ed/model_instance.hpp:1629-1633): *dim = 0; for the other
+ * formats the reduced density matrix of the first state on the sites (0-based)
+ * (density_matrix_mixed, src_ed/model_instance.hpp:1547-1625): *dim subsystem states, rho
+ * (dim x dim complex, column-major) and the subsystem states collapsed onto bits 0.. of
+ * each spin half (states, dim entries). Pass rho = NULL to query *dim. Trivial group. */
+int qed_ed_density_matrix(uint64_t label, int nsites, const int* sites, int* dim, double* rho, uint64_t* states);
 /* ED::susceptibility (src_ed/qcm_ED.cpp:307-315, model_instance.hpp:1024-1058): dynamic
  * susceptibility of operator `op` at nz complex frequencies z (interleaved re, im), the
  * Boltzmann-weighted sum over low-energy states of <psi|O (w - H + E)^-1 O|psi>, with the

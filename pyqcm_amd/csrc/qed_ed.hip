@@ -3026,13 +3026,73 @@ int qed_ed_get_combined_mcf(uint64_t label, int spin_down, int* p, int* M, int* 
   });
 }
 
-int qed_ed_density_matrix(uint64_t label, int nsites, const int* sites, int* dim) {
+int qed_ed_density_matrix(uint64_t label, int nsites, const int* sites, int* dim, double* rho, uint64_t* states) {
   std::lock_guard<std::recursive_mutex> lk(lib_mutex());
   return guard([&] {
-    inst(label);
-    (void)nsites;
-    (void)sites;
-    *dim = 0;  // the reference's factorized-format form is empty
+    // ED::density_matrix (src_ed/qcm_ED.cpp:205-216): empty for the factorized format
+    // (density_matrix_factorized), otherwise model_instance::density_matrix_mixed
+    // (src_ed/model_instance.hpp:1547-1625) on the first state: the basis states b
+    // (up | down << 32, in basis order) split into b & mask_A and the rest, subsystem
+    // states numbered by first appearance, rho(a, a') = sum over the rest of
+    // psi(a, B) psi(a', B), and the subsystem states collapsed onto bits 0.. of each half
+    Instance& I = inst(label);
+    *dim = 0;
+    if (gc("Hamiltonian_format") == 'F' && !spinor_mixing(I.mixing)) return;
+    if (I.model->symmetric()) fail("computing the density matrix requires symmetries to be turned off!");
+    low_energy_states(I);
+    if (I.states.empty() || !I.states[0]->psi.p) fail("the density matrix needs the ground-state vector");
+    const State& st = *I.states[0];
+    SectorPtr H = state_sector(I, st);
+    std::vector<uint64_t> basis(H->D);
+    if (spinor_mixing(I.mixing)) {
+      uint64_t n = 0;
+      if (qed_general_basis(H.get(), basis.data(), &n) != 0) fail(qed_last_error());
+    } else {
+      std::vector<uint32_t> bu(H->du), bd(H->dd);
+      QED_HIP(hipMemcpy(bu.data(), H->bup.p, H->du * 4, hipMemcpyDeviceToHost));
+      QED_HIP(hipMemcpy(bd.data(), H->bdn.p, H->dd * 4, hipMemcpyDeviceToHost));
+      for (uint64_t k = 0; k < H->D; ++k) basis[k] = (uint64_t)bu[k % H->du] | ((uint64_t)bd[k / H->du] << 32);
+    }
+    std::vector<double> psi(H->D);
+    QED_HIP(hipMemcpy(psi.data(), st.psi.p, H->D * 8, hipMemcpyDeviceToHost));
+    uint64_t mask = 0;
+    for (int i = 0; i < nsites; ++i) {
+      if (sites[i] < 0 || sites[i] >= I.model->L) fail("density_matrix: site out of range");
+      mask |= 1ull << sites[i];
+    }
+    mask |= mask << 32;
+    std::map<uint64_t, int> ia, ib;
+    std::vector<uint64_t> binA;
+    std::vector<std::vector<std::pair<int, uint64_t>>> listA;  // per B: (a, basis index)
+    for (uint64_t k = 0; k < H->D; ++k) {
+      const uint64_t A = basis[k] & mask, B = basis[k] ^ A;
+      auto a = ia.find(A);
+      if (a == ia.end()) {
+        a = ia.emplace(A, (int)binA.size()).first;
+        binA.push_back(A);
+      }
+      auto b = ib.find(B);
+      if (b == ib.end()) {
+        b = ib.emplace(B, (int)listA.size()).first;
+        listA.emplace_back();
+      }
+      listA[b->second].push_back({a->second, k});
+    }
+    const int dA = (int)binA.size();
+    *dim = dA;
+    if (!rho) return;
+    std::fill(rho, rho + (size_t)2 * dA * dA, 0.0);
+    for (const auto& v : listA)
+      for (const auto& x : v)
+        for (const auto& y : v) rho[2 * (x.first + (size_t)y.first * dA)] += psi[x.second] * psi[y.second];
+    for (int a = 0; a < dA; ++a) {
+      uint64_t c = 0;
+      for (int i = 0; i < nsites; ++i) {
+        if (binA[a] & (1ull << sites[i])) c |= 1ull << i;
+        if (binA[a] & (1ull << (sites[i] + 32))) c |= 1ull << (i + 32);
+      }
+      states[a] = c;
+    }
   });
 }
 

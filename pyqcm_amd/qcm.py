@@ -365,12 +365,20 @@ def print_wavefunction(label=0):
 
 
 def density_matrix(sites, label=0):
-    """(density matrix, basis) of the subsystem `sites`; empty for the factorized format,
-    as the reference's (src_python/qcm_ED_wrap.hpp:32-42, src_ed/model_instance.hpp:1629-1633)."""
+    """(density matrix, basis) of the subsystem `sites` (0-based) in the first state: empty
+    for the factorized format 'F', the reduced density matrix otherwise
+    (src_python/qcm_ED_wrap.hpp:32-42, src_ed/model_instance.hpp:1547-1633)."""
     s = np.ascontiguousarray(sites, np.int32)
     dim = ctypes.c_int()
-    call("qed_ed_density_matrix", int(label), len(s), ip(s), ctypes.byref(dim))
-    return np.zeros((0, 0), np.complex128), np.zeros(0, np.uint64)
+    call("qed_ed_density_matrix", int(label), len(s), ip(s), ctypes.byref(dim), None, None)
+    d = dim.value
+    if d == 0:
+        return np.zeros((0, 0), np.complex128), np.zeros(0, np.uint64)
+    rho = np.zeros(2 * d * d)
+    st = np.zeros(d, np.uint64)
+    call("qed_ed_density_matrix", int(label), len(s), ip(s), ctypes.byref(dim), dp(rho),
+         st.ctypes.data_as(ctypes.c_void_p))
+    return rho.view(np.complex128).reshape(d, d).T.copy(), st
 
 
 def susceptibility(op, w, label=0):

@@ -168,7 +168,49 @@ def test_cluster_averages_from_green_function(gpu_lib, tmp_path):
     Mslater = occ @ occ.T
     for op in ("t", "mu"):
         assert abs(got0[op][0] - _one_body_average(m, op, Mslater, 4)) < 1e-12, op
-    assert qcm.density_matrix([0, 1], a)[0].shape == (0, 0)
+    qcm.set_global_parameter("Hamiltonian_format", "F")
+    try:
+        assert qcm.density_matrix([0, 1], a)[0].shape == (0, 0)
+    finally:
+        qcm.set_global_parameter("Hamiltonian_format", "E")
+
+
+def test_reduced_density_matrix(gpu_lib):
+    """ED::density_matrix with the default (operator) format: the reduced density matrix of
+    the ground state on sites {0, 1} of the 4-site chain (density_matrix_mixed,
+    src_ed/model_instance.hpp:1547-1625: subsystem states numbered by first appearance in
+    the basis order, collapsed labels, no fermionic sign) against the dense ground state
+    restated the same way; unit trace, symmetric, positive."""
+    from pyqcm_amd import qcm
+    m, vals = TM.chain(4, U=4.0, mu=2.0, tp=-0.2)
+    a = register(with_sites(m, 4), vals, "R0:N4:S0")
+    rho, st = qcm.density_matrix([0, 1], a)
+    H = O.dense_hamiltonian(m, vals, 2, 2)[0]
+    psi = np.linalg.eigh(H)[1][:, 0]
+    labels = O.sector_states(4, 2, 2)  # up | down << 4, the factorized basis order
+    b64 = [(int(x) & 15) | ((int(x) >> 4) << 32) for x in labels]
+    mask = (3 | (3 << 32))
+    ia, ib, binA, members = {}, {}, [], []
+    for k, b in enumerate(b64):
+        A, B = b & mask, b & ~mask
+        if A not in ia:
+            ia[A] = len(binA)
+            binA.append(A)
+        if B not in ib:
+            ib[B] = len(members)
+            members.append([])
+        members[ib[B]].append((ia[A], k))
+    ref = np.zeros((len(binA), len(binA)))
+    for v in members:
+        for a_, i in v:
+            for b_, j in v:
+                ref[a_, b_] += psi[i] * psi[j]
+    # the sign of the eigenvector is arbitrary: rho is quadratic in it
+    np.testing.assert_allclose(rho.real, ref, atol=1e-10)
+    coll = [(A & 3) | (((A >> 32) & 3) << 32) for A in binA]
+    np.testing.assert_array_equal(st, np.array(coll, np.uint64))
+    assert abs(np.trace(rho).real - 1.0) < 1e-10
+    assert np.linalg.eigvalsh(rho.real).min() > -1e-12
 
 
 def test_print_wavefunction(gpu_lib):
