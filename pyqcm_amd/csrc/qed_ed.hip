@@ -2956,40 +2956,46 @@ int qed_ed_print_wavefunction(uint64_t label, char* buf, int buflen, int* needed
     // model_instance::print_wavefunction, src_ed/model_instance.hpp:1472-1485, and
     // state::write_wavefunction, src_ed/state.hpp:117-131: per state a header line, then,
     // when the sector dimension is at most max_dim_print, |psi_i|^2, psi_i and the basis
-    // state written as the down-spin then the up-spin occupations, lowest orbital first
-    // (ED_factorized_basis::print_state, src_ed/ED_basis.cpp:167-170; PrintBinaryDouble,
-    // src_ed/binary_state.cpp:104-112)
+    // state, lowest orbital first; for the factorized format 'F' the down-spin then the
+    // up-spin occupations (ED_factorized_basis::print_state, src_ed/ED_basis.cpp:167-170;
+    // PrintBinaryDouble, src_ed/binary_state.cpp:104-112).
+    // The operator formats (not 'F') print through the reference's mixed basis
+    // (ED_mixed_basis::print_state, src_ed/ED_basis.cpp:391-399, trivial group): the state
+    // b = up | down << 32 as PrintBinaryDouble writes it, the up-spin occupations first.
     Instance& I = inst(label);
     require_trivial_group(I, "print_wavefunction");
-    require_no_spinor(I, "print_wavefunction");
     low_energy_states(I);
     const int L = I.model->L;
+    const bool factorized = gc("Hamiltonian_format") == 'F' && !spinor_mixing(I.mixing);
     std::ostringstream o;
+    auto bits = [&](uint32_t v) {
+      std::string t(L, '0');
+      for (int k = 0; k < L; ++k) t[k] = (v >> k) & 1u ? '1' : '0';
+      return t;
+    };
     for (auto& st : I.states) {
-      o << "state\n" << sector_name(st->nup + st->ndn, st->nup - st->ndn) << '\t' << st->energy << '\t' << st->weight
-        << std::endl;
+      const bool sp = spinor_mixing(I.mixing);
+      o << "state\n" << (sp ? sector_name(st->secN, st->secS) : sector_name(st->nup + st->ndn, st->nup - st->ndn))
+        << '\t' << st->energy << '\t' << st->weight << std::endl;
       if (!st->psi.p) continue;
-      auto binom = [](int n, int k) {
-        uint64_t c = 1;
-        for (int i = 1; i <= k; ++i) c = c * (uint64_t)(n - k + i) / (uint64_t)i;
-        return c;
-      };
-      if (binom(L, st->nup) * binom(L, st->ndn) > (uint64_t)gi("max_dim_print")) continue;
-      SectorPtr H = make_sector(I, st->nup, st->ndn);
+      SectorPtr H = state_sector(I, *st);
+      if (H->D > (uint64_t)gi("max_dim_print")) continue;
       std::vector<double> psi(H->D);
-      std::vector<uint32_t> bu(H->du), bd(H->dd);
       QED_HIP(hipMemcpy(psi.data(), st->psi.p, H->D * 8, hipMemcpyDeviceToHost));
-      QED_HIP(hipMemcpy(bu.data(), H->bup.p, H->du * 4, hipMemcpyDeviceToHost));
-      QED_HIP(hipMemcpy(bd.data(), H->bdn.p, H->dd * 4, hipMemcpyDeviceToHost));
-      auto bits = [&](uint32_t v) {
-        std::string t(L, '0');
-        for (int k = 0; k < L; ++k) t[k] = (v >> k) & 1u ? '1' : '0';
-        return t;
-      };
+      std::vector<uint64_t> basis(H->D);
+      if (sp) {
+        uint64_t n = 0;
+        if (qed_general_basis(H.get(), basis.data(), &n) != 0) fail(qed_last_error());
+      } else {
+        std::vector<uint32_t> bu(H->du), bd(H->dd);
+        QED_HIP(hipMemcpy(bu.data(), H->bup.p, H->du * 4, hipMemcpyDeviceToHost));
+        QED_HIP(hipMemcpy(bd.data(), H->bdn.p, H->dd * 4, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < H->D; ++i) basis[i] = (uint64_t)bu[i % H->du] | ((uint64_t)bd[i / H->du] << 32);
+      }
       for (uint64_t i = 0; i < H->D; ++i) {
-        const uint32_t up = bu[i % H->du], dn = bd[i / H->du];
-        o << std::abs(psi[i]) * std::abs(psi[i]) << '\t' << psi[i] << '\t' << '|' << bits(dn) << ',' << bits(up) << '>'
-          << '\n';
+        const uint32_t up = (uint32_t)basis[i], dn = (uint32_t)(basis[i] >> 32);
+        o << std::abs(psi[i]) * std::abs(psi[i]) << '\t' << psi[i] << '\t' << '|'
+          << (factorized ? bits(dn) : bits(up)) << ',' << (factorized ? bits(up) : bits(dn)) << '>' << '\n';
       }
     }
     const std::string out = o.str();

@@ -215,13 +215,23 @@ def test_reduced_density_matrix(gpu_lib):
 
 def test_print_wavefunction(gpu_lib):
     """4-site chain, N = 4, S = 0 (D = 36 <= max_dim_print): the header, and per basis
-    state |psi|^2, psi and |down,up> with the lowest orbital first, against the oracle's
-    ground state (printed with six significant digits) and its half bases in rank order.
-    Above max_dim_print only the header is printed."""
+    state |psi|^2, psi and, for the factorized format 'F', |down,up> with the lowest
+    orbital first (the operator formats print |up,down>, the mixed basis' order), against
+    the oracle's ground state (printed with six significant digits) and its half bases in
+    rank order. Above max_dim_print only the header is printed."""
     from pyqcm_amd import qcm
     m, vals = TM.chain(4, U=4.0, mu=2.0)
     lab = register(with_sites(m, 4), vals, "R0:N4:S0")
-    text = qcm.print_wavefunction(lab)
+    text_e = qcm.print_wavefunction(lab)
+    qcm.set_global_parameter("Hamiltonian_format", "F")
+    try:
+        text = qcm.print_wavefunction(lab)
+    finally:
+        qcm.set_global_parameter("Hamiltonian_format", "E")
+    for le, lf in zip(text_e.strip("\n").split("\n")[2:], text.strip("\n").split("\n")[2:]):
+        se, sf = le.split("\t")[2], lf.split("\t")[2]
+        dn, up = sf[1:-1].split(",")
+        assert se == "|%s,%s>" % (up, dn)
     lines = text.strip("\n").split("\n")
     assert lines[0] == "state"
     sec, e, w = lines[1].split("\t")

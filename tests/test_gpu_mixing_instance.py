@@ -531,3 +531,29 @@ def test_hopping_matrix_full_with_a_paired_bath(gpu_lib):
     np.testing.assert_allclose(np.diag(Hd[d:, d:]).real, E, atol=1e-14)
     assert np.abs(H0[d:, d:] - np.diag(np.diag(H0[d:, d:]))).max() > 0.1
     np.testing.assert_allclose(np.linalg.eigvalsh(H0), np.linalg.eigvalsh(Hd), atol=1e-12)
+
+
+def test_nambu_print_wavefunction(gpu_lib):
+    """print_wavefunction of a Nambu instance: the sector string with its code ("R0:S0"),
+    and per basis state of the mixed-number sector |psi|^2, psi and |up,down> (the mixed
+    basis' PrintBinaryDouble, src_ed/ED_basis.cpp:391-399), against the dense ground state
+    of the same states in the same order."""
+    from pyqcm_amd import qcm
+    m, vals, n = chain(d=0.5, s=0.2)
+    lab = register(m, vals, "R0:S0", n)
+    qcm.set_global_parameter("max_dim_print", 100)
+    try:
+        lines = qcm.print_wavefunction(lab).strip("\n").split("\n")
+    finally:
+        qcm.set_global_parameter("max_dim_print", 64)
+    states = O.block_states(m.L, blocks(m.L, S=0))
+    psi = np.linalg.eigh(O.dense_hamiltonian(m, vals, 0, 0, states=states)[0])[1][:, 0]
+    assert lines[0] == "state" and lines[1].split("\t")[0] == "R0:S0"
+    assert len(lines) == 2 + len(states)
+    L = m.L
+    bits = lambda v: "".join("1" if (v >> k) & 1 else "0" for k in range(L))
+    for i, ln in enumerate(lines[2:]):
+        p2, amp, st = ln.split("\t")
+        up, dn = int(states[i]) & ((1 << L) - 1), int(states[i]) >> L
+        assert st == "|%s,%s>" % (bits(up), bits(dn))
+        assert abs(float(p2) - psi[i] ** 2) < 1e-5
