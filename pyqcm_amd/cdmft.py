@@ -247,8 +247,10 @@ class CDMFTProblem:
     def __init__(self, model, varia, sectors, grid=None, kside=32, graph=True, nambu=None):
         self.model = model
         # the spinor of the fit: None until the first solve tells the instance's mixing
-        # (1: pairing terms, the Nambu spinor; 0 or 4: one spin block)
+        # (1: pairing terms, the Nambu spinor; 2: spin flip, the spinor (c_up, c_down);
+        # 0 or 4: one spin block); nambu is true for both spinors, spinor_mixing says which
         self.nambu = nambu
+        self.spinor_mixing = 1 if nambu else 0
         self.graph = graph  # replay the batched Lanczos steps as hipGraphs
         self.varia = [v[:-2] if v.endswith("_1") else v for v in varia]
         self.sectors = sectors
@@ -265,11 +267,12 @@ class CDMFTProblem:
 
     def _check_mixing(self, inst):
         m = inst.mixing
-        if m not in (0, 1):
-            raise RuntimeError("CDMFT takes the normal and Nambu mixings (instance mixing %d)" % m)
+        if m not in (0, 1, 2):
+            raise RuntimeError("CDMFT takes the normal, Nambu and spin-flip mixings (instance mixing %d)" % m)
         if self.nambu is None:
-            self.nambu = m == 1
-        elif self.nambu != (m == 1):
+            self.nambu = m in (1, 2)
+            self.spinor_mixing = m if m in (1, 2) else 0
+        elif self.nambu != (m in (1, 2)) or (self.nambu and self.spinor_mixing != m):
             raise RuntimeError("the instance's mixing changed during CDMFT (%d)" % m)
 
     def host(self, inst):
@@ -301,7 +304,7 @@ class CDMFTProblem:
         return self.sectors(values) if callable(self.sectors) else self.sectors
 
     def distance(self, Ghost, X):
-        eb, tcb = self.model.diagonal_bath(self.varia, X, bool(self.nambu))
+        eb, tcb = self.model.diagonal_bath(self.varia, X, mixing=self.spinor_mixing)
         return LT.cdmft_distance_batch(self.grid.wr, self.grid.cdmft_weight, Ghost, eb, tcb,
                                        1 if self.nambu else 0)
 
@@ -310,7 +313,7 @@ class CDMFTProblem:
         one device call (qed_cdmft_distance_multi), values returned per request."""
         pts = np.concatenate([np.atleast_2d(p) for _, p in requests])
         host_of = np.concatenate([np.full(len(np.atleast_2d(p)), h, np.int32) for h, p in requests])
-        eb, tcb = self.model.diagonal_bath(self.varia, pts, bool(self.nambu))
+        eb, tcb = self.model.diagonal_bath(self.varia, pts, mixing=self.spinor_mixing)
         d = LT.cdmft_distance_multi(self.grid.wr, self.grid.cdmft_weight, np.asarray(hosts),
                                     host_of, eb, tcb, 1 if self.nambu else 0)
         out, o = [], 0

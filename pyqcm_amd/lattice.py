@@ -525,6 +525,27 @@ class LatticeModel:
         self.bath_ops[name] = (B, C)
         self.cluster_ops.append(name)
 
+    def spin_flip_bath_blocks(self, name):
+        """(B (2nb x 2nb), C (2n x 2nb)): the bath-bath and cluster-bath blocks of the spinor
+        (c_up, c_down) matrices of a one-body cluster operator at unit value, the spin-flip
+        mixing's rule (the operator's 2L x 2L matrix restricted to the orbitals,
+        one_body_operator.hpp:326-411)."""
+        el, kind = self.bath_elements[name]
+        if kind != "one-body":
+            raise RuntimeError("operator %s: the spin-flip mixing takes one-body bath operators" % name)
+        n, nb, L = self.n, self.n_bath, self.L
+        F = np.zeros((2 * L, 2 * L))
+        for r, c, v in el:
+            r, c = r - 1, c - 1
+            if r < c:
+                F[r, c] += v
+                F[c, r] += v
+            elif r == c:
+                F[r, r] += v
+        S = list(range(n)) + list(range(L, L + n))
+        Bt = list(range(n, L)) + list(range(L + n, 2 * L))
+        return F[np.ix_(Bt, Bt)].copy(), F[np.ix_(S, Bt)].copy()
+
     def nambu_bath_blocks(self, name):
         """(B (2nb x 2nb), C (2n x 2nb)): the bath-bath and cluster-bath blocks of the Nambu
         spinor matrices of a cluster operator at unit value, by the rules of the cluster's
@@ -565,14 +586,15 @@ class LatticeModel:
             Cn[n:, nb:] = -dd[S, Bt]
         return Bn, Cn
 
-    def diagonal_bath(self, names, X, nambu=False):
+    def diagonal_bath(self, names, X, nambu=False, mixing=None):
         """(eb (B x nb), tcb (B x n x nb)) for rows X (B x len(names)) of bath parameters;
         a non-diagonal bath matrix is diagonalized as in the reference's hybridization
         function (src_ed/model_instance.hpp:1004-1018). With nambu, the spinor forms:
         eb (B x 2nb), tcb (B x 2n x 2nb)."""
         X = np.atleast_2d(np.asarray(X, float))
-        if nambu:
-            blocks = [self.nambu_bath_blocks(nm) for nm in names]
+        mixing = (1 if nambu else 0) if mixing is None else mixing
+        if mixing in (1, 2):
+            blocks = [(self.nambu_bath_blocks if mixing == 1 else self.spin_flip_bath_blocks)(nm) for nm in names]
             E = np.einsum("bi,ijk->bjk", X, np.array([b[0] for b in blocks]))
             T = np.einsum("bi,ijk->bjk", X, np.array([b[1] for b in blocks]))
             eb, U = np.linalg.eigh(E)

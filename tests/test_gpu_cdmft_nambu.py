@@ -104,8 +104,7 @@ def test_nambu_cdmft_loop_reaches_the_reference_fixed_point(gpu_lib, golden):
 def test_nambu_fit_recovers_a_planted_paired_bath(gpu_lib):
     from pyqcm_amd import cdmft as C
     M = model()
-    P = C.CDMFTProblem(M, VARIA, "R0:S0", grid=C.frequency_grid("matsubara", (2, 50)))
-    P.nambu = True
+    P = C.CDMFTProblem(M, VARIA, "R0:S0", grid=C.frequency_grid("matsubara", (2, 50)), nambu=True)
     xs = np.array([0.9, -1.3, 0.55, 0.4, 0.25, 0.15])
     eb, tcb = M.diagonal_bath(VARIA, xs[None], nambu=True)
     z = 1j * P.grid.wr
@@ -140,3 +139,33 @@ def test_batched_nambu_cdmft_equals_plain_loop(gpu_lib):
         # difference can end the loop one iteration earlier
         np.testing.assert_allclose(batch.x[i], p.x, atol=2e-4)
         assert abs(p.x[4]) > 1e-3
+
+
+def test_spin_flip_cdmft_without_field_reaches_the_reference_fixed_point(gpu_lib, golden):
+    """The loop in the spin-flip spinor (c_up, c_down): the bath fit over the spinor's
+    blocks (LatticeModel.spin_flip_bath_blocks), distance factor 1. With a vanishing
+    field along x the loop reaches the recorded fixed point of cdmft.tsv (mu = 2), and
+    Gamma(z) from the fit's blocks equals the solver's."""
+    from pyqcm_amd import cdmft as C
+    from pyqcm_amd import lattice as LT
+    from pyqcm_amd import qcm
+    row = golden["chain2_4b_cdmft"]["rows"][0]
+    T = LT.chain_tiling(2).spin_flip("hx")
+    M = LT.LatticeModel("sfL2_4b_%d" % next(_names), T, n_bath=4, lattice_ops=("t", "mu", "hx"))
+    M.add_cluster_operator("tb1", [(1, 3, -1.0), (2, 4, -1.0), (7, 9, -1.0), (8, 10, -1.0)])
+    M.add_cluster_operator("tb2", [(1, 5, -1.0), (2, 6, -1.0), (7, 11, -1.0), (8, 12, -1.0)])
+    M.add_cluster_operator("eb1", [(3, 3, 1.0), (4, 4, 1.0), (9, 9, 1.0), (10, 10, 1.0)])
+    M.add_cluster_operator("eb2", [(5, 5, 1.0), (6, 6, 1.0), (11, 11, 1.0), (12, 12, 1.0)])
+    P = C.CDMFTProblem(M, VARIA[:4], "R0:N6", grid=C.frequency_grid("matsubara", (2, 50)))
+    vals = {"t": 1.0, "mu": 2.0, "U": 4.0, "hx": 1e-12}
+    x0 = np.array([1.0, -1.0, 0.5, 0.5])
+    inst = P.solve(vals, x0)
+    assert P.nambu and P.spinor_mixing == 2
+    z = np.array([0.3j, 1.1j])
+    eb, tcb = M.diagonal_bath(VARIA[:4], x0[None], mixing=2)
+    Gam = np.array([(tcb[0] / (zz - eb[0])) @ tcb[0].T for zz in z])
+    np.testing.assert_allclose(Gam, qcm.hybridization_functionC(z, 0, inst.label), atol=1e-12)
+    run = C.CDMFT(P, vals, x0)
+    ref = np.array([float(row[k]) for k in ("eb1_1", "eb2_1", "tb1_1", "tb2_1")])
+    print("spin-flip loop", run.x, "reference", ref, "iterations", run.niter)
+    np.testing.assert_allclose(run.x, ref, atol=2e-4)
