@@ -197,3 +197,56 @@ def test_batched_cdmft_falls_back_to_the_second_start():
     for i in range(3):
         np.testing.assert_allclose(B.x[i], [1.0, 1.0], atol=1e-4)
     assert B.niter[1] > 2  # both attempts counted
+
+
+def _ring_bdg(N, t, mu, D, S, hx):
+    """Real-space one-body matrix of an N-site ring in the spinor (c_up, c+_down) (Nambu,
+    D and S singlet pairing on the bonds and sites, v (c_i up c_j down + h.c.) giving -v in
+    both off-diagonal blocks) or (c_up, c_down) (spin flip, hx along x)."""
+    if hx is None:
+        H = np.zeros((2 * N, 2 * N))
+        for i in range(N):
+            j = (i + 1) % N
+            H[i, j] = H[j, i] = -t
+            H[N + i, N + j] = H[N + j, N + i] = t
+            H[i, i], H[N + i, N + i] = -mu, mu
+            H[i, N + i] = H[N + i, i] = -S
+            for a, b in ((i, j), (j, i)):
+                H[a, N + b] += -D
+                H[N + b, a] += -D
+        return H
+    H = np.zeros((2 * N, 2 * N))
+    for i in range(N):
+        j = (i + 1) % N
+        H[i, j] = H[j, i] = H[N + i, N + j] = H[N + j, N + i] = -t
+        H[i, i] = H[N + i, N + i] = -mu
+        H[i, N + i] = H[N + i, i] = hx
+    return H
+
+
+@pytest.mark.parametrize("form", ["nambu", "spin_flip"])
+def test_tiling_spinor_terms_synthesize_the_ring(form):
+    """The tiling's spinor Fourier terms of a 2-site cluster, summed over the two
+    superlattice wavevectors of a 4-site ring with phases exp(2 pi i k (R - R')), give the
+    ring's real-space spinor matrix built bond by bond: the inter-cluster pairing, hole
+    and spin-flip blocks and their phases."""
+    T = LT.chain_tiling(2)
+    if form == "nambu":
+        T.pairing("D", (1, 0, 0)).pairing("S", (0, 0, 0))
+        vals, mixing = {"t": 1.0, "mu": 0.6, "D": 0.3, "S": 0.2}, 1
+        ref = _ring_bdg(4, 1.0, 0.6, 0.3, 0.2, None)
+    else:
+        T.spin_flip("hx")
+        vals, mixing = {"t": 1.0, "mu": 0.6, "hx": 0.4}, 2
+        ref = _ring_bdg(4, 1.0, 0.6, 0.0, 0.0, 0.4)
+    k = np.array([[0.0, 0, 0], [0.5, 0, 0]])
+    hk = T.tk(vals, k, mixing=mixing)
+    got = np.zeros((8, 8), complex)
+    for s1 in range(4):
+        for s2 in range(4):
+            i, R, j, Rp = s1 % 2, s1 // 2, s2 % 2, s2 // 2
+            ph = np.exp(2j * np.pi * k[:, 0] * (R - Rp))
+            for h1 in (0, 1):
+                for h2 in (0, 1):
+                    got[s1 + 4 * h1, s2 + 4 * h2] = 0.5 * np.sum(ph * hk[:, i + 2 * h1, j + 2 * h2])
+    np.testing.assert_allclose(got, ref, atol=1e-14)
