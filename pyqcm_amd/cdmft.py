@@ -248,7 +248,8 @@ class CDMFTProblem:
         self.model = model
         # the spinor of the fit: None until the first solve tells the instance's mixing
         # (1: pairing terms, the Nambu spinor; 2: spin flip, the spinor (c_up, c_down);
-        # 0 or 4: one spin block); nambu is true for both spinors, spinor_mixing says which
+        # 4: the two spin blocks side by side; 0: one spin block); nambu is true for every
+        # two-block form, spinor_mixing says which
         self.nambu = nambu
         self.spinor_mixing = 1 if nambu else 0
         self.graph = graph  # replay the batched Lanczos steps as hipGraphs
@@ -267,15 +268,24 @@ class CDMFTProblem:
 
     def _check_mixing(self, inst):
         m = inst.mixing
-        if m not in (0, 1, 2):
-            raise RuntimeError("CDMFT takes the normal, Nambu and spin-flip mixings (instance mixing %d)" % m)
+        if m not in (0, 1, 2, 4):
+            raise RuntimeError("CDMFT takes the normal, Nambu, spin-flip and up/down mixings "
+                               "(instance mixing %d)" % m)
         if self.nambu is None:
-            self.nambu = m in (1, 2)
-            self.spinor_mixing = m if m in (1, 2) else 0
-        elif self.nambu != (m in (1, 2)) or (self.nambu and self.spinor_mixing != m):
+            self.nambu = m in (1, 2, 4)
+            self.spinor_mixing = m if m in (1, 2, 4) else 0
+        elif self.nambu != (m in (1, 2, 4)) or (self.nambu and self.spinor_mixing != m):
             raise RuntimeError("the instance's mixing changed during CDMFT (%d)" % m)
 
     def host(self, inst):
+        """G_host; for the up/down mixing the two spin blocks side by side (2n x 2n)."""
+        if inst.mixing == 4:
+            up = inst.cdmft_host(self.grid.wr, self.kside, 0)
+            dn = inst.cdmft_host(self.grid.wr, self.kside, 1)
+            n = up.shape[1]
+            out = np.zeros((up.shape[0], 2 * n, 2 * n), complex)
+            out[:, :n, :n], out[:, n:, n:] = up, dn
+            return out
         return inst.cdmft_host(self.grid.wr, self.kside)
 
     def solve_batch(self, values_list, x_list, graph=None):
@@ -306,7 +316,7 @@ class CDMFTProblem:
     def distance(self, Ghost, X):
         eb, tcb = self.model.diagonal_bath(self.varia, X, mixing=self.spinor_mixing)
         return LT.cdmft_distance_batch(self.grid.wr, self.grid.cdmft_weight, Ghost, eb, tcb,
-                                       1 if self.nambu else 0)
+                                       self.spinor_mixing)
 
     def distance_multi(self, hosts, requests):
         """requests: list of (host index, points (m x nvar)); all points are evaluated in
@@ -315,7 +325,7 @@ class CDMFTProblem:
         host_of = np.concatenate([np.full(len(np.atleast_2d(p)), h, np.int32) for h, p in requests])
         eb, tcb = self.model.diagonal_bath(self.varia, pts, mixing=self.spinor_mixing)
         d = LT.cdmft_distance_multi(self.grid.wr, self.grid.cdmft_weight, np.asarray(hosts),
-                                    host_of, eb, tcb, 1 if self.nambu else 0)
+                                    host_of, eb, tcb, self.spinor_mixing)
         out, o = [], 0
         for _, p in requests:
             m = len(np.atleast_2d(p))

@@ -370,6 +370,15 @@ def sef_batch(Geff, V0, nv, mats, k):
     return out
 
 
+def _distance_scale(mixing, wr, n):
+    """The distance's factor (src_qcm/CPT.cpp:548-552): 2 at mixing 0, 1/2 at mixing 3,
+    the frequency step, over dim^2; at mixing 4 the two spin blocks come side by side
+    (n = 2 dim), the reference adding their sums over its dim x dim matrices."""
+    f = {0: 2.0, 3: 0.5}.get(mixing, 1.0)
+    dim = n // 2 if mixing == 4 else n
+    return f * (wr[1] - wr[0]) / (dim * dim)
+
+
 def cdmft_distance_batch(wr, weights, Ghost, eb, tcb, mixing=0):
     """Distance function for B bath parameter sets at once (src_qcm/CPT.cpp:519-555):
     d = f dw / dim^2 sum_w weight(w) ||Gamma(i w) + G_host(i w)||_F^2, f = 2 at mixing 0,
@@ -384,7 +393,7 @@ def cdmft_distance_batch(wr, weights, Ghost, eb, tcb, mixing=0):
     B, n, nb = tcb.shape
     if eb.shape != (B, nb) or np.shape(Ghost) != (len(wr), n, n):
         raise ValueError("cdmft_distance_batch: inconsistent shapes")
-    scale = (2.0 if mixing == 0 else 1.0) * (wr[1] - wr[0]) / (n * n)
+    scale = _distance_scale(mixing, wr, n)
     d = np.zeros(B)
     tcm = np.ascontiguousarray(np.swapaxes(tcb, 1, 2)).ravel()  # (i, m) at i + n m
     _lib.call("qed_cdmft_distance_batch", n, nb, len(wr), _lib.dp(wr), _lib.dp(weights),
@@ -404,7 +413,7 @@ def cdmft_distance_multi(wr, weights, hosts, host_of, eb, tcb, mixing=0):
     host_of = np.ascontiguousarray(host_of, np.int32)
     if hosts.shape[1:] != (len(wr), n, n) or eb.shape != (B, nb) or host_of.shape != (B,):
         raise ValueError("cdmft_distance_multi: inconsistent shapes")
-    scale = (2.0 if mixing == 0 else 1.0) * (wr[1] - wr[0]) / (n * n)
+    scale = _distance_scale(mixing, wr, n)
     d = np.zeros(B)
     tcm = np.ascontiguousarray(np.swapaxes(tcb, 1, 2)).ravel()
     _lib.call("qed_cdmft_distance_multi", n, nb, len(wr), _lib.dp(wr), _lib.dp(weights),
@@ -429,7 +438,7 @@ def cdmft_fit_batch(wr, weights, hosts, host_of, E, T, X0, initial_step=0.1, xat
     host_of = np.ascontiguousarray(host_of, np.int32)
     if hosts.shape[1:] != (len(wr), n, n) or E.shape != (nvar, nb) or X0.shape[1] != nvar:
         raise ValueError("cdmft_fit_batch: inconsistent shapes")
-    scale = (2.0 if mixing == 0 else 1.0) * (wr[1] - wr[0]) / (n * n)
+    scale = _distance_scale(mixing, wr, n)
     Tcm = np.ascontiguousarray(np.swapaxes(T, 1, 2)).ravel()
     x = np.zeros((B, nvar))
     f = np.zeros(B)
@@ -525,6 +534,29 @@ class LatticeModel:
         self.bath_ops[name] = (B, C)
         self.cluster_ops.append(name)
 
+    def up_down_bath_blocks(self, name):
+        """(B (2nb x 2nb), C (2n x 2nb)): the spin-up and spin-down bath-bath and
+        cluster-bath blocks of a one-body cluster operator at unit value, side by side
+        (the up/down mixing: the two spins solved apart, src_ed/model_instance.hpp:424-475)."""
+        el, kind = self.bath_elements[name]
+        n, nb, L = self.n, self.n_bath, self.L
+        Bn = np.zeros((2 * nb, 2 * nb))
+        Cn = np.zeros((2 * n, 2 * nb))
+        if kind != "one-body":
+            return Bn, Cn
+        for r, c, v in el:
+            r, c = r - 1, c - 1
+            if (r >= L) != (c >= L):
+                raise RuntimeError("operator %s couples the two spins" % name)
+            sp = 1 if r >= L else 0
+            r, c = r - sp * L, c - sp * L
+            for a, b in ((r, c), (c, r)) if r != c else ((r, c),):
+                if a >= n and b >= n:
+                    Bn[sp * nb + a - n, sp * nb + b - n] += v
+                elif a < n <= b:
+                    Cn[sp * n + a, sp * nb + b - n] += v
+        return Bn, Cn
+
     def spin_flip_bath_blocks(self, name):
         """(B (2nb x 2nb), C (2n x 2nb)): the bath-bath and cluster-bath blocks of the spinor
         (c_up, c_down) matrices of a one-body cluster operator at unit value, the spin-flip
@@ -593,6 +625,13 @@ class LatticeModel:
         eb (B x 2nb), tcb (B x 2n x 2nb)."""
         X = np.atleast_2d(np.asarray(X, float))
         mixing = (1 if nambu else 0) if mixing is None else mixing
+        if mixing == 4:
+            # both spin blocks side by side (up, then down)
+            blocks = [self.up_down_bath_blocks(nm) for nm in names]
+            E = np.einsum("bi,ijk->bjk", X, np.array([b[0] for b in blocks]))
+            T = np.einsum("bi,ijk->bjk", X, np.array([b[1] for b in blocks]))
+            eb, U = np.linalg.eigh(E)
+            return eb, T @ U
         if mixing in (1, 2):
             blocks = [(self.nambu_bath_blocks if mixing == 1 else self.spin_flip_bath_blocks)(nm) for nm in names]
             E = np.einsum("bi,ijk->bjk", X, np.array([b[0] for b in blocks]))
@@ -735,11 +774,13 @@ class LatticeInstance:
         k = self.model.tiling.kgrid(kside, shift=0.5)
         return self.cpt_green(k, 1j * np.asarray(wr, float), mode="average")
 
-    def cdmft_host(self, wr, kside=32):
-        """G_host(i w) = Gproj^-1 + Sigma_c + t_c - i w (src_qcm/CPT.cpp:356-424)."""
+    def cdmft_host(self, wr, kside=32, spin_down=0):
+        """G_host(i w) = Gproj^-1 + Sigma_c + t_c - i w (src_qcm/CPT.cpp:356-424), of one
+        spin block."""
         z = 1j * np.asarray(wr, float)
-        Gp = self.projected_green(wr, kside)
-        S = qcm.self_energyC(z, 0, self.label)
-        tc = qcm.hopping_matrix(0, 0, self.label)
+        k = self.model.tiling.kgrid(kside, shift=0.5)
+        Gp = self.cpt_green(k, z, mode="average", spin_down=spin_down)
+        S = qcm.self_energyC(z, spin_down, self.label)
+        tc = qcm.hopping_matrix(spin_down, 0, self.label)
         n = Gp.shape[1]
         return np.linalg.inv(Gp) + S + tc[None] - z[:, None, None] * np.eye(n)[None]

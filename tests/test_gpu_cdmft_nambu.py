@@ -169,3 +169,39 @@ def test_spin_flip_cdmft_without_field_reaches_the_reference_fixed_point(gpu_lib
     ref = np.array([float(row[k]) for k in ("eb1_1", "eb2_1", "tb1_1", "tb2_1")])
     print("spin-flip loop", run.x, "reference", ref, "iterations", run.niter)
     np.testing.assert_allclose(run.x, ref, atol=2e-4)
+
+
+def test_up_down_cdmft_reaches_the_reference_fixed_point(gpu_lib, golden):
+    """The up/down mixing (a spin-dependent bath term, here a vanishing bath field along z):
+    both spin blocks' hosts side by side, the bath fit over both spins' blocks, the
+    distance summing both (src_qcm/CPT.cpp:519-552, no spin factor). Its distance equals
+    the normal one for spin-symmetric baths, and the loop reaches the recorded fixed point
+    of cdmft.tsv (mu = 2)."""
+    from pyqcm_amd import cdmft as C
+    from pyqcm_amd import lattice as LT
+    row = golden["chain2_4b_cdmft"]["rows"][0]
+
+    def mk():
+        M = LT.LatticeModel("udL2_4b_%d" % next(_names), LT.chain_tiling(2), n_bath=4, lattice_ops=("t", "mu"))
+        M.add_cluster_operator("tb1", [(1, 3, -1.0), (2, 4, -1.0), (7, 9, -1.0), (8, 10, -1.0)])
+        M.add_cluster_operator("tb2", [(1, 5, -1.0), (2, 6, -1.0), (7, 11, -1.0), (8, 12, -1.0)])
+        M.add_cluster_operator("eb1", [(3, 3, 1.0), (4, 4, 1.0), (9, 9, 1.0), (10, 10, 1.0)])
+        M.add_cluster_operator("eb2", [(5, 5, 1.0), (6, 6, 1.0), (11, 11, 1.0), (12, 12, 1.0)])
+        M.add_cluster_operator("hzb", [(b, b, 1.0) for b in range(3, 7)] + [(b + L, b + L, -1.0) for b in range(3, 7)])
+        return M
+    grid = C.frequency_grid("matsubara", (2, 50))
+    P4 = C.CDMFTProblem(mk(), VARIA[:4], "R0:N6:S0", grid=grid)
+    P0 = C.CDMFTProblem(mk(), VARIA[:4], "R0:N6:S0", grid=grid)
+    base = {"t": 1.0, "mu": 2.0, "U": 4.0}
+    x0 = np.array([1.0, -1.0, 0.5, 0.5])
+    i4 = P4.solve(dict(base, hzb_1=1e-12), x0)
+    i0 = P0.solve(base, x0)
+    assert P4.spinor_mixing == 4 and P0.spinor_mixing == 0
+    X = np.array([x0, [1.2, -0.9, 0.6, 0.4]])
+    d4, d0 = P4.distance(P4.host(i4), X), P0.distance(P0.host(i0), X)
+    print("up/down distance / normal:", d4 / d0)
+    np.testing.assert_allclose(d4, d0, rtol=1e-8)
+    run = C.CDMFT(P4, dict(base, hzb_1=1e-12), x0)
+    ref = np.array([float(row[k]) for k in ("eb1_1", "eb2_1", "tb1_1", "tb2_1")])
+    print("up/down loop", run.x, "reference", ref, "iterations", run.niter)
+    np.testing.assert_allclose(run.x, ref, atol=2e-4)
